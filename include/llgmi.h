@@ -1,0 +1,187 @@
+/*
+ * llgmi.h -- C ABI of the MI355X (gfx950) LLG hot-path library (libllgmi.so).
+ *
+ * Drop-in boundary for llg-sim's uniform-grid path.  Each entry point names
+ * the Rust function it replaces (file:line in the llg-sim source tree).  The
+ * host-side entry points take the reference's own layout: AoS f64 vector
+ * fields, one [x, y, z] triple per cell, cell index j*nx + i
+ * (grid.rs:54-58, vector_field.rs:24-27), so a Rust `extern "C"` block can
+ * pass `VectorField2D::data.as_ptr()` straight through.
+ *
+ * Conventions
+ *  - Every call returns an int status: LLGMI_OK (0) or an error code; the
+ *    message of the last failure on the calling thread is llgmi_last_error().
+ *    Nothing aborts the process; invalid arguments are rejected before any
+ *    device work is queued.
+ *  - `add_*` functions accumulate into the output field, `compute_*` and
+ *    `build_*` overwrite it, exactly as the Rust names do.
+ *  - A context binds one grid, one precision and one GPU.  It owns the
+ *    cached demag operators (the reference's process-global DEMAG_CACHE,
+ *    demag_fft_uniform.rs:44-108, and DEMAG_MG_CACHE, demag_poisson_mg.rs:3603),
+ *    the RK45 scratch and FSAL state (RK45Scratch, llg.rs:126-189), and one HIP
+ *    stream.  A context is not thread-safe: callers serialise on it, as the
+ *    reference serialises on each operator's Mutex.
+ *  - Host entry points are synchronous.  Device entry points (llgmi_dev_*)
+ *    take SoA device buffers in the context's precision (component-major:
+ *    x[0..n), y[0..n), z[0..n)), are ordered on the context stream and return
+ *    once the work is queued, except where a host-visible result is returned
+ *    (the RK45 step returns eps/accepted/dt_used and therefore synchronises).
+ */
+#ifndef LLGMI_H
+#define LLGMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define LLGMI_OK 0
+#define LLGMI_ERR_INVALID 1     /* bad argument (null pointer, size, enum) */
+#define LLGMI_ERR_HIP 2         /* HIP runtime error (message has the HIP text) */
+#define LLGMI_ERR_UNSUPPORTED 3 /* valid for the reference, not built here */
+#define LLGMI_ERR_ALLOC 4       /* device allocation failed */
+
+/* ---- enums ---------------------------------------------------------------- */
+/* FieldMask (effective_field/mod.rs:37-45) */
+#define LLGMI_MASK_EXCH_ANIS 0
+#define LLGMI_MASK_EXCH_ANIS_DMI 1
+#define LLGMI_MASK_FULL 2
+/* DemagMethod (params.rs:67-92) */
+#define LLGMI_DEMAG_FFT 0
+#define LLGMI_DEMAG_MG 1
+#define LLGMI_DEMAG_DST 2 /* retired in the reference: routed to MG (demag.rs:128-141) */
+/* arithmetic of a context */
+#define LLGMI_F32 0
+#define LLGMI_F64 1
+
+/* ---- parameter structs (plain data, 8-byte aligned) ------------------------ */
+/* Grid2D (grid.rs:5-31) */
+typedef struct llgmi_grid {
+    int64_t nx, ny;
+    double dx, dy, dz;
+} llgmi_grid;
+
+/* Material (params.rs:94-109).  `dmi` is Option<f64>: has_dmi = 0 is None. */
+typedef struct llgmi_material {
+    double ms;            /* A/m */
+    double a_ex;          /* J/m */
+    double k_u;           /* J/m^3 */
+    double easy_axis[3];
+    int32_t has_dmi;
+    int32_t demag;        /* include the demag field */
+    double dmi;           /* J/m^2 */
+    int32_t demag_method; /* LLGMI_DEMAG_* */
+    int32_t reserved;
+} llgmi_material;
+
+/* LLGParams (params.rs:137-144) */
+typedef struct llgmi_params {
+    double gamma;    /* rad/(s T) */
+    double alpha;
+    double dt;       /* s; in/out for the RK45 step */
+    double b_ext[3]; /* T */
+} llgmi_params;
+
+typedef struct llgmi_ctx llgmi_ctx;
+
+/* ---- library ------------------------------------------------------------- */
+const char *llgmi_last_error(void);
+int llgmi_version(void);
+/* Number of HIP devices visible (0 when no GPU). */
+int llgmi_device_count(int *count);
+
+/* ---- context ---------------------------------------------------------------- */
+/* Creates a context for `grid` in `precision` (LLGMI_F32 / LLGMI_F64) on HIP
+ * device `device`.  Replaces RK45Scratch::new (llg.rs:159) plus the lazily
+ * built operator caches. */
+int llgmi_ctx_create(const llgmi_grid *grid, int precision, int device, llgmi_ctx **out);
+int llgmi_ctx_destroy(llgmi_ctx *ctx);
+/* The context's HIP stream (hipStream_t), for callers that order their own
+ * work or events against the library's kernels. */
+int llgmi_ctx_stream(llgmi_ctx *ctx, void **hip_stream);
+int llgmi_ctx_synchronize(llgmi_ctx *ctx);
+
+/* ---- host entry points (AoS f64, synchronous) -------------------------------- */
+/* build_h_eff_masked (effective_field/mod.rs:48-75): overwrite b_eff with
+ * Zeeman + exchange + anisotropy (+ DMI) (+ demag). */
+int llgmi_build_h_eff_masked(llgmi_ctx *ctx, const double *m, double *b_eff,
+                             const llgmi_params *params, const llgmi_material *mat,
+                             int field_mask);
+/* build_h_eff_masked_geom (effective_field/mod.rs:78-101); geom_mask may be
+ * NULL (= None); one byte per cell, nonzero = material. */
+int llgmi_build_h_eff_masked_geom(llgmi_ctx *ctx, const double *m, double *b_eff,
+                                  const llgmi_params *params, const llgmi_material *mat,
+                                  int field_mask, const uint8_t *geom_mask);
+/* build_h_eff (effective_field/mod.rs:104-112): FieldMask::Full. */
+int llgmi_build_h_eff(llgmi_ctx *ctx, const double *m, double *b_eff,
+                      const llgmi_params *params, const llgmi_material *mat);
+
+/* demag::add_demag_field_pbc (effective_field/demag.rs:70-143): accumulate.
+ * Dispatch follows mat->demag_method (MG for open boundaries, FFT when
+ * pbc_x or pbc_y > 0 or when the method is FFT).  Periodic images
+ * (pbc > 0) return LLGMI_ERR_UNSUPPORTED. */
+int llgmi_add_demag_field_pbc(llgmi_ctx *ctx, const double *m, double *b_eff,
+                              const llgmi_material *mat, uint64_t pbc_x, uint64_t pbc_y);
+/* demag::compute_demag_field_pbc (effective_field/demag.rs:146-156): overwrite. */
+int llgmi_compute_demag_field_pbc(llgmi_ctx *ctx, const double *m, double *out,
+                                  const llgmi_material *mat, uint64_t pbc_x, uint64_t pbc_y);
+/* demag::add_demag_field (effective_field/demag.rs:47-55), open boundaries. */
+int llgmi_add_demag_field(llgmi_ctx *ctx, const double *m, double *b_eff,
+                          const llgmi_material *mat);
+
+/* step_llg_rk45_recompute_field_masked_adaptive (llg.rs:795-1026).
+ * m (AoS f64) is advanced in place when the step is accepted; params->dt is
+ * replaced by the next step size; eps, accepted (0/1) and dt_used are the
+ * Rust tuple (f64, bool, f64).  FSAL state lives in the context. */
+int llgmi_rk45_step(llgmi_ctx *ctx, double *m, llgmi_params *params,
+                    const llgmi_material *mat, int field_mask, double max_err,
+                    double headroom, double dt_min, double dt_max, double *eps,
+                    int *accepted, double *dt_used);
+/* step_llg_rk45_recompute_field_adaptive (llg.rs:1029-1050): FieldMask::Full. */
+int llgmi_rk45_step_adaptive(llgmi_ctx *ctx, double *m, llgmi_params *params,
+                             const llgmi_material *mat, double max_err, double headroom,
+                             double dt_min, double dt_max, double *eps, int *accepted,
+                             double *dt_used);
+/* Drops the FSAL cache (what a fresh RK45Scratch::new gives). */
+int llgmi_rk45_reset(llgmi_ctx *ctx);
+
+/* ---- device-resident entry points (SoA, context precision) -------------------- */
+/* Same as llgmi_build_h_eff_masked_geom, on device buffers of 3*n elements.
+ * geom_mask_dev: device bytes or NULL.  Asynchronous on the context stream. */
+int llgmi_dev_build_h_eff(llgmi_ctx *ctx, const void *m_dev, void *b_dev,
+                          const llgmi_params *params, const llgmi_material *mat,
+                          int field_mask, const uint8_t *geom_mask_dev);
+/* Same as llgmi_add_demag_field_pbc on device buffers (accumulate). */
+int llgmi_dev_add_demag_field_pbc(llgmi_ctx *ctx, const void *m_dev, void *b_dev,
+                                  const llgmi_material *mat, uint64_t pbc_x, uint64_t pbc_y);
+/* Same as llgmi_rk45_step with the state in device memory (SoA). */
+int llgmi_dev_rk45_step(llgmi_ctx *ctx, void *m_dev, llgmi_params *params,
+                        const llgmi_material *mat, int field_mask, double max_err,
+                        double headroom, double dt_min, double dt_max, double *eps,
+                        int *accepted, double *dt_used);
+/* AoS f64 host <-> SoA device copies in the context precision (synchronous). */
+int llgmi_dev_upload(llgmi_ctx *ctx, const double *host_aos, void *dev_soa);
+int llgmi_dev_download(llgmi_ctx *ctx, const void *dev_soa, double *host_aos);
+
+/* ---- per-kernel timing (HIP events on the context stream) ---------------------- */
+/* Kernel classes timed when profiling is on. */
+#define LLGMI_K_FFT_ROW_FWD 0   /* row R2C of Ms*m (zero padded) */
+#define LLGMI_K_FFT_COL 1       /* column FFT, k-space tensor multiply, inverse column FFT */
+#define LLGMI_K_FFT_ROW_INV 2   /* row C2R to B_demag */
+#define LLGMI_K_LOCAL_FIELD 3   /* Zeeman + exchange + anisotropy + DMI (+ torque) */
+#define LLGMI_K_RK45_STAGE 4    /* stage state combination + normalise */
+#define LLGMI_K_RK45_ERR 5      /* error-norm reduction */
+#define LLGMI_K_MG 6            /* all multigrid kernels */
+#define LLGMI_K_COUNT 8
+int llgmi_profile_enable(llgmi_ctx *ctx, int on);
+int llgmi_profile_reset(llgmi_ctx *ctx);
+/* Sum of event-timed durations (ms) and number of launches of a class. */
+int llgmi_profile_read(llgmi_ctx *ctx, int kernel_class, double *total_ms, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

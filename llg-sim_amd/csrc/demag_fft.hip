@@ -1,0 +1,506 @@
+// FFT demag for the uniform 2D grid (Nz = 1), open boundaries.
+//
+// Same operator as llg-sim's Demag2D (effective_field/demag_fft_uniform.rs:434-769):
+//   B_i = IFFT( K^_ij * FFT(Ms m_j) ) restricted to the physical region,
+// with the MuMax-like face-charge kernel (demag_fft_uniform.rs:1013-1253)
+// evaluated in FP64 on the GPU.
+//
+// MI355X layout (differs from the reference's six full complex 2D FFTs):
+//  * Padded size px x py is the next power of two >= 2nx x 2ny.  A circular
+//    convolution on any period >= 2n-1 equals the reference's linear
+//    (2n-padded) convolution on the physical region.
+//  * The kernels are even (xx, yy, zz) or odd-odd (xy) in real space, so their
+//    transforms are real and quadrant-symmetric: only (px/2+1) x (py/2+1)
+//    real 4-vectors are stored (xy changes sign for ky > py/2).
+//  * Pass 1 (row R2C): two real rows per complex FFT, rows >= ny and
+//    columns >= nx are zero and never read.  The half spectrum (px/2+1
+//    columns) is written column-major, S[c][kx][j], j < ny only.
+//  * Pass 2 (column): per kx, the three components are loaded into LDS,
+//    zero-extended to py, transformed, multiplied by the 4-component tensor
+//    (scale 1/(px py) folded in), inverse transformed; only j < ny is stored.
+//  * Pass 3 (row C2R): two Hermitian rows per complex inverse FFT; the first
+//    nx outputs of each row are written as B_demag (SoA).
+#include "common.hpp"
+#include "internal.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace llgmi {
+
+static constexpr int kEmax = 32;  // complex elements held per thread in an LDS FFT
+
+// ---------------------------------------------------------------------------
+// Pass 1: row R2C.  Block = `pairs` pairs of rows of one component.
+// dst[comp*dst_cstride + k*nrows + j] for k <= L/2, j < nrows.
+// ---------------------------------------------------------------------------
+template <typename T, typename Tin>
+__global__ void __launch_bounds__(512)
+k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int ncols_valid,
+          int nrows, T scale, int L, int log2L, int pairs, const cpx<T> *__restrict__ tw,
+          cpx<T> *__restrict__ dst, int64_t dst_cstride) {
+    extern __shared__ unsigned char lds_raw[];
+    cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
+    const int comp = blockIdx.y;
+    const int j0 = blockIdx.x * pairs * 2;
+    const Tin *s = src + comp * src_cstride;
+    const int tot = pairs << log2L;
+    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        int p = idx >> log2L, i = idx & (L - 1);
+        int ja = j0 + 2 * p, jb = ja + 1;
+        T re = T(0), im = T(0);
+        if (i < ncols_valid) {
+            if (ja < nrows) re = scale * T(s[(int64_t)ja * row_stride + i]);
+            if (jb < nrows) im = scale * T(s[(int64_t)jb * row_stride + i]);
+        }
+        buf[idx] = {re, im};
+    }
+    __syncthreads();
+    lds_fft<T, false, kEmax>(buf, L, log2L, pairs, L, tw);
+    const int hx = (L >> 1) + 1;
+    const int nr = 2 * pairs;
+    cpx<T> *d = dst + comp * dst_cstride;
+    for (int idx = threadIdx.x; idx < hx * nr; idx += blockDim.x) {
+        int r = idx % nr, k = idx / nr;
+        int j = j0 + r;
+        if (j >= nrows) continue;
+        int p = r >> 1;
+        cpx<T> z = buf[(p << log2L) + k];
+        cpx<T> zc = buf[(p << log2L) + ((L - k) & (L - 1))];
+        cpx<T> X;
+        if ((r & 1) == 0) X = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
+        else X = {T(0.5) * (z.y + zc.y), T(0.5) * (zc.x - z.x)};
+        d[(int64_t)k * nrows + j] = X;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 2: column FFT, tensor multiply, inverse column FFT.  Block = one kx.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(512)
+k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, int L, int log2L,
+            const cpx<T> *__restrict__ tw, const quad4<T> *__restrict__ kern, T scale) {
+    extern __shared__ unsigned char lds_raw[];
+    cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
+    const int kx = blockIdx.x;
+    for (int idx = threadIdx.x; idx < 3 * L; idx += blockDim.x) {
+        int c = idx >> log2L, j = idx & (L - 1);
+        buf[idx] = j < ny ? S[((int64_t)c * hx + kx) * ny + j] : cpx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    lds_fft<T, false, kEmax>(buf, L, log2L, 3, L, tw);
+    const int hy = (L >> 1) + 1;
+    const quad4<T> *kc = kern + (int64_t)kx * hy;
+    for (int ky = threadIdx.x; ky < L; ky += blockDim.x) {
+        int kq = ky < hy ? ky : L - ky;
+        quad4<T> K = kc[kq];
+        T kxy = ky < hy ? K.xy : -K.xy;
+        cpx<T> mx = buf[ky], my = buf[L + ky], mz = buf[2 * L + ky];
+        buf[ky] = {scale * (K.xx * mx.x + kxy * my.x), scale * (K.xx * mx.y + kxy * my.y)};
+        buf[L + ky] = {scale * (kxy * mx.x + K.yy * my.x), scale * (kxy * mx.y + K.yy * my.y)};
+        buf[2 * L + ky] = {scale * (K.zz * mz.x), scale * (K.zz * mz.y)};
+    }
+    __syncthreads();
+    lds_fft<T, true, kEmax>(buf, L, log2L, 3, L, tw);
+    for (int idx = threadIdx.x; idx < 3 * ny; idx += blockDim.x) {
+        int c = idx / ny, j = idx - c * ny;
+        S[((int64_t)c * hx + kx) * ny + j] = buf[c * L + j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3: row C2R.  Block = `pairs` pairs of rows of one component.  The k = 0
+// and k = L/2 bins use their real parts only, which is what taking Re() of
+// the reference's full complex inverse transform amounts to.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(512)
+k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, int L, int log2L, int pairs,
+          const cpx<T> *__restrict__ tw, T *__restrict__ out, int64_t n) {
+    extern __shared__ unsigned char lds_raw[];
+    cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
+    const int comp = blockIdx.y;
+    const int j0 = blockIdx.x * pairs * 2;
+    const cpx<T> *s = S + (int64_t)comp * hx * ny;
+    const int half = L >> 1;
+    for (int idx = threadIdx.x; idx < hx * pairs; idx += blockDim.x) {
+        int p = idx % pairs, k = idx / pairs;
+        int ja = j0 + 2 * p, jb = ja + 1;
+        cpx<T> xa = ja < ny ? s[(int64_t)k * ny + ja] : cpx<T>{T(0), T(0)};
+        cpx<T> xb = jb < ny ? s[(int64_t)k * ny + jb] : cpx<T>{T(0), T(0)};
+        if (k == 0 || k == half) { xa.y = T(0); xb.y = T(0); }
+        cpx<T> *row = buf + (p << log2L);
+        row[k] = {xa.x - xb.y, xa.y + xb.x};
+        if (k != 0 && k != half) row[L - k] = {xa.x + xb.y, xb.x - xa.y};
+    }
+    __syncthreads();
+    lds_fft<T, true, kEmax>(buf, L, log2L, pairs, L, tw);
+    T *o = out + comp * n;
+    for (int idx = threadIdx.x; idx < pairs * nx; idx += blockDim.x) {
+        int p = idx / nx, i = idx - p * nx;
+        int ja = j0 + 2 * p;
+        cpx<T> z = buf[(p << log2L) + i];
+        if (ja < ny) o[(int64_t)ja * nx + i] = z.x;
+        if (ja + 1 < ny) o[(int64_t)(ja + 1) * nx + i] = z.y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One-time k-space kernel: forward column FFT (FP64) of the row spectra of the
+// real-space kernels and extraction of the real quadrant.  Block = (kx, comp).
+// ---------------------------------------------------------------------------
+template <typename Tout>
+__global__ void __launch_bounds__(512)
+k_col_extract(const cpx<double> *__restrict__ Sk, int hx, int L, int log2L,
+              const cpx<double> *__restrict__ tw, Tout *__restrict__ kern) {
+    extern __shared__ unsigned char lds_raw[];
+    cpx<double> *buf = reinterpret_cast<cpx<double> *>(lds_raw);
+    const int kx = blockIdx.x, comp = blockIdx.y;
+    const cpx<double> *s = Sk + ((int64_t)comp * hx + kx) * L;
+    for (int j = threadIdx.x; j < L; j += blockDim.x) buf[j] = s[j];
+    __syncthreads();
+    lds_fft<double, false, kEmax>(buf, L, log2L, 1, L, tw);
+    const int hy = (L >> 1) + 1;
+    for (int ky = threadIdx.x; ky < hy; ky += blockDim.x)
+        kern[((int64_t)kx * hy + ky) * 4 + comp] = Tout(buf[ky].x);
+}
+
+// ---------------------------------------------------------------------------
+// MuMax-like face-charge quadrature (demag_fft_uniform.rs:1139-1253), FP64.
+// ---------------------------------------------------------------------------
+struct QuadSetup {
+    int64_t nv, nw, nx, ny, nz;
+    double charge, pu1, cell[3];
+    int u, v, w;
+};
+
+__host__ __device__ inline double delta_cell_d(int64_t d) {
+    double a = (double)(d < 0 ? -d : d);
+    if (a > 0.0) a -= 1.0;
+    return a;
+}
+__host__ __device__ inline int64_t mumax_n_d(double x) {
+    return (int64_t)floor(fmax(x, 1.0) + 0.5);
+}
+
+__host__ __device__ inline QuadSetup quad_setup(int u, double dx, double dy, double dz,
+                                                int64_t sx, int64_t sy, double accuracy) {
+#pragma clang fp contract(off)
+    QuadSetup q;
+    q.u = u; q.v = (u + 1) % 3; q.w = (u + 2) % 3;
+    q.cell[0] = dx; q.cell[1] = dy; q.cell[2] = dz;
+    double lmin = fmin(fmin(dx, dy), dz);
+    double dxm = delta_cell_d(sx) * dx, dym = delta_cell_d(sy) * dy, dzm = 0.0 * dz;
+    double d = sqrt(dxm * dxm + dym * dym + dzm * dzm);
+    if (d == 0.0) d = lmin;
+    double max_size = d / accuracy;
+    q.nx = mumax_n_d(dx / max_size);
+    q.ny = mumax_n_d(dy / max_size);
+    q.nz = mumax_n_d(dz / max_size);
+    q.nv = 2 * mumax_n_d(q.cell[q.v] / max_size);
+    q.nw = 2 * mumax_n_d(q.cell[q.w] / max_size);
+    double scale = 1.0 / (double)(q.nv * q.nw * q.nx * q.ny * q.nz);
+    q.charge = q.cell[q.v] * q.cell[q.w] * scale;
+    q.pu1 = 0.5 * q.cell[u];
+    return q;
+}
+
+__host__ __device__ inline int64_t quad_points(double dx, double dy, double dz, int64_t sx,
+                                               int64_t sy, double accuracy) {
+    int64_t t = 0;
+    for (int u = 0; u < 3; ++u) {
+        QuadSetup q = quad_setup(u, dx, dy, dz, sx, sy, accuracy);
+        t += q.nv * q.nw * q.nx * q.ny * q.nz;
+    }
+    return t;
+}
+
+// Sum over sample points (ax, ay, az) for the pole pair at (pv, pw).
+__device__ inline void quad_inner(const QuadSetup &q, const double rc[3], double pv, double pw,
+                                  double h[3]) {
+#pragma clang fp contract(off)
+    const double PI_D = 3.14159265358979323846;
+    double pole[3] = {0.0, 0.0, 0.0};
+    pole[q.v] = pv;
+    pole[q.w] = pw;
+    for (int64_t ax = 0; ax < q.nx; ++ax) {
+        double rx = rc[0] - 0.5 * q.cell[0] + q.cell[0] / (2.0 * (double)q.nx) + (double)ax * (q.cell[0] / (double)q.nx);
+        for (int64_t ay = 0; ay < q.ny; ++ay) {
+            double ry = rc[1] - 0.5 * q.cell[1] + q.cell[1] / (2.0 * (double)q.ny) + (double)ay * (q.cell[1] / (double)q.ny);
+            for (int64_t az = 0; az < q.nz; ++az) {
+                double rz = rc[2] - 0.5 * q.cell[2] + q.cell[2] / (2.0 * (double)q.nz) + (double)az * (q.cell[2] / (double)q.nz);
+                pole[q.u] = q.pu1;
+                double r1x = rx - pole[0], r1y = ry - pole[1], r1z = rz - pole[2];
+                double r1 = sqrt(r1x * r1x + r1y * r1y + r1z * r1z);
+                double q1 = q.charge / (4.0 * PI_D * r1 * r1 * r1);
+                double hx1 = r1x * q1, hy1 = r1y * q1, hz1 = r1z * q1;
+                pole[q.u] = -q.pu1;
+                double r2x = rx - pole[0], r2y = ry - pole[1], r2z = rz - pole[2];
+                double r2 = sqrt(r2x * r2x + r2y * r2y + r2z * r2z);
+                double q2 = -q.charge / (4.0 * PI_D * r2 * r2 * r2);
+                h[0] += hx1 + r2x * q2;
+                h[1] += hy1 + r2y * q2;
+                h[2] += hz1 + r2z * q2;
+            }
+        }
+    }
+}
+
+__device__ inline double pole_pos(double c, int64_t n, int64_t i) {
+#pragma clang fp contract(off)
+    return -0.5 * c + c / (2.0 * (double)n) + (double)i * (c / (double)n);
+}
+
+// One thread per (base-quadrant displacement, source axis), summing serially in
+// the reference's loop order with FP contraction off, so every real-space
+// kernel value is the reference's own rounding of the face-charge sum (these
+// sums cancel heavily near the origin; a reordered sum differs at ~1e-12).
+// Threads are ordered by displacement, so the few heavy near-field sums share
+// the first waves.
+__global__ void __launch_bounds__(256)
+k_quad(double *__restrict__ base, int nx, int ny, double dx, double dy, double dz,
+       double accuracy, double mu0) {
+#pragma clang fp contract(off)
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)nx * ny * 3) return;
+    int64_t t = g / 3;
+    int u = (int)(g - 3 * t);
+    int64_t sx = t % nx, sy = t / nx;
+    double rc[3] = {(double)sx * dx, (double)sy * dy, 0.0};
+    QuadSetup q = quad_setup(u, dx, dy, dz, sx, sy, accuracy);
+    double h[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < q.nv; ++i) {
+        double pv = pole_pos(q.cell[q.v], q.nv, i);
+        for (int64_t j = 0; j < q.nw; ++j)
+            quad_inner(q, rc, pv, pole_pos(q.cell[q.w], q.nw, j), h);
+    }
+    if (u == 0) base[t * 4 + 0] = mu0 * h[0];
+    else if (u == 1) { base[t * 4 + 1] = mu0 * h[0]; base[t * 4 + 2] = mu0 * h[1]; }
+    else base[t * 4 + 3] = mu0 * h[2];
+}
+
+// Parity reflection into the padded real-space arrays (demag_fft_uniform.rs:1029-1071):
+// xx, yy, zz even; xy odd-odd and zero on both axes; bins outside |s| <= n-1 zero.
+__global__ void k_reflect(const double *__restrict__ base, int nx, int ny, int px, int py,
+                          double *__restrict__ kr) {
+    int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    int iy = blockIdx.y;
+    if (ix >= px) return;
+    int sx, sy;
+    bool okx = true, oky = true;
+    if (ix < nx) sx = ix;
+    else if (ix >= px - nx + 1) sx = ix - px;
+    else { sx = 0; okx = false; }
+    if (iy < ny) sy = iy;
+    else if (iy >= py - ny + 1) sy = iy - py;
+    else { sy = 0; oky = false; }
+    int64_t np = (int64_t)px * py, o = (int64_t)iy * px + ix;
+    if (!okx || !oky) {
+        for (int c = 0; c < 4; ++c) kr[c * np + o] = 0.0;
+        return;
+    }
+    int ax = sx < 0 ? -sx : sx, ay = sy < 0 ? -sy : sy;
+    const double *k = base + ((int64_t)ay * nx + ax) * 4;
+    double sg = (sx == 0 || sy == 0) ? 0.0 : (double)((sx > 0) == (sy > 0) ? 1 : -1);
+    kr[0 * np + o] = k[0];
+    kr[1 * np + o] = sg * k[1];
+    kr[2 * np + o] = k[2];
+    kr[3 * np + o] = k[3];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+template <typename T>
+static cpx<T> *make_twiddles(int L) {
+    std::vector<cpx<T>> h(L);
+    for (int t = 0; t < L; ++t) {
+        double a = -2.0 * M_PI * (double)t / (double)L;
+        h[t] = {T(std::cos(a)), T(std::sin(a))};
+    }
+    cpx<T> *d = nullptr;
+    LLGMI_HIP(hipMalloc(&d, sizeof(cpx<T>) * L));
+    LLGMI_HIP(hipMemcpy(d, h.data(), sizeof(cpx<T>) * L, hipMemcpyHostToDevice));
+    return d;
+}
+
+// Threads for an LDS FFT holding `elems` complex values: keep elems/threads <= kEmax.
+static int fft_threads(int64_t elems) {
+    int nt = 256;
+    while ((int64_t)nt * kEmax < elems && nt < 512) nt *= 2;
+    if ((int64_t)nt * kEmax < elems) throw Error(3, "FFT length too large for the LDS kernels");
+    return nt;
+}
+
+static constexpr size_t kLdsBudget = 64 * 1024;  // bytes per block for the row passes
+static constexpr size_t kLdsMax = 160 * 1024;
+
+// Dynamic LDS above the default per-launch limit must be opted into.
+template <typename K>
+static void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        LLGMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+template <typename T>
+static int row_pairs(int L, int nrows) {
+    int pairs = (int)(kLdsBudget / (sizeof(cpx<T>) * (size_t)L));
+    if (pairs < 1) pairs = 1;
+    int need = (nrows + 1) / 2;
+    if (pairs > need) pairs = need;
+    if (pairs > 8) pairs = 8;
+    return pairs;
+}
+
+template <typename T>
+DemagFFT<T>::~DemagFFT() {
+    release();
+}
+
+template <typename T>
+void DemagFFT<T>::release() {
+    void *ps[] = {tw_x, tw_y, kern, spec};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    tw_x = tw_y = nullptr;
+    kern = nullptr;
+    spec = nullptr;
+    built = false;
+}
+
+template <typename T>
+void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double dz_,
+                        hipStream_t stream) {
+    release();
+    nx = nx_; ny = ny_; dx = dx_; dy = dy_; dz = dz_;
+    px = next_pow2(2 * nx);
+    py = next_pow2(2 * ny);
+    const size_t maxL = kLdsMax / (3 * sizeof(cpx<T>));
+    if ((size_t)py > maxL || (size_t)px > 8192 || (size_t)py > 8192)
+        throw Error(3, "FFT demag: padded size " + std::to_string(px) + "x" + std::to_string(py) +
+                           " exceeds the LDS column pass of this precision");
+    if ((size_t)px > kLdsMax / sizeof(cpx<double>) || (size_t)py > kLdsMax / sizeof(cpx<double>))
+        throw Error(3, "FFT demag: padded size too large for the FP64 kernel build");
+    hx = px / 2 + 1;
+    hy = py / 2 + 1;
+    lx = ilog2(px);
+    ly = ilog2(py);
+    const double mu0 = 4.0 * M_PI * 1e-7;
+    const double acc = 10.0;  // DEMAG_ACCURACY (demag_fft_uniform.rs:432)
+
+    // 1) base-quadrant quadrature (FP64)
+    double *base = nullptr, *kr = nullptr;
+    cpx<double> *sk = nullptr, *twx64 = nullptr, *twy64 = nullptr;
+    try {
+        const int64_t nb = nx * ny;
+        LLGMI_HIP(hipMalloc(&base, sizeof(double) * 4 * nb));
+        {
+            int nt = 256;
+            int64_t nbk = (3 * nb + nt - 1) / nt;
+            hipLaunchKernelGGL(k_quad, dim3((unsigned)nbk), dim3(nt), 0, stream, base, (int)nx,
+                               (int)ny, dx, dy, dz, acc, mu0);
+            LLGMI_LAUNCH_CHECK();
+        }
+        // 2) reflect into padded real arrays
+        const int64_t np = px * py;
+        LLGMI_HIP(hipMalloc(&kr, sizeof(double) * 4 * np));
+        hipLaunchKernelGGL(k_reflect, dim3((unsigned)((px + 255) / 256), (unsigned)py), dim3(256), 0,
+                           stream, base, (int)nx, (int)ny, (int)px, (int)py, kr);
+        LLGMI_LAUNCH_CHECK();
+        // 3) row R2C (FP64) of the four arrays
+        twx64 = make_twiddles<double>((int)px);
+        twy64 = make_twiddles<double>((int)py);
+        LLGMI_HIP(hipMalloc(&sk, sizeof(cpx<double>) * 4 * hx * py));
+        {
+            int pairs = row_pairs<double>((int)px, (int)py);
+            int nt = fft_threads((int64_t)pairs * px);
+            size_t lds = sizeof(cpx<double>) * pairs * px;
+            dim3 grid((unsigned)((py + 2 * pairs - 1) / (2 * pairs)), 4);
+            hipLaunchKernelGGL((k_row_r2c<double, double>), grid, dim3(nt), lds, stream, kr, np,
+                               (int)px, (int)px, (int)py, 1.0, (int)px, lx, pairs, twx64, sk,
+                               (int64_t)hx * py);
+            LLGMI_LAUNCH_CHECK();
+        }
+        // 4) column FFT (FP64), real quadrant -> kern (T)
+        LLGMI_HIP(hipMalloc(&kern, sizeof(quad4<T>) * hx * hy));
+        {
+            int nt = fft_threads(py);
+            size_t lds = sizeof(cpx<double>) * py;
+            hipLaunchKernelGGL((k_col_extract<T>), dim3((unsigned)hx, 4), dim3(nt), lds, stream, sk,
+                               (int)hx, (int)py, ly, twy64, reinterpret_cast<T *>(kern));
+            LLGMI_LAUNCH_CHECK();
+        }
+        LLGMI_HIP(hipStreamSynchronize(stream));
+    } catch (...) {
+        if (base) (void)hipFree(base);
+        if (kr) (void)hipFree(kr);
+        if (sk) (void)hipFree(sk);
+        if (twx64) (void)hipFree(twx64);
+        if (twy64) (void)hipFree(twy64);
+        release();
+        throw;
+    }
+    (void)hipFree(base);
+    (void)hipFree(kr);
+    (void)hipFree(sk);
+    (void)hipFree(twx64);
+    (void)hipFree(twy64);
+
+    tw_x = make_twiddles<T>((int)px);
+    tw_y = make_twiddles<T>((int)py);
+    LLGMI_HIP(hipMalloc(&spec, sizeof(cpx<T>) * 3 * hx * ny));
+    pairs_fwd = row_pairs<T>((int)px, (int)ny);
+    nt_row = fft_threads((int64_t)pairs_fwd * px);
+    nt_col = fft_threads(3 * py);
+    allow_lds(k_col_demag<T>, sizeof(cpx<T>) * 3 * py);
+    built = true;
+}
+
+template <typename T>
+void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *prof) {
+    const int64_t n = nx * ny;
+    const int pairs = pairs_fwd;
+    const dim3 grid_rows((unsigned)((ny + 2 * pairs - 1) / (2 * pairs)), 3);
+    const size_t lds_row = sizeof(cpx<T>) * pairs * px;
+    {
+        ProfScope ps(prof, 0, stream);
+        hipLaunchKernelGGL((k_row_r2c<T, T>), grid_rows, dim3(nt_row), lds_row, stream, m, n,
+                           (int)nx, (int)nx, (int)ny, ms, (int)px, lx, pairs, tw_x, spec,
+                           (int64_t)hx * ny);
+        LLGMI_LAUNCH_CHECK();
+    }
+    {
+        ProfScope ps(prof, 1, stream);
+        const T scale = T(1.0 / ((double)px * (double)py));
+        hipLaunchKernelGGL((k_col_demag<T>), dim3((unsigned)hx), dim3(nt_col),
+                           sizeof(cpx<T>) * 3 * py, stream, spec, (int)hx, (int)ny, (int)py, ly,
+                           tw_y, kern, scale);
+        LLGMI_LAUNCH_CHECK();
+    }
+    {
+        ProfScope ps(prof, 2, stream);
+        hipLaunchKernelGGL((k_row_c2r<T>), grid_rows, dim3(nt_row), lds_row, stream, spec,
+                           (int)hx, (int)nx, (int)ny, (int)px, lx, pairs, tw_x, bd, n);
+        LLGMI_LAUNCH_CHECK();
+    }
+}
+
+template <typename T>
+void DemagFFT<T>::read_kernel(std::vector<double> &out) const {
+    std::vector<quad4<T>> h((size_t)(hx * hy));
+    LLGMI_HIP(hipMemcpy(h.data(), kern, sizeof(quad4<T>) * h.size(), hipMemcpyDeviceToHost));
+    out.resize(h.size() * 4);
+    for (size_t i = 0; i < h.size(); ++i) {
+        out[4 * i + 0] = (double)h[i].xx;
+        out[4 * i + 1] = (double)h[i].xy;
+        out[4 * i + 2] = (double)h[i].yy;
+        out[4 * i + 3] = (double)h[i].zz;
+    }
+}
+
+template struct DemagFFT<float>;
+template struct DemagFFT<double>;
+
+}  // namespace llgmi

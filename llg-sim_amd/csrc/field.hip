@@ -1,0 +1,360 @@
+// Local effective-field kernel (Zeeman, exchange, uniaxial anisotropy,
+// interfacial DMI, optional demag addend, optional LLG torque) and the RK45
+// stage / error-norm kernels.
+//
+// Field terms follow llg-sim's effective_field/ (zeeman.rs:6-11,
+// exchange.rs:45-222, anisotropy.rs:10-25, dmi.rs:67-209, summed in the order
+// of build_h_eff_masked, mod.rs:48-75); the torque is llg_rhs (llg.rs:20-32);
+// the stage combination and error norm are those of
+// step_llg_rk45_recompute_field_masked_adaptive (llg.rs:869-997).
+//
+// One thread per cell, SoA component-major arrays; neighbour reads hit the
+// same or adjacent cache lines (x) or a row already pulled into L2 by the
+// neighbouring block row (y), so the kernel streams m once and B once.
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace llgmi {
+
+template <typename T>
+__device__ __forceinline__ void ghost_x(const T *m, T n_x, T eta, T dx, T *o) {
+    o[0] = m[0] + n_x * dx * (-eta * m[2]);
+    o[1] = m[1];
+    o[2] = m[2] + n_x * dx * (eta * m[0]);
+}
+template <typename T>
+__device__ __forceinline__ void ghost_y(const T *m, T n_y, T eta, T dy, T *o) {
+    o[0] = m[0];
+    o[1] = m[1] + n_y * dy * (-eta * m[2]);
+    o[2] = m[2] + n_y * dy * (eta * m[1]);
+}
+
+template <typename T, bool GEOM, bool DEMAG, bool TORQUE>
+__global__ void __launch_bounds__(256)
+k_local_field(LocalArgs<T> a, const T *__restrict__ m, const T *__restrict__ bd,
+              const uint8_t *__restrict__ geom, T *__restrict__ b, T *__restrict__ kout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int nx = a.nx, ny = a.ny;
+    if (i >= nx) return;
+    const int64_t n = (int64_t)nx * ny;
+    const int64_t c = (int64_t)j * nx + i;
+    const T *mx = m, *my = m + n, *mz = m + 2 * n;
+    T m0[3] = {mx[c], my[c], mz[c]};
+    T bv[3] = {a.bext[0], a.bext[1], a.bext[2]};  // b = 0 + B_ext
+    const bool inside = !GEOM || geom[c] != 0;
+
+    if ((a.exch_on || a.dmi_on) && inside) {
+        // raw neighbours (valid = exists in the domain and is material)
+        T nl[3], nr[3], nd[3], nu[3];
+        bool vl = false, vr = false, vd = false, vu = false;
+        bool el = nx > 1 && i == 0, er = nx > 1 && i == nx - 1;   // domain edges
+        bool ed = ny > 1 && j == 0, eu = ny > 1 && j == ny - 1;
+        if (nx > 1 && i > 0) {
+            int64_t q = c - 1;
+            vl = !GEOM || geom[q] != 0;
+            if (vl) { nl[0] = mx[q]; nl[1] = my[q]; nl[2] = mz[q]; }
+        }
+        if (nx > 1 && i < nx - 1) {
+            int64_t q = c + 1;
+            vr = !GEOM || geom[q] != 0;
+            if (vr) { nr[0] = mx[q]; nr[1] = my[q]; nr[2] = mz[q]; }
+        }
+        if (ny > 1 && j > 0) {
+            int64_t q = c - nx;
+            vd = !GEOM || geom[q] != 0;
+            if (vd) { nd[0] = mx[q]; nd[1] = my[q]; nd[2] = mz[q]; }
+        }
+        if (ny > 1 && j < ny - 1) {
+            int64_t q = c + nx;
+            vu = !GEOM || geom[q] != 0;
+            if (vu) { nu[0] = mx[q]; nu[1] = my[q]; nu[2] = mz[q]; }
+        }
+        if (a.exch_on) {
+            // exchange.rs:96-204: domain edge or masked neighbour -> chiral ghost
+            // when D != 0, else the centre value; nx == 1 -> centre value.
+            T xm[3], xp[3], ym[3], yp[3];
+            if (vl) { xm[0] = nl[0]; xm[1] = nl[1]; xm[2] = nl[2]; }
+            else if (a.has_eta && nx > 1) ghost_x(m0, T(-1), a.eta, a.dx, xm);
+            else { xm[0] = m0[0]; xm[1] = m0[1]; xm[2] = m0[2]; }
+            if (vr) { xp[0] = nr[0]; xp[1] = nr[1]; xp[2] = nr[2]; }
+            else if (a.has_eta && nx > 1) ghost_x(m0, T(1), a.eta, a.dx, xp);
+            else { xp[0] = m0[0]; xp[1] = m0[1]; xp[2] = m0[2]; }
+            if (vd) { ym[0] = nd[0]; ym[1] = nd[1]; ym[2] = nd[2]; }
+            else if (a.has_eta && ny > 1) ghost_y(m0, T(-1), a.eta, a.dy, ym);
+            else { ym[0] = m0[0]; ym[1] = m0[1]; ym[2] = m0[2]; }
+            if (vu) { yp[0] = nu[0]; yp[1] = nu[1]; yp[2] = nu[2]; }
+            else if (a.has_eta && ny > 1) ghost_y(m0, T(1), a.eta, a.dy, yp);
+            else { yp[0] = m0[0]; yp[1] = m0[1]; yp[2] = m0[2]; }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                T d2x = nx > 1 ? (xp[q] - T(2) * m0[q] + xm[q]) * a.inv_dx2 : T(0);
+                T d2y = ny > 1 ? (yp[q] - T(2) * m0[q] + ym[q]) * a.inv_dy2 : T(0);
+                bv[q] += a.exch_coeff * (d2x + d2y);
+            }
+        }
+        if (a.anis_on) {
+            T mdotu = m0[0] * a.u[0] + m0[1] * a.u[1] + m0[2] * a.u[2];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) bv[q] += a.anis_coeff * mdotu * a.u[q];
+        }
+        if (a.dmi_on) {
+            // dmi.rs:123-181: domain edge -> chiral ghost; masked neighbour -> centre.
+            T xm[3], xp[3], ym[3], yp[3];
+            if (el) ghost_x(m0, T(-1), a.eta, a.dx, xm);
+            else if (vl) { xm[0] = nl[0]; xm[1] = nl[1]; xm[2] = nl[2]; }
+            else { xm[0] = m0[0]; xm[1] = m0[1]; xm[2] = m0[2]; }
+            if (er) ghost_x(m0, T(1), a.eta, a.dx, xp);
+            else if (vr) { xp[0] = nr[0]; xp[1] = nr[1]; xp[2] = nr[2]; }
+            else { xp[0] = m0[0]; xp[1] = m0[1]; xp[2] = m0[2]; }
+            if (ed) ghost_y(m0, T(-1), a.eta, a.dy, ym);
+            else if (vd) { ym[0] = nd[0]; ym[1] = nd[1]; ym[2] = nd[2]; }
+            else { ym[0] = m0[0]; ym[1] = m0[1]; ym[2] = m0[2]; }
+            if (eu) ghost_y(m0, T(1), a.eta, a.dy, yp);
+            else if (vu) { yp[0] = nu[0]; yp[1] = nu[1]; yp[2] = nu[2]; }
+            else { yp[0] = m0[0]; yp[1] = m0[1]; yp[2] = m0[2]; }
+            T dmz_dx = nx > 1 ? (xp[2] - xm[2]) * a.inv_2dx : T(0);
+            T dmz_dy = ny > 1 ? (yp[2] - ym[2]) * a.inv_2dy : T(0);
+            T dmx_dx = nx > 1 ? (xp[0] - xm[0]) * a.inv_2dx : T(0);
+            T dmy_dy = ny > 1 ? (yp[1] - ym[1]) * a.inv_2dy : T(0);
+            bv[0] += a.dmi_pref * dmz_dx;
+            bv[1] += a.dmi_pref * dmz_dy;
+            bv[2] += -a.dmi_pref * (dmx_dx + dmy_dy);
+        }
+    } else if (a.anis_on) {
+        // anisotropy is not masked (mod.rs:91)
+        T mdotu = m0[0] * a.u[0] + m0[1] * a.u[1] + m0[2] * a.u[2];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) bv[q] += a.anis_coeff * mdotu * a.u[q];
+    }
+    if (DEMAG) {
+        bv[0] += bd[c];
+        bv[1] += bd[n + c];
+        bv[2] += bd[2 * n + c];
+    }
+    b[c] = bv[0];
+    b[n + c] = bv[1];
+    b[2 * n + c] = bv[2];
+    if (TORQUE) {
+        // llg_rhs: -gamma/(1+alpha^2) [m x B + alpha m x (m x B)]
+        T cx = m0[1] * bv[2] - m0[2] * bv[1];
+        T cy = m0[2] * bv[0] - m0[0] * bv[2];
+        T cz = m0[0] * bv[1] - m0[1] * bv[0];
+        T ccx = m0[1] * cz - m0[2] * cy;
+        T ccy = m0[2] * cx - m0[0] * cz;
+        T ccz = m0[0] * cy - m0[1] * cx;
+        kout[c] = a.rhs_pref * (cx + a.alpha * ccx);
+        kout[n + c] = a.rhs_pref * (cy + a.alpha * ccy);
+        kout[2 * n + c] = a.rhs_pref * (cz + a.alpha * ccz);
+    }
+}
+
+template <typename T>
+void launch_local_field(const LocalArgs<T> &a, const T *m, const T *bd, const uint8_t *geom,
+                        T *b, T *k, hipStream_t s) {
+    dim3 blk(256), grid((unsigned)((a.nx + 255) / 256), (unsigned)a.ny);
+    const bool g = geom != nullptr, d = bd != nullptr, t = a.torque != 0;
+#define LLGMI_LF(G, D, Q) \
+    hipLaunchKernelGGL((k_local_field<T, G, D, Q>), grid, blk, 0, s, a, m, bd, geom, b, k)
+    if (g) {
+        if (d) { if (t) LLGMI_LF(true, true, true); else LLGMI_LF(true, true, false); }
+        else { if (t) LLGMI_LF(true, false, true); else LLGMI_LF(true, false, false); }
+    } else {
+        if (d) { if (t) LLGMI_LF(false, true, true); else LLGMI_LF(false, true, false); }
+        else { if (t) LLGMI_LF(false, false, true); else LLGMI_LF(false, false, false); }
+    }
+#undef LLGMI_LF
+    LLGMI_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion between the reference's AoS f64 and device SoA
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_aos_to_soa(const double *__restrict__ aos, T *__restrict__ soa, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    soa[i] = T(aos[3 * i]);
+    soa[n + i] = T(aos[3 * i + 1]);
+    soa[2 * n + i] = T(aos[3 * i + 2]);
+}
+template <typename T>
+__global__ void k_soa_to_aos(const T *__restrict__ soa, double *__restrict__ aos, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    aos[3 * i] = (double)soa[i];
+    aos[3 * i + 1] = (double)soa[n + i];
+    aos[3 * i + 2] = (double)soa[2 * n + i];
+}
+template <typename T>
+__global__ void k_soa_add_to_aos(const T *__restrict__ soa, double *__restrict__ aos, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    aos[3 * i] += (double)soa[i];
+    aos[3 * i + 1] += (double)soa[n + i];
+    aos[3 * i + 2] += (double)soa[2 * n + i];
+}
+
+template <typename T>
+void launch_aos_to_soa(const double *aos, T *soa, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_aos_to_soa<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, aos, soa, n);
+    LLGMI_LAUNCH_CHECK();
+}
+template <typename T>
+void launch_soa_to_aos(const T *soa, double *aos, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_soa_to_aos<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, soa, aos, n);
+    LLGMI_LAUNCH_CHECK();
+}
+template <typename T>
+void launch_soa_add_to_aos(const T *soa, double *aos, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_soa_add_to_aos<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, soa, aos, n);
+    LLGMI_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void k_accumulate(const T *__restrict__ src, T *__restrict__ dst, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] += src[i];
+}
+template <typename T>
+void launch_soa_accumulate(const T *src, T *dst, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL((k_accumulate<T>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, src, dst, count);
+    LLGMI_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// RK45 stage state: v = m; v = v + c_q k_q (q in order); normalize (zero -> +z)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_rk_stage(const T *__restrict__ m, StageArgs<T> a, T *__restrict__ out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T v0 = m[i], v1 = m[n + i], v2 = m[2 * n + i];
+    for (int q = 0; q < a.nk; ++q) {
+        const T *k = a.k[q];
+        T c = a.c[q];
+        v0 = v0 + c * k[i];
+        v1 = v1 + c * k[n + i];
+        v2 = v2 + c * k[2 * n + i];
+    }
+    T n2 = v0 * v0 + v1 * v1 + v2 * v2;
+    if (n2 == T(0)) {
+        v0 = T(0); v1 = T(0); v2 = T(1);
+    } else {
+        T inv = T(1) / sqrt(n2);
+        v0 *= inv; v1 *= inv; v2 *= inv;
+    }
+    out[i] = v0;
+    out[n + i] = v1;
+    out[2 * n + i] = v2;
+}
+
+template <typename T>
+void launch_rk_stage(const T *m, const StageArgs<T> &a, T *out, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_rk_stage<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, a, out, n);
+    LLGMI_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Error norm (llg.rs:948-997), FP64.  reference form: |tau5 - tau4| with the
+// two weighted sums formed separately (FP64 contexts, bit-for-bit the
+// reference's arithmetic order); combined form: |sum_j (b_j - bh_j) k_j| with
+// FP64 coefficients (FP32 contexts, where forming tau5 and tau4 from FP32
+// torques would cancel the estimate away).
+// ---------------------------------------------------------------------------
+struct DP {
+    static constexpr double B1 = 35.0 / 384.0, B3 = 500.0 / 1113.0, B4 = 125.0 / 192.0,
+                            B5 = -2187.0 / 6784.0, B6 = 11.0 / 84.0;
+    static constexpr double BH1 = 5179.0 / 57600.0, BH3 = 7571.0 / 16695.0, BH4 = 393.0 / 640.0,
+                            BH5 = -92097.0 / 339200.0, BH6 = 187.0 / 2100.0, BH7 = 1.0 / 40.0;
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+template <typename T, bool COMBINED>
+__global__ void __launch_bounds__(256)
+k_rk_err(const T *__restrict__ k1, const T *__restrict__ k3, const T *__restrict__ k4,
+         const T *__restrict__ k5, const T *__restrict__ k6, const T *__restrict__ k7, int64_t n,
+         double *__restrict__ partial) {
+    __shared__ double red[4];
+    double local = 0.0;
+    const int64_t n3 = 3 * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        double d = 0.0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            int64_t o = q * n + t;
+            double a1 = k1[o], a3 = k3[o], a4 = k4[o], a5 = k5[o], a6 = k6[o], a7 = k7[o];
+            double e;
+            if (COMBINED) {
+                e = (DP::B1 - DP::BH1) * a1 + (DP::B3 - DP::BH3) * a3 + (DP::B4 - DP::BH4) * a4 +
+                    (DP::B5 - DP::BH5) * a5 + (DP::B6 - DP::BH6) * a6 - DP::BH7 * a7;
+            } else {
+                double th = DP::B1 * a1 + DP::B3 * a3 + DP::B4 * a4 + DP::B5 * a5 + DP::B6 * a6;
+                double tl = DP::BH1 * a1 + DP::BH3 * a3 + DP::BH4 * a4 + DP::BH5 * a5 + DP::BH6 * a6 + DP::BH7 * a7;
+                e = th - tl;
+            }
+            d = q == 0 ? fabs(e) : fmax(d, fabs(e));
+        }
+        if (d > local) local = d;
+    }
+    (void)n3;
+    local = wave_max(local);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, red[w]);
+        partial[blockIdx.x] = v;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_max_reduce(const double *__restrict__ partial, int np,
+                                                     double *__restrict__ out) {
+    __shared__ double red[16];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) v = fmax(v, partial[i]);
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, red[w]);
+        out[0] = r;
+    }
+}
+
+template <typename T>
+void launch_rk_err(const T *k1, const T *k3, const T *k4, const T *k5, const T *k6, const T *k7,
+                   int64_t n, bool combined, double *partial, double *out, hipStream_t s) {
+    int64_t want = (n + 255) / 256;
+    int nb = (int)(want < kErrBlocks ? (want > 0 ? want : 1) : kErrBlocks);
+    if (combined)
+        hipLaunchKernelGGL((k_rk_err<T, true>), dim3(nb), dim3(256), 0, s, k1, k3, k4, k5, k6, k7, n, partial);
+    else
+        hipLaunchKernelGGL((k_rk_err<T, false>), dim3(nb), dim3(256), 0, s, k1, k3, k4, k5, k6, k7, n, partial);
+    LLGMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(1024), 0, s, partial, nb, out);
+    LLGMI_LAUNCH_CHECK();
+}
+
+#define LLGMI_INST(T)                                                                        \
+    template void launch_local_field<T>(const LocalArgs<T> &, const T *, const T *,          \
+                                        const uint8_t *, T *, T *, hipStream_t);             \
+    template void launch_aos_to_soa<T>(const double *, T *, int64_t, hipStream_t);           \
+    template void launch_soa_to_aos<T>(const T *, double *, int64_t, hipStream_t);           \
+    template void launch_soa_add_to_aos<T>(const T *, double *, int64_t, hipStream_t);       \
+    template void launch_soa_accumulate<T>(const T *, T *, int64_t, hipStream_t);            \
+    template void launch_rk_stage<T>(const T *, const StageArgs<T> &, T *, int64_t, hipStream_t); \
+    template void launch_rk_err<T>(const T *, const T *, const T *, const T *, const T *,    \
+                                   const T *, int64_t, bool, double *, double *, hipStream_t);
+LLGMI_INST(float)
+LLGMI_INST(double)
+#undef LLGMI_INST
+
+}  // namespace llgmi

@@ -1,0 +1,153 @@
+// Host-side declarations shared by the translation units of libllgmi.so.
+#pragma once
+
+#include "common.hpp"
+
+#include <vector>
+
+namespace llgmi {
+
+// ---------------------------------------------------------------------------
+// Per-kernel-class timing with HIP events recorded on the launch stream.
+// ---------------------------------------------------------------------------
+struct Profiler {
+    bool on = false;
+    struct Pending {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms[8] = {0};
+    int64_t count[8] = {0};
+
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        LLGMI_HIP(hipEventCreate(&e));
+        return e;
+    }
+    void drain() {
+        for (auto &p : pending) {
+            LLGMI_HIP(hipEventSynchronize(p.b));
+            float ms = 0.f;
+            LLGMI_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+            total_ms[p.cls] += ms;
+            count[p.cls] += 1;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        drain();
+        for (int i = 0; i < 8; ++i) { total_ms[i] = 0; count[i] = 0; }
+    }
+    ~Profiler() {
+        for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+struct ProfScope {
+    Profiler *p;
+    int cls;
+    hipStream_t s;
+    hipEvent_t a{}, b{};
+    ProfScope(Profiler *p_, int c, hipStream_t s_) : p(p_), cls(c), s(s_) {
+        if (p && p->on) {
+            if (p->pending.size() > 4096) p->drain();
+            a = p->get();
+            b = p->get();
+            LLGMI_HIP(hipEventRecord(a, s));
+        }
+    }
+    ~ProfScope() {
+        if (p && p->on) {
+            if (hipEventRecord(b, s) == hipSuccess) p->pending.push_back({cls, a, b});
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// FFT demag operator for one grid (Ms independent), precision T.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DemagFFT {
+    bool built = false;
+    int64_t nx = 0, ny = 0, px = 0, py = 0, hx = 0, hy = 0;
+    int lx = 0, ly = 0;
+    double dx = 0, dy = 0, dz = 0;
+    cpx<T> *tw_x = nullptr, *tw_y = nullptr;
+    quad4<T> *kern = nullptr;  // [hx][hy]
+    cpx<T> *spec = nullptr;    // [3][hx][ny]
+    int pairs_fwd = 1, nt_row = 256, nt_col = 256;
+
+    ~DemagFFT();
+    void release();
+    void build(int64_t nx, int64_t ny, double dx, double dy, double dz, hipStream_t stream);
+    // bd = B_demag (overwrite) for M = ms * m; m and bd are SoA with 3*nx*ny entries.
+    void field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *prof);
+    void read_kernel(std::vector<double> &out) const;
+};
+
+// ---------------------------------------------------------------------------
+// Local field kernel arguments (all coefficients precomputed in FP64).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LocalArgs {
+    int nx, ny;
+    T bext[3];
+    int exch_on;           // A != 0 && Ms != 0
+    T exch_coeff;          // 2A/Ms
+    T inv_dx2, inv_dy2;
+    int has_eta;           // DMI != 0 (chiral ghosts in exchange)
+    T eta, dx, dy;
+    int anis_on;
+    T anis_coeff;          // 2Ku/Ms
+    T u[3];
+    int dmi_on;            // mask includes DMI and D != 0, A != 0, Ms != 0
+    T dmi_pref;            // 2D/Ms
+    T inv_2dx, inv_2dy;
+    int torque;            // also write k = llg_rhs(m, B)
+    T rhs_pref;            // -gamma/(1+alpha^2)
+    T alpha;
+};
+
+template <typename T>
+void launch_local_field(const LocalArgs<T> &a, const T *m, const T *bd, const uint8_t *geom,
+                        T *b, T *k, hipStream_t s);
+
+template <typename T>
+void launch_aos_to_soa(const double *aos, T *soa, int64_t n, hipStream_t s);
+template <typename T>
+void launch_soa_to_aos(const T *soa, double *aos, int64_t n, hipStream_t s);
+template <typename T>
+void launch_soa_add_to_aos(const T *soa, double *aos, int64_t n, hipStream_t s);
+// dst[i] += src[i], i < count
+template <typename T>
+void launch_soa_accumulate(const T *src, T *dst, int64_t count, hipStream_t s);
+
+// RK45 stage state: out = normalize(m + sum_q c[q] k[q]) applied in order q = 0..nk-1.
+template <typename T>
+struct StageArgs {
+    const T *k[6];
+    T c[6];
+    int nk;
+};
+template <typename T>
+void launch_rk_stage(const T *m, const StageArgs<T> &a, T *out, int64_t n, hipStream_t s);
+
+// Error norm max_{cell,comp} |tau5 - tau4| over k1,k3,k4,k5,k6,k7 (FP64 arithmetic).
+// partial must hold kErrBlocks doubles; result written to out[0].
+static constexpr int kErrBlocks = 1024;
+template <typename T>
+void launch_rk_err(const T *k1, const T *k3, const T *k4, const T *k5, const T *k6,
+                   const T *k7, int64_t n, bool combined, double *partial, double *out,
+                   hipStream_t s);
+
+}  // namespace llgmi

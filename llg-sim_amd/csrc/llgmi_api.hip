@@ -1,0 +1,639 @@
+// C ABI of libllgmi.so (include/llgmi.h): contexts, argument checks, dispatch
+// to the per-precision engine, and the RK45 driver with its FP64 controller.
+#include "../../include/llgmi.h"
+#include "common.hpp"
+#include "internal.hpp"
+#include "mg.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace llgmi;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+// LLG_DEMAG_METHOD, parsed once (demag.rs:35-44, params.rs:76-83).
+int env_demag_method() {
+    static int v = [] {
+        const char *s = std::getenv("LLG_DEMAG_METHOD");
+        if (!s) return -1;
+        std::string t(s);
+        auto b = t.find_first_not_of(" \t\r\n");
+        auto e = t.find_last_not_of(" \t\r\n");
+        t = b == std::string::npos ? std::string() : t.substr(b, e - b + 1);
+        for (auto &ch : t) ch = (char)std::tolower((unsigned char)ch);
+        if (t == "fft" || t == "fftuniform" || t == "uniform") return LLGMI_DEMAG_FFT;
+        if (t == "mg" || t == "multigrid" || t == "poissonmg" || t == "poisson_mg") return LLGMI_DEMAG_MG;
+        if (t == "dst" || t == "poissondst" || t == "poisson_dst" || t == "dst_open") return LLGMI_DEMAG_DST;
+        return -1;
+    }();
+    return v;
+}
+
+int resolved_demag_method(const llgmi_material &mat) {
+    int e = env_demag_method();
+    return e >= 0 ? e : mat.demag_method;
+}
+
+struct EngineBase {
+    virtual ~EngineBase() = default;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Engine: all device state of a context, in precision T.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Engine : EngineBase {
+    llgmi_grid g;
+    int64_t n;
+    hipStream_t stream;
+    Profiler *prof;
+
+    T *d_m = nullptr, *d_b = nullptr, *d_bd = nullptr;
+    double *d_aos = nullptr;
+    uint8_t *d_geom = nullptr;
+    std::vector<double> h_tmp;
+
+    DemagFFT<T> fft;
+    std::unique_ptr<MGSolver> mg;
+
+    // RK45 scratch (RK45Scratch, llg.rs:126-189): k1..k7, stage state, m7.
+    T *d_k[8] = {nullptr};
+    T *d_stage = nullptr, *d_m7 = nullptr;
+    double *d_partial = nullptr, *d_err = nullptr;
+    double *h_err = nullptr;  // pinned
+    bool fsal_valid = false;
+    int last_mask = -1;
+    double last_bext[3] = {0, 0, 0};
+
+    Engine(const llgmi_grid &g_, hipStream_t s, Profiler *p) : g(g_), n(g_.nx * g_.ny), stream(s), prof(p) {}
+
+    ~Engine() override {
+        void *ps[] = {d_m, d_b, d_bd, d_aos, d_geom, d_stage, d_m7, d_partial, d_err};
+        for (void *q : ps)
+            if (q) (void)hipFree(q);
+        for (int i = 0; i < 8; ++i)
+            if (d_k[i]) (void)hipFree(d_k[i]);
+        if (h_err) (void)hipHostFree(h_err);
+    }
+
+    template <typename U>
+    void alloc(U *&p, size_t count) {
+        if (!p) LLGMI_HIP(hipMalloc(&p, sizeof(U) * std::max<size_t>(count, 1)));
+    }
+
+    void ensure_io() {
+        alloc(d_m, 3 * n);
+        alloc(d_b, 3 * n);
+        alloc(d_aos, 3 * n);
+    }
+
+    void ensure_fft() {
+        if (!fft.built) fft.build(g.nx, g.ny, g.dx, g.dy, g.dz, stream);
+    }
+
+    MGSolver &ensure_mg() {
+        if (!mg) mg.reset(new MGSolver(g.nx, g.ny, g.dx, g.dy, g.dz, stream, prof));
+        return *mg;
+    }
+
+    // B_demag into d_bd (overwrite).  method resolved by the caller.
+    void demag_into_bd(const T *m, const llgmi_material &mat, int method, uint64_t pbc_x,
+                       uint64_t pbc_y) {
+        alloc(d_bd, 3 * n);
+        bool use_fft = method == LLGMI_DEMAG_FFT || pbc_x > 0 || pbc_y > 0;
+        if (use_fft) {
+            if (pbc_x > 0 || pbc_y > 0)
+                throw Error(LLGMI_ERR_UNSUPPORTED, "FFT demag with periodic images (pbc_x/pbc_y > 0) is not built");
+            ensure_fft();
+            fft.field(m, T(mat.ms), d_bd, stream, prof);
+        } else {
+            ensure_mg().template demag_field<T>(m, mat.ms, d_bd);
+        }
+    }
+
+    LocalArgs<T> local_args(const llgmi_params &p, const llgmi_material &mat, int mask, bool torque) const {
+        LocalArgs<T> a{};
+        a.nx = (int)g.nx;
+        a.ny = (int)g.ny;
+        for (int q = 0; q < 3; ++q) a.bext[q] = T(p.b_ext[q]);
+        a.exch_on = (mat.a_ex != 0.0 && mat.ms != 0.0) ? 1 : 0;
+        a.exch_coeff = T(mat.ms != 0.0 ? 2.0 * mat.a_ex / mat.ms : 0.0);
+        a.inv_dx2 = T(1.0 / (g.dx * g.dx));
+        a.inv_dy2 = T(1.0 / (g.dy * g.dy));
+        bool dnz = mat.has_dmi && mat.dmi != 0.0;
+        a.has_eta = (dnz && mat.a_ex != 0.0) ? 1 : 0;
+        a.eta = T(a.has_eta ? mat.dmi / (2.0 * mat.a_ex) : 0.0);
+        a.dx = T(g.dx);
+        a.dy = T(g.dy);
+        a.anis_on = (mat.k_u != 0.0 && mat.ms != 0.0) ? 1 : 0;
+        a.anis_coeff = T(mat.ms != 0.0 ? 2.0 * mat.k_u / mat.ms : 0.0);
+        for (int q = 0; q < 3; ++q) a.u[q] = T(mat.easy_axis[q]);
+        bool inc_dmi = mask == LLGMI_MASK_EXCH_ANIS_DMI || mask == LLGMI_MASK_FULL;
+        a.dmi_on = (inc_dmi && dnz && mat.ms != 0.0 && mat.a_ex != 0.0) ? 1 : 0;
+        a.dmi_pref = T(mat.ms != 0.0 ? 2.0 * mat.dmi / mat.ms : 0.0);
+        a.inv_2dx = T(1.0 / (2.0 * g.dx));
+        a.inv_2dy = T(1.0 / (2.0 * g.dy));
+        a.torque = torque ? 1 : 0;
+        a.rhs_pref = T(-p.gamma / (1.0 + p.alpha * p.alpha));
+        a.alpha = T(p.alpha);
+        return a;
+    }
+
+    // build_h_eff_masked[_geom] on device buffers; optional torque output.
+    void field(const T *m, T *b, const llgmi_params &p, const llgmi_material &mat, int mask,
+               const uint8_t *geom, T *kout) {
+        const bool demag = mask == LLGMI_MASK_FULL && mat.demag;
+        if (demag) demag_into_bd(m, mat, resolved_demag_method(mat), 0, 0);
+        LocalArgs<T> a = local_args(p, mat, mask, kout != nullptr);
+        ProfScope ps(prof, LLGMI_K_LOCAL_FIELD, stream);
+        launch_local_field<T>(a, m, demag ? d_bd : nullptr, geom, b, kout, stream);
+    }
+
+    // The staging buffers must exist before a destination is named: callers
+    // pass d_m etc. by value, so ensure_io() runs before the call (a null dst
+    // is rejected here rather than written through on the device).
+    void upload(const double *h_aos, T *dst) {
+        if (!dst || !d_aos) throw Error(LLGMI_ERR_INVALID, "internal: upload before allocation");
+        LLGMI_HIP(hipMemcpyAsync(d_aos, h_aos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, stream));
+        launch_aos_to_soa<T>(d_aos, dst, n, stream);
+    }
+    void download(const T *src, double *h_aos) {
+        if (!src || !d_aos) throw Error(LLGMI_ERR_INVALID, "internal: download before allocation");
+        launch_soa_to_aos<T>(src, d_aos, n, stream);
+        LLGMI_HIP(hipMemcpyAsync(h_aos, d_aos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, stream));
+        LLGMI_HIP(hipStreamSynchronize(stream));
+    }
+    const uint8_t *upload_geom(const uint8_t *geom) {
+        if (!geom) return nullptr;
+        alloc(d_geom, n);
+        LLGMI_HIP(hipMemcpyAsync(d_geom, geom, (size_t)n, hipMemcpyHostToDevice, stream));
+        return d_geom;
+    }
+
+    void host_build(const double *m, double *b, const llgmi_params &p, const llgmi_material &mat,
+                    int mask, const uint8_t *geom) {
+        ensure_io();
+        upload(m, d_m);
+        const uint8_t *dg = upload_geom(geom);
+        field(d_m, d_b, p, mat, mask, dg, nullptr);
+        download(d_b, b);
+    }
+
+    void host_add_demag(const double *m, double *b, const llgmi_material &mat, uint64_t pbc_x,
+                        uint64_t pbc_y) {
+        if (!mat.demag) return;
+        ensure_io();
+        upload(m, d_m);
+        demag_into_bd(d_m, mat, resolved_demag_method(mat), pbc_x, pbc_y);
+        h_tmp.resize((size_t)(3 * n));
+        download(d_bd, h_tmp.data());
+        for (int64_t i = 0; i < 3 * n; ++i) b[i] += h_tmp[(size_t)i];
+    }
+
+    void ensure_rk() {
+        for (int s = 1; s <= 7; ++s) alloc(d_k[s], 3 * n);
+        alloc(d_stage, 3 * n);
+        alloc(d_m7, 3 * n);
+        alloc(d_b, 3 * n);
+        alloc(d_partial, kErrBlocks);
+        alloc(d_err, 1);
+        if (!h_err) LLGMI_HIP(hipHostMalloc((void **)&h_err, sizeof(double), hipHostMallocDefault));
+    }
+
+    // step_llg_rk45_recompute_field_masked_adaptive (llg.rs:795-1026) on device state m.
+    void rk45(T *m, llgmi_params &p, const llgmi_material &mat, int mask, double max_err,
+              double headroom, double dt_min, double dt_max, double &eps_out, int &acc_out,
+              double &dtu_out) {
+        static const double A[7][5] = {
+            {0}, {0},
+            {1.0 / 5.0},
+            {3.0 / 40.0, 9.0 / 40.0},
+            {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0},
+            {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0},
+            {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0},
+        };
+        static const double Bw[5] = {35.0 / 384.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0};
+        static const int Bk[5] = {1, 3, 4, 5, 6};
+        ensure_rk();
+        const double dt0 = p.dt;
+        bool fsal_ok = fsal_valid && last_mask == mask && last_bext[0] == p.b_ext[0] &&
+                       last_bext[1] == p.b_ext[1] && last_bext[2] == p.b_ext[2];
+        if (fsal_ok) std::swap(d_k[1], d_k[7]);
+        else field(m, d_b, p, mat, mask, nullptr, d_k[1]);
+        for (int s = 2; s <= 7; ++s) {
+            StageArgs<T> sa{};
+            if (s < 7) {
+                sa.nk = s - 1;
+                for (int j = 1; j < s; ++j) {
+                    sa.k[j - 1] = d_k[j];
+                    sa.c[j - 1] = T(dt0 * A[s][j - 1]);
+                }
+            } else {
+                sa.nk = 5;
+                for (int q = 0; q < 5; ++q) {
+                    sa.k[q] = d_k[Bk[q]];
+                    sa.c[q] = T(dt0 * Bw[q]);
+                }
+            }
+            T *out = s == 7 ? d_m7 : d_stage;
+            {
+                ProfScope ps(prof, LLGMI_K_RK45_STAGE, stream);
+                launch_rk_stage<T>(m, sa, out, n, stream);
+            }
+            field(out, d_b, p, mat, mask, nullptr, d_k[s]);
+        }
+        {
+            ProfScope ps(prof, LLGMI_K_RK45_ERR, stream);
+            launch_rk_err<T>(d_k[1], d_k[3], d_k[4], d_k[5], d_k[6], d_k[7], n,
+                             sizeof(T) == sizeof(float), d_partial, d_err, stream);
+        }
+        LLGMI_HIP(hipMemcpyAsync(h_err, d_err, sizeof(double), hipMemcpyDeviceToHost, stream));
+        LLGMI_HIP(hipStreamSynchronize(stream));
+        const double err_inf = *h_err;
+        // controller (llg.rs:999-1025), FP64 on the host
+        const double eps = err_inf * dt0;
+        double dt_next = eps > 0.0 ? dt0 * headroom * std::pow(max_err / eps, 0.2) : dt0 * 2.0;
+        if (dt_next < dt_min) dt_next = dt_min;
+        if (dt_next > dt_max) dt_next = dt_max;
+        const bool accept = (eps <= max_err) || (dt0 <= dt_min * 1.0000000001);
+        if (accept) {
+            LLGMI_HIP(hipMemcpyAsync(m, d_m7, sizeof(T) * 3 * n, hipMemcpyDeviceToDevice, stream));
+            fsal_valid = true;
+            last_mask = mask;
+            for (int q = 0; q < 3; ++q) last_bext[q] = p.b_ext[q];
+        } else {
+            fsal_valid = false;
+            last_mask = -1;
+        }
+        p.dt = dt_next;
+        eps_out = eps;
+        acc_out = accept ? 1 : 0;
+        dtu_out = dt0;
+    }
+};
+
+struct llgmi_ctx {
+    llgmi_grid grid{};
+    int precision = LLGMI_F64;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Profiler prof;
+    std::unique_ptr<Engine<float>> ef;
+    std::unique_ptr<Engine<double>> ed;
+};
+
+// ---------------------------------------------------------------------------
+// argument checks
+// ---------------------------------------------------------------------------
+namespace {
+
+std::string check_grid(const llgmi_grid *g) {
+    if (!g) return "grid is NULL";
+    if (g->nx <= 0 || g->ny <= 0) return "grid nx and ny must be positive";
+    if (g->nx > (1 << 20) || g->ny > (1 << 20) || g->nx * g->ny > (int64_t(1) << 31))
+        return "grid too large";
+    if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0) || !std::isfinite(g->dx) ||
+        !std::isfinite(g->dy) || !std::isfinite(g->dz))
+        return "grid spacings must be positive and finite";
+    return "";
+}
+
+std::string check_mat(const llgmi_material *m) {
+    if (!m) return "material is NULL";
+    if (m->demag_method < 0 || m->demag_method > 2) return "unknown demag_method";
+    return "";
+}
+
+std::string check_mask(int mask) {
+    if (mask < 0 || mask > 2) return "unknown field mask";
+    return "";
+}
+
+// Device pointer check: the range [p, p+bytes) must be device memory owned by
+// one allocation, so a host pointer or a short buffer is rejected instead of
+// faulting the GPU.
+std::string check_dev(const void *p, size_t bytes, const char *what) {
+    if (!p) return std::string(what) + " is NULL";
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return std::string(what) + " is not a HIP device pointer";
+    }
+    if (at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged)
+        return std::string(what) + " is not device memory";
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        return std::string(what) + ": cannot query its allocation";
+    }
+    const char *b = (const char *)base, *q = (const char *)p;
+    if (q < b || q + bytes > b + size) return std::string(what) + " is smaller than the grid needs";
+    return "";
+}
+
+template <typename F>
+int guarded(llgmi_ctx *ctx, F &&f) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    try {
+        LLGMI_HIP(hipSetDevice(ctx->device));
+        f();
+        return LLGMI_OK;
+    } catch (const Error &e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(LLGMI_ERR_ALLOC, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(LLGMI_ERR_INVALID, e.what());
+    }
+}
+
+template <typename F>
+void with_engine(llgmi_ctx *ctx, F &&f) {
+    if (ctx->precision == LLGMI_F32) f(*ctx->ef);
+    else f(*ctx->ed);
+}
+
+size_t elem_size(const llgmi_ctx *ctx) { return ctx->precision == LLGMI_F32 ? 4 : 8; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *llgmi_last_error(void) { return g_last_error.c_str(); }
+int llgmi_version(void) { return 1; }
+
+int llgmi_device_count(int *count) {
+    if (!count) return fail(LLGMI_ERR_INVALID, "count is NULL");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        (void)hipGetLastError();
+        c = 0;
+    }
+    *count = c;
+    return LLGMI_OK;
+}
+
+int llgmi_ctx_create(const llgmi_grid *grid, int precision, int device, llgmi_ctx **out) {
+    if (!out) return fail(LLGMI_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    std::string e = check_grid(grid);
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    if (precision != LLGMI_F32 && precision != LLGMI_F64) return fail(LLGMI_ERR_INVALID, "unknown precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return fail(LLGMI_ERR_HIP, "no HIP device available");
+    }
+    if (device < 0 || device >= ndev) return fail(LLGMI_ERR_INVALID, "device index out of range");
+    std::unique_ptr<llgmi_ctx> c(new (std::nothrow) llgmi_ctx());
+    if (!c) return fail(LLGMI_ERR_ALLOC, "host allocation failed");
+    c->grid = *grid;
+    c->precision = precision;
+    c->device = device;
+    try {
+        LLGMI_HIP(hipSetDevice(device));
+        LLGMI_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        if (precision == LLGMI_F32) c->ef.reset(new Engine<float>(*grid, c->stream, &c->prof));
+        else c->ed.reset(new Engine<double>(*grid, c->stream, &c->prof));
+    } catch (const Error &ex) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        return fail(ex.code, ex.what());
+    }
+    *out = c.release();
+    return LLGMI_OK;
+}
+
+int llgmi_ctx_destroy(llgmi_ctx *ctx) {
+    if (!ctx) return LLGMI_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->ef.reset();
+    ctx->ed.reset();
+    ctx->prof.pending.clear();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LLGMI_OK;
+}
+
+int llgmi_ctx_stream(llgmi_ctx *ctx, void **s) {
+    if (!ctx || !s) return fail(LLGMI_ERR_INVALID, "NULL argument");
+    *s = (void *)ctx->stream;
+    return LLGMI_OK;
+}
+
+int llgmi_ctx_synchronize(llgmi_ctx *ctx) {
+    return guarded(ctx, [&] { LLGMI_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int llgmi_build_h_eff_masked_geom(llgmi_ctx *ctx, const double *m, double *b_eff,
+                                  const llgmi_params *params, const llgmi_material *mat,
+                                  int field_mask, const uint8_t *geom_mask) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!m || !b_eff || !params) return fail(LLGMI_ERR_INVALID, "NULL field or params");
+    std::string e = check_mat(mat);
+    if (e.empty()) e = check_mask(field_mask);
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) { eng.host_build(m, b_eff, *params, *mat, field_mask, geom_mask); });
+    });
+}
+
+int llgmi_build_h_eff_masked(llgmi_ctx *ctx, const double *m, double *b_eff,
+                             const llgmi_params *params, const llgmi_material *mat,
+                             int field_mask) {
+    return llgmi_build_h_eff_masked_geom(ctx, m, b_eff, params, mat, field_mask, nullptr);
+}
+
+int llgmi_build_h_eff(llgmi_ctx *ctx, const double *m, double *b_eff, const llgmi_params *params,
+                      const llgmi_material *mat) {
+    return llgmi_build_h_eff_masked_geom(ctx, m, b_eff, params, mat, LLGMI_MASK_FULL, nullptr);
+}
+
+int llgmi_add_demag_field_pbc(llgmi_ctx *ctx, const double *m, double *b_eff,
+                              const llgmi_material *mat, uint64_t pbc_x, uint64_t pbc_y) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!m || !b_eff) return fail(LLGMI_ERR_INVALID, "NULL field");
+    std::string e = check_mat(mat);
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) { eng.host_add_demag(m, b_eff, *mat, pbc_x, pbc_y); });
+    });
+}
+
+int llgmi_compute_demag_field_pbc(llgmi_ctx *ctx, const double *m, double *out,
+                                  const llgmi_material *mat, uint64_t pbc_x, uint64_t pbc_y) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!out) return fail(LLGMI_ERR_INVALID, "NULL field");
+    std::memset(out, 0, sizeof(double) * 3 * (size_t)(ctx->grid.nx * ctx->grid.ny));
+    return llgmi_add_demag_field_pbc(ctx, m, out, mat, pbc_x, pbc_y);
+}
+
+int llgmi_add_demag_field(llgmi_ctx *ctx, const double *m, double *b_eff, const llgmi_material *mat) {
+    return llgmi_add_demag_field_pbc(ctx, m, b_eff, mat, 0, 0);
+}
+
+int llgmi_rk45_step(llgmi_ctx *ctx, double *m, llgmi_params *params, const llgmi_material *mat,
+                    int field_mask, double max_err, double headroom, double dt_min, double dt_max,
+                    double *eps, int *accepted, double *dt_used) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!m || !params || !eps || !accepted || !dt_used) return fail(LLGMI_ERR_INVALID, "NULL argument");
+    std::string e = check_mat(mat);
+    if (e.empty()) e = check_mask(field_mask);
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    if (dt_min > dt_max) return fail(LLGMI_ERR_INVALID, "dt_min > dt_max");
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            eng.ensure_io();
+            eng.upload(m, eng.d_m);
+            eng.rk45(eng.d_m, *params, *mat, field_mask, max_err, headroom, dt_min, dt_max, *eps,
+                     *accepted, *dt_used);
+            if (*accepted) eng.download(eng.d_m, m);
+        });
+    });
+}
+
+int llgmi_rk45_step_adaptive(llgmi_ctx *ctx, double *m, llgmi_params *params,
+                             const llgmi_material *mat, double max_err, double headroom,
+                             double dt_min, double dt_max, double *eps, int *accepted,
+                             double *dt_used) {
+    return llgmi_rk45_step(ctx, m, params, mat, LLGMI_MASK_FULL, max_err, headroom, dt_min, dt_max,
+                           eps, accepted, dt_used);
+}
+
+int llgmi_rk45_reset(llgmi_ctx *ctx) {
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            eng.fsal_valid = false;
+            eng.last_mask = -1;
+        });
+    });
+}
+
+int llgmi_dev_build_h_eff(llgmi_ctx *ctx, const void *m_dev, void *b_dev, const llgmi_params *params,
+                          const llgmi_material *mat, int field_mask, const uint8_t *geom_mask_dev) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!params) return fail(LLGMI_ERR_INVALID, "params is NULL");
+    std::string e = check_mat(mat);
+    if (e.empty()) e = check_mask(field_mask);
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    if (e.empty()) e = check_dev(m_dev, 3 * n * elem_size(ctx), "m");
+    if (e.empty()) e = check_dev(b_dev, 3 * n * elem_size(ctx), "b");
+    if (e.empty() && geom_mask_dev) e = check_dev(geom_mask_dev, n, "geom_mask");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.field((const T *)m_dev, (T *)b_dev, *params, *mat, field_mask, geom_mask_dev, nullptr);
+        });
+    });
+}
+
+int llgmi_dev_add_demag_field_pbc(llgmi_ctx *ctx, const void *m_dev, void *b_dev,
+                                  const llgmi_material *mat, uint64_t pbc_x, uint64_t pbc_y) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    std::string e = check_mat(mat);
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    if (e.empty()) e = check_dev(m_dev, 3 * n * elem_size(ctx), "m");
+    if (e.empty()) e = check_dev(b_dev, 3 * n * elem_size(ctx), "b");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    if (!mat->demag) return LLGMI_OK;
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.demag_into_bd((const T *)m_dev, *mat, resolved_demag_method(*mat), pbc_x, pbc_y);
+            launch_soa_accumulate<T>(eng.d_bd, (T *)b_dev, 3 * eng.n, eng.stream);
+        });
+    });
+}
+
+int llgmi_dev_rk45_step(llgmi_ctx *ctx, void *m_dev, llgmi_params *params, const llgmi_material *mat,
+                        int field_mask, double max_err, double headroom, double dt_min,
+                        double dt_max, double *eps, int *accepted, double *dt_used) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!params || !eps || !accepted || !dt_used) return fail(LLGMI_ERR_INVALID, "NULL argument");
+    std::string e = check_mat(mat);
+    if (e.empty()) e = check_mask(field_mask);
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    if (e.empty()) e = check_dev(m_dev, 3 * n * elem_size(ctx), "m");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    if (dt_min > dt_max) return fail(LLGMI_ERR_INVALID, "dt_min > dt_max");
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.rk45((T *)m_dev, *params, *mat, field_mask, max_err, headroom, dt_min, dt_max, *eps,
+                     *accepted, *dt_used);
+        });
+    });
+}
+
+int llgmi_dev_upload(llgmi_ctx *ctx, const double *host_aos, void *dev_soa) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!host_aos) return fail(LLGMI_ERR_INVALID, "host buffer is NULL");
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    std::string e = check_dev(dev_soa, 3 * n * elem_size(ctx), "dev_soa");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.ensure_io();
+            eng.upload(host_aos, (T *)dev_soa);
+            LLGMI_HIP(hipStreamSynchronize(eng.stream));
+        });
+    });
+}
+
+int llgmi_dev_download(llgmi_ctx *ctx, const void *dev_soa, double *host_aos) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!host_aos) return fail(LLGMI_ERR_INVALID, "host buffer is NULL");
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    std::string e = check_dev(dev_soa, 3 * n * elem_size(ctx), "dev_soa");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.ensure_io();
+            eng.download((const T *)dev_soa, host_aos);
+        });
+    });
+}
+
+int llgmi_profile_enable(llgmi_ctx *ctx, int on) {
+    return guarded(ctx, [&] { ctx->prof.on = on != 0; });
+}
+
+int llgmi_profile_reset(llgmi_ctx *ctx) {
+    return guarded(ctx, [&] { ctx->prof.reset(); });
+}
+
+int llgmi_profile_read(llgmi_ctx *ctx, int cls, double *total_ms, int64_t *count) {
+    if (cls < 0 || cls >= LLGMI_K_COUNT) return fail(LLGMI_ERR_INVALID, "unknown kernel class");
+    if (!total_ms || !count) return fail(LLGMI_ERR_INVALID, "NULL argument");
+    return guarded(ctx, [&] {
+        ctx->prof.drain();
+        *total_ms = ctx->prof.total_ms[cls];
+        *count = ctx->prof.count[cls];
+    });
+}
+
+}  // extern "C"

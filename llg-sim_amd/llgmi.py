@@ -1,0 +1,346 @@
+"""Python binding of libllgmi.so (the C ABI in include/llgmi.h) via ctypes.
+
+Mirrors the Rust API of llg-sim's hot path by name:
+  Grid2D (grid.rs:5-31), Material / LLGParams (params.rs:94-144),
+  FieldMask (effective_field/mod.rs:37-45),
+  Context.build_h_eff_masked[_geom] (mod.rs:48-101),
+  Context.add_demag_field_pbc (demag.rs:70-143),
+  Context.step_llg_rk45_recompute_field_masked_adaptive (llg.rs:795-1026).
+
+Host-side methods take numpy float64 arrays of shape (n, 3) or (ny, nx, 3)
+(AoS, cell j*nx + i).  Device methods take torch tensors (or raw pointers)
+in SoA layout (3, ny, nx) in the context precision.
+
+The library is required: importing works without it, but creating a context
+raises if libllgmi.so is missing, and there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libllgmi.so")
+
+MU0 = 4.0 * np.pi * 1e-7
+GAMMA_E_RAD_PER_S_T = 1.76085963023e11
+
+OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_ALLOC = 0, 1, 2, 3, 4
+F32, F64 = 0, 1
+
+K_FFT_ROW_FWD, K_FFT_COL, K_FFT_ROW_INV, K_LOCAL_FIELD, K_RK45_STAGE, K_RK45_ERR, K_MG = range(7)
+KERNEL_CLASS_NAMES = {
+    K_FFT_ROW_FWD: "fft_row_fwd",
+    K_FFT_COL: "fft_col_mul",
+    K_FFT_ROW_INV: "fft_row_inv",
+    K_LOCAL_FIELD: "local_field",
+    K_RK45_STAGE: "rk45_stage",
+    K_RK45_ERR: "rk45_err",
+    K_MG: "mg",
+}
+
+
+class FieldMask:
+    """FieldMask (effective_field/mod.rs:37-45)."""
+    ExchAnis = 0
+    ExchAnisDmi = 1
+    Full = 2
+
+
+class DemagMethod:
+    """DemagMethod (params.rs:67-92)."""
+    FftUniform = 0
+    PoissonMG = 1
+    PoissonDst = 2
+
+
+class LlgmiError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"llgmi error {code}: {msg}")
+        self.code = code
+
+
+class _Grid(C.Structure):
+    _fields_ = [("nx", C.c_int64), ("ny", C.c_int64), ("dx", C.c_double), ("dy", C.c_double),
+                ("dz", C.c_double)]
+
+
+class _Material(C.Structure):
+    _fields_ = [("ms", C.c_double), ("a_ex", C.c_double), ("k_u", C.c_double),
+                ("easy_axis", C.c_double * 3), ("has_dmi", C.c_int32), ("demag", C.c_int32),
+                ("dmi", C.c_double), ("demag_method", C.c_int32), ("reserved", C.c_int32)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("alpha", C.c_double), ("dt", C.c_double),
+                ("b_ext", C.c_double * 3)]
+
+
+@dataclass
+class Grid2D:
+    nx: int
+    ny: int
+    dx: float
+    dy: float
+    dz: float
+
+    def n_cells(self) -> int:
+        return self.nx * self.ny
+
+    def cell_volume(self) -> float:
+        return self.dx * self.dy * self.dz
+
+    def idx(self, i: int, j: int) -> int:
+        return j * self.nx + i
+
+    def _c(self) -> _Grid:
+        return _Grid(self.nx, self.ny, self.dx, self.dy, self.dz)
+
+
+@dataclass
+class Material:
+    ms: float
+    a_ex: float = 0.0
+    k_u: float = 0.0
+    easy_axis: Tuple[float, float, float] = (0.0, 0.0, 1.0)
+    dmi: Optional[float] = None
+    demag: bool = False
+    demag_method: int = DemagMethod.FftUniform
+
+    def _c(self) -> _Material:
+        return _Material(self.ms, self.a_ex, self.k_u, (C.c_double * 3)(*self.easy_axis),
+                         0 if self.dmi is None else 1, 1 if self.demag else 0,
+                         0.0 if self.dmi is None else float(self.dmi), int(self.demag_method), 0)
+
+
+@dataclass
+class LLGParams:
+    gamma: float = GAMMA_E_RAD_PER_S_T
+    alpha: float = 0.0
+    dt: float = 1e-13
+    b_ext: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+
+    def _c(self) -> _Params:
+        return _Params(self.gamma, self.alpha, self.dt, (C.c_double * 3)(*self.b_ext))
+
+
+_lib = None
+
+# Every symbol include/llgmi.h declares (checked by the CPU test suite).
+EXPORTED = [
+    "llgmi_last_error", "llgmi_version", "llgmi_device_count", "llgmi_ctx_create",
+    "llgmi_ctx_destroy", "llgmi_ctx_stream", "llgmi_ctx_synchronize",
+    "llgmi_build_h_eff_masked", "llgmi_build_h_eff_masked_geom", "llgmi_build_h_eff",
+    "llgmi_add_demag_field_pbc", "llgmi_compute_demag_field_pbc", "llgmi_add_demag_field",
+    "llgmi_rk45_step", "llgmi_rk45_step_adaptive", "llgmi_rk45_reset",
+    "llgmi_dev_build_h_eff", "llgmi_dev_add_demag_field_pbc", "llgmi_dev_rk45_step",
+    "llgmi_dev_upload", "llgmi_dev_download", "llgmi_profile_enable", "llgmi_profile_reset",
+    "llgmi_profile_read",
+]
+
+
+def lib():
+    """Loads libllgmi.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libllgmi.so not found at {LIB_PATH}: run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    dp, ip, vp, u8p = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p, C.POINTER(C.c_uint8)
+    G, M, P = C.POINTER(_Grid), C.POINTER(_Material), C.POINTER(_Params)
+    L.llgmi_last_error.restype = C.c_char_p
+    L.llgmi_device_count.argtypes = [ip]
+    L.llgmi_ctx_create.argtypes = [G, C.c_int, C.c_int, C.POINTER(vp)]
+    L.llgmi_ctx_destroy.argtypes = [vp]
+    L.llgmi_ctx_stream.argtypes = [vp, C.POINTER(vp)]
+    L.llgmi_ctx_synchronize.argtypes = [vp]
+    L.llgmi_build_h_eff_masked.argtypes = [vp, dp, dp, P, M, C.c_int]
+    L.llgmi_build_h_eff_masked_geom.argtypes = [vp, dp, dp, P, M, C.c_int, u8p]
+    L.llgmi_build_h_eff.argtypes = [vp, dp, dp, P, M]
+    L.llgmi_add_demag_field_pbc.argtypes = [vp, dp, dp, M, C.c_uint64, C.c_uint64]
+    L.llgmi_compute_demag_field_pbc.argtypes = [vp, dp, dp, M, C.c_uint64, C.c_uint64]
+    L.llgmi_add_demag_field.argtypes = [vp, dp, dp, M]
+    step_args = [vp, dp, P, M, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, ip, dp]
+    L.llgmi_rk45_step.argtypes = step_args
+    L.llgmi_rk45_step_adaptive.argtypes = [vp, dp, P, M, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, dp, ip, dp]
+    L.llgmi_rk45_reset.argtypes = [vp]
+    L.llgmi_dev_build_h_eff.argtypes = [vp, vp, vp, P, M, C.c_int, vp]
+    L.llgmi_dev_add_demag_field_pbc.argtypes = [vp, vp, vp, M, C.c_uint64, C.c_uint64]
+    L.llgmi_dev_rk45_step.argtypes = [vp, vp, P, M, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, dp, ip, dp]
+    L.llgmi_dev_upload.argtypes = [vp, dp, vp]
+    L.llgmi_dev_download.argtypes = [vp, vp, dp]
+    L.llgmi_profile_enable.argtypes = [vp, C.c_int]
+    L.llgmi_profile_reset.argtypes = [vp]
+    L.llgmi_profile_read.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_int64)]
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        raise LlgmiError(rc, lib().llgmi_last_error().decode())
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    _check(lib().llgmi_device_count(C.byref(c)))
+    return c.value
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _ptr(x) -> Optional[int]:
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return int(x.data_ptr())
+
+
+class Context:
+    """A grid + precision + GPU: the reference's operator caches and RK45Scratch."""
+
+    def __init__(self, grid: Grid2D, precision: int = F64, device: int = 0):
+        self.grid = grid
+        self.precision = precision
+        self._h = C.c_void_p()
+        g = grid._c()
+        _check(lib().llgmi_ctx_create(C.byref(g), precision, device, C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().llgmi_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n(self) -> int:
+        return self.grid.nx * self.grid.ny
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        _check(lib().llgmi_ctx_stream(self._h, C.byref(s)))
+        return s.value or 0
+
+    def synchronize(self) -> None:
+        _check(lib().llgmi_ctx_synchronize(self._h))
+
+    # ---- host (AoS f64) ----
+    def build_h_eff_masked_geom(self, m, params: LLGParams, mat: Material, mask: int,
+                                geom_mask=None) -> np.ndarray:
+        m = _f64(m).reshape(-1, 3)
+        b = np.zeros_like(m)
+        p, cm = params._c(), mat._c()
+        gm = None
+        if geom_mask is not None:
+            g8 = np.ascontiguousarray(np.asarray(geom_mask).reshape(-1).astype(np.uint8))
+            gm = g8.ctypes.data_as(C.POINTER(C.c_uint8))
+        _check(lib().llgmi_build_h_eff_masked_geom(self._h, _dp(m), _dp(b), C.byref(p),
+                                                   C.byref(cm), mask, gm))
+        return b
+
+    def build_h_eff_masked(self, m, params: LLGParams, mat: Material, mask: int) -> np.ndarray:
+        return self.build_h_eff_masked_geom(m, params, mat, mask, None)
+
+    def build_h_eff(self, m, params: LLGParams, mat: Material) -> np.ndarray:
+        return self.build_h_eff_masked_geom(m, params, mat, FieldMask.Full, None)
+
+    def add_demag_field_pbc(self, m, b: np.ndarray, mat: Material, pbc_x: int = 0,
+                            pbc_y: int = 0) -> None:
+        m = _f64(m).reshape(-1, 3)
+        assert b.dtype == np.float64 and b.flags.c_contiguous and b.size == m.size
+        cm = mat._c()
+        _check(lib().llgmi_add_demag_field_pbc(self._h, _dp(m), _dp(b), C.byref(cm), pbc_x, pbc_y))
+
+    def add_demag_field(self, m, b: np.ndarray, mat: Material) -> None:
+        self.add_demag_field_pbc(m, b, mat, 0, 0)
+
+    def compute_demag_field(self, m, mat: Material) -> np.ndarray:
+        m = _f64(m).reshape(-1, 3)
+        b = np.zeros_like(m)
+        self.add_demag_field_pbc(m, b, mat, 0, 0)
+        return b
+
+    def step_llg_rk45_recompute_field_masked_adaptive(self, m: np.ndarray, params: LLGParams,
+                                                      mat: Material, mask: int, max_err: float,
+                                                      headroom: float, dt_min: float,
+                                                      dt_max: float):
+        """Advances m (float64, C-contiguous, 3n) in place; updates params.dt.
+        Returns (eps, accepted, dt_used)."""
+        assert m.dtype == np.float64 and m.flags.c_contiguous and m.size == 3 * self.n
+        p, cm = params._c(), mat._c()
+        eps, dtu, acc = C.c_double(), C.c_double(), C.c_int()
+        _check(lib().llgmi_rk45_step(self._h, _dp(m), C.byref(p), C.byref(cm), mask, max_err,
+                                     headroom, dt_min, dt_max, C.byref(eps), C.byref(acc),
+                                     C.byref(dtu)))
+        params.dt = p.dt
+        return eps.value, bool(acc.value), dtu.value
+
+    def step_llg_rk45_recompute_field_adaptive(self, m, params, mat, max_err, headroom, dt_min,
+                                               dt_max):
+        return self.step_llg_rk45_recompute_field_masked_adaptive(
+            m, params, mat, FieldMask.Full, max_err, headroom, dt_min, dt_max)
+
+    def rk45_reset(self) -> None:
+        _check(lib().llgmi_rk45_reset(self._h))
+
+    # ---- device (SoA, context precision) ----
+    def dev_build_h_eff(self, m_dev, b_dev, params: LLGParams, mat: Material, mask: int,
+                        geom_dev=None) -> None:
+        p, cm = params._c(), mat._c()
+        _check(lib().llgmi_dev_build_h_eff(self._h, _ptr(m_dev), _ptr(b_dev), C.byref(p),
+                                           C.byref(cm), mask, _ptr(geom_dev)))
+
+    def dev_add_demag_field_pbc(self, m_dev, b_dev, mat: Material, pbc_x=0, pbc_y=0) -> None:
+        cm = mat._c()
+        _check(lib().llgmi_dev_add_demag_field_pbc(self._h, _ptr(m_dev), _ptr(b_dev), C.byref(cm),
+                                                   pbc_x, pbc_y))
+
+    def dev_rk45_step(self, m_dev, params: LLGParams, mat: Material, mask: int, max_err: float,
+                      headroom: float, dt_min: float, dt_max: float):
+        p, cm = params._c(), mat._c()
+        eps, dtu, acc = C.c_double(), C.c_double(), C.c_int()
+        _check(lib().llgmi_dev_rk45_step(self._h, _ptr(m_dev), C.byref(p), C.byref(cm), mask,
+                                         max_err, headroom, dt_min, dt_max, C.byref(eps),
+                                         C.byref(acc), C.byref(dtu)))
+        params.dt = p.dt
+        return eps.value, bool(acc.value), dtu.value
+
+    def dev_upload(self, host_aos, dev_soa) -> None:
+        h = _f64(host_aos).reshape(-1)
+        _check(lib().llgmi_dev_upload(self._h, _dp(h), _ptr(dev_soa)))
+
+    def dev_download(self, dev_soa) -> np.ndarray:
+        h = np.zeros((self.n, 3))
+        _check(lib().llgmi_dev_download(self._h, _ptr(dev_soa), _dp(h)))
+        return h
+
+    # ---- timing ----
+    def profile_enable(self, on: bool = True) -> None:
+        _check(lib().llgmi_profile_enable(self._h, 1 if on else 0))
+
+    def profile_reset(self) -> None:
+        _check(lib().llgmi_profile_reset(self._h))
+
+    def profile_read(self, cls: int):
+        t, c = C.c_double(), C.c_int64()
+        _check(lib().llgmi_profile_read(self._h, cls, C.byref(t), C.byref(c)))
+        return t.value, c.value

@@ -1,0 +1,156 @@
+"""GPU parity of the Dormand-Prince RK45 stepper (llg.rs:795-1026) through the C ABI.
+
+Bars: FP64 context: the accepted state, eps and next dt of single steps within
+1e-12 of the oracle; spatially averaged m(t) within 1e-4 of the oracle over a
+trajectory (dt sequence and accept/reject decisions must follow the oracle's).
+Reference test mirrored: macrospin_precession_quarter_turn_rk45_adaptive
+(tests/validation.rs:646-745).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(nx, ny, seed, dmi=None, demag=True, k_u=0.0):
+    from oracle import oracle as O
+    import llgmi as L
+    from initial_states import smoothed_random
+    m0 = smoothed_random(nx, ny, seed).reshape(-1, 3).copy()
+    kw = dict(ms=8.0e5, a_ex=1.3e-11, k_u=k_u, dmi=dmi, demag=demag)
+    return m0, O.Material(**kw), L.Material(**kw)
+
+
+def test_rk45_single_steps_match_oracle(gpu_lib):
+    from oracle import oracle as O
+    L = gpu_lib
+    nx, ny, cell = 32, 16, (4e-9, 4e-9, 3e-9)
+    m0, om, gm = _setup(nx, ny, 5, dmi=1e-3, k_u=2e4)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    gc = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    op = O.Params(alpha=0.02, dt=1e-13, b_ext=(-24.6e-3, 4.3e-3, 0.0))
+    gp = L.LLGParams(alpha=0.02, dt=1e-13, b_ext=(-24.6e-3, 4.3e-3, 0.0))
+    mo, mg = m0.copy(), m0.copy()
+    for step in range(6):
+        eo, ao, uo = oc.rk45_step(mo, op, om, 2, 1e-6, 0.8, 1e-19, 1e-11)
+        eg, ag, ug = gc.step_llg_rk45_recompute_field_masked_adaptive(mg, gp, gm, 2, 1e-6, 0.8, 1e-19, 1e-11)
+        assert ao == ag, step
+        assert uo == ug
+        assert abs(eg - eo) <= 1e-9 * abs(eo) + 1e-30, (step, eg, eo)
+        assert abs(gp.dt - op.dt) <= 1e-9 * op.dt
+        assert np.abs(mg - mo).max() <= 1e-12, step
+        # keep both on the oracle's dt so later steps compare like with like
+        gp.dt = op.dt
+
+
+def test_rk45_reject_path_and_fsal(gpu_lib):
+    """A huge first dt is rejected (state unchanged, dt reduced); FSAL is only
+    used after an accepted step with the same mask and B_ext."""
+    from oracle import oracle as O
+    L = gpu_lib
+    nx, ny, cell = 16, 8, (4e-9, 4e-9, 3e-9)
+    m0, om, gm = _setup(nx, ny, 9)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    gc = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    op, gp = O.Params(alpha=0.1, dt=5e-12), L.LLGParams(alpha=0.1, dt=5e-12)
+    mo, mg = m0.copy(), m0.copy()
+    eo, ao, _ = oc.rk45_step(mo, op, om, 2, 1e-7, 0.8, 1e-18, 1e-11)
+    eg, ag, _ = gc.step_llg_rk45_recompute_field_masked_adaptive(mg, gp, gm, 2, 1e-7, 0.8, 1e-18, 1e-11)
+    assert not ao and not ag
+    assert (mg == m0).all()
+    assert abs(gp.dt - op.dt) <= 1e-9 * op.dt
+    for _ in range(3):
+        gp.dt = op.dt
+        oc.rk45_step(mo, op, om, 2, 1e-7, 0.8, 1e-18, 1e-11)
+        gc.step_llg_rk45_recompute_field_masked_adaptive(mg, gp, gm, 2, 1e-7, 0.8, 1e-18, 1e-11)
+    assert np.abs(mg - mo).max() <= 1e-12
+    # changing B_ext invalidates FSAL in both
+    op.b_ext = gp.b_ext = (0.0, 0.0, 0.05)
+    gp.dt = op.dt
+    oc.rk45_step(mo, op, om, 2, 1e-7, 0.8, 1e-18, 1e-11)
+    gc.step_llg_rk45_recompute_field_masked_adaptive(mg, gp, gm, 2, 1e-7, 0.8, 1e-18, 1e-11)
+    assert np.abs(mg - mo).max() <= 1e-12
+
+
+def _trajectory(stepper, m, params, t_end, dt_out):
+    """Driver of bin/uniform_film_field_rk45.rs:316-343: clamp dt to land on
+    each output time; returns <m> at every output time."""
+    out = []
+    t = 0.0
+    n_out = int(round(t_end / dt_out))
+    for k in range(1, n_out + 1):
+        t_target = k * dt_out
+        while t + 1e-18 < t_target:
+            remaining = t_target - t
+            if params.dt > remaining:
+                params.dt = remaining
+            eps, acc, dtu = stepper(m, params)
+            if acc:
+                t += dtu
+        t = t_target
+        out.append(m.reshape(-1, 3).mean(axis=0).copy())
+    return np.array(out)
+
+
+def test_rk45_trajectory_matches_oracle(gpu_lib):
+    """Plain film (exchange, Zeeman, FFT demag), 0.1 ns, FP64 on both sides."""
+    from oracle import oracle as O
+    L = gpu_lib
+    nx, ny, cell = 64, 32, (4e-9, 4e-9, 3e-9)
+    m0, om, gm = _setup(nx, ny, 1)
+    bext = (-24.6e-3, 4.3e-3, 0.0)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    gc = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    op = O.Params(alpha=0.02, dt=1e-13, b_ext=bext)
+    gp = L.LLGParams(alpha=0.02, dt=1e-13, b_ext=bext)
+    args = (1e-6, 0.8, 1e-19, 1e-11)
+    ro = _trajectory(lambda m, p: oc.rk45_step(m, p, om, 2, *args), m0.copy(), op, 1e-10, 1e-11)
+    rg = _trajectory(lambda m, p: gc.step_llg_rk45_recompute_field_masked_adaptive(m, p, gm, 2, *args),
+                     m0.copy(), gp, 1e-10, 1e-11)
+    assert np.abs(rg - ro).max() <= 1e-4
+
+
+def test_macrospin_precession_quarter_turn_rk45_adaptive(gpu_lib):
+    """tests/validation.rs:646-745 through the C ABI, plus the closed form."""
+    L = gpu_lib
+    grid = L.Grid2D(1, 1, 1.0, 1.0, 1.0)
+    b0 = 0.1
+    params = L.LLGParams(alpha=0.0, dt=1e-14, b_ext=(0.0, 0.0, b0))
+    mat = L.Material(ms=8.0e5)
+    ctx = L.Context(grid, precision=L.F64)
+    m = np.array([[1.0, 0.0, 0.0]])
+    t_target = (math.pi / 2) / (params.gamma * b0)
+    t, attempts = 0.0, 0
+    while t < t_target:
+        attempts += 1
+        assert attempts < 200000
+        remaining = t_target - t
+        if params.dt > remaining:
+            params.dt = remaining
+        _, acc, dtu = ctx.step_llg_rk45_recompute_field_adaptive(m, params, mat, 1e-5, 0.8, 1e-18, 1e-11)
+        if acc:
+            t += dtu
+    v = m[0]
+    assert abs(v[2]) < 0.1 and abs(v[1]) > 0.9 and abs(v[0]) < 0.3
+    # closed form: dm/dt = -gamma m x B with m = x, B = b0 z gives +gamma b0 y,
+    # so m = (cos phi, sin phi, 0) with phi = gamma b0 t
+    phi = params.gamma * b0 * t
+    np.testing.assert_allclose(v, [math.cos(phi), math.sin(phi), 0.0], atol=1e-4)
+
+
+def test_rk45_fp32_matches_fp64_over_steps(gpu_lib):
+    """FP32 context (FP64 error norm + controller) against the FP64 context."""
+    L = gpu_lib
+    nx, ny, cell = 64, 32, (4e-9, 4e-9, 3e-9)
+    m0, _, gm = _setup(nx, ny, 2)
+    c64 = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    c32 = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F32)
+    bext = (-24.6e-3, 4.3e-3, 0.0)
+    p64 = L.LLGParams(alpha=0.02, dt=1e-13, b_ext=bext)
+    p32 = L.LLGParams(alpha=0.02, dt=1e-13, b_ext=bext)
+    args = (1e-6, 0.8, 1e-19, 1e-11)
+    r64 = _trajectory(lambda m, p: c64.step_llg_rk45_recompute_field_adaptive(m, p, gm, *args), m0.copy(), p64, 5e-11, 1e-11)
+    r32 = _trajectory(lambda m, p: c32.step_llg_rk45_recompute_field_adaptive(m, p, gm, *args), m0.copy(), p32, 5e-11, 1e-11)
+    assert np.abs(r32 - r64).max() <= 1e-4

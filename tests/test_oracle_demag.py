@@ -1,0 +1,180 @@
+"""Pins the oracle's FFT demag (demag_fft_uniform.rs) on the CPU:
+
+* the FFT (stand-in for rustfft 6.4.1) against the DFT definition and numpy.fft;
+* the quadrature self term against Aharoni's closed-form demag factors of a
+  rectangular prism (J. Appl. Phys. 83, 3432 (1998)), itself checked by the
+  cube value 1/3 and the sum rule Nxx + Nyy + Nzz = 1;
+* far-field kernel against the point-dipole tensor;
+* the parity reflection against direct quadrature at negative displacements;
+* the FFT convolution against a brute-force real-space sum with the same kernel;
+* reference tests: single cube Bz ~ -mu0/3 (demag_fft_uniform.rs:1261-1293),
+  2x2 symmetry and factors (tests/validation.rs:478-615), demag energy >= 0
+  (tests/validation.rs:617-644).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+MU0 = O.MU0
+
+
+def aharoni_n(a, b, c):
+    """Demag factor along the third axis of a 2a x 2b x 2c prism."""
+    r = np.sqrt(a * a + b * b + c * c)
+    ab, bc, ac = np.sqrt(a * a + b * b), np.sqrt(b * b + c * c), np.sqrt(a * a + c * c)
+    s = ((b * b - c * c) / (2 * b * c)) * np.log((r - a) / (r + a)) \
+        + ((a * a - c * c) / (2 * a * c)) * np.log((r - b) / (r + b)) \
+        + (b / (2 * c)) * np.log((ab + a) / (ab - a)) + (a / (2 * c)) * np.log((ab + b) / (ab - b)) \
+        + (c / (2 * a)) * np.log((bc - b) / (bc + b)) + (c / (2 * b)) * np.log((ac - a) / (ac + a)) \
+        + 2 * np.arctan(a * b / (c * r)) + (a ** 3 + b ** 3 - 2 * c ** 3) / (3 * a * b * c) \
+        + ((a * a + b * b - 2 * c * c) / (3 * a * b * c)) * r + (c / (a * b)) * (ac + bc) \
+        - (ab ** 3 + bc ** 3 + ac ** 3) / (3 * a * b * c)
+    return s / np.pi
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 12, 30, 64, 97, 210, 256, 1000])
+def test_fft1d_matches_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.fft(x)
+    if n <= 64:
+        k = np.arange(n)
+        ref_dft = np.exp(-2j * np.pi * np.outer(k, k) / n) @ x
+        np.testing.assert_allclose(ref, ref_dft, atol=1e-12 * n)
+    np.testing.assert_allclose(O.fft1d(x), ref, rtol=0, atol=1e-13 * n * np.abs(x).max())
+    np.testing.assert_allclose(O.fft1d(x, True), np.fft.ifft(x) * n, rtol=0, atol=1e-13 * n * np.abs(x).max())
+
+
+@pytest.mark.parametrize("shape", [(4, 6), (8, 8), (10, 3), (1, 5), (64, 32)])
+def test_fft2_forward_and_scaled_inverse(shape):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    np.testing.assert_allclose(O.fft2(x), np.fft.fft2(x), atol=1e-12 * x.size)
+    np.testing.assert_allclose(O.fft2(x, True), np.fft.ifft2(x), atol=1e-14)
+
+
+def test_aharoni_reference_is_sound():
+    assert abs(aharoni_n(1, 1, 1) - 1 / 3) < 1e-14
+    for a, b, c in [(2, 2, 1.5), (1, 2, 3), (0.5, 1.5, 0.2)]:
+        assert abs(aharoni_n(a, b, c) + aharoni_n(b, c, a) + aharoni_n(c, a, b) - 1) < 1e-12
+
+
+@pytest.mark.parametrize("cell", [(1.0, 1.0, 1.0), (4e-9, 4e-9, 3e-9), (2e-9, 2e-9, 1e-9),
+                                  (5e-9, 3e-9, 2e-9), (3e-9, 3e-9, 3e-9)])
+def test_self_term_matches_aharoni(cell):
+    dx, dy, dz = cell
+    k = O.kernel_components(dx, dy, dz, 0, 0)
+    nxx, nyy, nzz = -k[0] / MU0, -k[2] / MU0, -k[3] / MU0
+    assert abs(nxx - aharoni_n(dy / 2, dz / 2, dx / 2)) < 2e-3 * nxx
+    assert abs(nyy - aharoni_n(dz / 2, dx / 2, dy / 2)) < 2e-3 * nyy
+    assert abs(nzz - aharoni_n(dx / 2, dy / 2, dz / 2)) < 2e-3 * nzz
+    assert abs(k[1]) < 1e-12 * MU0  # no xy coupling on the self term
+
+
+@pytest.mark.parametrize("s", [(40, 0), (0, 37), (25, 30), (60, 11)])
+def test_far_field_is_point_dipole(s):
+    dx, dy, dz = 4e-9, 3e-9, 2e-9
+    sx, sy = s
+    k = O.kernel_components(dx, dy, dz, sx, sy)
+    r = np.array([sx * dx, sy * dy, 0.0])
+    rn = np.linalg.norm(r)
+    rh = r / rn
+    V = dx * dy * dz
+    N = V / (4 * np.pi * rn ** 3) * (3 * np.outer(rh, rh) - np.eye(3))
+    tol = 5e-3 * V / (4 * np.pi * rn ** 3) * MU0
+    assert abs(k[0] - MU0 * N[0, 0]) < tol
+    assert abs(k[1] - MU0 * N[0, 1]) < tol
+    assert abs(k[2] - MU0 * N[1, 1]) < tol
+    assert abs(k[3] - MU0 * N[2, 2]) < tol
+
+
+@pytest.mark.parametrize("s", [(1, 2), (3, 1), (2, 5), (0, 3), (4, 0)])
+def test_reflection_parity_against_direct_quadrature(s):
+    """Kernels are even (xx, yy, zz) or odd-odd (xy): the reflected base-quadrant
+    values equal direct quadrature at negative displacements."""
+    dx, dy, dz = 4e-9, 4e-9, 3e-9
+    g = O.Grid(6, 7, dx, dy, dz)
+    px, py = 12, 14
+    kr = O.kernel_realspace(g, px, py)
+    sx, sy = s
+    for ax in (1, -1):
+        for ay in (1, -1):
+            direct = O.kernel_components(dx, dy, dz, ax * sx, ay * sy)
+            ix, iy = (ax * sx) % px, (ay * sy) % py
+            scale = np.abs(direct).max()
+            for c, idx in ((0, 0), (2, 2), (3, 3)):
+                assert abs(kr[c, iy, ix] - direct[idx]) <= 1e-9 * scale
+            if sx != 0 and sy != 0:
+                assert abs(kr[1, iy, ix] - direct[1]) <= 1e-9 * scale
+            else:
+                assert kr[1, iy, ix] == 0.0
+
+
+def _brute_force_demag(grid, m, ms):
+    """B_i = sum_j K(x_i - x_j) M_j with the oracle's real-space kernel."""
+    nx, ny = grid.nx, grid.ny
+    px, py = 2 * nx, 2 * ny
+    kr = O.kernel_realspace(grid, px, py)
+    M = ms * m.reshape(ny, nx, 3)
+    B = np.zeros((ny, nx, 3))
+    for j in range(ny):
+        for i in range(nx):
+            for jj in range(ny):
+                for ii in range(nx):
+                    ix, iy = (i - ii) % px, (j - jj) % py
+                    kxx, kxy, kyy, kzz = kr[:, iy, ix]
+                    mm = M[jj, ii]
+                    B[j, i, 0] += kxx * mm[0] + kxy * mm[1]
+                    B[j, i, 1] += kxy * mm[0] + kyy * mm[1]
+                    B[j, i, 2] += kzz * mm[2]
+    return B.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 2), (3, 5), (6, 4), (7, 1)])
+def test_fft_convolution_equals_direct_sum(shape):
+    nx, ny = shape
+    g = O.Grid(nx, ny, 3e-9, 2e-9, 1e-9)
+    m = np.random.default_rng(nx * 7 + ny).standard_normal((nx * ny, 3))
+    b = np.zeros_like(m)
+    O.Demag(g).add_field(m, b, 8e5)
+    ref = _brute_force_demag(g, m, 8e5)
+    assert np.abs(b - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_demag_single_cell_cube_is_close_to_minus_mu0_over_3():
+    c = O.Context(O.Grid(1, 1, 1.0, 1.0, 1.0))
+    b = np.zeros((1, 3))
+    c.add_demag(np.array([[0.0, 0.0, 1.0]]), b, O.Material(ms=1.0, demag=True))
+    assert abs(b[0, 2] + MU0 / 3) < 1e-3
+
+
+def test_demag_uniform_2x2_symmetry_and_factors():
+    c = O.Context(O.Grid(2, 2, 1.0, 1.0, 1.0))
+    mat = O.Material(ms=1.0, demag=True)
+    n = []
+    for comp in range(3):
+        m = np.zeros((4, 3))
+        m[:, comp] = 1.0
+        b = np.zeros((4, 3))
+        c.add_demag(m, b, mat)
+        if comp == 2:
+            assert np.abs(b[:, :2]).max() < 1e-10
+        assert np.ptp(b[:, comp]) < 1e-10
+        assert b[0, comp] < 0
+        n.append(-b[0, comp] / MU0)
+    assert 0.05 < n[2] < 1.05 and 0.5 < sum(n) < 2.5
+    # tighter: each cell's factors of a 2x2x1 cube array sum to ~1 (sum rule)
+    assert abs(sum(n) - 1) < 2e-3
+
+
+def test_demag_energy_is_nonnegative_for_uniform_state():
+    g = O.Grid(32, 32, 5e-9, 5e-9, 1e-9)
+    c = O.Context(g)
+    m = np.tile([0.0, 0.0, 1.0], (32 * 32, 1))
+    b = np.zeros_like(m)
+    c.add_demag(m, b, O.Material(ms=1.0, demag=True))
+    e = -0.5 * 1.0 * (m * b).sum() * g.dx * g.dy * g.dz
+    assert e >= 0.0
+    # thin film limit: Nzz -> 1 in the centre
+    assert 0.8 < -b[16 * 32 + 16, 2] / MU0 < 1.0
