@@ -1,0 +1,245 @@
+"""Benchmark: field evaluations/s of the uniform-grid LLG hot path on one MI355X.
+
+Workload (1024 x 1024 x 1 film, FP32): dx = dy = 2 nm, dz = 1 nm, Ms = 5.8e5 A/m,
+A = 1.5e-11 J/m, Ku = 8e5 J/m^3 along z, interfacial D = 3e-3 J/m^2,
+alpha = 0.3, B_ext = (-24.6, 4.3, 0) mT, FFT demag zero-padded to 2048 x 2048,
+m0 = seeded +-z Voronoi domains (2048 points) with 10 % angular noise.
+
+One step = one full field evaluation (build_h_eff_masked, FieldMask::Full:
+Zeeman + exchange + anisotropy + DMI + FFT demag) with m already resident in
+HBM, called through the C ABI (llgmi_dev_build_h_eff).
+
+The JSON line also carries:
+  roofline      the dominant kernel (column FFT * tensor * inverse column FFT),
+                timed live with HIP events on the library stream during the
+                timed region; algorithmic bytes per launch as below;
+  cpu_baseline  the CPU oracle (C FP64 restatement of the reference path,
+                OpenMP over the host cores) on the same 1024^2 workload, a few
+                evaluations after an untimed warm-up that builds its kernel;
+  rel_err_vs_fp64  FP32 against the FP64 GPU path on the same m, per term;
+  kernels_us    per-kernel average durations from a separate event-timed pass;
+  rk45          accepted RK45 steps/s (FSAL, 6 field evaluations per step).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "llg-sim_amd"))
+
+METRIC = "field evals/s at 1024² FP32 vs FP64 plain (rel err ≤1e-5); % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+NX = NY = 1024
+CELL = (2e-9, 2e-9, 1e-9)
+MAT = dict(ms=5.8e5, a_ex=1.5e-11, k_u=8e5, easy_axis=(0.0, 0.0, 1.0), dmi=3e-3, demag=True)
+BEXT = (-24.6e-3, 4.3e-3, 0.0)
+ALPHA = 0.3
+
+
+def col_pass_bytes(nx, ny, elem):
+    """Algorithmic bytes of one column-pass launch: read + write the half
+    spectrum (3 components x (px/2+1) columns x ny rows, complex) and read the
+    real quadrant-symmetric kernel ((px/2+1) x (py/2+1) x 4 reals)."""
+    px, py = 2 * nx, 2 * ny
+    hx, hy = px // 2 + 1, py // 2 + 1
+    return 2 * 3 * hx * ny * 2 * elem + hx * hy * 4 * elem
+
+
+def field_eval_bytes(nx, ny, elem):
+    """Algorithmic bytes of the whole pipeline per field evaluation."""
+    n = nx * ny
+    hx = nx + 1
+    spec = 3 * hx * ny * 2 * elem
+    row_fwd = 3 * n * elem + spec
+    row_inv = spec + 3 * n * elem
+    local = 3 * 3 * n * elem
+    return row_fwd + col_pass_bytes(nx, ny, elem) + row_inv + local
+
+
+def rel_err(a, b):
+    import numpy as np
+    a = np.asarray(a).reshape(-1, 3)
+    b = np.asarray(b).reshape(-1, 3)
+    den = np.sqrt((b * b).sum(1)).max()
+    return float(np.sqrt(((a - b) ** 2).sum(1)).max() / den) if den > 0 else 0.0
+
+
+def accuracy_report(L, m_host):
+    """FP32 vs FP64 GPU path on the same m, total and per field term."""
+    grid = L.Grid2D(NX, NY, *CELL)
+    c32 = L.Context(grid, precision=L.F32)
+    c64 = L.Context(grid, precision=L.F64)
+    p = L.LLGParams(alpha=ALPHA, b_ext=BEXT)
+    out = {}
+    full = L.Material(**MAT)
+    out["total"] = rel_err(c32.build_h_eff(m_host, p, full), c64.build_h_eff(m_host, p, full))
+    p0 = L.LLGParams(alpha=ALPHA)
+    ex = L.Material(ms=MAT["ms"], a_ex=MAT["a_ex"])
+    out["exchange"] = rel_err(c32.build_h_eff(m_host, p0, ex), c64.build_h_eff(m_host, p0, ex))
+    an = L.Material(ms=MAT["ms"], k_u=MAT["k_u"], easy_axis=MAT["easy_axis"])
+    out["anisotropy"] = rel_err(c32.build_h_eff(m_host, p0, an), c64.build_h_eff(m_host, p0, an))
+    dm = L.Material(ms=MAT["ms"], a_ex=MAT["a_ex"], dmi=MAT["dmi"])
+    d32 = c32.build_h_eff_masked(m_host, p0, dm, L.FieldMask.ExchAnisDmi) - c32.build_h_eff_masked(m_host, p0, dm, L.FieldMask.ExchAnis)
+    d64 = c64.build_h_eff_masked(m_host, p0, dm, L.FieldMask.ExchAnisDmi) - c64.build_h_eff_masked(m_host, p0, dm, L.FieldMask.ExchAnis)
+    out["dmi"] = rel_err(d32, d64)
+    out["demag"] = rel_err(c32.compute_demag_field(m_host, full), c64.compute_demag_field(m_host, full))
+    bx = np.array(BEXT)
+    out["zeeman"] = float(np.linalg.norm(bx.astype(np.float32).astype(np.float64) - bx) / np.linalg.norm(bx))
+    c32.close()
+    c64.close()
+    return out
+
+
+def cpu_baseline(m_host, n_eval=2):
+    """CPU oracle (C FP64 restatement of the reference path) on this workload."""
+    from oracle import oracle as O
+    threads = O.get_threads()
+    ctx = O.Context(O.Grid(NX, NY, *CELL))
+    p = O.Params(alpha=ALPHA, b_ext=BEXT)
+    mat = O.Material(**MAT)
+    t0 = time.perf_counter()
+    ctx.build_h_eff(m_host, p, mat, O.FULL)  # warm-up: builds the kernel, untimed
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(n_eval):
+        ctx.build_h_eff(m_host, p, mat, O.FULL)
+    dt = time.perf_counter() - t0
+    return {"value": n_eval / dt, "unit": "field evals/s", "cores": threads, "kind": "port",
+            "sample": f"{n_eval} full field evaluations (1024x1024 FP64, FFT demag 2048^2) after "
+                      f"one untimed warm-up that built the kernel ({t_build:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--rk45-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU slab "
+              "decomposition is a design, not a build", file=sys.stderr)
+        sys.exit(2)
+
+    global np
+    import numpy as np
+    import torch
+    import llgmi as L
+    from initial_states import voronoi_domains
+
+    assert torch.cuda.is_available(), "no GPU visible to torch"
+    m_host = voronoi_domains(NX, NY, seed=2048, n_points=2048, noise=0.1).reshape(-1, 3)
+    print(f"[bench] workload 1024x1024 FP32, steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+
+    grid = L.Grid2D(NX, NY, *CELL)
+    ctx = L.Context(grid, precision=L.F32)
+    dev = torch.device("cuda:0")
+    m = torch.empty((3, NY, NX), dtype=torch.float32, device=dev)
+    b = torch.empty((3, NY, NX), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    ctx.dev_upload(m_host, m)
+    params = L.LLGParams(alpha=ALPHA, b_ext=BEXT)
+    mat = L.Material(**MAT)
+
+    for _ in range(args.warmup):
+        ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
+    ctx.synchronize()
+
+    # timed region: the dominant kernel is event-timed on the library stream
+    ctx.profile_reset()
+    ctx.profile_enable([L.K_FFT_COL])
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.profile_enable([])
+    col_ms, col_n = ctx.profile_read(L.K_FFT_COL)
+    elapsed = t1 - t0
+    value = args.steps / elapsed
+    col_us = 1e3 * col_ms / max(col_n, 1)
+    col_bytes = col_pass_bytes(NX, NY, 4)
+    achieved = col_bytes / (col_us * 1e-6) / 1e9
+
+    # per-kernel breakdown (separate pass, all classes timed)
+    ctx.profile_reset()
+    ctx.profile_enable(None)
+    for _ in range(50):
+        ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
+    ctx.synchronize()
+    ctx.profile_enable([])
+    kernels = {}
+    for cls, name in L.KERNEL_CLASS_NAMES.items():
+        t, c = ctx.profile_read(cls)
+        if c:
+            kernels[name] = round(1e3 * t / c, 2)
+
+    # RK45 accepted steps/s (FSAL; state resident)
+    rk = {}
+    if args.rk45_steps > 0:
+        mr = m.clone()
+        rp = L.LLGParams(alpha=ALPHA, dt=1e-14, b_ext=BEXT)
+        for _ in range(3):
+            ctx.dev_rk45_step(mr, rp, mat, L.FieldMask.Full, 1e-5, 0.8, 1e-18, 1e-12)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        acc = 0
+        att = 0
+        while acc < args.rk45_steps and att < 4 * args.rk45_steps:
+            _, a, _ = ctx.dev_rk45_step(mr, rp, mat, L.FieldMask.Full, 1e-5, 0.8, 1e-18, 1e-12)
+            acc += int(a)
+            att += 1
+        ctx.synchronize()
+        dtr = time.perf_counter() - t0
+        rk = {"accepted_steps_per_s": round(acc / dtr, 1), "attempts": att, "accepted": acc,
+              "max_err": 1e-5, "last_dt": rp.dt}
+
+    acc_rep = accuracy_report(L, m_host)
+    print(f"[bench] accuracy done: {acc_rep}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(m_host)
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "field evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "uniform film 1024x1024x1, dx=dy=2nm dz=1nm, exchange+anisotropy+"
+                               "interfacial DMI+Zeeman+FFT demag (padded 2048^2), Voronoi +-z m0",
+                   "nx": NX, "ny": NY, "precision": "fp32", "field_mask": "Full"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_col_demag (column FFT x tensor x inverse column FFT)",
+                     "bytes_per_launch": col_bytes, "avg_us": round(col_us, 2)},
+        "cpu_baseline": cpu,
+        "pipeline_bytes_per_eval": field_eval_bytes(NX, NY, 4),
+        "pipeline_frac_hbm": round(field_eval_bytes(NX, NY, 4) * value / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels_us": kernels,
+        "rel_err_vs_fp64": acc_rep,
+        "rk45": rk,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -176,7 +176,8 @@ int llgmi_dev_download(llgmi_ctx *ctx, const void *dev_soa, double *host_aos);
 #define LLGMI_K_RK45_ERR 5      /* error-norm reduction */
 #define LLGMI_K_MG 6            /* all multigrid kernels */
 #define LLGMI_K_COUNT 8
-int llgmi_profile_enable(llgmi_ctx *ctx, int on);
+/* Times the kernel classes whose bit (1 << class) is set in class_mask; 0 stops. */
+int llgmi_profile_enable(llgmi_ctx *ctx, int class_mask);
 int llgmi_profile_reset(llgmi_ctx *ctx);
 /* Sum of event-timed durations (ms) and number of launches of a class. */
 int llgmi_profile_read(llgmi_ctx *ctx, int kernel_class, double *total_ms, int64_t *count);

@@ -11,7 +11,7 @@ namespace llgmi {
 // Per-kernel-class timing with HIP events recorded on the launch stream.
 // ---------------------------------------------------------------------------
 struct Profiler {
-    bool on = false;
+    unsigned mask = 0;  // bit c set: time kernel class c
     struct Pending {
         int cls;
         hipEvent_t a, b;
@@ -59,7 +59,7 @@ struct ProfScope {
     hipStream_t s;
     hipEvent_t a{}, b{};
     ProfScope(Profiler *p_, int c, hipStream_t s_) : p(p_), cls(c), s(s_) {
-        if (p && p->on) {
+        if (p && (p->mask >> cls) & 1u) {
             if (p->pending.size() > 4096) p->drain();
             a = p->get();
             b = p->get();
@@ -67,7 +67,7 @@ struct ProfScope {
         }
     }
     ~ProfScope() {
-        if (p && p->on) {
+        if (p && (p->mask >> cls) & 1u) {
             if (hipEventRecord(b, s) == hipSuccess) p->pending.push_back({cls, a, b});
         }
     }

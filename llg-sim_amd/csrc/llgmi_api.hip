@@ -618,8 +618,8 @@ int llgmi_dev_download(llgmi_ctx *ctx, const void *dev_soa, double *host_aos) {
     });
 }
 
-int llgmi_profile_enable(llgmi_ctx *ctx, int on) {
-    return guarded(ctx, [&] { ctx->prof.on = on != 0; });
+int llgmi_profile_enable(llgmi_ctx *ctx, int class_mask) {
+    return guarded(ctx, [&] { ctx->prof.mask = (unsigned)class_mask & ((1u << LLGMI_K_COUNT) - 1u); });
 }
 
 int llgmi_profile_reset(llgmi_ctx *ctx) {

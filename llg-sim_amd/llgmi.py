@@ -334,8 +334,15 @@ class Context:
         return h
 
     # ---- timing ----
-    def profile_enable(self, on: bool = True) -> None:
-        _check(lib().llgmi_profile_enable(self._h, 1 if on else 0))
+    def profile_enable(self, classes=None) -> None:
+        """Times the given kernel classes (iterable of K_*), all when None; [] stops."""
+        if classes is None:
+            mask = (1 << 8) - 1
+        else:
+            mask = 0
+            for c in classes:
+                mask |= 1 << int(c)
+        _check(lib().llgmi_profile_enable(self._h, mask))
 
     def profile_reset(self) -> None:
         _check(lib().llgmi_profile_reset(self._h))
