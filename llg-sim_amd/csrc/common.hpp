@@ -111,88 +111,6 @@ __device__ __forceinline__ void dftR(cpx<T> *v) {
     else dft8<T, INV>(v);
 }
 
-// ---------------------------------------------------------------------------
-// One Stockham stage over `nseq` sequences of length L held in LDS at
-// buf[s*sstride + i].  Stage with span Ns and radix R:
-//   read   v_r = buf[j + r*L/R] * w^(r*k), k = j mod Ns, w = exp(-2 pi i/(Ns R))
-//   DFT_R(v)
-//   write  buf[(j/Ns)*Ns*R + k + r*Ns] = v_r
-// All reads of a stage complete (barrier) before any write, so one buffer
-// suffices; each thread keeps at most EMAX/R butterflies in registers.
-// tw[t] = exp(-2 pi i t / L) (t < L), precomputed in FP64 and rounded to T.
-// ---------------------------------------------------------------------------
-template <typename T, bool INV, int R, int EMAX>
-__device__ __forceinline__ void stockham_stage(cpx<T> *buf, int L, int log2L, int nseq,
-                                               int sstride, int Ns, int log2Ns,
-                                               const cpx<T> *__restrict__ tw) {
-    constexpr int BPT = (EMAX + R - 1) / R;
-    const int nt = blockDim.x;
-    const int tid = threadIdx.x;
-    const int log2R = R == 8 ? 3 : (R == 4 ? 2 : 1);
-    const int log2nb1 = log2L - log2R;  // butterflies per sequence = L/R
-    const int nb = nseq << log2nb1;
-    const int twstep_log2 = log2L - log2Ns - log2R;  // L/(Ns R)
-    cpx<T> v[BPT][R];
-#pragma unroll
-    for (int q = 0; q < BPT; ++q) {
-        int b = tid + q * nt;
-        if (b < nb) {
-            int s = b >> log2nb1;
-            int j = b & ((1 << log2nb1) - 1);
-            int k = j & (Ns - 1);
-            const cpx<T> *src = buf + s * sstride + j;
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = src[r << log2nb1];
-            if (Ns > 1) {
-#pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    cpx<T> w = tw[(r * k) << twstep_log2];
-                    v[q][r] = INV ? cmulc(v[q][r], w) : cmul(v[q][r], w);
-                }
-            }
-            dftR<T, INV, R>(v[q]);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < BPT; ++q) {
-        int b = tid + q * nt;
-        if (b < nb) {
-            int s = b >> log2nb1;
-            int j = b & ((1 << log2nb1) - 1);
-            int k = j & (Ns - 1);
-            int idxD = ((j >> log2Ns) << (log2Ns + log2R)) + k;
-            cpx<T> *dst = buf + s * sstride + idxD;
-#pragma unroll
-            for (int r = 0; r < R; ++r) dst[r << log2Ns] = v[q][r];
-        }
-    }
-    __syncthreads();
-}
-
-// Full transform of nseq sequences of length L = 2^log2L in LDS (unnormalised).
-// Stages: radix 8 while >= 3 bits remain except that a remainder of 1 bit after
-// radix-8 stages is folded into two radix-4 stages (e.g. 2^10 = 8*8*4*4).
-// Caller guarantees nseq*L <= EMAX*blockDim.x and a barrier before the call.
-template <typename T, bool INV, int EMAX>
-__device__ void lds_fft(cpx<T> *buf, int L, int log2L, int nseq, int sstride,
-                        const cpx<T> *__restrict__ tw) {
-    int rem = log2L;
-    int Ns = 1, log2Ns = 0;
-    while (rem > 0) {
-        if (rem >= 3 && rem != 4) {
-            stockham_stage<T, INV, 8, EMAX>(buf, L, log2L, nseq, sstride, Ns, log2Ns, tw);
-            rem -= 3; Ns <<= 3; log2Ns += 3;
-        } else if (rem >= 2) {
-            stockham_stage<T, INV, 4, EMAX>(buf, L, log2L, nseq, sstride, Ns, log2Ns, tw);
-            rem -= 2; Ns <<= 2; log2Ns += 2;
-        } else {
-            stockham_stage<T, INV, 2, EMAX>(buf, L, log2L, nseq, sstride, Ns, log2Ns, tw);
-            rem -= 1; Ns <<= 1; log2Ns += 1;
-        }
-    }
-}
-
 inline int ilog2(int64_t v) {
     int r = 0;
     while ((int64_t(1) << r) < v) ++r;

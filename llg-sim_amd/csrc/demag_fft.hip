@@ -21,6 +21,7 @@
 //  * Pass 3 (row C2R): two Hermitian rows per complex inverse FFT; the first
 //    nx outputs of each row are written as B_demag (SoA).
 #include "common.hpp"
+#include "fft_lds.hpp"
 #include "internal.hpp"
 
 #include <cmath>
@@ -28,45 +29,77 @@
 
 namespace llgmi {
 
-static constexpr int kEmax = 32;  // complex elements held per thread in an LDS FFT
+// Block shapes, fixed per (precision, log2 length).  Row passes hold PAIRS
+// pairs of rows (one complex FFT per pair) in about 64 KB of LDS; the column
+// pass holds the three components of one column.  About 16 complex values per
+// thread.
+__host__ __device__ constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <typename T, int LOG2L>
+struct RowCfg {
+    static constexpr int L = 1 << LOG2L;
+    static constexpr int PAIRS = clampi((int)(65536 / (sizeof(cpx<T>) * L)), 1, 8);
+    static constexpr int NT = clampi((PAIRS * L / 16 + 63) / 64 * 64, 64, 512);
+    static constexpr int SLEN = lds_seq_len(L);
+    static constexpr size_t LDS = sizeof(cpx<T>) * PAIRS * SLEN;
+};
+
+template <typename T, int LOG2L>
+struct ColCfg {
+    static constexpr int L = 1 << LOG2L;
+    static constexpr int NSEQ = 3;
+    static constexpr int NT = clampi((NSEQ * L / 16 + 63) / 64 * 64, 64, 512);
+    static constexpr int SLEN = lds_seq_len(L);
+    static constexpr size_t LDS = sizeof(cpx<T>) * NSEQ * SLEN;
+};
+
+template <int LOG2L>
+struct ExtractCfg {
+    static constexpr int L = 1 << LOG2L;
+    static constexpr int NT = clampi((L / 16 + 63) / 64 * 64, 64, 512);
+    static constexpr int SLEN = lds_seq_len(L);
+    static constexpr size_t LDS = sizeof(cpx<double>) * SLEN;
+};
 
 // ---------------------------------------------------------------------------
-// Pass 1: row R2C.  Block = `pairs` pairs of rows of one component.
+// Pass 1: row R2C.  Block = PAIRS pairs of rows of one component; two real rows
+// form one complex sequence.  Columns >= ncols_valid and rows >= nrows are zero.
 // dst[comp*dst_cstride + k*nrows + j] for k <= L/2, j < nrows.
 // ---------------------------------------------------------------------------
-template <typename T, typename Tin>
-__global__ void __launch_bounds__(512)
+template <typename T, typename Tin, int LOG2L>
+__global__ void __launch_bounds__((RowCfg<T, LOG2L>::NT))
 k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int ncols_valid,
-          int nrows, T scale, int L, int log2L, int pairs, const cpx<T> *__restrict__ tw,
-          cpx<T> *__restrict__ dst, int64_t dst_cstride) {
+          int nrows, T scale, const cpx<T> *__restrict__ tw, cpx<T> *__restrict__ dst,
+          int64_t dst_cstride) {
+    using C = RowCfg<T, LOG2L>;
+    constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int comp = blockIdx.y;
-    const int j0 = blockIdx.x * pairs * 2;
+    const int j0 = blockIdx.x * PAIRS * 2;
     const Tin *s = src + comp * src_cstride;
-    const int tot = pairs << log2L;
-    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-        int p = idx >> log2L, i = idx & (L - 1);
-        int ja = j0 + 2 * p, jb = ja + 1;
+    for (int idx = threadIdx.x; idx < PAIRS * L; idx += NT) {
+        const int p = idx >> LOG2L, i = idx & (L - 1);
+        const int ja = j0 + 2 * p, jb = ja + 1;
         T re = T(0), im = T(0);
         if (i < ncols_valid) {
             if (ja < nrows) re = scale * T(s[(int64_t)ja * row_stride + i]);
             if (jb < nrows) im = scale * T(s[(int64_t)jb * row_stride + i]);
         }
-        buf[idx] = {re, im};
+        buf[p * SLEN + lds_pad(i)] = {re, im};
     }
     __syncthreads();
-    lds_fft<T, false, kEmax>(buf, L, log2L, pairs, L, tw);
-    const int hx = (L >> 1) + 1;
-    const int nr = 2 * pairs;
+    lds_fft_ct<T, false, LOG2L, PAIRS, NT>(buf, tw);
+    constexpr int HX = L / 2 + 1;
+    constexpr int NR = 2 * PAIRS;
     cpx<T> *d = dst + comp * dst_cstride;
-    for (int idx = threadIdx.x; idx < hx * nr; idx += blockDim.x) {
-        int r = idx % nr, k = idx / nr;
-        int j = j0 + r;
+    for (int idx = threadIdx.x; idx < HX * NR; idx += NT) {
+        const int r = idx % NR, k = idx / NR;
+        const int j = j0 + r;
         if (j >= nrows) continue;
-        int p = r >> 1;
-        cpx<T> z = buf[(p << log2L) + k];
-        cpx<T> zc = buf[(p << log2L) + ((L - k) & (L - 1))];
+        const int p = r >> 1;
+        const cpx<T> z = buf[p * SLEN + lds_pad(k)];
+        const cpx<T> zc = buf[p * SLEN + lds_pad((L - k) & (L - 1))];
         cpx<T> X;
         if ((r & 1) == 0) X = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
         else X = {T(0.5) * (z.y + zc.y), T(0.5) * (zc.x - z.x)};
@@ -77,70 +110,75 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
 // ---------------------------------------------------------------------------
 // Pass 2: column FFT, tensor multiply, inverse column FFT.  Block = one kx.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(512)
-k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, int L, int log2L,
-            const cpx<T> *__restrict__ tw, const quad4<T> *__restrict__ kern, T scale) {
+template <typename T, int LOG2L>
+__global__ void __launch_bounds__((ColCfg<T, LOG2L>::NT))
+k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, const cpx<T> *__restrict__ tw,
+            const quad4<T> *__restrict__ kern, T scale) {
+    using C = ColCfg<T, LOG2L>;
+    constexpr int L = C::L, NT = C::NT, SLEN = C::SLEN;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int kx = blockIdx.x;
-    for (int idx = threadIdx.x; idx < 3 * L; idx += blockDim.x) {
-        int c = idx >> log2L, j = idx & (L - 1);
-        buf[idx] = j < ny ? S[((int64_t)c * hx + kx) * ny + j] : cpx<T>{T(0), T(0)};
+    for (int idx = threadIdx.x; idx < 3 * L; idx += NT) {
+        const int c = idx >> LOG2L, j = idx & (L - 1);
+        buf[c * SLEN + lds_pad(j)] = j < ny ? S[((int64_t)c * hx + kx) * ny + j] : cpx<T>{T(0), T(0)};
     }
     __syncthreads();
-    lds_fft<T, false, kEmax>(buf, L, log2L, 3, L, tw);
-    const int hy = (L >> 1) + 1;
-    const quad4<T> *kc = kern + (int64_t)kx * hy;
-    for (int ky = threadIdx.x; ky < L; ky += blockDim.x) {
-        int kq = ky < hy ? ky : L - ky;
-        quad4<T> K = kc[kq];
-        T kxy = ky < hy ? K.xy : -K.xy;
-        cpx<T> mx = buf[ky], my = buf[L + ky], mz = buf[2 * L + ky];
-        buf[ky] = {scale * (K.xx * mx.x + kxy * my.x), scale * (K.xx * mx.y + kxy * my.y)};
-        buf[L + ky] = {scale * (kxy * mx.x + K.yy * my.x), scale * (kxy * mx.y + K.yy * my.y)};
-        buf[2 * L + ky] = {scale * (K.zz * mz.x), scale * (K.zz * mz.y)};
+    lds_fft_ct<T, false, LOG2L, 3, NT>(buf, tw);
+    constexpr int HY = L / 2 + 1;
+    const quad4<T> *kc = kern + (int64_t)kx * HY;
+    for (int ky = threadIdx.x; ky < L; ky += NT) {
+        const int kq = ky < HY ? ky : L - ky;
+        const quad4<T> K = kc[kq];
+        const T kxy = ky < HY ? K.xy : -K.xy;
+        const int o = lds_pad(ky);
+        const cpx<T> mx = buf[o], my = buf[SLEN + o], mz = buf[2 * SLEN + o];
+        buf[o] = {scale * (K.xx * mx.x + kxy * my.x), scale * (K.xx * mx.y + kxy * my.y)};
+        buf[SLEN + o] = {scale * (kxy * mx.x + K.yy * my.x), scale * (kxy * mx.y + K.yy * my.y)};
+        buf[2 * SLEN + o] = {scale * (K.zz * mz.x), scale * (K.zz * mz.y)};
     }
     __syncthreads();
-    lds_fft<T, true, kEmax>(buf, L, log2L, 3, L, tw);
-    for (int idx = threadIdx.x; idx < 3 * ny; idx += blockDim.x) {
-        int c = idx / ny, j = idx - c * ny;
-        S[((int64_t)c * hx + kx) * ny + j] = buf[c * L + j];
+    lds_fft_ct<T, true, LOG2L, 3, NT>(buf, tw);
+    for (int idx = threadIdx.x; idx < 3 * ny; idx += NT) {
+        const int c = idx / ny, j = idx - c * ny;
+        S[((int64_t)c * hx + kx) * ny + j] = buf[c * SLEN + lds_pad(j)];
     }
 }
 
 // ---------------------------------------------------------------------------
-// Pass 3: row C2R.  Block = `pairs` pairs of rows of one component.  The k = 0
-// and k = L/2 bins use their real parts only, which is what taking Re() of
-// the reference's full complex inverse transform amounts to.
+// Pass 3: row C2R.  Block = PAIRS pairs of rows of one component.  The k = 0
+// and k = L/2 bins use their real parts only, which is what taking Re() of the
+// reference's full complex inverse transform amounts to.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(512)
-k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, int L, int log2L, int pairs,
-          const cpx<T> *__restrict__ tw, T *__restrict__ out, int64_t n) {
+template <typename T, int LOG2L>
+__global__ void __launch_bounds__((RowCfg<T, LOG2L>::NT))
+k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, const cpx<T> *__restrict__ tw,
+          T *__restrict__ out, int64_t n) {
+    using C = RowCfg<T, LOG2L>;
+    constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int comp = blockIdx.y;
-    const int j0 = blockIdx.x * pairs * 2;
+    const int j0 = blockIdx.x * PAIRS * 2;
     const cpx<T> *s = S + (int64_t)comp * hx * ny;
-    const int half = L >> 1;
-    for (int idx = threadIdx.x; idx < hx * pairs; idx += blockDim.x) {
-        int p = idx % pairs, k = idx / pairs;
-        int ja = j0 + 2 * p, jb = ja + 1;
+    constexpr int HALF = L >> 1;
+    for (int idx = threadIdx.x; idx < (HALF + 1) * PAIRS; idx += NT) {
+        const int p = idx % PAIRS, k = idx / PAIRS;
+        const int ja = j0 + 2 * p, jb = ja + 1;
         cpx<T> xa = ja < ny ? s[(int64_t)k * ny + ja] : cpx<T>{T(0), T(0)};
         cpx<T> xb = jb < ny ? s[(int64_t)k * ny + jb] : cpx<T>{T(0), T(0)};
-        if (k == 0 || k == half) { xa.y = T(0); xb.y = T(0); }
-        cpx<T> *row = buf + (p << log2L);
-        row[k] = {xa.x - xb.y, xa.y + xb.x};
-        if (k != 0 && k != half) row[L - k] = {xa.x + xb.y, xb.x - xa.y};
+        if (k == 0 || k == HALF) { xa.y = T(0); xb.y = T(0); }
+        cpx<T> *row = buf + p * SLEN;
+        row[lds_pad(k)] = {xa.x - xb.y, xa.y + xb.x};
+        if (k != 0 && k != HALF) row[lds_pad(L - k)] = {xa.x + xb.y, xb.x - xa.y};
     }
     __syncthreads();
-    lds_fft<T, true, kEmax>(buf, L, log2L, pairs, L, tw);
+    lds_fft_ct<T, true, LOG2L, PAIRS, NT>(buf, tw);
     T *o = out + comp * n;
-    for (int idx = threadIdx.x; idx < pairs * nx; idx += blockDim.x) {
-        int p = idx / nx, i = idx - p * nx;
-        int ja = j0 + 2 * p;
-        cpx<T> z = buf[(p << log2L) + i];
+    for (int idx = threadIdx.x; idx < PAIRS * nx; idx += NT) {
+        const int p = idx / nx, i = idx - p * nx;
+        const int ja = j0 + 2 * p;
+        const cpx<T> z = buf[p * SLEN + lds_pad(i)];
         if (ja < ny) o[(int64_t)ja * nx + i] = z.x;
         if (ja + 1 < ny) o[(int64_t)(ja + 1) * nx + i] = z.y;
     }
@@ -150,20 +188,22 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, int L, int log2L
 // One-time k-space kernel: forward column FFT (FP64) of the row spectra of the
 // real-space kernels and extraction of the real quadrant.  Block = (kx, comp).
 // ---------------------------------------------------------------------------
-template <typename Tout>
-__global__ void __launch_bounds__(512)
-k_col_extract(const cpx<double> *__restrict__ Sk, int hx, int L, int log2L,
-              const cpx<double> *__restrict__ tw, Tout *__restrict__ kern) {
+template <typename Tout, int LOG2L>
+__global__ void __launch_bounds__((ExtractCfg<LOG2L>::NT))
+k_col_extract(const cpx<double> *__restrict__ Sk, int hx, const cpx<double> *__restrict__ tw,
+              Tout *__restrict__ kern) {
+    using C = ExtractCfg<LOG2L>;
+    constexpr int L = C::L, NT = C::NT;
     extern __shared__ unsigned char lds_raw[];
     cpx<double> *buf = reinterpret_cast<cpx<double> *>(lds_raw);
     const int kx = blockIdx.x, comp = blockIdx.y;
     const cpx<double> *s = Sk + ((int64_t)comp * hx + kx) * L;
-    for (int j = threadIdx.x; j < L; j += blockDim.x) buf[j] = s[j];
+    for (int j = threadIdx.x; j < L; j += NT) buf[lds_pad(j)] = s[j];
     __syncthreads();
-    lds_fft<double, false, kEmax>(buf, L, log2L, 1, L, tw);
-    const int hy = (L >> 1) + 1;
-    for (int ky = threadIdx.x; ky < hy; ky += blockDim.x)
-        kern[((int64_t)kx * hy + ky) * 4 + comp] = Tout(buf[ky].x);
+    lds_fft_ct<double, false, LOG2L, 1, NT>(buf, tw);
+    constexpr int HY = L / 2 + 1;
+    for (int ky = threadIdx.x; ky < HY; ky += NT)
+        kern[((int64_t)kx * HY + ky) * 4 + comp] = Tout(buf[lds_pad(ky)].x);
 }
 
 // ---------------------------------------------------------------------------
@@ -325,34 +365,23 @@ static cpx<T> *make_twiddles(int L) {
     return d;
 }
 
-// Threads for an LDS FFT holding `elems` complex values: keep elems/threads <= kEmax.
-static int fft_threads(int64_t elems) {
-    int nt = 256;
-    while ((int64_t)nt * kEmax < elems && nt < 512) nt *= 2;
-    if ((int64_t)nt * kEmax < elems) throw Error(3, "FFT length too large for the LDS kernels");
-    return nt;
-}
-
-static constexpr size_t kLdsBudget = 64 * 1024;  // bytes per block for the row passes
 static constexpr size_t kLdsMax = 160 * 1024;
 
 // Dynamic LDS above the default per-launch limit must be opted into.
 template <typename K>
 static void allow_lds(K kernel, size_t bytes) {
+    if (bytes > kLdsMax) throw Error(3, "FFT block needs more LDS than a CU has");
     if (bytes > 64 * 1024)
         LLGMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-template <typename T>
-static int row_pairs(int L, int nrows) {
-    int pairs = (int)(kLdsBudget / (sizeof(cpx<T>) * (size_t)L));
-    if (pairs < 1) pairs = 1;
-    int need = (nrows + 1) / 2;
-    if (pairs > need) pairs = need;
-    if (pairs > 8) pairs = 8;
-    return pairs;
-}
+// Built ranges: rows up to 2^13 (FP32) / 2^12 (FP64), columns up to 2^12 / 2^11.
+template <typename T> struct Range;
+template <> struct Range<float> { static constexpr int ROW = 13, COL = 12; };
+template <> struct Range<double> { static constexpr int ROW = 12, COL = 11; };
+static constexpr int kRowF64Build = 13;  // FP64 row pass of the kernel build (FP32 contexts)
+static constexpr int kExtractMax = 12;
 
 template <typename T>
 DemagFFT<T>::~DemagFFT() {
@@ -377,23 +406,20 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     nx = nx_; ny = ny_; dx = dx_; dy = dy_; dz = dz_;
     px = next_pow2(2 * nx);
     py = next_pow2(2 * ny);
-    const size_t maxL = kLdsMax / (3 * sizeof(cpx<T>));
-    if ((size_t)py > maxL || (size_t)px > 8192 || (size_t)py > 8192)
-        throw Error(3, "FFT demag: padded size " + std::to_string(px) + "x" + std::to_string(py) +
-                           " exceeds the LDS column pass of this precision");
-    if ((size_t)px > kLdsMax / sizeof(cpx<double>) || (size_t)py > kLdsMax / sizeof(cpx<double>))
-        throw Error(3, "FFT demag: padded size too large for the FP64 kernel build");
     hx = px / 2 + 1;
     hy = py / 2 + 1;
     lx = ilog2(px);
     ly = ilog2(py);
+    if (lx > Range<T>::ROW || ly > Range<T>::COL || ly > kExtractMax)
+        throw Error(3, "FFT demag: padded size " + std::to_string(px) + "x" + std::to_string(py) +
+                           " is beyond the built FFT lengths of this precision");
     const double mu0 = 4.0 * M_PI * 1e-7;
     const double acc = 10.0;  // DEMAG_ACCURACY (demag_fft_uniform.rs:432)
 
-    // 1) base-quadrant quadrature (FP64)
     double *base = nullptr, *kr = nullptr;
     cpx<double> *sk = nullptr, *twx64 = nullptr, *twy64 = nullptr;
     try {
+        // 1) base-quadrant quadrature (FP64)
         const int64_t nb = nx * ny;
         LLGMI_HIP(hipMalloc(&base, sizeof(double) * 4 * nb));
         {
@@ -413,25 +439,25 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         twx64 = make_twiddles<double>((int)px);
         twy64 = make_twiddles<double>((int)py);
         LLGMI_HIP(hipMalloc(&sk, sizeof(cpx<double>) * 4 * hx * py));
-        {
-            int pairs = row_pairs<double>((int)px, (int)py);
-            int nt = fft_threads((int64_t)pairs * px);
-            size_t lds = sizeof(cpx<double>) * pairs * px;
-            dim3 grid((unsigned)((py + 2 * pairs - 1) / (2 * pairs)), 4);
-            hipLaunchKernelGGL((k_row_r2c<double, double>), grid, dim3(nt), lds, stream, kr, np,
-                               (int)px, (int)px, (int)py, 1.0, (int)px, lx, pairs, twx64, sk,
-                               (int64_t)hx * py);
+        dispatch_log2<1, kRowF64Build>(lx, [&](auto lg) {
+            constexpr int LG = decltype(lg)::value;
+            using C = RowCfg<double, LG>;
+            allow_lds(k_row_r2c<double, double, LG>, C::LDS);
+            dim3 grid((unsigned)((py + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 4);
+            hipLaunchKernelGGL((k_row_r2c<double, double, LG>), grid, dim3(C::NT), C::LDS, stream, kr,
+                               np, (int)px, (int)px, (int)py, 1.0, twx64, sk, (int64_t)hx * py);
             LLGMI_LAUNCH_CHECK();
-        }
+        });
         // 4) column FFT (FP64), real quadrant -> kern (T)
         LLGMI_HIP(hipMalloc(&kern, sizeof(quad4<T>) * hx * hy));
-        {
-            int nt = fft_threads(py);
-            size_t lds = sizeof(cpx<double>) * py;
-            hipLaunchKernelGGL((k_col_extract<T>), dim3((unsigned)hx, 4), dim3(nt), lds, stream, sk,
-                               (int)hx, (int)py, ly, twy64, reinterpret_cast<T *>(kern));
+        dispatch_log2<1, kExtractMax>(ly, [&](auto lg) {
+            constexpr int LG = decltype(lg)::value;
+            using C = ExtractCfg<LG>;
+            allow_lds(k_col_extract<T, LG>, C::LDS);
+            hipLaunchKernelGGL((k_col_extract<T, LG>), dim3((unsigned)hx, 4), dim3(C::NT), C::LDS,
+                               stream, sk, (int)hx, twy64, reinterpret_cast<T *>(kern));
             LLGMI_LAUNCH_CHECK();
-        }
+        });
         LLGMI_HIP(hipStreamSynchronize(stream));
     } catch (...) {
         if (base) (void)hipFree(base);
@@ -451,40 +477,49 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     tw_x = make_twiddles<T>((int)px);
     tw_y = make_twiddles<T>((int)py);
     LLGMI_HIP(hipMalloc(&spec, sizeof(cpx<T>) * 3 * hx * ny));
-    pairs_fwd = row_pairs<T>((int)px, (int)ny);
-    nt_row = fft_threads((int64_t)pairs_fwd * px);
-    nt_col = fft_threads(3 * py);
-    allow_lds(k_col_demag<T>, sizeof(cpx<T>) * 3 * py);
+    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        allow_lds(k_row_r2c<T, T, LG>, RowCfg<T, LG>::LDS);
+        allow_lds(k_row_c2r<T, LG>, RowCfg<T, LG>::LDS);
+        pairs_fwd = RowCfg<T, LG>::PAIRS;
+    });
+    dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        allow_lds(k_col_demag<T, LG>, ColCfg<T, LG>::LDS);
+    });
     built = true;
 }
 
 template <typename T>
 void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *prof) {
     const int64_t n = nx * ny;
-    const int pairs = pairs_fwd;
-    const dim3 grid_rows((unsigned)((ny + 2 * pairs - 1) / (2 * pairs)), 3);
-    const size_t lds_row = sizeof(cpx<T>) * pairs * px;
-    {
+    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        using C = RowCfg<T, LG>;
+        const dim3 grid((unsigned)((ny + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 3);
         ProfScope ps(prof, 0, stream);
-        hipLaunchKernelGGL((k_row_r2c<T, T>), grid_rows, dim3(nt_row), lds_row, stream, m, n,
-                           (int)nx, (int)nx, (int)ny, ms, (int)px, lx, pairs, tw_x, spec,
-                           (int64_t)hx * ny);
+        hipLaunchKernelGGL((k_row_r2c<T, T, LG>), grid, dim3(C::NT), C::LDS, stream, m, n, (int)nx,
+                           (int)nx, (int)ny, ms, tw_x, spec, (int64_t)hx * ny);
         LLGMI_LAUNCH_CHECK();
-    }
-    {
+    });
+    dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        using C = ColCfg<T, LG>;
         ProfScope ps(prof, 1, stream);
         const T scale = T(1.0 / ((double)px * (double)py));
-        hipLaunchKernelGGL((k_col_demag<T>), dim3((unsigned)hx), dim3(nt_col),
-                           sizeof(cpx<T>) * 3 * py, stream, spec, (int)hx, (int)ny, (int)py, ly,
-                           tw_y, kern, scale);
+        hipLaunchKernelGGL((k_col_demag<T, LG>), dim3((unsigned)hx), dim3(C::NT), C::LDS, stream,
+                           spec, (int)hx, (int)ny, tw_y, kern, scale);
         LLGMI_LAUNCH_CHECK();
-    }
-    {
+    });
+    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        using C = RowCfg<T, LG>;
+        const dim3 grid((unsigned)((ny + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 3);
         ProfScope ps(prof, 2, stream);
-        hipLaunchKernelGGL((k_row_c2r<T>), grid_rows, dim3(nt_row), lds_row, stream, spec,
-                           (int)hx, (int)nx, (int)ny, (int)px, lx, pairs, tw_x, bd, n);
+        hipLaunchKernelGGL((k_row_c2r<T, LG>), grid, dim3(C::NT), C::LDS, stream, spec, (int)hx,
+                           (int)nx, (int)ny, tw_x, bd, n);
         LLGMI_LAUNCH_CHECK();
-    }
+    });
 }
 
 template <typename T>
