@@ -120,11 +120,28 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--rk45-steps", type=int, default=20)
+    ap.add_argument("--quick", action="store_true",
+                    help="field-evaluation loop only (no accuracy check, CPU leg or RK45); for profilers")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 256 MiB (x10) to scale HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU slab "
               "decomposition is a design, not a build", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4,
+                          "launches": 10}), flush=True)
+        return
+    if args.quick:
+        args.no_cpu = True
+        args.rk45_steps = 0
 
     global np
     import numpy as np
@@ -202,8 +219,10 @@ def main():
         rk = {"accepted_steps_per_s": round(acc / dtr, 1), "attempts": att, "accepted": acc,
               "max_err": 1e-5, "last_dt": rp.dt}
 
-    acc_rep = accuracy_report(L, m_host)
-    print(f"[bench] accuracy done: {acc_rep}", file=sys.stderr, flush=True)
+    acc_rep = None
+    if not args.quick:
+        acc_rep = accuracy_report(L, m_host)
+        print(f"[bench] accuracy done: {acc_rep}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(m_host)

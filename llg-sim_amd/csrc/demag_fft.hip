@@ -21,7 +21,7 @@
 //  * Pass 3 (row C2R): two Hermitian rows per complex inverse FFT; the first
 //    nx outputs of each row are written as B_demag (SoA).
 #include "common.hpp"
-#include "fft_lds.hpp"
+#include "fft_reg.hpp"
 #include "internal.hpp"
 
 #include <cmath>
@@ -29,41 +29,43 @@
 
 namespace llgmi {
 
-// Block shapes, fixed per (precision, log2 length).  Row passes hold PAIRS
-// pairs of rows (one complex FFT per pair) in about 64 KB of LDS; the column
-// pass holds the three components of one column.  About 16 complex values per
-// thread.
+// Block shapes, fixed per (precision, log2 length).  One sequence of length L
+// is transformed by TPS = L/R0 threads holding R0 values each (fft_reg.hpp).
+// Row passes hold PAIRS pairs of rows (one complex sequence per pair) in at
+// most ~80 KB of LDS; the column pass holds the three components of one column.
 __host__ __device__ constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__host__ __device__ constexpr int mini(int a, int b) { return a < b ? a : b; }
 
 template <typename T, int LOG2L>
 struct RowCfg {
-    static constexpr int L = 1 << LOG2L;
-    static constexpr int PAIRS = clampi((int)(65536 / (sizeof(cpx<T>) * L)), 1, 8);
-    static constexpr int NT = clampi((PAIRS * L / 16 + 63) / 64 * 64, 64, 512);
-    static constexpr int SLEN = lds_seq_len(L);
+    using P = RegPlan<LOG2L>;
+    static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
+    static constexpr int PAIRS =
+        clampi(mini((int)(80 * 1024 / (sizeof(cpx<T>) * SLEN)), 512 / TPS), 1, 64);
+    static constexpr int NT = PAIRS * TPS;
     static constexpr size_t LDS = sizeof(cpx<T>) * PAIRS * SLEN;
 };
 
 template <typename T, int LOG2L>
 struct ColCfg {
-    static constexpr int L = 1 << LOG2L;
-    static constexpr int NSEQ = 3;
-    static constexpr int NT = clampi((NSEQ * L / 16 + 63) / 64 * 64, 64, 512);
-    static constexpr int SLEN = lds_seq_len(L);
-    static constexpr size_t LDS = sizeof(cpx<T>) * NSEQ * SLEN;
+    using P = RegPlan<LOG2L>;
+    static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
+    static constexpr int NT = 3 * TPS;
+    static constexpr size_t LDS = sizeof(cpx<T>) * 3 * SLEN;
 };
 
 template <int LOG2L>
 struct ExtractCfg {
-    static constexpr int L = 1 << LOG2L;
-    static constexpr int NT = clampi((L / 16 + 63) / 64 * 64, 64, 512);
-    static constexpr int SLEN = lds_seq_len(L);
+    using P = RegPlan<LOG2L>;
+    static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
+    static constexpr int NT = TPS;
     static constexpr size_t LDS = sizeof(cpx<double>) * SLEN;
 };
 
 // ---------------------------------------------------------------------------
 // Pass 1: row R2C.  Block = PAIRS pairs of rows of one component; two real rows
-// form one complex sequence.  Columns >= ncols_valid and rows >= nrows are zero.
+// form one complex sequence, loaded straight into registers.  Columns
+// >= ncols_valid and rows >= nrows are zero and not read.
 // dst[comp*dst_cstride + k*nrows + j] for k <= L/2, j < nrows.
 // ---------------------------------------------------------------------------
 template <typename T, typename Tin, int LOG2L>
@@ -72,24 +74,33 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
           int nrows, T scale, const cpx<T> *__restrict__ tw, cpx<T> *__restrict__ dst,
           int64_t dst_cstride) {
     using C = RowCfg<T, LOG2L>;
-    constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN;
+    constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
+    constexpr int R0 = RegPlan<LOG2L>::R0;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int comp = blockIdx.y;
     const int j0 = blockIdx.x * PAIRS * 2;
-    const Tin *s = src + comp * src_cstride;
-    for (int idx = threadIdx.x; idx < PAIRS * L; idx += NT) {
-        const int p = idx >> LOG2L, i = idx & (L - 1);
-        const int ja = j0 + 2 * p, jb = ja + 1;
+    const int p = threadIdx.x / TPS, t = threadIdx.x - p * TPS;
+    const int ja = j0 + 2 * p, jb = ja + 1;
+    const Tin *ra = src + comp * src_cstride + (int64_t)ja * row_stride;
+    const Tin *rb = ra + row_stride;
+    cpx<T> v[R0];
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const int i = t + r * TPS;
         T re = T(0), im = T(0);
         if (i < ncols_valid) {
-            if (ja < nrows) re = scale * T(s[(int64_t)ja * row_stride + i]);
-            if (jb < nrows) im = scale * T(s[(int64_t)jb * row_stride + i]);
+            if (ja < nrows) re = scale * T(ra[i]);
+            if (jb < nrows) im = scale * T(rb[i]);
         }
-        buf[p * SLEN + lds_pad(i)] = {re, im};
+        v[r] = {re, im};
     }
+    cpx<T> *seq = buf + p * SLEN;
+    reg_fft<T, false, LOG2L>(v, seq, t, tw);
     __syncthreads();
-    lds_fft_ct<T, false, LOG2L, PAIRS, NT>(buf, tw);
+#pragma unroll
+    for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
+    __syncthreads();
     constexpr int HX = L / 2 + 1;
     constexpr int NR = 2 * PAIRS;
     cpx<T> *d = dst + comp * dst_cstride;
@@ -97,9 +108,9 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
         const int r = idx % NR, k = idx / NR;
         const int j = j0 + r;
         if (j >= nrows) continue;
-        const int p = r >> 1;
-        const cpx<T> z = buf[p * SLEN + lds_pad(k)];
-        const cpx<T> zc = buf[p * SLEN + lds_pad((L - k) & (L - 1))];
+        const int pp = r >> 1;
+        const cpx<T> z = buf[pp * SLEN + lds_pad(k)];
+        const cpx<T> zc = buf[pp * SLEN + lds_pad((L - k) & (L - 1))];
         cpx<T> X;
         if ((r & 1) == 0) X = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
         else X = {T(0.5) * (z.y + zc.y), T(0.5) * (zc.x - z.x)};
@@ -108,40 +119,60 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
 }
 
 // ---------------------------------------------------------------------------
-// Pass 2: column FFT, tensor multiply, inverse column FFT.  Block = one kx.
+// Pass 2: column FFT, tensor multiply, inverse column FFT.  Block = one kx;
+// threads [c*TPS, (c+1)*TPS) own component c.  The forward outputs stay in
+// registers; x and y exchange through LDS for the off-diagonal Kxy terms.
 // ---------------------------------------------------------------------------
 template <typename T, int LOG2L>
 __global__ void __launch_bounds__((ColCfg<T, LOG2L>::NT))
 k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, const cpx<T> *__restrict__ tw,
             const quad4<T> *__restrict__ kern, T scale) {
     using C = ColCfg<T, LOG2L>;
-    constexpr int L = C::L, NT = C::NT, SLEN = C::SLEN;
+    constexpr int L = C::L, SLEN = C::SLEN, TPS = C::TPS;
+    constexpr int R0 = RegPlan<LOG2L>::R0;
+    constexpr int HY = L / 2 + 1;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int kx = blockIdx.x;
-    for (int idx = threadIdx.x; idx < 3 * L; idx += NT) {
-        const int c = idx >> LOG2L, j = idx & (L - 1);
-        buf[c * SLEN + lds_pad(j)] = j < ny ? S[((int64_t)c * hx + kx) * ny + j] : cpx<T>{T(0), T(0)};
+    const int c = threadIdx.x / TPS, t = threadIdx.x - c * TPS;
+    cpx<T> *col = S + ((int64_t)c * hx + kx) * ny;
+    cpx<T> v[R0];
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const int j = t + r * TPS;
+        v[r] = j < ny ? col[j] : cpx<T>{T(0), T(0)};
+    }
+    cpx<T> *seq = buf + c * SLEN;
+    reg_fft<T, false, LOG2L>(v, seq, t, tw);
+    __syncthreads();
+    if (c < 2) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
     }
     __syncthreads();
-    lds_fft_ct<T, false, LOG2L, 3, NT>(buf, tw);
-    constexpr int HY = L / 2 + 1;
     const quad4<T> *kc = kern + (int64_t)kx * HY;
-    for (int ky = threadIdx.x; ky < L; ky += NT) {
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const int ky = t + r * TPS;
         const int kq = ky < HY ? ky : L - ky;
         const quad4<T> K = kc[kq];
         const T kxy = ky < HY ? K.xy : -K.xy;
-        const int o = lds_pad(ky);
-        const cpx<T> mx = buf[o], my = buf[SLEN + o], mz = buf[2 * SLEN + o];
-        buf[o] = {scale * (K.xx * mx.x + kxy * my.x), scale * (K.xx * mx.y + kxy * my.y)};
-        buf[SLEN + o] = {scale * (kxy * mx.x + K.yy * my.x), scale * (kxy * mx.y + K.yy * my.y)};
-        buf[2 * SLEN + o] = {scale * (K.zz * mz.x), scale * (K.zz * mz.y)};
+        const cpx<T> own = v[r];
+        if (c == 0) {
+            const cpx<T> my = buf[SLEN + lds_pad(ky)];
+            v[r] = {scale * (K.xx * own.x + kxy * my.x), scale * (K.xx * own.y + kxy * my.y)};
+        } else if (c == 1) {
+            const cpx<T> mx = buf[lds_pad(ky)];
+            v[r] = {scale * (kxy * mx.x + K.yy * own.x), scale * (kxy * mx.y + K.yy * own.y)};
+        } else {
+            v[r] = {scale * (K.zz * own.x), scale * (K.zz * own.y)};
+        }
     }
-    __syncthreads();
-    lds_fft_ct<T, true, LOG2L, 3, NT>(buf, tw);
-    for (int idx = threadIdx.x; idx < 3 * ny; idx += NT) {
-        const int c = idx / ny, j = idx - c * ny;
-        S[((int64_t)c * hx + kx) * ny + j] = buf[c * SLEN + lds_pad(j)];
+    reg_fft<T, true, LOG2L>(v, seq, t, tw);
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const int j = t + r * TPS;
+        if (j < ny) col[j] = v[r];
     }
 }
 
@@ -155,7 +186,8 @@ __global__ void __launch_bounds__((RowCfg<T, LOG2L>::NT))
 k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, const cpx<T> *__restrict__ tw,
           T *__restrict__ out, int64_t n) {
     using C = RowCfg<T, LOG2L>;
-    constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN;
+    constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
+    constexpr int R0 = RegPlan<LOG2L>::R0;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int comp = blockIdx.y;
@@ -173,14 +205,21 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, const cpx<T> *__
         if (k != 0 && k != HALF) row[lds_pad(L - k)] = {xa.x + xb.y, xb.x - xa.y};
     }
     __syncthreads();
-    lds_fft_ct<T, true, LOG2L, PAIRS, NT>(buf, tw);
+    const int p = threadIdx.x / TPS, t = threadIdx.x - p * TPS;
+    cpx<T> *seq = buf + p * SLEN;
+    cpx<T> v[R0];
+#pragma unroll
+    for (int r = 0; r < R0; ++r) v[r] = seq[lds_pad(t + r * TPS)];
+    reg_fft<T, true, LOG2L>(v, seq, t, tw);
+    const int ja = j0 + 2 * p;
     T *o = out + comp * n;
-    for (int idx = threadIdx.x; idx < PAIRS * nx; idx += NT) {
-        const int p = idx / nx, i = idx - p * nx;
-        const int ja = j0 + 2 * p;
-        const cpx<T> z = buf[p * SLEN + lds_pad(i)];
-        if (ja < ny) o[(int64_t)ja * nx + i] = z.x;
-        if (ja + 1 < ny) o[(int64_t)(ja + 1) * nx + i] = z.y;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const int i = t + r * TPS;
+        if (i < nx) {
+            if (ja < ny) o[(int64_t)ja * nx + i] = v[r].x;
+            if (ja + 1 < ny) o[(int64_t)(ja + 1) * nx + i] = v[r].y;
+        }
     }
 }
 
@@ -193,17 +232,23 @@ __global__ void __launch_bounds__((ExtractCfg<LOG2L>::NT))
 k_col_extract(const cpx<double> *__restrict__ Sk, int hx, const cpx<double> *__restrict__ tw,
               Tout *__restrict__ kern) {
     using C = ExtractCfg<LOG2L>;
-    constexpr int L = C::L, NT = C::NT;
+    constexpr int L = C::L, TPS = C::TPS;
+    constexpr int R0 = RegPlan<LOG2L>::R0;
     extern __shared__ unsigned char lds_raw[];
     cpx<double> *buf = reinterpret_cast<cpx<double> *>(lds_raw);
     const int kx = blockIdx.x, comp = blockIdx.y;
+    const int t = threadIdx.x;
     const cpx<double> *s = Sk + ((int64_t)comp * hx + kx) * L;
-    for (int j = threadIdx.x; j < L; j += NT) buf[lds_pad(j)] = s[j];
-    __syncthreads();
-    lds_fft_ct<double, false, LOG2L, 1, NT>(buf, tw);
+    cpx<double> v[R0];
+#pragma unroll
+    for (int r = 0; r < R0; ++r) v[r] = s[t + r * TPS];
+    reg_fft<double, false, LOG2L>(v, buf, t, tw);
     constexpr int HY = L / 2 + 1;
-    for (int ky = threadIdx.x; ky < HY; ky += NT)
-        kern[((int64_t)kx * HY + ky) * 4 + comp] = Tout(buf[lds_pad(ky)].x);
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const int ky = t + r * TPS;
+        if (ky < HY) kern[((int64_t)kx * HY + ky) * 4 + comp] = Tout(v[r].x);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -376,7 +421,8 @@ static void allow_lds(K kernel, size_t bytes) {
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-// Built ranges: rows up to 2^13 (FP32) / 2^12 (FP64), columns up to 2^12 / 2^11.
+// Built ranges: rows up to 2^13 (FP32) / 2^12 (FP64), columns up to 2^12 / 2^11
+// (the column pass holds 3 padded sequences in LDS and 3*TPS <= 1024 threads).
 template <typename T> struct Range;
 template <> struct Range<float> { static constexpr int ROW = 13, COL = 12; };
 template <> struct Range<double> { static constexpr int ROW = 12, COL = 11; };

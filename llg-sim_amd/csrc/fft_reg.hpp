@@ -1,0 +1,174 @@
+// Register-resident Stockham FFTs for gfx950 (power-of-two lengths).
+//
+// One sequence of length L = 2^LOG2L is transformed by TPS = L / R0 threads,
+// each holding R0 complex values in registers.  The stage radices are R0 for
+// the first and the last stage and at most R0 in between; middle stages go
+// through LDS.  Because the first and the last stage have the same radix and
+// every thread owns exactly one butterfly in both, a thread's register view of
+// the sequence is the same before the first stage and after the last one:
+//
+//     v[r] = x[t + r * TPS],  r < R0,  t = thread index within the sequence.
+//
+// So a transform can start from values a thread loaded itself from HBM and
+// end with values it can store (or multiply and inverse-transform) directly:
+// a forward transform, a pointwise product and an inverse transform need no
+// reordering pass.  L <= 16 is one in-register DFT with no LDS at all.
+//
+// LDS layout of a sequence: element i at lds_pad(i) = i + i/16, which spreads
+// the stride-R0 writes of the first stage over the banks.
+#pragma once
+
+#include "common.hpp"
+
+namespace llgmi {
+
+__host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lds_seq_len(int L) { return L + (L >> 4); }
+
+// In-register DFT16 as 4 x 4: X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 x[4 n1 + n2] W4^(n1 k1)
+template <typename T, bool INV>
+__device__ __forceinline__ void dft16(cpx<T> *v) {
+    const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173);  // cos, sin pi/8
+    const T h = T(0.70710678118654752440);
+    cpx<T> a[4][4];  // a[n2][k1]
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        cpx<T> t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+        dft4<T, INV>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+    }
+    // twiddles W16^(n2 k1), W16 = exp(-i pi/8) (conjugate for INV)
+    auto tw = [&](cpx<T> x, T c, T s) -> cpx<T> {
+        // multiply by (c - i s) forward, (c + i s) inverse
+        return INV ? cpx<T>{x.x * c - x.y * s, x.x * s + x.y * c}
+                   : cpx<T>{x.x * c + x.y * s, x.y * c - x.x * s};
+    };
+    a[1][1] = tw(a[1][1], c1, s1);                 // W^1
+    a[1][2] = tw(a[1][2], h, h);                   // W^2
+    a[1][3] = tw(a[1][3], s1, c1);                 // W^3
+    a[2][1] = tw(a[2][1], h, h);                   // W^2
+    a[2][2] = mul_mi<T, INV>(a[2][2]);             // W^4 = -i
+    a[2][3] = tw(a[2][3], -h, h);                  // W^6
+    a[3][1] = tw(a[3][1], s1, c1);                 // W^3
+    a[3][2] = tw(a[3][2], -h, h);                  // W^6
+    a[3][3] = tw(a[3][3], -c1, -s1);               // W^9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cpx<T> t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+        dft4<T, INV>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+}
+
+template <typename T, bool INV, int R>
+__device__ __forceinline__ void dft_reg(cpx<T> *v) {
+    if constexpr (R == 1) {
+    } else if constexpr (R == 2) dft2<T, INV>(v);
+    else if constexpr (R == 4) dft4<T, INV>(v);
+    else if constexpr (R == 8) dft8<T, INV>(v);
+    else dft16<T, INV>(v);
+}
+
+template <int LOG2L>
+struct RegPlan {
+    static constexpr int L = 1 << LOG2L;
+    static constexpr int LOG2R0 = LOG2L <= 4 ? LOG2L : (LOG2L >= 8 ? 4 : (LOG2L >= 6 ? 3 : 2));
+    static constexpr int R0 = 1 << LOG2R0;
+    static constexpr int TPS = L / R0;                                   // threads per sequence
+    static constexpr int MID = LOG2L <= 4 ? 0 : LOG2L - 2 * LOG2R0;      // bits of the middle stages
+    // radix (log2) of the stage starting with LOG2NS bits done
+    static constexpr int log2r(int log2ns) {
+        if (log2ns == 0) return LOG2R0;
+        int rem = LOG2L - log2ns;
+        if (rem == LOG2R0) return LOG2R0;                                // last stage
+        int mid_rem = rem - LOG2R0;
+        return mid_rem >= LOG2R0 ? LOG2R0 : mid_rem;
+    }
+};
+
+// Twiddle of stage (Ns, R) for butterfly k and leg r: tw[(r k) L/(Ns R)].
+template <typename T, bool INV>
+__device__ __forceinline__ cpx<T> twiddle_mul(cpx<T> x, const cpx<T> *__restrict__ tw, int idx) {
+    const cpx<T> w = tw[idx];
+    return INV ? cmulc(x, w) : cmul(x, w);
+}
+
+// Middle/last stages.  Called after the first stage wrote its outputs to LDS.
+// If this is the last stage its outputs stay in v.
+template <typename T, bool INV, int LOG2L, int LOG2NS>
+__device__ __forceinline__ void reg_fft_rest(cpx<T> *v, cpx<T> *seq, int t,
+                                             const cpx<T> *__restrict__ tw) {
+    using P = RegPlan<LOG2L>;
+    if constexpr (LOG2NS < LOG2L) {
+        constexpr int LOG2R = P::log2r(LOG2NS);
+        constexpr int R = 1 << LOG2R;
+        constexpr int NS = 1 << LOG2NS;
+        constexpr int L = P::L, TPS = P::TPS, R0 = P::R0;
+        constexpr int BPT = R0 / R;                  // butterflies per thread
+        constexpr int NB = L / R;                    // butterflies per sequence
+        constexpr int TWS = LOG2L - LOG2NS - LOG2R;
+        constexpr bool LAST = LOG2NS + LOG2R == LOG2L;
+        __syncthreads();  // previous stage's LDS writes are visible
+#pragma unroll
+        for (int q = 0; q < BPT; ++q) {
+            const int j = t + q * TPS;
+            const int k = j & (NS - 1);
+            cpx<T> *u = v + q * R;
+#pragma unroll
+            for (int r = 0; r < R; ++r) u[r] = seq[lds_pad(j + r * NB)];
+#pragma unroll
+            for (int r = 1; r < R; ++r) u[r] = twiddle_mul<T, INV>(u[r], tw, (r * k) << TWS);
+            dft_reg<T, INV, R>(u);
+        }
+        if constexpr (!LAST) {
+            __syncthreads();  // all reads of this stage done before it overwrites
+#pragma unroll
+            for (int q = 0; q < BPT; ++q) {
+                const int j = t + q * TPS;
+                const int k = j & (NS - 1);
+                const int idxD = ((j >> LOG2NS) << (LOG2NS + LOG2R)) + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) seq[lds_pad(idxD + r * NS)] = v[q * R + r];
+            }
+            reg_fft_rest<T, INV, LOG2L, LOG2NS + LOG2R>(v, seq, t, tw);
+        }
+        // LAST: BPT == 1, j == t, outputs v[r] = X[t + r*TPS]
+    }
+}
+
+// Transform of one sequence held as v[r] = x[t + r*TPS] by its TPS threads,
+// using `seq` (lds_seq_len(L) slots) as exchange space.  On return
+// v[r] = X[t + r*TPS].  All threads of the block must call it (it contains
+// barriers); the caller must not touch `seq` between the call and a barrier
+// that follows it.  The first LDS write happens after a leading barrier, so
+// `seq` may still be read by the caller before the call.
+template <typename T, bool INV, int LOG2L>
+__device__ __forceinline__ void reg_fft(cpx<T> *v, cpx<T> *seq, int t,
+                                        const cpx<T> *__restrict__ tw) {
+    using P = RegPlan<LOG2L>;
+    dft_reg<T, INV, P::R0>(v);
+    if constexpr (LOG2L > P::LOG2R0) {
+        __syncthreads();  // the caller may have been reading seq
+#pragma unroll
+        for (int r = 0; r < P::R0; ++r) seq[lds_pad(t * P::R0 + r)] = v[r];
+        reg_fft_rest<T, INV, LOG2L, P::LOG2R0>(v, seq, t, tw);
+    }
+}
+
+// Compile-time dispatch over log2 lengths LO..HI.
+template <int LO, int HI, typename F>
+inline void dispatch_log2(int lg, F &&f) {
+    if constexpr (LO <= HI) {
+        if (lg == LO) {
+            f(std::integral_constant<int, LO>{});
+            return;
+        }
+        dispatch_log2<LO + 1, HI>(lg, static_cast<F &&>(f));
+    } else {
+        throw Error(3, "FFT length 2^" + std::to_string(lg) + " is outside the built range");
+    }
+}
+
+}  // namespace llgmi
