@@ -68,7 +68,8 @@ struct ExtractCfg {
 // >= ncols_valid and rows >= nrows are zero and not read.
 // dst[comp*dst_cstride + k*nrows + j] for k <= L/2, j < nrows.
 // ---------------------------------------------------------------------------
-template <typename T, typename Tin, int LOG2L>
+// HALF: ncols_valid == L/2, so the upper half of every row is a compile-time zero.
+template <typename T, typename Tin, int LOG2L, bool HALF>
 __global__ void __launch_bounds__((RowCfg<T, LOG2L>::NT))
 k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int ncols_valid,
           int nrows, T scale, const cpx<T> *__restrict__ tw, cpx<T> *__restrict__ dst,
@@ -89,7 +90,7 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
     for (int r = 0; r < R0; ++r) {
         const int i = t + r * TPS;
         T re = T(0), im = T(0);
-        if (i < ncols_valid) {
+        if (HALF ? r < R0 / 2 : i < ncols_valid) {
             if (ja < nrows) re = scale * T(ra[i]);
             if (jb < nrows) im = scale * T(rb[i]);
         }
@@ -123,7 +124,9 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
 // threads [c*TPS, (c+1)*TPS) own component c.  The forward outputs stay in
 // registers; x and y exchange through LDS for the off-diagonal Kxy terms.
 // ---------------------------------------------------------------------------
-template <typename T, int LOG2L>
+// HALF: ny == L/2, so the upper half of each column is a compile-time zero on
+// input and is not stored on output.
+template <typename T, int LOG2L, bool HALF>
 __global__ void __launch_bounds__((ColCfg<T, LOG2L>::NT))
 k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, const cpx<T> *__restrict__ tw,
             const quad4<T> *__restrict__ kern, T scale) {
@@ -140,7 +143,7 @@ k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, const cpx<T> *__restrict__ t
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
         const int j = t + r * TPS;
-        v[r] = j < ny ? col[j] : cpx<T>{T(0), T(0)};
+        v[r] = (HALF ? r < R0 / 2 : j < ny) ? col[j] : cpx<T>{T(0), T(0)};
     }
     cpx<T> *seq = buf + c * SLEN;
     reg_fft<T, false, LOG2L>(v, seq, t, tw);
@@ -172,7 +175,7 @@ k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, const cpx<T> *__restrict__ t
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
         const int j = t + r * TPS;
-        if (j < ny) col[j] = v[r];
+        if (HALF ? r < R0 / 2 : j < ny) col[j] = v[r];
     }
 }
 
@@ -181,7 +184,9 @@ k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, const cpx<T> *__restrict__ t
 // and k = L/2 bins use their real parts only, which is what taking Re() of the
 // reference's full complex inverse transform amounts to.
 // ---------------------------------------------------------------------------
-template <typename T, int LOG2L>
+// HALF: nx == L/2, so only the lower half of each inverse-transformed row is
+// formed and stored.
+template <typename T, int LOG2L, bool HALF>
 __global__ void __launch_bounds__((RowCfg<T, LOG2L>::NT))
 k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, const cpx<T> *__restrict__ tw,
           T *__restrict__ out, int64_t n) {
@@ -193,16 +198,16 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, const cpx<T> *__
     const int comp = blockIdx.y;
     const int j0 = blockIdx.x * PAIRS * 2;
     const cpx<T> *s = S + (int64_t)comp * hx * ny;
-    constexpr int HALF = L >> 1;
-    for (int idx = threadIdx.x; idx < (HALF + 1) * PAIRS; idx += NT) {
+    constexpr int NYQ = L >> 1;
+    for (int idx = threadIdx.x; idx < (NYQ + 1) * PAIRS; idx += NT) {
         const int p = idx % PAIRS, k = idx / PAIRS;
         const int ja = j0 + 2 * p, jb = ja + 1;
         cpx<T> xa = ja < ny ? s[(int64_t)k * ny + ja] : cpx<T>{T(0), T(0)};
         cpx<T> xb = jb < ny ? s[(int64_t)k * ny + jb] : cpx<T>{T(0), T(0)};
-        if (k == 0 || k == HALF) { xa.y = T(0); xb.y = T(0); }
+        if (k == 0 || k == NYQ) { xa.y = T(0); xb.y = T(0); }
         cpx<T> *row = buf + p * SLEN;
         row[lds_pad(k)] = {xa.x - xb.y, xa.y + xb.x};
-        if (k != 0 && k != HALF) row[lds_pad(L - k)] = {xa.x + xb.y, xb.x - xa.y};
+        if (k != 0 && k != NYQ) row[lds_pad(L - k)] = {xa.x + xb.y, xb.x - xa.y};
     }
     __syncthreads();
     const int p = threadIdx.x / TPS, t = threadIdx.x - p * TPS;
@@ -216,7 +221,7 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, const cpx<T> *__
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
         const int i = t + r * TPS;
-        if (i < nx) {
+        if (HALF ? r < R0 / 2 : i < nx) {
             if (ja < ny) o[(int64_t)ja * nx + i] = v[r].x;
             if (ja + 1 < ny) o[(int64_t)(ja + 1) * nx + i] = v[r].y;
         }
@@ -488,9 +493,9 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         dispatch_log2<1, kRowF64Build>(lx, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
             using C = RowCfg<double, LG>;
-            allow_lds(k_row_r2c<double, double, LG>, C::LDS);
+            allow_lds(k_row_r2c<double, double, LG, false>, C::LDS);
             dim3 grid((unsigned)((py + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 4);
-            hipLaunchKernelGGL((k_row_r2c<double, double, LG>), grid, dim3(C::NT), C::LDS, stream, kr,
+            hipLaunchKernelGGL((k_row_r2c<double, double, LG, false>), grid, dim3(C::NT), C::LDS, stream, kr,
                                np, (int)px, (int)px, (int)py, 1.0, twx64, sk, (int64_t)hx * py);
             LLGMI_LAUNCH_CHECK();
         });
@@ -525,13 +530,16 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     LLGMI_HIP(hipMalloc(&spec, sizeof(cpx<T>) * 3 * hx * ny));
     dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
-        allow_lds(k_row_r2c<T, T, LG>, RowCfg<T, LG>::LDS);
-        allow_lds(k_row_c2r<T, LG>, RowCfg<T, LG>::LDS);
+        allow_lds(k_row_r2c<T, T, LG, true>, RowCfg<T, LG>::LDS);
+        allow_lds(k_row_r2c<T, T, LG, false>, RowCfg<T, LG>::LDS);
+        allow_lds(k_row_c2r<T, LG, true>, RowCfg<T, LG>::LDS);
+        allow_lds(k_row_c2r<T, LG, false>, RowCfg<T, LG>::LDS);
         pairs_fwd = RowCfg<T, LG>::PAIRS;
     });
     dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
-        allow_lds(k_col_demag<T, LG>, ColCfg<T, LG>::LDS);
+        allow_lds(k_col_demag<T, LG, true>, ColCfg<T, LG>::LDS);
+        allow_lds(k_col_demag<T, LG, false>, ColCfg<T, LG>::LDS);
     });
     built = true;
 }
@@ -544,8 +552,12 @@ void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *p
         using C = RowCfg<T, LG>;
         const dim3 grid((unsigned)((ny + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 3);
         ProfScope ps(prof, 0, stream);
-        hipLaunchKernelGGL((k_row_r2c<T, T, LG>), grid, dim3(C::NT), C::LDS, stream, m, n, (int)nx,
-                           (int)nx, (int)ny, ms, tw_x, spec, (int64_t)hx * ny);
+        if (2 * nx == px)
+            hipLaunchKernelGGL((k_row_r2c<T, T, LG, true>), grid, dim3(C::NT), C::LDS, stream, m, n,
+                               (int)nx, (int)nx, (int)ny, ms, tw_x, spec, (int64_t)hx * ny);
+        else
+            hipLaunchKernelGGL((k_row_r2c<T, T, LG, false>), grid, dim3(C::NT), C::LDS, stream, m, n,
+                               (int)nx, (int)nx, (int)ny, ms, tw_x, spec, (int64_t)hx * ny);
         LLGMI_LAUNCH_CHECK();
     });
     dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
@@ -553,8 +565,12 @@ void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *p
         using C = ColCfg<T, LG>;
         ProfScope ps(prof, 1, stream);
         const T scale = T(1.0 / ((double)px * (double)py));
-        hipLaunchKernelGGL((k_col_demag<T, LG>), dim3((unsigned)hx), dim3(C::NT), C::LDS, stream,
-                           spec, (int)hx, (int)ny, tw_y, kern, scale);
+        if (2 * ny == py)
+            hipLaunchKernelGGL((k_col_demag<T, LG, true>), dim3((unsigned)hx), dim3(C::NT), C::LDS,
+                               stream, spec, (int)hx, (int)ny, tw_y, kern, scale);
+        else
+            hipLaunchKernelGGL((k_col_demag<T, LG, false>), dim3((unsigned)hx), dim3(C::NT), C::LDS,
+                               stream, spec, (int)hx, (int)ny, tw_y, kern, scale);
         LLGMI_LAUNCH_CHECK();
     });
     dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
@@ -562,8 +578,12 @@ void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *p
         using C = RowCfg<T, LG>;
         const dim3 grid((unsigned)((ny + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 3);
         ProfScope ps(prof, 2, stream);
-        hipLaunchKernelGGL((k_row_c2r<T, LG>), grid, dim3(C::NT), C::LDS, stream, spec, (int)hx,
-                           (int)nx, (int)ny, tw_x, bd, n);
+        if (2 * nx == px)
+            hipLaunchKernelGGL((k_row_c2r<T, LG, true>), grid, dim3(C::NT), C::LDS, stream, spec,
+                               (int)hx, (int)nx, (int)ny, tw_x, bd, n);
+        else
+            hipLaunchKernelGGL((k_row_c2r<T, LG, false>), grid, dim3(C::NT), C::LDS, stream, spec,
+                               (int)hx, (int)nx, (int)ny, tw_x, bd, n);
         LLGMI_LAUNCH_CHECK();
     });
 }

@@ -88,11 +88,34 @@ struct RegPlan {
     }
 };
 
-// Twiddle of stage (Ns, R) for butterfly k and leg r: tw[(r k) L/(Ns R)].
-template <typename T, bool INV>
-__device__ __forceinline__ cpx<T> twiddle_mul(cpx<T> x, const cpx<T> *__restrict__ tw, int idx) {
-    const cpx<T> w = tw[idx];
-    return INV ? cmulc(x, w) : cmul(x, w);
+// Twiddles of stage (Ns, R) for butterfly k: w^r with w = tw[k L/(Ns R)], r < R.
+// Only w, w^4 and w^8 are gathered from the table; the other powers are
+// products of at most three table values (a few ulp in FP32), which replaces
+// R-1 scattered loads per butterfly by at most three.
+template <typename T, bool INV, int R>
+__device__ __forceinline__ void apply_twiddles(cpx<T> *u, const cpx<T> *__restrict__ tw, int base) {
+    if constexpr (R == 2) {
+        const cpx<T> w1 = tw[base];
+        u[1] = INV ? cmulc(u[1], w1) : cmul(u[1], w1);
+    } else {
+        cpx<T> w[R];
+        w[1] = tw[base];
+        w[2] = cmul(w[1], w[1]);
+        w[3] = cmul(w[2], w[1]);
+        if constexpr (R >= 8) {
+            w[4] = tw[4 * base];
+            w[5] = cmul(w[4], w[1]);
+            w[6] = cmul(w[4], w[2]);
+            w[7] = cmul(w[4], w[3]);
+        }
+        if constexpr (R == 16) {
+            w[8] = tw[8 * base];
+#pragma unroll
+            for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+        }
+#pragma unroll
+        for (int r = 1; r < R; ++r) u[r] = INV ? cmulc(u[r], w[r]) : cmul(u[r], w[r]);
+    }
 }
 
 // Middle/last stages.  Called after the first stage wrote its outputs to LDS.
@@ -118,8 +141,7 @@ __device__ __forceinline__ void reg_fft_rest(cpx<T> *v, cpx<T> *seq, int t,
             cpx<T> *u = v + q * R;
 #pragma unroll
             for (int r = 0; r < R; ++r) u[r] = seq[lds_pad(j + r * NB)];
-#pragma unroll
-            for (int r = 1; r < R; ++r) u[r] = twiddle_mul<T, INV>(u[r], tw, (r * k) << TWS);
+            apply_twiddles<T, INV, R>(u, tw, k << TWS);
             dft_reg<T, INV, R>(u);
         }
         if constexpr (!LAST) {
