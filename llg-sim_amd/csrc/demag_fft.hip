@@ -13,11 +13,16 @@
 //    transforms are real and quadrant-symmetric: only (px/2+1) x (py/2+1)
 //    real 4-vectors are stored (xy changes sign for ky > py/2).
 //  * Pass 1 (row R2C): two real rows per complex FFT, rows >= ny and
-//    columns >= nx are zero and never read.  The half spectrum (px/2+1
-//    columns) is written column-major, S[c][kx][j], j < ny only.
-//  * Pass 2 (column): per kx, the three components are loaded into LDS,
-//    zero-extended to py, transformed, multiplied by the 4-component tensor
-//    (scale 1/(px py) folded in), inverse transformed; only j < ny is stored.
+//    columns >= nx are zero and never read.  The half spectrum is written
+//    column-major, S[c][kx][j], j < ny only, in packed form: px/2 complex
+//    columns, column 0 holding Re X[0] + i Re X[px/2] (both are real for a
+//    real row), so the column pass has exactly px/2 independent columns.
+//  * Pass 2 (column): per kx, the three components are zero-extended to py,
+//    transformed, multiplied by the 4-component tensor (scale 1/(px py)
+//    folded in), inverse transformed; only j < ny is stored.  The packed
+//    column 0 carries two real-data transforms at once; Kxy vanishes on both
+//    kx = 0 and kx = px/2, so it needs only the diagonal entries of the two
+//    kernel rows, combined through the Hermitian partner ky <-> py - ky.
 //  * Pass 3 (row C2R): two Hermitian rows per complex inverse FFT; the first
 //    nx outputs of each row are written as B_demag (SoA).
 #include "common.hpp"
@@ -102,19 +107,26 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
 #pragma unroll
     for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
     __syncthreads();
-    constexpr int HX = L / 2 + 1;
+    constexpr int HXS = L / 2;  // packed columns
     constexpr int NR = 2 * PAIRS;
     cpx<T> *d = dst + comp * dst_cstride;
-    for (int idx = threadIdx.x; idx < HX * NR; idx += NT) {
+    for (int idx = threadIdx.x; idx < HXS * NR; idx += NT) {
         const int r = idx % NR, k = idx / NR;
         const int j = j0 + r;
         if (j >= nrows) continue;
         const int pp = r >> 1;
-        const cpx<T> z = buf[pp * SLEN + lds_pad(k)];
-        const cpx<T> zc = buf[pp * SLEN + lds_pad((L - k) & (L - 1))];
         cpx<T> X;
-        if ((r & 1) == 0) X = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
-        else X = {T(0.5) * (z.y + zc.y), T(0.5) * (zc.x - z.x)};
+        if (k == 0) {
+            // X_a[0] = Re Z[0], X_b[0] = Im Z[0]; X_a[L/2] = Re Z[L/2], X_b[L/2] = Im Z[L/2]
+            const cpx<T> z0 = buf[pp * SLEN + lds_pad(0)];
+            const cpx<T> zn = buf[pp * SLEN + lds_pad(L / 2)];
+            X = (r & 1) == 0 ? cpx<T>{z0.x, zn.x} : cpx<T>{z0.y, zn.y};
+        } else {
+            const cpx<T> z = buf[pp * SLEN + lds_pad(k)];
+            const cpx<T> zc = buf[pp * SLEN + lds_pad(L - k)];
+            if ((r & 1) == 0) X = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
+            else X = {T(0.5) * (z.y + zc.y), T(0.5) * (zc.x - z.x)};
+        }
         d[(int64_t)k * nrows + j] = X;
     }
 }
@@ -128,7 +140,7 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
 // input and is not stored on output.
 template <typename T, int LOG2L, bool HALF>
 __global__ void __launch_bounds__((ColCfg<T, LOG2L>::NT))
-k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, const cpx<T> *__restrict__ tw,
+k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
             const quad4<T> *__restrict__ kern, T scale) {
     using C = ColCfg<T, LOG2L>;
     constexpr int L = C::L, SLEN = C::SLEN, TPS = C::TPS;
@@ -138,7 +150,7 @@ k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, const cpx<T> *__restrict__ t
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int kx = blockIdx.x;
     const int c = threadIdx.x / TPS, t = threadIdx.x - c * TPS;
-    cpx<T> *col = S + ((int64_t)c * hx + kx) * ny;
+    cpx<T> *col = S + ((int64_t)c * hxs + kx) * ny;
     cpx<T> v[R0];
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
@@ -148,27 +160,49 @@ k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, const cpx<T> *__restrict__ t
     cpx<T> *seq = buf + c * SLEN;
     reg_fft<T, false, LOG2L>(v, seq, t, tw);
     __syncthreads();
-    if (c < 2) {
+    if (kx != 0) {
+        if (c < 2) {
+#pragma unroll
+            for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
+        }
+        __syncthreads();
+        const quad4<T> *kc = kern + (int64_t)kx * HY;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int ky = t + r * TPS;
+            const int kq = ky < HY ? ky : L - ky;
+            const quad4<T> K = kc[kq];
+            const T kxy = ky < HY ? K.xy : -K.xy;
+            const cpx<T> own = v[r];
+            if (c == 0) {
+                const cpx<T> my = buf[SLEN + lds_pad(ky)];
+                v[r] = {scale * (K.xx * own.x + kxy * my.x), scale * (K.xx * own.y + kxy * my.y)};
+            } else if (c == 1) {
+                const cpx<T> mx = buf[lds_pad(ky)];
+                v[r] = {scale * (kxy * mx.x + K.yy * own.x), scale * (kxy * mx.y + K.yy * own.y)};
+            } else {
+                v[r] = {scale * (K.zz * own.x), scale * (K.zz * own.y)};
+            }
+        }
+    } else {
+        // packed column: Z = A + iB with A, B the transforms of the real kx = 0
+        // and kx = px/2 columns; Y = Ka A + i Kb B = ((Ka+Kb)/2) Z + ((Ka-Kb)/2) conj(Z[-ky])
 #pragma unroll
         for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
-    }
-    __syncthreads();
-    const quad4<T> *kc = kern + (int64_t)kx * HY;
+        __syncthreads();
+        const quad4<T> *ka = kern;
+        const quad4<T> *kb = kern + (int64_t)hxs * HY;
 #pragma unroll
-    for (int r = 0; r < R0; ++r) {
-        const int ky = t + r * TPS;
-        const int kq = ky < HY ? ky : L - ky;
-        const quad4<T> K = kc[kq];
-        const T kxy = ky < HY ? K.xy : -K.xy;
-        const cpx<T> own = v[r];
-        if (c == 0) {
-            const cpx<T> my = buf[SLEN + lds_pad(ky)];
-            v[r] = {scale * (K.xx * own.x + kxy * my.x), scale * (K.xx * own.y + kxy * my.y)};
-        } else if (c == 1) {
-            const cpx<T> mx = buf[lds_pad(ky)];
-            v[r] = {scale * (kxy * mx.x + K.yy * own.x), scale * (kxy * mx.y + K.yy * own.y)};
-        } else {
-            v[r] = {scale * (K.zz * own.x), scale * (K.zz * own.y)};
+        for (int r = 0; r < R0; ++r) {
+            const int ky = t + r * TPS;
+            const int kq = ky < HY ? ky : L - ky;
+            const quad4<T> A = ka[kq], B = kb[kq];
+            const T a = c == 0 ? A.xx : (c == 1 ? A.yy : A.zz);
+            const T b = c == 0 ? B.xx : (c == 1 ? B.yy : B.zz);
+            const T c1 = T(0.5) * (a + b), c2 = T(0.5) * (a - b);
+            const cpx<T> z = v[r];
+            const cpx<T> zp = seq[lds_pad((L - ky) & (L - 1))];
+            v[r] = {scale * (c1 * z.x + c2 * zp.x), scale * (c1 * z.y - c2 * zp.y)};
         }
     }
     reg_fft<T, true, LOG2L>(v, seq, t, tw);
@@ -188,7 +222,7 @@ k_col_demag(cpx<T> *__restrict__ S, int hx, int ny, const cpx<T> *__restrict__ t
 // formed and stored.
 template <typename T, int LOG2L, bool HALF>
 __global__ void __launch_bounds__((RowCfg<T, LOG2L>::NT))
-k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, const cpx<T> *__restrict__ tw,
+k_row_c2r(const cpx<T> *__restrict__ S, int hxs, int nx, int ny, const cpx<T> *__restrict__ tw,
           T *__restrict__ out, int64_t n) {
     using C = RowCfg<T, LOG2L>;
     constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
@@ -197,17 +231,23 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, const cpx<T> *__
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int comp = blockIdx.y;
     const int j0 = blockIdx.x * PAIRS * 2;
-    const cpx<T> *s = S + (int64_t)comp * hx * ny;
+    const cpx<T> *s = S + (int64_t)comp * hxs * ny;
     constexpr int NYQ = L >> 1;
-    for (int idx = threadIdx.x; idx < (NYQ + 1) * PAIRS; idx += NT) {
+    for (int idx = threadIdx.x; idx < NYQ * PAIRS; idx += NT) {
         const int p = idx % PAIRS, k = idx / PAIRS;
         const int ja = j0 + 2 * p, jb = ja + 1;
-        cpx<T> xa = ja < ny ? s[(int64_t)k * ny + ja] : cpx<T>{T(0), T(0)};
-        cpx<T> xb = jb < ny ? s[(int64_t)k * ny + jb] : cpx<T>{T(0), T(0)};
-        if (k == 0 || k == NYQ) { xa.y = T(0); xb.y = T(0); }
+        const cpx<T> xa = ja < ny ? s[(int64_t)k * ny + ja] : cpx<T>{T(0), T(0)};
+        const cpx<T> xb = jb < ny ? s[(int64_t)k * ny + jb] : cpx<T>{T(0), T(0)};
         cpx<T> *row = buf + p * SLEN;
-        row[lds_pad(k)] = {xa.x - xb.y, xa.y + xb.x};
-        if (k != 0 && k != NYQ) row[lds_pad(L - k)] = {xa.x + xb.y, xb.x - xa.y};
+        if (k == 0) {
+            // packed: xa = (Re X_a[0], Re X_a[L/2]); the imaginary parts of the
+            // two real bins are dropped, as Re() of a full inverse transform does
+            row[lds_pad(0)] = {xa.x, xb.x};
+            row[lds_pad(NYQ)] = {xa.y, xb.y};
+        } else {
+            row[lds_pad(k)] = {xa.x - xb.y, xa.y + xb.x};
+            row[lds_pad(L - k)] = {xa.x + xb.y, xb.x - xa.y};
+        }
     }
     __syncthreads();
     const int p = threadIdx.x / TPS, t = threadIdx.x - p * TPS;
@@ -234,7 +274,7 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hx, int nx, int ny, const cpx<T> *__
 // ---------------------------------------------------------------------------
 template <typename Tout, int LOG2L>
 __global__ void __launch_bounds__((ExtractCfg<LOG2L>::NT))
-k_col_extract(const cpx<double> *__restrict__ Sk, int hx, const cpx<double> *__restrict__ tw,
+k_col_extract(const cpx<double> *__restrict__ Sk, int hxs, const cpx<double> *__restrict__ tw,
               Tout *__restrict__ kern) {
     using C = ExtractCfg<LOG2L>;
     constexpr int L = C::L, TPS = C::TPS;
@@ -243,7 +283,7 @@ k_col_extract(const cpx<double> *__restrict__ Sk, int hx, const cpx<double> *__r
     cpx<double> *buf = reinterpret_cast<cpx<double> *>(lds_raw);
     const int kx = blockIdx.x, comp = blockIdx.y;
     const int t = threadIdx.x;
-    const cpx<double> *s = Sk + ((int64_t)comp * hx + kx) * L;
+    const cpx<double> *s = Sk + ((int64_t)comp * hxs + kx) * L;
     cpx<double> v[R0];
 #pragma unroll
     for (int r = 0; r < R0; ++r) v[r] = s[t + r * TPS];
@@ -252,7 +292,12 @@ k_col_extract(const cpx<double> *__restrict__ Sk, int hx, const cpx<double> *__r
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
         const int ky = t + r * TPS;
-        if (ky < HY) kern[((int64_t)kx * HY + ky) * 4 + comp] = Tout(v[r].x);
+        if (ky < HY) {
+            kern[((int64_t)kx * HY + ky) * 4 + comp] = Tout(v[r].x);
+            // packed column 0: the imaginary part is the transform of the kx = px/2
+            // column (both transforms are real for the even/odd-odd kernels)
+            if (kx == 0) kern[((int64_t)hxs * HY + ky) * 4 + comp] = Tout(v[r].y);
+        }
     }
 }
 
@@ -489,14 +534,15 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         // 3) row R2C (FP64) of the four arrays
         twx64 = make_twiddles<double>((int)px);
         twy64 = make_twiddles<double>((int)py);
-        LLGMI_HIP(hipMalloc(&sk, sizeof(cpx<double>) * 4 * hx * py));
+        const int64_t hxs = px / 2;
+        LLGMI_HIP(hipMalloc(&sk, sizeof(cpx<double>) * 4 * hxs * py));
         dispatch_log2<1, kRowF64Build>(lx, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
             using C = RowCfg<double, LG>;
             allow_lds(k_row_r2c<double, double, LG, false>, C::LDS);
             dim3 grid((unsigned)((py + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 4);
             hipLaunchKernelGGL((k_row_r2c<double, double, LG, false>), grid, dim3(C::NT), C::LDS, stream, kr,
-                               np, (int)px, (int)px, (int)py, 1.0, twx64, sk, (int64_t)hx * py);
+                               np, (int)px, (int)px, (int)py, 1.0, twx64, sk, (int64_t)hxs * py);
             LLGMI_LAUNCH_CHECK();
         });
         // 4) column FFT (FP64), real quadrant -> kern (T)
@@ -505,8 +551,8 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
             constexpr int LG = decltype(lg)::value;
             using C = ExtractCfg<LG>;
             allow_lds(k_col_extract<T, LG>, C::LDS);
-            hipLaunchKernelGGL((k_col_extract<T, LG>), dim3((unsigned)hx, 4), dim3(C::NT), C::LDS,
-                               stream, sk, (int)hx, twy64, reinterpret_cast<T *>(kern));
+            hipLaunchKernelGGL((k_col_extract<T, LG>), dim3((unsigned)hxs, 4), dim3(C::NT), C::LDS,
+                               stream, sk, (int)hxs, twy64, reinterpret_cast<T *>(kern));
             LLGMI_LAUNCH_CHECK();
         });
         LLGMI_HIP(hipStreamSynchronize(stream));
@@ -527,7 +573,7 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
 
     tw_x = make_twiddles<T>((int)px);
     tw_y = make_twiddles<T>((int)py);
-    LLGMI_HIP(hipMalloc(&spec, sizeof(cpx<T>) * 3 * hx * ny));
+    LLGMI_HIP(hipMalloc(&spec, sizeof(cpx<T>) * 3 * (px / 2) * ny));
     dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         allow_lds(k_row_r2c<T, T, LG, true>, RowCfg<T, LG>::LDS);
@@ -554,10 +600,10 @@ void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *p
         ProfScope ps(prof, 0, stream);
         if (2 * nx == px)
             hipLaunchKernelGGL((k_row_r2c<T, T, LG, true>), grid, dim3(C::NT), C::LDS, stream, m, n,
-                               (int)nx, (int)nx, (int)ny, ms, tw_x, spec, (int64_t)hx * ny);
+                               (int)nx, (int)nx, (int)ny, ms, tw_x, spec, (int64_t)(px / 2) * ny);
         else
             hipLaunchKernelGGL((k_row_r2c<T, T, LG, false>), grid, dim3(C::NT), C::LDS, stream, m, n,
-                               (int)nx, (int)nx, (int)ny, ms, tw_x, spec, (int64_t)hx * ny);
+                               (int)nx, (int)nx, (int)ny, ms, tw_x, spec, (int64_t)(px / 2) * ny);
         LLGMI_LAUNCH_CHECK();
     });
     dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
@@ -566,11 +612,11 @@ void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *p
         ProfScope ps(prof, 1, stream);
         const T scale = T(1.0 / ((double)px * (double)py));
         if (2 * ny == py)
-            hipLaunchKernelGGL((k_col_demag<T, LG, true>), dim3((unsigned)hx), dim3(C::NT), C::LDS,
-                               stream, spec, (int)hx, (int)ny, tw_y, kern, scale);
+            hipLaunchKernelGGL((k_col_demag<T, LG, true>), dim3((unsigned)(px / 2)), dim3(C::NT), C::LDS,
+                               stream, spec, (int)(px / 2), (int)ny, tw_y, kern, scale);
         else
-            hipLaunchKernelGGL((k_col_demag<T, LG, false>), dim3((unsigned)hx), dim3(C::NT), C::LDS,
-                               stream, spec, (int)hx, (int)ny, tw_y, kern, scale);
+            hipLaunchKernelGGL((k_col_demag<T, LG, false>), dim3((unsigned)(px / 2)), dim3(C::NT), C::LDS,
+                               stream, spec, (int)(px / 2), (int)ny, tw_y, kern, scale);
         LLGMI_LAUNCH_CHECK();
     });
     dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
@@ -580,10 +626,10 @@ void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *p
         ProfScope ps(prof, 2, stream);
         if (2 * nx == px)
             hipLaunchKernelGGL((k_row_c2r<T, LG, true>), grid, dim3(C::NT), C::LDS, stream, spec,
-                               (int)hx, (int)nx, (int)ny, tw_x, bd, n);
+                               (int)(px / 2), (int)nx, (int)ny, tw_x, bd, n);
         else
             hipLaunchKernelGGL((k_row_c2r<T, LG, false>), grid, dim3(C::NT), C::LDS, stream, spec,
-                               (int)hx, (int)nx, (int)ny, tw_x, bd, n);
+                               (int)(px / 2), (int)nx, (int)ny, tw_x, bd, n);
         LLGMI_LAUNCH_CHECK();
     });
 }
