@@ -111,6 +111,18 @@ __device__ __forceinline__ void dftR(cpx<T> *v) {
     else dft8<T, INV>(v);
 }
 
+// XCD-aware block order.  Workgroups are dealt round-robin over the eight XCDs
+// of an MI355X (block p runs on XCD p % 8), and each XCD has its own L2.  The
+// map below gives each XCD a contiguous range of logical blocks, so blocks
+// that share cache lines (neighbouring rows, row pairs of one line) are
+// resident in the same L2.  It is a bijection on [0, n).
+__device__ __forceinline__ int xcd_remap(int p, int n) {
+    constexpr int NX = 8;
+    const int q = n / NX, r = n % NX;
+    const int xcd = p % NX, i = p / NX;
+    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + i;
+}
+
 inline int ilog2(int64_t v) {
     int r = 0;
     while ((int64_t(1) << r) < v) ++r;

@@ -84,8 +84,9 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
     constexpr int R0 = RegPlan<LOG2L>::R0;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
-    const int comp = blockIdx.y;
-    const int j0 = blockIdx.x * PAIRS * 2;
+    const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int comp = lb / gridDim.x;
+    const int j0 = (lb - comp * gridDim.x) * PAIRS * 2;
     const int p = threadIdx.x / TPS, t = threadIdx.x - p * TPS;
     const int ja = j0 + 2 * p, jb = ja + 1;
     const Tin *ra = src + comp * src_cstride + (int64_t)ja * row_stride;
@@ -229,8 +230,9 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hxs, int nx, int ny, const cpx<T> *_
     constexpr int R0 = RegPlan<LOG2L>::R0;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
-    const int comp = blockIdx.y;
-    const int j0 = blockIdx.x * PAIRS * 2;
+    const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int comp = lb / gridDim.x;
+    const int j0 = (lb - comp * gridDim.x) * PAIRS * 2;
     const cpx<T> *s = S + (int64_t)comp * hxs * ny;
     constexpr int NYQ = L >> 1;
     for (int idx = threadIdx.x; idx < NYQ * PAIRS; idx += NT) {

@@ -23,7 +23,9 @@
 namespace llgmi {
 
 __host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 4); }
-__host__ __device__ constexpr int lds_seq_len(int L) { return L + (L >> 4); }
+// One extra slot per sequence: consecutive sequences then start on different
+// banks (padded lengths of powers of two are multiples of the bank count).
+__host__ __device__ constexpr int lds_seq_len(int L) { return L + (L >> 4) + 1; }
 
 // In-register DFT16 as 4 x 4: X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 x[4 n1 + n2] W4^(n1 k1)
 template <typename T, bool INV>

@@ -33,8 +33,10 @@ template <typename T, bool GEOM, bool DEMAG, bool TORQUE>
 __global__ void __launch_bounds__(256)
 k_local_field(LocalArgs<T> a, const T *__restrict__ m, const T *__restrict__ bd,
               const uint8_t *__restrict__ geom, T *__restrict__ b, T *__restrict__ kout) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
+    // neighbouring rows share one XCD's L2 (their halo rows are reused there)
+    const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int j = lb / gridDim.x;
+    const int i = (lb - j * gridDim.x) * blockDim.x + threadIdx.x;
     const int nx = a.nx, ny = a.ny;
     if (i >= nx) return;
     const int64_t n = (int64_t)nx * ny;
