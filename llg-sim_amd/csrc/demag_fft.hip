@@ -41,9 +41,9 @@ namespace llgmi {
 __host__ __device__ constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 __host__ __device__ constexpr int mini(int a, int b) { return a < b ? a : b; }
 
-template <typename T, int LOG2L>
+template <typename T, int LOG2L, int RQ = 0>
 struct RowCfg {
-    using P = RegPlan<LOG2L>;
+    using P = RegPlan<LOG2L, RQ>;
     static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
     static constexpr int PAIRS =
         clampi(mini((int)(80 * 1024 / (sizeof(cpx<T>) * SLEN)), 512 / TPS), 1, 64);
@@ -51,9 +51,9 @@ struct RowCfg {
     static constexpr size_t LDS = sizeof(cpx<T>) * PAIRS * SLEN;
 };
 
-template <typename T, int LOG2L>
+template <typename T, int LOG2L, int RQ = 0>
 struct ColCfg {
-    using P = RegPlan<LOG2L>;
+    using P = RegPlan<LOG2L, RQ>;
     static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
     static constexpr int NT = 3 * TPS;
     static constexpr size_t LDS = sizeof(cpx<T>) * 3 * SLEN;
@@ -74,14 +74,14 @@ struct ExtractCfg {
 // dst[comp*dst_cstride + k*nrows + j] for k <= L/2, j < nrows.
 // ---------------------------------------------------------------------------
 // HALF: ncols_valid == L/2, so the upper half of every row is a compile-time zero.
-template <typename T, typename Tin, int LOG2L, bool HALF>
-__global__ void __launch_bounds__((RowCfg<T, LOG2L>::NT))
+template <typename T, typename Tin, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((RowCfg<T, LOG2L, RQ>::NT))
 k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int ncols_valid,
           int nrows, T scale, const cpx<T> *__restrict__ tw, cpx<T> *__restrict__ dst,
           int64_t dst_cstride) {
-    using C = RowCfg<T, LOG2L>;
+    using C = RowCfg<T, LOG2L, RQ>;
     constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
-    constexpr int R0 = RegPlan<LOG2L>::R0;
+    constexpr int R0 = C::P::R0;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
@@ -103,7 +103,7 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
         v[r] = {re, im};
     }
     cpx<T> *seq = buf + p * SLEN;
-    reg_fft<T, false, LOG2L>(v, seq, t, tw);
+    reg_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
@@ -139,13 +139,13 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
 // ---------------------------------------------------------------------------
 // HALF: ny == L/2, so the upper half of each column is a compile-time zero on
 // input and is not stored on output.
-template <typename T, int LOG2L, bool HALF>
-__global__ void __launch_bounds__((ColCfg<T, LOG2L>::NT))
+template <typename T, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((ColCfg<T, LOG2L, RQ>::NT))
 k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
             const quad4<T> *__restrict__ kern, T scale) {
-    using C = ColCfg<T, LOG2L>;
+    using C = ColCfg<T, LOG2L, RQ>;
     constexpr int L = C::L, SLEN = C::SLEN, TPS = C::TPS;
-    constexpr int R0 = RegPlan<LOG2L>::R0;
+    constexpr int R0 = C::P::R0;
     constexpr int HY = L / 2 + 1;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
@@ -159,7 +159,7 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
         v[r] = (HALF ? r < R0 / 2 : j < ny) ? col[j] : cpx<T>{T(0), T(0)};
     }
     cpx<T> *seq = buf + c * SLEN;
-    reg_fft<T, false, LOG2L>(v, seq, t, tw);
+    reg_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
     __syncthreads();
     if (kx != 0) {
         if (c < 2) {
@@ -206,7 +206,7 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
             v[r] = {scale * (c1 * z.x + c2 * zp.x), scale * (c1 * z.y - c2 * zp.y)};
         }
     }
-    reg_fft<T, true, LOG2L>(v, seq, t, tw);
+    reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
         const int j = t + r * TPS;
@@ -221,13 +221,13 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
 // ---------------------------------------------------------------------------
 // HALF: nx == L/2, so only the lower half of each inverse-transformed row is
 // formed and stored.
-template <typename T, int LOG2L, bool HALF>
-__global__ void __launch_bounds__((RowCfg<T, LOG2L>::NT))
+template <typename T, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((RowCfg<T, LOG2L, RQ>::NT))
 k_row_c2r(const cpx<T> *__restrict__ S, int hxs, int nx, int ny, const cpx<T> *__restrict__ tw,
           T *__restrict__ out, int64_t n) {
-    using C = RowCfg<T, LOG2L>;
+    using C = RowCfg<T, LOG2L, RQ>;
     constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
-    constexpr int R0 = RegPlan<LOG2L>::R0;
+    constexpr int R0 = C::P::R0;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
@@ -257,7 +257,7 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hxs, int nx, int ny, const cpx<T> *_
     cpx<T> v[R0];
 #pragma unroll
     for (int r = 0; r < R0; ++r) v[r] = seq[lds_pad(t + r * TPS)];
-    reg_fft<T, true, LOG2L>(v, seq, t, tw);
+    reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
     const int ja = j0 + 2 * p;
     T *o = out + comp * n;
 #pragma unroll
@@ -473,6 +473,10 @@ static void allow_lds(K kernel, size_t bytes) {
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+// A column block holds the three components, 3 * L/R0 threads (at most 1024).
+template <int LG, int RQ>
+constexpr bool col_rq_ok() { return 3 * ((1 << LG) / (1 << RegPlan<LG, RQ>::LOG2R0)) <= 1024; }
+
 // Built ranges: rows up to 2^13 (FP32) / 2^12 (FP64), columns up to 2^12 / 2^11
 // (the column pass holds 3 padded sequences in LDS and 3*TPS <= 1024 threads).
 template <typename T> struct Range;
@@ -540,10 +544,10 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         LLGMI_HIP(hipMalloc(&sk, sizeof(cpx<double>) * 4 * hxs * py));
         dispatch_log2<1, kRowF64Build>(lx, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
-            using C = RowCfg<double, LG>;
-            allow_lds(k_row_r2c<double, double, LG, false>, C::LDS);
+            using C = RowCfg<double, LG, 0>;
+            allow_lds(k_row_r2c<double, double, LG, false, 0>, C::LDS);
             dim3 grid((unsigned)((py + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 4);
-            hipLaunchKernelGGL((k_row_r2c<double, double, LG, false>), grid, dim3(C::NT), C::LDS, stream, kr,
+            hipLaunchKernelGGL((k_row_r2c<double, double, LG, false, 0>), grid, dim3(C::NT), C::LDS, stream, kr,
                                np, (int)px, (int)px, (int)py, 1.0, twx64, sk, (int64_t)hxs * py);
             LLGMI_LAUNCH_CHECK();
         });
@@ -578,62 +582,163 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     LLGMI_HIP(hipMalloc(&spec, sizeof(cpx<T>) * 3 * (px / 2) * ny));
     dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
-        allow_lds(k_row_r2c<T, T, LG, true>, RowCfg<T, LG>::LDS);
-        allow_lds(k_row_r2c<T, T, LG, false>, RowCfg<T, LG>::LDS);
-        allow_lds(k_row_c2r<T, LG, true>, RowCfg<T, LG>::LDS);
-        allow_lds(k_row_c2r<T, LG, false>, RowCfg<T, LG>::LDS);
-        pairs_fwd = RowCfg<T, LG>::PAIRS;
+        allow_lds(k_row_r2c<T, T, LG, true, 3>, RowCfg<T, LG, 3>::LDS);
+        allow_lds(k_row_r2c<T, T, LG, false, 3>, RowCfg<T, LG, 3>::LDS);
+        allow_lds(k_row_c2r<T, LG, true, 3>, RowCfg<T, LG, 3>::LDS);
+        allow_lds(k_row_c2r<T, LG, false, 3>, RowCfg<T, LG, 3>::LDS);
+        allow_lds(k_row_r2c<T, T, LG, true, 4>, RowCfg<T, LG, 4>::LDS);
+        allow_lds(k_row_r2c<T, T, LG, false, 4>, RowCfg<T, LG, 4>::LDS);
+        allow_lds(k_row_c2r<T, LG, true, 4>, RowCfg<T, LG, 4>::LDS);
+        allow_lds(k_row_c2r<T, LG, false, 4>, RowCfg<T, LG, 4>::LDS);
     });
     dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
-        allow_lds(k_col_demag<T, LG, true>, ColCfg<T, LG>::LDS);
-        allow_lds(k_col_demag<T, LG, false>, ColCfg<T, LG>::LDS);
+        if constexpr (col_rq_ok<LG, 3>()) {
+            allow_lds(k_col_demag<T, LG, true, 3>, ColCfg<T, LG, 3>::LDS);
+            allow_lds(k_col_demag<T, LG, false, 3>, ColCfg<T, LG, 3>::LDS);
+        }
+        allow_lds(k_col_demag<T, LG, true, 4>, ColCfg<T, LG, 4>::LDS);
+        allow_lds(k_col_demag<T, LG, false, 4>, ColCfg<T, LG, 4>::LDS);
     });
     built = true;
+    autotune(stream);
+}
+
+template <typename T>
+void DemagFFT<T>::launch_row_fwd(const T *m, T ms, hipStream_t stream, int rq) {
+    const int64_t n = nx * ny;
+    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        auto go = [&](auto rqc) {
+            constexpr int RQ = decltype(rqc)::value;
+            using C = RowCfg<T, LG, RQ>;
+            const dim3 grid((unsigned)((ny + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 3);
+            if (2 * nx == px)
+                hipLaunchKernelGGL((k_row_r2c<T, T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, stream,
+                                   m, n, (int)nx, (int)nx, (int)ny, ms, tw_x, spec, (int64_t)(px / 2) * ny);
+            else
+                hipLaunchKernelGGL((k_row_r2c<T, T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, stream,
+                                   m, n, (int)nx, (int)nx, (int)ny, ms, tw_x, spec, (int64_t)(px / 2) * ny);
+        };
+        if (rq == 3) go(std::integral_constant<int, 3>{});
+        else go(std::integral_constant<int, 4>{});
+        LLGMI_LAUNCH_CHECK();
+    });
+}
+
+template <typename T>
+void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
+    dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        const T scale = T(1.0 / ((double)px * (double)py));
+        auto go = [&](auto rqc) {
+            constexpr int RQ = decltype(rqc)::value;
+            using C = ColCfg<T, LG, RQ>;
+            if (2 * ny == py)
+                hipLaunchKernelGGL((k_col_demag<T, LG, true, RQ>), dim3((unsigned)(px / 2)), dim3(C::NT),
+                                   C::LDS, stream, spec, (int)(px / 2), (int)ny, tw_y, kern, scale);
+            else
+                hipLaunchKernelGGL((k_col_demag<T, LG, false, RQ>), dim3((unsigned)(px / 2)), dim3(C::NT),
+                                   C::LDS, stream, spec, (int)(px / 2), (int)ny, tw_y, kern, scale);
+        };
+        if constexpr (col_rq_ok<LG, 3>()) {
+            if (rq == 3) go(std::integral_constant<int, 3>{});
+            else go(std::integral_constant<int, 4>{});
+        } else {
+            go(std::integral_constant<int, 4>{});
+        }
+        LLGMI_LAUNCH_CHECK();
+    });
+}
+
+template <typename T>
+void DemagFFT<T>::launch_row_inv(T *bd, hipStream_t stream, int rq) {
+    const int64_t n = nx * ny;
+    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        auto go = [&](auto rqc) {
+            constexpr int RQ = decltype(rqc)::value;
+            using C = RowCfg<T, LG, RQ>;
+            const dim3 grid((unsigned)((ny + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 3);
+            if (2 * nx == px)
+                hipLaunchKernelGGL((k_row_c2r<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, stream, spec,
+                                   (int)(px / 2), (int)nx, (int)ny, tw_x, bd, n);
+            else
+                hipLaunchKernelGGL((k_row_c2r<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, stream, spec,
+                                   (int)(px / 2), (int)nx, (int)ny, tw_x, bd, n);
+        };
+        if (rq == 3) go(std::integral_constant<int, 3>{});
+        else go(std::integral_constant<int, 4>{});
+        LLGMI_LAUNCH_CHECK();
+    });
+}
+
+// Per grid, time the radix-8 and radix-16 variants of each pass (5 launches
+// each after one warm-up) and keep the faster: which one wins depends on the
+// lengths through occupancy (registers) versus LDS round trips.
+template <typename T>
+void DemagFFT<T>::autotune(hipStream_t stream) {
+    const int64_t n = nx * ny;
+    T *m = nullptr, *bd = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+        if (m) (void)hipFree(m);
+        if (bd) (void)hipFree(bd);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    try {
+        LLGMI_HIP(hipMalloc(&m, sizeof(T) * 3 * n));
+        LLGMI_HIP(hipMalloc(&bd, sizeof(T) * 3 * n));
+        LLGMI_HIP(hipMemsetAsync(m, 0, sizeof(T) * 3 * n, stream));
+        LLGMI_HIP(hipEventCreate(&e0));
+        LLGMI_HIP(hipEventCreate(&e1));
+        auto time = [&](auto &&launch) {
+            launch();
+            LLGMI_HIP(hipEventRecord(e0, stream));
+            for (int i = 0; i < 5; ++i) launch();
+            LLGMI_HIP(hipEventRecord(e1, stream));
+            LLGMI_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            LLGMI_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return ms;
+        };
+        float best = 1e30f;
+        for (int rq = 3; rq <= 4; ++rq) {
+            float t = time([&] { launch_row_fwd(m, T(1), stream, rq); });
+            if (t < best) { best = t; rq_row = rq; }
+        }
+        best = 1e30f;
+        for (int rq = 3; rq <= 4; ++rq) {
+            float t = time([&] { launch_col(stream, rq); });
+            if (t < best) { best = t; rq_col = rq; }
+        }
+        best = 1e30f;
+        for (int rq = 3; rq <= 4; ++rq) {
+            float t = time([&] { launch_row_inv(bd, stream, rq); });
+            if (t < best) { best = t; rq_row_inv = rq; }
+        }
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
 }
 
 template <typename T>
 void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *prof) {
-    const int64_t n = nx * ny;
-    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
-        constexpr int LG = decltype(lg)::value;
-        using C = RowCfg<T, LG>;
-        const dim3 grid((unsigned)((ny + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 3);
+    {
         ProfScope ps(prof, 0, stream);
-        if (2 * nx == px)
-            hipLaunchKernelGGL((k_row_r2c<T, T, LG, true>), grid, dim3(C::NT), C::LDS, stream, m, n,
-                               (int)nx, (int)nx, (int)ny, ms, tw_x, spec, (int64_t)(px / 2) * ny);
-        else
-            hipLaunchKernelGGL((k_row_r2c<T, T, LG, false>), grid, dim3(C::NT), C::LDS, stream, m, n,
-                               (int)nx, (int)nx, (int)ny, ms, tw_x, spec, (int64_t)(px / 2) * ny);
-        LLGMI_LAUNCH_CHECK();
-    });
-    dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
-        constexpr int LG = decltype(lg)::value;
-        using C = ColCfg<T, LG>;
+        launch_row_fwd(m, ms, stream, rq_row);
+    }
+    {
         ProfScope ps(prof, 1, stream);
-        const T scale = T(1.0 / ((double)px * (double)py));
-        if (2 * ny == py)
-            hipLaunchKernelGGL((k_col_demag<T, LG, true>), dim3((unsigned)(px / 2)), dim3(C::NT), C::LDS,
-                               stream, spec, (int)(px / 2), (int)ny, tw_y, kern, scale);
-        else
-            hipLaunchKernelGGL((k_col_demag<T, LG, false>), dim3((unsigned)(px / 2)), dim3(C::NT), C::LDS,
-                               stream, spec, (int)(px / 2), (int)ny, tw_y, kern, scale);
-        LLGMI_LAUNCH_CHECK();
-    });
-    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
-        constexpr int LG = decltype(lg)::value;
-        using C = RowCfg<T, LG>;
-        const dim3 grid((unsigned)((ny + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 3);
+        launch_col(stream, rq_col);
+    }
+    {
         ProfScope ps(prof, 2, stream);
-        if (2 * nx == px)
-            hipLaunchKernelGGL((k_row_c2r<T, LG, true>), grid, dim3(C::NT), C::LDS, stream, spec,
-                               (int)(px / 2), (int)nx, (int)ny, tw_x, bd, n);
-        else
-            hipLaunchKernelGGL((k_row_c2r<T, LG, false>), grid, dim3(C::NT), C::LDS, stream, spec,
-                               (int)(px / 2), (int)nx, (int)ny, tw_x, bd, n);
-        LLGMI_LAUNCH_CHECK();
-    });
+        launch_row_inv(bd, stream, rq_row_inv);
+    }
 }
 
 template <typename T>

@@ -23,9 +23,12 @@
 namespace llgmi {
 
 __host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 4); }
-// One extra slot per sequence: consecutive sequences then start on different
-// banks (padded lengths of powers of two are multiples of the bank count).
-__host__ __device__ constexpr int lds_seq_len(int L) { return L + (L >> 4) + 1; }
+// Eight extra slots per sequence: padded power-of-two lengths are multiples of
+// the 64 four-byte banks, so without the offset every sequence of a block
+// starts on bank 0 and the row passes' column-wise reads and writes (one
+// element from each of several sequences) collide.  An offset of 8 complex
+// values (16 banks) puts four consecutive sequences on disjoint bank ranges.
+__host__ __device__ constexpr int lds_seq_len(int L) { return L + (L >> 4) + 8; }
 
 // In-register DFT16 as 4 x 4: X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 x[4 n1 + n2] W4^(n1 k1)
 template <typename T, bool INV>
@@ -73,10 +76,13 @@ __device__ __forceinline__ void dft_reg(cpx<T> *v) {
     else dft16<T, INV>(v);
 }
 
-template <int LOG2L>
+// RQ: requested log2 of the first/last radix (0 = default); ignored unless
+// both stages fit (2*RQ <= LOG2L).
+template <int LOG2L, int RQ = 0>
 struct RegPlan {
     static constexpr int L = 1 << LOG2L;
-    static constexpr int LOG2R0 = LOG2L <= 4 ? LOG2L : (LOG2L >= 8 ? 4 : (LOG2L >= 6 ? 3 : 2));
+    static constexpr int DEF = LOG2L <= 4 ? LOG2L : (LOG2L >= 8 ? 4 : (LOG2L >= 6 ? 3 : 2));
+    static constexpr int LOG2R0 = (RQ > 0 && 2 * RQ <= LOG2L) ? RQ : DEF;
     static constexpr int R0 = 1 << LOG2R0;
     static constexpr int TPS = L / R0;                                   // threads per sequence
     static constexpr int MID = LOG2L <= 4 ? 0 : LOG2L - 2 * LOG2R0;      // bits of the middle stages
@@ -122,10 +128,10 @@ __device__ __forceinline__ void apply_twiddles(cpx<T> *u, const cpx<T> *__restri
 
 // Middle/last stages.  Called after the first stage wrote its outputs to LDS.
 // If this is the last stage its outputs stay in v.
-template <typename T, bool INV, int LOG2L, int LOG2NS>
+template <typename T, bool INV, int LOG2L, int RQ, int LOG2NS>
 __device__ __forceinline__ void reg_fft_rest(cpx<T> *v, cpx<T> *seq, int t,
                                              const cpx<T> *__restrict__ tw) {
-    using P = RegPlan<LOG2L>;
+    using P = RegPlan<LOG2L, RQ>;
     if constexpr (LOG2NS < LOG2L) {
         constexpr int LOG2R = P::log2r(LOG2NS);
         constexpr int R = 1 << LOG2R;
@@ -156,7 +162,7 @@ __device__ __forceinline__ void reg_fft_rest(cpx<T> *v, cpx<T> *seq, int t,
 #pragma unroll
                 for (int r = 0; r < R; ++r) seq[lds_pad(idxD + r * NS)] = v[q * R + r];
             }
-            reg_fft_rest<T, INV, LOG2L, LOG2NS + LOG2R>(v, seq, t, tw);
+            reg_fft_rest<T, INV, LOG2L, RQ, LOG2NS + LOG2R>(v, seq, t, tw);
         }
         // LAST: BPT == 1, j == t, outputs v[r] = X[t + r*TPS]
     }
@@ -168,16 +174,16 @@ __device__ __forceinline__ void reg_fft_rest(cpx<T> *v, cpx<T> *seq, int t,
 // barriers); the caller must not touch `seq` between the call and a barrier
 // that follows it.  The first LDS write happens after a leading barrier, so
 // `seq` may still be read by the caller before the call.
-template <typename T, bool INV, int LOG2L>
+template <typename T, bool INV, int LOG2L, int RQ = 0>
 __device__ __forceinline__ void reg_fft(cpx<T> *v, cpx<T> *seq, int t,
                                         const cpx<T> *__restrict__ tw) {
-    using P = RegPlan<LOG2L>;
+    using P = RegPlan<LOG2L, RQ>;
     dft_reg<T, INV, P::R0>(v);
     if constexpr (LOG2L > P::LOG2R0) {
         __syncthreads();  // the caller may have been reading seq
 #pragma unroll
         for (int r = 0; r < P::R0; ++r) seq[lds_pad(t * P::R0 + r)] = v[r];
-        reg_fft_rest<T, INV, LOG2L, P::LOG2R0>(v, seq, t, tw);
+        reg_fft_rest<T, INV, LOG2L, RQ, P::LOG2R0>(v, seq, t, tw);
     }
 }
 

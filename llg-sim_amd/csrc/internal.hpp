@@ -85,13 +85,18 @@ struct DemagFFT {
     cpx<T> *tw_x = nullptr, *tw_y = nullptr;
     quad4<T> *kern = nullptr;  // [hx][hy]
     cpx<T> *spec = nullptr;    // [3][hx][ny]
-    int pairs_fwd = 1, nt_row = 256, nt_col = 256;
+    int rq_row = 4, rq_col = 4, rq_row_inv = 4;  // log2 first/last radix, picked by autotune()
 
     ~DemagFFT();
     void release();
     void build(int64_t nx, int64_t ny, double dx, double dy, double dz, hipStream_t stream);
     // bd = B_demag (overwrite) for M = ms * m; m and bd are SoA with 3*nx*ny entries.
     void field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *prof);
+    // Times the radix-8 and radix-16 variants of each pass on this grid and keeps the faster.
+    void autotune(hipStream_t stream);
+    void launch_row_fwd(const T *m, T ms, hipStream_t stream, int rq);
+    void launch_col(hipStream_t stream, int rq);
+    void launch_row_inv(T *bd, hipStream_t stream, int rq);
     void read_kernel(std::vector<double> &out) const;
 };
 
