@@ -27,6 +27,8 @@
 //    nx outputs of each row are written as B_demag (SoA).
 #include "common.hpp"
 #include "fft_reg.hpp"
+
+#include <cstdlib>
 #include "internal.hpp"
 
 #include <cmath>
@@ -55,8 +57,8 @@ template <typename T, int LOG2L, int RQ = 0>
 struct ColCfg {
     using P = RegPlan<LOG2L, RQ>;
     static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
-    static constexpr int NT = 3 * TPS;
-    static constexpr size_t LDS = sizeof(cpx<T>) * 3 * SLEN;
+    static constexpr int NT = 2 * TPS;
+    static constexpr size_t LDS = sizeof(cpx<T>) * 2 * SLEN;
 };
 
 template <int LOG2L>
@@ -133,12 +135,14 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
 }
 
 // ---------------------------------------------------------------------------
-// Pass 2: column FFT, tensor multiply, inverse column FFT.  Block = one kx;
-// threads [c*TPS, (c+1)*TPS) own component c.  The forward outputs stay in
+// Pass 2: column FFT, tensor multiply, inverse column FFT.
+// Every block transforms two sequences, with TPS threads each:
+//   blocks [0, hxs):            (x, y) of column kx = block   (coupled by Kxy)
+//   blocks [hxs, hxs + hxs/2):  z of columns 2b' and 2b'+1    (Kzz only)
+// Two sequences per block (instead of three) keep the block at 2*TPS threads
+// and ~35 KB of LDS, so two blocks fit per CU.  The forward outputs stay in
 // registers; x and y exchange through LDS for the off-diagonal Kxy terms.
 // ---------------------------------------------------------------------------
-// HALF: ny == L/2, so the upper half of each column is a compile-time zero on
-// input and is not stored on output.
 template <typename T, int LOG2L, bool HALF, int RQ>
 __global__ void __launch_bounds__((ColCfg<T, LOG2L, RQ>::NT))
 k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
@@ -149,25 +153,27 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
     constexpr int HY = L / 2 + 1;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
-    const int kx = blockIdx.x;
-    const int c = threadIdx.x / TPS, t = threadIdx.x - c * TPS;
-    cpx<T> *col = S + ((int64_t)c * hxs + kx) * ny;
+    const int sidx = threadIdx.x / TPS, t = threadIdx.x - sidx * TPS;
+    const bool xy = (int)blockIdx.x < hxs;
+    const int comp = xy ? sidx : 2;
+    const int kx = xy ? (int)blockIdx.x : 2 * ((int)blockIdx.x - hxs) + sidx;
+    const bool active = kx < hxs;  // the last z block may have one column only
+    cpx<T> *col = S + ((int64_t)comp * hxs + (active ? kx : 0)) * ny;
     cpx<T> v[R0];
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
         const int j = t + r * TPS;
-        v[r] = (HALF ? r < R0 / 2 : j < ny) ? col[j] : cpx<T>{T(0), T(0)};
+        v[r] = (active && (HALF ? r < R0 / 2 : j < ny)) ? col[j] : cpx<T>{T(0), T(0)};
     }
-    cpx<T> *seq = buf + c * SLEN;
+    cpx<T> *seq = buf + sidx * SLEN;
     reg_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
     __syncthreads();
-    if (kx != 0) {
-        if (c < 2) {
 #pragma unroll
-            for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
-        }
-        __syncthreads();
-        const quad4<T> *kc = kern + (int64_t)kx * HY;
+    for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
+    __syncthreads();
+    if (kx != 0) {
+        const quad4<T> *kc = kern + (int64_t)(active ? kx : 0) * HY;
+        const cpx<T> *other = buf + (sidx ^ 1) * SLEN;
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int ky = t + r * TPS;
@@ -175,11 +181,11 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
             const quad4<T> K = kc[kq];
             const T kxy = ky < HY ? K.xy : -K.xy;
             const cpx<T> own = v[r];
-            if (c == 0) {
-                const cpx<T> my = buf[SLEN + lds_pad(ky)];
+            if (comp == 0) {
+                const cpx<T> my = other[lds_pad(ky)];
                 v[r] = {scale * (K.xx * own.x + kxy * my.x), scale * (K.xx * own.y + kxy * my.y)};
-            } else if (c == 1) {
-                const cpx<T> mx = buf[lds_pad(ky)];
+            } else if (comp == 1) {
+                const cpx<T> mx = other[lds_pad(ky)];
                 v[r] = {scale * (kxy * mx.x + K.yy * own.x), scale * (kxy * mx.y + K.yy * own.y)};
             } else {
                 v[r] = {scale * (K.zz * own.x), scale * (K.zz * own.y)};
@@ -187,10 +193,8 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
         }
     } else {
         // packed column: Z = A + iB with A, B the transforms of the real kx = 0
-        // and kx = px/2 columns; Y = Ka A + i Kb B = ((Ka+Kb)/2) Z + ((Ka-Kb)/2) conj(Z[-ky])
-#pragma unroll
-        for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
-        __syncthreads();
+        // and kx = px/2 columns; Kxy vanishes on both, so per component
+        // Y = Ka A + i Kb B = ((Ka+Kb)/2) Z + ((Ka-Kb)/2) conj(Z[-ky])
         const quad4<T> *ka = kern;
         const quad4<T> *kb = kern + (int64_t)hxs * HY;
 #pragma unroll
@@ -198,8 +202,8 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
             const int ky = t + r * TPS;
             const int kq = ky < HY ? ky : L - ky;
             const quad4<T> A = ka[kq], B = kb[kq];
-            const T a = c == 0 ? A.xx : (c == 1 ? A.yy : A.zz);
-            const T b = c == 0 ? B.xx : (c == 1 ? B.yy : B.zz);
+            const T a = comp == 0 ? A.xx : (comp == 1 ? A.yy : A.zz);
+            const T b = comp == 0 ? B.xx : (comp == 1 ? B.yy : B.zz);
             const T c1 = T(0.5) * (a + b), c2 = T(0.5) * (a - b);
             const cpx<T> z = v[r];
             const cpx<T> zp = seq[lds_pad((L - ky) & (L - 1))];
@@ -207,10 +211,12 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
         }
     }
     reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
+    if (active) {
 #pragma unroll
-    for (int r = 0; r < R0; ++r) {
-        const int j = t + r * TPS;
-        if (HALF ? r < R0 / 2 : j < ny) col[j] = v[r];
+        for (int r = 0; r < R0; ++r) {
+            const int j = t + r * TPS;
+            if (HALF ? r < R0 / 2 : j < ny) col[j] = v[r];
+        }
     }
 }
 
@@ -473,9 +479,9 @@ static void allow_lds(K kernel, size_t bytes) {
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-// A column block holds the three components, 3 * L/R0 threads (at most 1024).
+// A column block holds two sequences, 2 * L/R0 threads (at most 1024).
 template <int LG, int RQ>
-constexpr bool col_rq_ok() { return 3 * ((1 << LG) / (1 << RegPlan<LG, RQ>::LOG2R0)) <= 1024; }
+constexpr bool col_rq_ok() { return 2 * ((1 << LG) / (1 << RegPlan<LG, RQ>::LOG2R0)) <= 1024; }
 
 // Built ranges: rows up to 2^13 (FP32) / 2^12 (FP64), columns up to 2^12 / 2^11
 // (the column pass holds 3 padded sequences in LDS and 3*TPS <= 1024 threads).
@@ -501,9 +507,19 @@ void DemagFFT<T>::release() {
     built = false;
 }
 
+static int forced_radix_log2() {
+    const char *f = std::getenv("LLGMI_FFT_RADIX");
+    if (!f || !*f) return 0;
+    const std::string v(f);
+    if (v == "8") return 3;
+    if (v == "16") return 4;
+    throw Error(1, "LLGMI_FFT_RADIX must be 8 or 16, got '" + v + "'");
+}
+
 template <typename T>
 void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double dz_,
                         hipStream_t stream) {
+    const int forced = forced_radix_log2();
     release();
     nx = nx_; ny = ny_; dx = dx_; dy = dy_; dz = dz_;
     px = next_pow2(2 * nx);
@@ -601,7 +617,7 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         allow_lds(k_col_demag<T, LG, false, 4>, ColCfg<T, LG, 4>::LDS);
     });
     built = true;
-    autotune(stream);
+    autotune(stream, forced);
 }
 
 template <typename T>
@@ -634,12 +650,14 @@ void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
         auto go = [&](auto rqc) {
             constexpr int RQ = decltype(rqc)::value;
             using C = ColCfg<T, LG, RQ>;
+            const int hxs = (int)(px / 2);
+            const dim3 grid((unsigned)(hxs + (hxs + 1) / 2));  // (x,y) blocks + z-pair blocks
             if (2 * ny == py)
-                hipLaunchKernelGGL((k_col_demag<T, LG, true, RQ>), dim3((unsigned)(px / 2)), dim3(C::NT),
-                                   C::LDS, stream, spec, (int)(px / 2), (int)ny, tw_y, kern, scale);
+                hipLaunchKernelGGL((k_col_demag<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, stream,
+                                   spec, hxs, (int)ny, tw_y, kern, scale);
             else
-                hipLaunchKernelGGL((k_col_demag<T, LG, false, RQ>), dim3((unsigned)(px / 2)), dim3(C::NT),
-                                   C::LDS, stream, spec, (int)(px / 2), (int)ny, tw_y, kern, scale);
+                hipLaunchKernelGGL((k_col_demag<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, stream,
+                                   spec, hxs, (int)ny, tw_y, kern, scale);
         };
         if constexpr (col_rq_ok<LG, 3>()) {
             if (rq == 3) go(std::integral_constant<int, 3>{});
@@ -673,11 +691,15 @@ void DemagFFT<T>::launch_row_inv(T *bd, hipStream_t stream, int rq) {
     });
 }
 
-// Per grid, time the radix-8 and radix-16 variants of each pass (5 launches
-// each after one warm-up) and keep the faster: which one wins depends on the
-// lengths through occupancy (registers) versus LDS round trips.
+// Per grid, time the radix-8 and radix-16 variants of each pass and keep the
+// faster: which one wins depends on the lengths through occupancy (registers)
+// versus LDS round trips.  LLGMI_FFT_RADIX=8 or 16 forces one variant for all
+// passes (used by tests to cover both); grids below 256 x 256 padded points
+// are launch-bound and keep radix 16 without timing.
 template <typename T>
-void DemagFFT<T>::autotune(hipStream_t stream) {
+void DemagFFT<T>::autotune(hipStream_t stream, int forced) {
+    rq_row = rq_col = rq_row_inv = forced ? forced : 4;
+    if (forced || px * py < 256 * 256) return;
     const int64_t n = nx * ny;
     T *m = nullptr, *bd = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -694,30 +716,28 @@ void DemagFFT<T>::autotune(hipStream_t stream) {
         LLGMI_HIP(hipEventCreate(&e0));
         LLGMI_HIP(hipEventCreate(&e1));
         auto time = [&](auto &&launch) {
-            launch();
             LLGMI_HIP(hipEventRecord(e0, stream));
-            for (int i = 0; i < 5; ++i) launch();
+            for (int i = 0; i < 8; ++i) launch();
             LLGMI_HIP(hipEventRecord(e1, stream));
             LLGMI_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
             LLGMI_HIP(hipEventElapsedTime(&ms, e0, e1));
             return ms;
         };
-        float best = 1e30f;
-        for (int rq = 3; rq <= 4; ++rq) {
-            float t = time([&] { launch_row_fwd(m, T(1), stream, rq); });
-            if (t < best) { best = t; rq_row = rq; }
-        }
-        best = 1e30f;
-        for (int rq = 3; rq <= 4; ++rq) {
-            float t = time([&] { launch_col(stream, rq); });
-            if (t < best) { best = t; rq_col = rq; }
-        }
-        best = 1e30f;
-        for (int rq = 3; rq <= 4; ++rq) {
-            float t = time([&] { launch_row_inv(bd, stream, rq); });
-            if (t < best) { best = t; rq_row_inv = rq; }
-        }
+        // variants alternate over three rounds (the first round also warms
+        // caches and code); the minimum per variant decides
+        auto pick = [&](auto &&launch_rq) {
+            float best[2] = {1e30f, 1e30f};
+            for (int round = 0; round < 3; ++round)
+                for (int v = 0; v < 2; ++v) {
+                    float t = time([&] { launch_rq(3 + v); });
+                    if (t < best[v]) best[v] = t;
+                }
+            return best[0] < best[1] ? 3 : 4;
+        };
+        rq_row = pick([&](int rq) { launch_row_fwd(m, T(1), stream, rq); });
+        rq_col = pick([&](int rq) { launch_col(stream, rq); });
+        rq_row_inv = pick([&](int rq) { launch_row_inv(bd, stream, rq); });
     } catch (...) {
         cleanup();
         throw;

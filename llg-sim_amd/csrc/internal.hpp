@@ -93,7 +93,7 @@ struct DemagFFT {
     // bd = B_demag (overwrite) for M = ms * m; m and bd are SoA with 3*nx*ny entries.
     void field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *prof);
     // Times the radix-8 and radix-16 variants of each pass on this grid and keeps the faster.
-    void autotune(hipStream_t stream);
+    void autotune(hipStream_t stream, int forced);
     void launch_row_fwd(const T *m, T ms, hipStream_t stream, int rq);
     void launch_col(hipStream_t stream, int rq);
     void launch_row_inv(T *bd, hipStream_t stream, int rq);

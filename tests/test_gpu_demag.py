@@ -110,3 +110,53 @@ def test_full_field_fp32_vs_fp64(gpu_lib, shape, cell):
     d64 = c64.compute_demag_field(m, mat)
     d32 = c32.compute_demag_field(m, mat)
     assert rel_err(d32, d64) <= 1e-5
+
+
+@pytest.mark.parametrize("radix", ["8", "16"])
+@pytest.mark.parametrize("shape,cell", [
+    ((128, 64), (4e-9, 4e-9, 3e-9)),
+    ((300, 3), (5e-9, 5e-9, 5e-9)),
+    ((129, 130), (3e-9, 2e-9, 1e-9)),
+])
+def test_demag_forced_radix_fp64_matches_oracle(gpu_lib, monkeypatch, radix, shape, cell):
+    """Both first/last-stage radices of the FFT passes (LLGMI_FFT_RADIX)."""
+    from oracle import oracle as O
+    monkeypatch.setenv("LLGMI_FFT_RADIX", radix)
+    L = gpu_lib
+    nx, ny = shape
+    m = _rand_m(nx * ny, 7 * nx + ny)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    b_ref = np.zeros_like(m)
+    oc.add_demag(m, b_ref, O.Material(ms=8e5, demag=True))
+    ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    b = ctx.compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(b, b_ref) <= 1e-12
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_demag_radix_variants_agree_1024(gpu_lib, monkeypatch, prec):
+    """At the 1024 x 1024 workload the radix-8 and radix-16 passes agree to
+    rounding, and the FP32 result stays within 1e-5 of FP64."""
+    L = gpu_lib
+    nx = ny = 1024
+    cell = (2e-9, 2e-9, 1e-9)
+    m = _rand_m(nx * ny, 11)
+    mat = L.Material(ms=8e5, demag=True)
+    out = {}
+    for radix in ("8", "16"):
+        monkeypatch.setenv("LLGMI_FFT_RADIX", radix)
+        out[radix] = L.Context(L.Grid2D(nx, ny, *cell), precision=prec).compute_demag_field(m, mat)
+    tol = 1e-13 if prec == 1 else 2e-6
+    assert rel_err(out["8"], out["16"]) <= tol
+    if prec == 0:
+        monkeypatch.setenv("LLGMI_FFT_RADIX", "16")
+        ref = L.Context(L.Grid2D(nx, ny, *cell), precision=1).compute_demag_field(m, mat)
+        assert rel_err(out["8"], ref) <= 1e-5
+
+
+def test_fft_radix_env_rejects_other_values(gpu_lib, monkeypatch):
+    L = gpu_lib
+    monkeypatch.setenv("LLGMI_FFT_RADIX", "32")
+    ctx = L.Context(L.Grid2D(8, 8, 1e-9, 1e-9, 1e-9), precision=L.F64)
+    with pytest.raises(Exception, match="LLGMI_FFT_RADIX"):
+        ctx.compute_demag_field(np.tile([0.0, 0.0, 1.0], (64, 1)), L.Material(ms=8e5, demag=True))
