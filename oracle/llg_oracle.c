@@ -17,6 +17,7 @@
  * DFT definition and numpy.fft, not against rustfft's rounding.
  */
 #include "llg_oracle.h"
+#include "mg_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -622,6 +623,8 @@ void orc_demag_add_field(orc_demag *d, const double *m, double *b, double ms) {
 struct orc_ctx {
     orc_grid g;
     orc_demag *demag;
+    orc_mg *mg;
+    orc_mg_cfg mg_cfg;
     double *ms_[8];      /* m2..m7 at [2..7] */
     double *bs[8];       /* b1..b7 */
     double *ks[8];       /* k1..k7 */
@@ -641,24 +644,42 @@ orc_ctx *orc_ctx_new(const orc_grid *g) {
         if (s >= 2) c->ms_[s] = (double *)calloc(n3, sizeof(double));
     }
     c->last_mask = -1;
+    orc_mg_cfg_default(&c->mg_cfg);
     return c;
 }
 
 void orc_ctx_free(orc_ctx *c) {
     if (!c) return;
     orc_demag_free(c->demag);
+    orc_mg_free(c->mg);
     for (int s = 1; s <= 7; ++s) { free(c->bs[s]); free(c->ks[s]); free(c->ms_[s]); }
     free(c);
 }
 
 int64_t orc_ctx_field_builds(const orc_ctx *c) { return c->builds; }
 
-/* demag.rs:47-67 + demag_fft_uniform.rs:157-172 (FFT method, open BC). */
+/* demag.rs:47-67, 109-142: FFT (demag_fft_uniform.rs:157-172, open BC), or
+ * MG for demag_method mg / dst (add_demag_field_poisson_mg,
+ * demag_poisson_mg.rs:3605-3635: cached solver, rebuilt when the structure
+ * changes, runtime knobs applied per call). */
 void orc_add_demag(orc_ctx *c, const double *m, double *b, const orc_material *mat) {
     if (!mat->demag) return;
+    if (mat->demag_method == 1 || mat->demag_method == 2) {
+        if (c->mg && !orc_mg_same_structure(c->mg, &c->g, &c->mg_cfg)) {
+            orc_mg_free(c->mg);
+            c->mg = NULL;
+        }
+        if (!c->mg) c->mg = orc_mg_new(&c->g, &c->mg_cfg);
+        orc_mg_apply_cfg(c->mg, &c->mg_cfg);
+        orc_mg_add_field(c->mg, m, b, mat->ms);
+        return;
+    }
     if (!c->demag) c->demag = orc_demag_new(&c->g);
     orc_demag_add_field(c->demag, m, b, mat->ms);
 }
+
+void orc_ctx_set_mg_cfg(orc_ctx *c, const orc_mg_cfg *cfg) { c->mg_cfg = *cfg; }
+orc_mg *orc_ctx_mg(orc_ctx *c) { return c->mg; }
 
 /* build_h_eff_masked (mod.rs:48-75) / build_h_eff_masked_geom (mod.rs:78-101) */
 void orc_build_h_eff(orc_ctx *c, const double *m, double *b, const orc_params *p,

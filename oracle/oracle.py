@@ -82,8 +82,79 @@ def lib():
         L.orc_ctx_field_builds.restype = C.c_int64
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
+        _bind_mg(L)
         _lib = L
     return _lib
+
+
+class _MGCfg(C.Structure):
+    _fields_ = [("pad_xy", C.c_int64), ("n_vac_z", C.c_int64), ("v_cycles", C.c_int64),
+                ("v_cycles_max", C.c_int64), ("has_tol_abs", C.c_int32), ("has_tol_rel", C.c_int32),
+                ("tol_abs", C.c_double), ("tol_rel", C.c_double), ("pre_smooth", C.c_int64),
+                ("post_smooth", C.c_int64), ("smoother", C.c_int32), ("warm_start", C.c_int32),
+                ("omega", C.c_double), ("sor_omega", C.c_double), ("bc", C.c_int32),
+                ("stencil", C.c_int32), ("tree_theta", C.c_double), ("tree_leaf", C.c_int64),
+                ("tree_max_depth", C.c_int64), ("prolong", C.c_int32), ("coarse_op", C.c_int32),
+                ("iso27_alpha", C.c_double)]
+
+
+MG_MAX_ST = 729
+
+
+class _Stencil(C.Structure):
+    _fields_ = [("center", C.c_double), ("diag", C.c_double), ("n", C.c_int32),
+                ("offs", (C.c_int32 * 3) * MG_MAX_ST), ("coeffs", C.c_double * MG_MAX_ST)]
+
+
+class _MGStats(C.Structure):
+    _fields_ = [("cycles", C.c_int64), ("rhs_max", C.c_double), ("tol_target", C.c_double),
+                ("last_residual", C.c_double), ("diverged", C.c_int32), ("n_charges", C.c_int32)]
+
+
+def _bind_mg(L):
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    i64 = C.c_int64
+    sp = C.POINTER(_Stencil)
+    L.orc_mg_cfg_default.argtypes = [C.POINTER(_MGCfg)]
+    L.orc_mg_sanitize.argtypes = [C.POINTER(_MGCfg)]
+    L.orc_stencil_seven_point.argtypes = [C.c_double] * 3 + [sp]
+    L.orc_stencil_iso9_plus_z.argtypes = [C.c_double] * 3 + [sp]
+    L.orc_stencil_iso27.argtypes = [C.c_double] * 4 + [sp]
+    L.orc_stencil_galerkin.argtypes = [sp, C.c_int, C.c_int, C.c_int, C.c_int, sp]
+    L.orc_mg_stamp_bc.argtypes = [dp, dp, i64, i64, i64]
+    L.orc_mg_apply_stencil.argtypes = [sp, dp, dp, i64, i64, i64]
+    L.orc_mg_smooth_wj.argtypes = [dp, dp, dp, dp, i64, i64, i64, sp, i64, C.c_double]
+    L.orc_mg_smooth_rb.argtypes = [dp, dp, dp, dp, i64, i64, i64, C.c_double, C.c_double,
+                                   C.c_double, i64, C.c_double]
+    L.orc_mg_residual.argtypes = [dp, dp, dp, i64, i64, i64, sp]
+    L.orc_mg_residual.restype = C.c_double
+    L.orc_mg_restrict.argtypes = [dp, i64, i64, i64, dp, dp, i64, i64, i64]
+    L.orc_mg_prolong_add.argtypes = [dp, i64, i64, i64, dp, i64, i64, i64, C.c_int]
+    L.orc_mg_build_charges.argtypes = [dp, i64, i64, i64, C.c_double, C.c_double, C.c_double, dp]
+    L.orc_mg_build_charges.restype = i64
+    L.orc_mg_treecode_eval.argtypes = [dp, i64, dp, C.c_double, i64, C.c_double, i64, dp, i64, dp]
+    L.orc_mg_treecode_bc.argtypes = [dp, dp, i64, i64, i64, C.c_double, C.c_double, C.c_double,
+                                     i64, C.c_double, i64]
+    L.orc_mg_dipole_bc.argtypes = [dp, dp, i64, i64, i64, C.c_double, C.c_double, C.c_double]
+    L.orc_mg_new.argtypes = [C.POINTER(_Grid), C.POINTER(_MGCfg)]
+    L.orc_mg_new.restype = vp
+    L.orc_mg_free.argtypes = [vp]
+    L.orc_mg_dims.argtypes = [vp, C.POINTER(i64)]
+    L.orc_mg_levels.argtypes = [vp, C.POINTER(i64), dp]
+    L.orc_mg_levels.restype = C.c_int
+    L.orc_mg_level_stencil.argtypes = [vp, C.c_int, sp]
+    L.orc_mg_build_rhs.argtypes = [vp, dp, C.c_double]
+    L.orc_mg_solve.argtypes = [vp]
+    L.orc_mg_add_b.argtypes = [vp, dp, dp]
+    L.orc_mg_add_field.argtypes = [vp, dp, dp, C.c_double]
+    L.orc_mg_finest.argtypes = [vp, C.c_int]
+    L.orc_mg_finest.restype = dp
+    L.orc_mg_stats_get.argtypes = [vp, C.POINTER(_MGStats)]
+    L.orc_mg_apply_cfg.argtypes = [vp, C.POINTER(_MGCfg)]
+    L.orc_ctx_set_mg_cfg.argtypes = [vp, C.POINTER(_MGCfg)]
+    L.orc_ctx_mg.argtypes = [vp]
+    L.orc_ctx_mg.restype = vp
 
 
 def _dp(a: np.ndarray):
@@ -294,3 +365,235 @@ class Context:
 
     def field_builds(self) -> int:
         return int(lib().orc_ctx_field_builds(self._h))
+
+    def set_mg_config(self, cfg) -> None:
+        """MG configuration used when mat.demag_method is mg (1) or dst (2)."""
+        lib().orc_ctx_set_mg_cfg(self._h, C.byref(cfg))
+
+
+# --- multigrid Poisson demag (demag_poisson_mg.rs) --------------------------
+SMOOTHER_WJ, SMOOTHER_RB = 0, 1
+BC_ZERO, BC_DIPOLE, BC_TREECODE = 0, 1, 2
+STENCIL_7, STENCIL_ISO9, STENCIL_ISO27 = 0, 1, 2
+PROLONG_INJECT, PROLONG_TRILINEAR = 0, 1
+COARSE_REDISCRETIZE, COARSE_GALERKIN = 0, 1
+
+
+def mg_config(**kw) -> _MGCfg:
+    """DemagPoissonMGConfig::default() + default operator, fields overridden by kw.
+    tol_abs / tol_rel = None switch the tolerance off."""
+    c = _MGCfg()
+    lib().orc_mg_cfg_default(C.byref(c))
+    for k, v in kw.items():
+        if k in ("tol_abs", "tol_rel"):
+            setattr(c, "has_" + k, 0 if v is None else 1)
+            setattr(c, k, 0.0 if v is None else float(v))
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def _stencil_dict(s: _Stencil) -> dict:
+    n = s.n
+    offs = np.array([[s.offs[q][0], s.offs[q][1], s.offs[q][2]] for q in range(n)], dtype=np.int64).reshape(n, 3)
+    coeffs = np.array([s.coeffs[q] for q in range(n)])
+    return {"center": s.center, "diag": s.diag, "offs": offs, "coeffs": coeffs}
+
+
+def _stencil_c(d: dict) -> _Stencil:
+    s = _Stencil()
+    s.center, s.diag, s.n = d["center"], d["diag"], len(d["coeffs"])
+    for q in range(s.n):
+        for a in range(3):
+            s.offs[q][a] = int(d["offs"][q][a])
+        s.coeffs[q] = float(d["coeffs"][q])
+    return s
+
+
+def stencil(kind: int, dx: float, dy: float, dz: float, alpha: float = 0.0) -> dict:
+    s = _Stencil()
+    if kind == STENCIL_7:
+        lib().orc_stencil_seven_point(dx, dy, dz, C.byref(s))
+    elif kind == STENCIL_ISO9:
+        lib().orc_stencil_iso9_plus_z(dx, dy, dz, C.byref(s))
+    else:
+        lib().orc_stencil_iso27(dx, dy, dz, alpha, C.byref(s))
+    return _stencil_dict(s)
+
+
+def galerkin(fine: dict, rx: int, ry: int, rz: int, trilinear: bool) -> dict:
+    f, o = _stencil_c(fine), _Stencil()
+    lib().orc_stencil_galerkin(C.byref(f), rx, ry, rz, int(trilinear), C.byref(o))
+    return _stencil_dict(o)
+
+
+def _box(a) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    assert a.ndim == 3
+    return a
+
+
+def mg_apply_stencil(st: dict, phi) -> np.ndarray:
+    phi = _box(phi)
+    nz, ny, nx = phi.shape
+    out = np.zeros_like(phi)
+    s = _stencil_c(st)
+    lib().orc_mg_apply_stencil(C.byref(s), _dp(phi), _dp(out), nx, ny, nz)
+    return out
+
+
+def mg_smooth_wj(phi, rhs, bc, st: dict, iters: int, omega: float) -> np.ndarray:
+    phi = _box(phi).copy()
+    rhs, bc = _box(rhs), _box(bc)
+    nz, ny, nx = phi.shape
+    tmp = np.zeros_like(phi)
+    s = _stencil_c(st)
+    lib().orc_mg_smooth_wj(_dp(phi), _dp(tmp), _dp(rhs), _dp(bc), nx, ny, nz, C.byref(s), iters, omega)
+    return phi
+
+
+def mg_smooth_rb(phi, rhs, bc, spacing, iters: int, omega: float) -> np.ndarray:
+    phi = _box(phi).copy()
+    rhs, bc = _box(rhs), _box(bc)
+    nz, ny, nx = phi.shape
+    tmp = np.zeros_like(phi)
+    dx, dy, dz = spacing
+    lib().orc_mg_smooth_rb(_dp(phi), _dp(tmp), _dp(rhs), _dp(bc), nx, ny, nz, 1.0 / (dx * dx),
+                           1.0 / (dy * dy), 1.0 / (dz * dz), iters, omega)
+    return phi
+
+
+def mg_residual(phi, rhs, st: dict):
+    phi, rhs = _box(phi), _box(rhs)
+    nz, ny, nx = phi.shape
+    res = np.zeros_like(phi)
+    s = _stencil_c(st)
+    mx = lib().orc_mg_residual(_dp(phi), _dp(rhs), _dp(res), nx, ny, nz, C.byref(s))
+    return res, mx
+
+
+def mg_restrict(fine_res, coarse_shape):
+    fr = _box(fine_res)
+    fnz, fny, fnx = fr.shape
+    cnz, cny, cnx = coarse_shape
+    crhs = np.full(coarse_shape, np.nan)
+    cphi = np.full(coarse_shape, np.nan)
+    lib().orc_mg_restrict(_dp(fr), fnx, fny, fnz, _dp(crhs), _dp(cphi), cnx, cny, cnz)
+    return crhs, cphi
+
+
+def mg_prolong_add(coarse, fine, trilinear: bool) -> np.ndarray:
+    c = _box(coarse)
+    f = _box(fine).copy()
+    cnz, cny, cnx = c.shape
+    fnz, fny, fnx = f.shape
+    lib().orc_mg_prolong_add(_dp(c), cnx, cny, cnz, _dp(f), fnx, fny, fnz, int(trilinear))
+    return f
+
+
+def mg_charges(rhs, spacing) -> np.ndarray:
+    """(n, 4): x, y, z, q of build_charges_from_rhs."""
+    rhs = _box(rhs)
+    nz, ny, nx = rhs.shape
+    n = lib().orc_mg_build_charges(_dp(rhs), nx, ny, nz, *spacing, None)
+    out = np.zeros((n, 4))
+    if n:
+        lib().orc_mg_build_charges(_dp(rhs), nx, ny, nz, *spacing, _dp(out))
+    return out
+
+
+def treecode_eval(charges, targets, root_center, root_half, leaf=64, theta=0.6, max_depth=20):
+    pq = np.ascontiguousarray(np.asarray(charges, dtype=np.float64).reshape(-1, 4))
+    tg = np.ascontiguousarray(np.asarray(targets, dtype=np.float64).reshape(-1, 3))
+    rc = np.ascontiguousarray(np.asarray(root_center, dtype=np.float64))
+    out = np.zeros(len(tg))
+    lib().orc_mg_treecode_eval(_dp(pq), len(pq), _dp(rc), root_half, leaf, theta, max_depth,
+                               _dp(tg), len(tg), _dp(out))
+    return out
+
+
+def mg_treecode_bc(rhs, spacing, leaf=64, theta=0.6, max_depth=20) -> np.ndarray:
+    rhs = _box(rhs)
+    nz, ny, nx = rhs.shape
+    bc = np.zeros_like(rhs)
+    lib().orc_mg_treecode_bc(_dp(bc), _dp(rhs), nx, ny, nz, *spacing, leaf, theta, max_depth)
+    return bc
+
+
+def mg_dipole_bc(rhs, spacing) -> np.ndarray:
+    rhs = _box(rhs)
+    nz, ny, nx = rhs.shape
+    bc = np.zeros_like(rhs)
+    lib().orc_mg_dipole_bc(_dp(bc), _dp(rhs), nx, ny, nz, *spacing)
+    return bc
+
+
+class MG:
+    """DemagPoissonMG (demag_poisson_mg.rs:1376-2098)."""
+
+    def __init__(self, grid: Grid, cfg: Optional[_MGCfg] = None):
+        self.grid = grid
+        self.cfg = cfg if cfg is not None else mg_config()
+        g = grid.c()
+        self._h = lib().orc_mg_new(C.byref(g), C.byref(self.cfg))
+        d = (C.c_int64 * 6)()
+        lib().orc_mg_dims(self._h, d)
+        self.px, self.py, self.pz, self.offx, self.offy, self.offz = list(d)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_mg_free(self._h)
+            self._h = None
+
+    def levels(self):
+        dims = (C.c_int64 * (3 * 33))()
+        sp = (C.c_double * (3 * 33))()
+        n = lib().orc_mg_levels(self._h, dims, sp)
+        return [((dims[3 * l + 2], dims[3 * l + 1], dims[3 * l]), (sp[3 * l], sp[3 * l + 1], sp[3 * l + 2]))
+                for l in range(n)]
+
+    def level_stencil(self, level: int) -> dict:
+        s = _Stencil()
+        lib().orc_mg_level_stencil(self._h, level, C.byref(s))
+        return _stencil_dict(s)
+
+    def apply_cfg(self, cfg: _MGCfg) -> None:
+        self.cfg = cfg
+        lib().orc_mg_apply_cfg(self._h, C.byref(cfg))
+
+    def finest(self, which: str) -> np.ndarray:
+        """Copy of the finest-level array 'phi', 'rhs', 'res' or 'bc', shape (pz, py, px)."""
+        k = {"phi": 0, "rhs": 1, "res": 2, "bc": 3}[which]
+        p = lib().orc_mg_finest(self._h, k)
+        n = self.px * self.py * self.pz
+        return np.ctypeslib.as_array(p, shape=(n,)).copy().reshape(self.pz, self.py, self.px)
+
+    def set_finest(self, which: str, arr) -> None:
+        """Overwrite a finest-level array (shape (pz, py, px))."""
+        k = {"phi": 0, "rhs": 1, "res": 2, "bc": 3}[which]
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64)).reshape(-1)
+        assert a.size == self.px * self.py * self.pz
+        p = lib().orc_mg_finest(self._h, k)
+        C.memmove(p, a.ctypes.data, a.nbytes)
+
+    def build_rhs(self, m, ms: float) -> None:
+        m = _f(m)
+        lib().orc_mg_build_rhs(self._h, _dp(m), ms)
+
+    def solve(self) -> None:
+        lib().orc_mg_solve(self._h)
+
+    def add_b(self, m, b: np.ndarray) -> None:
+        m = _f(m)
+        lib().orc_mg_add_b(self._h, _dp(m), _dp(b))
+
+    def add_field(self, m, b: np.ndarray, ms: float) -> None:
+        m = _f(m)
+        lib().orc_mg_add_field(self._h, _dp(m), _dp(b), ms)
+
+    def stats(self) -> dict:
+        st = _MGStats()
+        lib().orc_mg_stats_get(self._h, C.byref(st))
+        return {"cycles": st.cycles, "rhs_max": st.rhs_max, "tol_target": st.tol_target,
+                "last_residual": st.last_residual, "diverged": bool(st.diverged),
+                "n_charges": st.n_charges}
