@@ -166,6 +166,68 @@ int llgmi_dev_rk45_step(llgmi_ctx *ctx, void *m_dev, llgmi_params *params,
 int llgmi_dev_upload(llgmi_ctx *ctx, const double *host_aos, void *dev_soa);
 int llgmi_dev_download(llgmi_ctx *ctx, const void *dev_soa, double *host_aos);
 
+/* ---- multigrid Poisson demag (demag_poisson_mg.rs) ------------------------------ */
+/* DemagPoissonMGConfig (demag_poisson_mg.rs:115-185) together with the
+ * operator choice MGOperatorSettings (:727-804).  144 bytes, no padding. */
+typedef struct llgmi_mg_config {
+    int64_t pad_xy;          /* vacuum cells on each side in x and y */
+    int64_t n_vac_z;         /* vacuum layers below and above the film */
+    int64_t v_cycles;        /* minimum V-cycles (exact count without a tolerance) */
+    int64_t v_cycles_max;
+    int32_t has_tol_abs;     /* Option<f64> tol_abs: 0 = None */
+    int32_t has_tol_rel;     /* Option<f64> tol_rel: 0 = None */
+    double tol_abs;          /* max-norm residual bound (A/m^2) */
+    double tol_rel;          /* relative to max |rhs| */
+    int64_t pre_smooth, post_smooth;
+    int32_t smoother;        /* 0 weighted Jacobi, 1 red-black SOR (MGSmoother :97-112) */
+    int32_t warm_start;      /* 0/1 */
+    double omega;            /* weighted-Jacobi factor */
+    double sor_omega;        /* red-black SOR factor */
+    int32_t bc;              /* 0 Dirichlet zero, 1 monopole+dipole, 2 treecode (:81-94) */
+    int32_t stencil;         /* 0 seven-point, 1 iso9+z, 2 iso27 (:708-712) */
+    double tree_theta;
+    int64_t tree_leaf;
+    int64_t tree_max_depth;
+    int32_t prolong;         /* 0 injection, 1 trilinear (:714-718) */
+    int32_t coarse_op;       /* 0 rediscretize, 1 Galerkin (:720-724) */
+    double iso27_alpha;      /* > 0: LLG_DEMAG_MG_ISO27_FLUX_ALPHA override; else none */
+} llgmi_mg_config;
+
+/* Outcome of the last MG solve of a context. */
+typedef struct llgmi_mg_stats {
+    int64_t cycles;          /* V-cycles run */
+    double rhs_max;          /* max |rhs| over interior cells (0 without tol_rel) */
+    double tol_target;       /* max(tol_abs, tol_rel * rhs_max) */
+    double last_residual;    /* max |r| at the last check, -1 if none was made */
+    int32_t diverged;        /* the NaN/Inf guard reset phi (:2021-2032) */
+    int32_t n_charges;       /* treecode point charges */
+    int32_t tree_rebuilt;    /* the treecode octree was rebuilt for this solve */
+    int32_t levels;          /* grid levels of the hierarchy */
+    int64_t box[3];          /* finest padded box px, py, pz */
+} llgmi_mg_stats;
+
+/* DemagPoissonMGConfig::default() with the default operator (iso27 stencil,
+ * trilinear prolongation, Galerkin coarse operators). */
+int llgmi_mg_config_default(llgmi_mg_config *cfg);
+/* DemagPoissonMGConfig::from_env (:188-267) + MGOperatorSettings::from_env
+ * (:741-794): the LLG_DEMAG_MG_* variables, with the reference's aliases,
+ * clamps and fallbacks. */
+int llgmi_mg_config_from_env(llgmi_mg_config *cfg);
+/* Pins the MG configuration of a context.  NULL restores the reference
+ * behaviour: the configuration is read from the environment at every call
+ * (add_demag_field_poisson_mg, :3612-3613).  A smoother that the operator
+ * does not allow is overridden as sanitize_cfg_for_op does (:1406-1417). */
+int llgmi_ctx_set_mg_config(llgmi_ctx *ctx, const llgmi_mg_config *cfg);
+/* add_demag_field_poisson_mg (demag_poisson_mg.rs:3605-3635): accumulate the
+ * MG demag field; the solver (hierarchy, warm-start potential, treecode) is
+ * cached in the context and rebuilt when grid, padding or operator change. */
+int llgmi_add_demag_field_poisson_mg(llgmi_ctx *ctx, const double *m, double *b_eff,
+                                     const llgmi_material *mat);
+/* compute_demag_field_poisson_mg (:3638-3646): overwrite. */
+int llgmi_compute_demag_field_poisson_mg(llgmi_ctx *ctx, const double *m, double *out,
+                                         const llgmi_material *mat);
+int llgmi_mg_last_stats(llgmi_ctx *ctx, llgmi_mg_stats *out);
+
 /* ---- per-kernel timing (HIP events on the context stream) ---------------------- */
 /* Kernel classes timed when profiling is on. */
 #define LLGMI_K_FFT_ROW_FWD 0   /* row R2C of Ms*m (zero padded) */
@@ -174,7 +236,8 @@ int llgmi_dev_download(llgmi_ctx *ctx, const void *dev_soa, double *host_aos);
 #define LLGMI_K_LOCAL_FIELD 3   /* Zeeman + exchange + anisotropy + DMI (+ torque) */
 #define LLGMI_K_RK45_STAGE 4    /* stage state combination + normalise */
 #define LLGMI_K_RK45_ERR 5      /* error-norm reduction */
-#define LLGMI_K_MG 6            /* all multigrid kernels */
+#define LLGMI_K_MG 6            /* multigrid RHS, smoothing, transfers, residuals, gradient */
+#define LLGMI_K_MG_BC 7         /* multigrid boundary values (treecode / dipole) */
 #define LLGMI_K_COUNT 8
 /* Times the kernel classes whose bit (1 << class) is set in class_mask; 0 stops. */
 int llgmi_profile_enable(llgmi_ctx *ctx, int class_mask);

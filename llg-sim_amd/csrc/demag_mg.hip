@@ -1,0 +1,1602 @@
+// Multigrid Poisson demag on the GPU (FP64).
+//
+// Reference: demag_poisson_mg.rs (config :115-267, stencils :840-1189, levels
+// :1425-1577, RHS :1595-1802, boundary :1804-1833, V-cycle :1877-1945, solve
+// :1947-2057, gradient :2063-2098), mg_kernels.rs (smoothers, residual,
+// transfers, gradient), mg_treecode.rs (charges, Barnes-Hut octree, dipole).
+//
+// Every kernel evaluates the reference's expressions in the reference's order
+// with floating-point contraction off, so a solve reproduces the CPU
+// restatement to the last bit wherever the reference's order is sequential
+// per element (smoothers, residuals, transfers, RHS, gradient, treecode
+// moments and evaluation).  Two reductions differ in order only: the
+// residual max-norm (max is exact in any order) and the dipole moments of the
+// DirichletDipole boundary (summed in a fixed parallel order).
+//
+// Layout: one FP64 array per level and quantity, (k*ny + j)*nx + i.  The
+// residual is never stored: the restriction recomputes it per fine cell
+// (each fine cell belongs to exactly one coarse cell), and the convergence
+// check only needs its max-norm.
+#pragma clang fp contract(off)
+
+#include "mg.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <type_traits>
+
+namespace llgmi {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kMu0 = 4.0 * kPi * 1e-7;   // params.rs:15
+constexpr int kMaxSt = 729;                  // Galerkin probe box 9^3
+constexpr int kMaxDepth = 64;                // tree_max_depth clamp (from_env :259-261)
+#ifdef LLGMI_MG_RED_BLOCKS
+constexpr int kRedBlocks = LLGMI_MG_RED_BLOCKS;
+#else
+constexpr int kRedBlocks = 1024;             // blocks of the grid-stride reductions
+#endif
+constexpr int kNT = 256;
+
+struct DevStencil {
+    double center, diag;
+    int n, pad_;
+    double c[kMaxSt];
+    int off[kMaxSt][3];
+};
+
+struct TNode {            // octree node in pre-order; leaf iff count > 0
+    double cx, cy, cz, half;
+    int skip, depth, start, count;
+};
+
+inline unsigned nblocks(int64_t n, int nt = kNT) {
+    int64_t b = (n + nt - 1) / nt;
+    if (b > 0x7fffffffLL) throw Error(3, "multigrid launch exceeds the grid limit");
+    return (unsigned)(b > 0 ? b : 1);
+}
+
+__device__ __forceinline__ int64_t idx3(int i, int j, int k, int nx, int ny) {
+    return ((int64_t)k * ny + j) * nx + i;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// offdiag_sum_at (mg_kernels.rs:72-94)
+__device__ __forceinline__ double offdiag(const double *__restrict__ phi, int nx, int ny, int nz, int i,
+                                          int j, int k, const DevStencil *__restrict__ st) {
+    double sum = 0.0;
+    const int n = st->n;
+    for (int q = 0; q < n; ++q) {
+        const int ii = clampi(i + st->off[q][0], 0, nx - 1);
+        const int jj = clampi(j + st->off[q][1], 0, ny - 1);
+        const int kk = clampi(k + st->off[q][2], 0, nz - 1);
+        sum += st->c[q] * phi[idx3(ii, jj, kk, nx, ny)];
+    }
+    return sum;
+}
+
+// Cell t of the box boundary (z faces, then y faces, then x faces).
+__device__ __forceinline__ void boundary_cell(int64_t t, int nx, int ny, int nz, int &i, int &j, int &k) {
+    const int64_t a = (int64_t)nx * ny;
+    if (t < 2 * a) {
+        k = t < a ? 0 : nz - 1;
+        t = t < a ? t : t - a;
+        j = (int)(t / nx);
+        i = (int)(t % nx);
+        return;
+    }
+    t -= 2 * a;
+    const int64_t b = (int64_t)nx * (nz - 2);
+    if (t < 2 * b) {
+        j = t < b ? 0 : ny - 1;
+        t = t < b ? t : t - b;
+        k = 1 + (int)(t / nx);
+        i = (int)(t % nx);
+        return;
+    }
+    t -= 2 * b;
+    const int64_t c = (int64_t)(ny - 2) * (nz - 2);
+    i = t < c ? 0 : nx - 1;
+    t = t < c ? t : t - c;
+    j = 1 + (int)(t % (ny - 2));
+    k = 1 + (int)(t / (ny - 2));
+}
+
+__host__ __device__ inline int64_t boundary_count(int nx, int ny, int nz) {
+    return 2 * (int64_t)nx * ny + 2 * (int64_t)nx * (nz - 2) + 2 * (int64_t)(ny - 2) * (nz - 2);
+}
+
+// ---------------------------------------------------------------------------
+// RHS: build_rhs_from_m (demag_poisson_mg.rs:1595-1802) on the three planes
+// k = offz-1, offz, offz+1 that can be nonzero (the rest of rhs stays 0).
+// ---------------------------------------------------------------------------
+struct RhsGeom {
+    int px, py, pz, offx, offy, offz, nxm, nym;
+    double dx, dy, dz, ms;
+};
+
+template <typename T>
+__device__ __forceinline__ bool m_at(const T *__restrict__ m, int64_t nm, const RhsGeom &g, int pi, int pj,
+                                     int pk, double out[3]) {
+    out[0] = out[1] = out[2] = 0.0;
+    if (pi < 0 || pj < 0 || pk < 0) return false;
+    if (pi >= g.px || pj >= g.py || pk >= g.pz) return false;
+    if (pk != g.offz) return false;
+    if (pi < g.offx || pi >= g.offx + g.nxm || pj < g.offy || pj >= g.offy + g.nym) return false;
+    const int64_t id = (int64_t)(pj - g.offy) * g.nxm + (pi - g.offx);
+    const double v0 = (double)m[id], v1 = (double)m[nm + id], v2 = (double)m[2 * nm + id];
+    const double n2 = v0 * v0 + v1 * v1 + v2 * v2;
+    if (n2 < 1e-30) return false;
+    out[0] = g.ms * v0;
+    out[1] = g.ms * v1;
+    out[2] = g.ms * v2;
+    return true;
+}
+
+__device__ __forceinline__ double face_val(bool ia, double a, bool ib, double b) {
+    if (ia && ib) return 0.5 * (a + b);
+    if (ia) return a;
+    if (ib) return b;
+    return 0.0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_mg_rhs(const T *__restrict__ m, int64_t nm, RhsGeom g, int k_lo,
+                                                int nk, double *__restrict__ rhs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)(g.px - 2) * (g.py - 2);
+    if (t >= plane * nk) return;
+    const int k = k_lo + (int)(t / plane);
+    const int64_t r = t % plane;
+    const int j = 1 + (int)(r / (g.px - 2));
+    const int i = 1 + (int)(r % (g.px - 2));
+    double c[3], xp[3], xm[3], yp[3], ym[3], zp[3], zm[3];
+    const bool ci = m_at(m, nm, g, i, j, k, c);
+    const bool xpi = m_at(m, nm, g, i + 1, j, k, xp);
+    const bool xmi = m_at(m, nm, g, i - 1, j, k, xm);
+    const bool ypi = m_at(m, nm, g, i, j + 1, k, yp);
+    const bool ymi = m_at(m, nm, g, i, j - 1, k, ym);
+    const bool zpi = m_at(m, nm, g, i, j, k + 1, zp);
+    const bool zmi = m_at(m, nm, g, i, j, k - 1, zm);
+    const double mx_p = face_val(ci, c[0], xpi, xp[0]);
+    const double mx_m = face_val(xmi, xm[0], ci, c[0]);
+    const double my_p = face_val(ci, c[1], ypi, yp[1]);
+    const double my_m = face_val(ymi, ym[1], ci, c[1]);
+    const double mz_p = face_val(ci, c[2], zpi, zp[2]);
+    const double mz_m = face_val(zmi, zm[2], ci, c[2]);
+    rhs[idx3(i, j, k, g.px, g.py)] = (mx_p - mx_m) / g.dx + (my_p - my_m) / g.dy + (mz_p - mz_m) / g.dz;
+}
+
+// ---------------------------------------------------------------------------
+// Smoothers (mg_kernels.rs:121-240), residual (:249-285), transfers (:293-439)
+// ---------------------------------------------------------------------------
+// One colour of smooth_rb_sor, in place: a cell of one colour reads only
+// cells of the other colour and itself, so the reference's tmp buffer is not
+// needed.  Thread -> (interior row, h), i = first cell of the colour + 2h.
+__global__ void __launch_bounds__(kNT) k_mg_rb(double *__restrict__ phi, const double *__restrict__ rhs,
+                                               int nx, int ny, int nz, double sx, double sy, double sz,
+                                               double denom, double omega, int color) {
+    const int H = (nx - 1) / 2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t rows = (int64_t)(ny - 2) * (nz - 2);
+    if (t >= rows * H) return;
+    const int64_t row = t / H;
+    const int h = (int)(t - row * H);
+    const int j = 1 + (int)(row % (ny - 2));
+    const int k = 1 + (int)(row / (ny - 2));
+    const int i = 1 + ((1 + j + k + color) & 1) + 2 * h;
+    if (i > nx - 2) return;
+    const int64_t id = idx3(i, j, k, nx, ny);
+    const int64_t plane = (int64_t)nx * ny;
+    const double off = sx * (phi[id - 1] + phi[id + 1]) + sy * (phi[id - nx] + phi[id + nx]) +
+                       sz * (phi[id - plane] + phi[id + plane]);
+    const double pn = (off - rhs[id]) / denom;
+    const double po = phi[id];
+    phi[id] = po + omega * (pn - po);
+}
+
+// One sweep of smooth_weighted_jacobi from src into dst; boundary cells of dst
+// take the Dirichlet values (bc, or 0 when bc is null).
+__global__ void __launch_bounds__(kNT) k_mg_wj(const double *__restrict__ src, double *__restrict__ dst,
+                                               const double *__restrict__ rhs, const double *__restrict__ bc,
+                                               const DevStencil *__restrict__ st, double omega, int nx,
+                                               int ny, int nz) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (int64_t)nx * ny * nz) return;
+    const int i = (int)(id % nx);
+    const int64_t r = id / nx;
+    const int j = (int)(r % ny);
+    const int k = (int)(r / ny);
+    if (i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || k == 0 || k == nz - 1) {
+        dst[id] = bc ? bc[id] : 0.0;
+        return;
+    }
+    const double off = offdiag(src, nx, ny, nz, i, j, k, st);
+    const double s = src[id];
+    const double gs = fabs(st->diag) > 1e-30 ? (off - rhs[id]) / st->diag : s;
+    dst[id] = (1.0 - omega) * s + omega * gs;
+}
+
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double red[kNT / 64];
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmax(m, red[q]);
+        v = m;
+    }
+    return v;
+}
+
+// max |rhs - A phi| over interior cells, per block (compute_residual's return value)
+__global__ void __launch_bounds__(kNT) k_mg_resid_max(const double *__restrict__ phi,
+                                                      const double *__restrict__ rhs,
+                                                      const DevStencil *__restrict__ st, int nx, int ny,
+                                                      int nz, double *__restrict__ partial) {
+    const int64_t n = (int64_t)(nx - 2) * (ny - 2) * (nz - 2);
+    double mx = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(t % (nx - 2));
+        const int64_t r = t / (nx - 2);
+        const int j = 1 + (int)(r % (ny - 2));
+        const int k = 1 + (int)(r / (ny - 2));
+        const int64_t id = idx3(i, j, k, nx, ny);
+        const double a = st->center * phi[id] + offdiag(phi, nx, ny, nz, i, j, k, st);
+        mx = fmax(mx, fabs(rhs[id] - a));
+    }
+    mx = block_max(mx);
+    if (threadIdx.x == 0) partial[blockIdx.x] = mx;
+}
+
+// Residual + restrict_residual: coarse rhs = mean over the fine block of
+// r = rhs - A phi; coarse phi = 0 (interior; the coarse boundary stays 0).
+__global__ void __launch_bounds__(kNT) k_mg_resid_restrict(const double *__restrict__ phi,
+                                                           const double *__restrict__ rhs,
+                                                           const DevStencil *__restrict__ st, int fnx,
+                                                           int fny, int fnz, double *__restrict__ crhs,
+                                                           double *__restrict__ cphi, int cnx, int cny,
+                                                           int cnz, int rx, int ry, int rz, double w) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)(cnx - 2) * (cny - 2) * (cnz - 2);
+    if (t >= n) return;
+    const int ic = 1 + (int)(t % (cnx - 2));
+    const int64_t r = t / (cnx - 2);
+    const int jc = 1 + (int)(r % (cny - 2));
+    const int kc = 1 + (int)(r / (cny - 2));
+    double sum = 0.0;
+    for (int dk = 0; dk < rz; ++dk)
+        for (int dj = 0; dj < ry; ++dj)
+            for (int di = 0; di < rx; ++di) {
+                const int i = rx * ic + di, j = ry * jc + dj, k = rz * kc + dk;
+                const int64_t id = idx3(i, j, k, fnx, fny);
+                const double a = st->center * phi[id] + offdiag(phi, fnx, fny, fnz, i, j, k, st);
+                sum += rhs[id] - a;
+            }
+    const int64_t cid = idx3(ic, jc, kc, cnx, cny);
+    crhs[cid] = w * sum;
+    cphi[cid] = 0.0;
+}
+
+__device__ __forceinline__ void interp_1d(int ic, int ri, int nc, int r, int &i0o, int &i1o, double &w0,
+                                          double &w1) {
+    const int i0 = ic < nc - 1 ? ic : nc - 1;
+    if (r == 1) {
+        i0o = i1o = i0;
+        w0 = 1.0;
+        w1 = 0.0;
+        return;
+    }
+    const int i1 = ri == 0 ? (i0 > 0 ? i0 - 1 : 0) : (i0 + 1 < nc - 1 ? i0 + 1 : nc - 1);
+    if (i1 == i0) {
+        i0o = i1o = i0;
+        w0 = 1.0;
+        w1 = 0.0;
+    } else {
+        i0o = i0;
+        i1o = i1;
+        w0 = 0.75;
+        w1 = 0.25;
+    }
+}
+
+// prolongate_add (mg_kernels.rs:377-439), one thread per fine interior cell
+__global__ void __launch_bounds__(kNT) k_mg_prolong(const double *__restrict__ c, int cnx, int cny, int cnz,
+                                                    double *__restrict__ f, int fnx, int fny, int fnz,
+                                                    int trilinear) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)(fnx - 2) * (fny - 2) * (fnz - 2);
+    if (t >= n) return;
+    const int i = 1 + (int)(t % (fnx - 2));
+    const int64_t r = t / (fnx - 2);
+    const int j = 1 + (int)(r % (fny - 2));
+    const int k = 1 + (int)(r / (fny - 2));
+    const int rx = fnx / cnx, ry = fny / cny, rz = fnz / cnz;
+    const int64_t id = idx3(i, j, k, fnx, fny);
+    if (trilinear) {
+        int k0, k1, j0, j1, i0, i1;
+        double wk0, wk1, wj0, wj1, wi0, wi1;
+        interp_1d(k / rz, k % rz, cnz, rz, k0, k1, wk0, wk1);
+        interp_1d(j / ry, j % ry, cny, ry, j0, j1, wj0, wj1);
+        interp_1d(i / rx, i % rx, cnx, rx, i0, i1, wi0, wi1);
+        const double v = wi0 * wj0 * wk0 * c[idx3(i0, j0, k0, cnx, cny)]
+                       + wi1 * wj0 * wk0 * c[idx3(i1, j0, k0, cnx, cny)]
+                       + wi0 * wj1 * wk0 * c[idx3(i0, j1, k0, cnx, cny)]
+                       + wi1 * wj1 * wk0 * c[idx3(i1, j1, k0, cnx, cny)]
+                       + wi0 * wj0 * wk1 * c[idx3(i0, j0, k1, cnx, cny)]
+                       + wi1 * wj0 * wk1 * c[idx3(i1, j0, k1, cnx, cny)]
+                       + wi0 * wj1 * wk1 * c[idx3(i0, j1, k1, cnx, cny)]
+                       + wi1 * wj1 * wk1 * c[idx3(i1, j1, k1, cnx, cny)];
+        f[id] += v;
+    } else {
+        const int ic = i / rx, jc = j / ry, kc = k / rz;
+        if (ic == 0 || ic + 1 >= cnx || jc == 0 || jc + 1 >= cny || kc == 0 || kc + 1 >= cnz) return;
+        f[id] += c[idx3(ic, jc, kc, cnx, cny)];
+    }
+}
+
+// Dirichlet stamp of the boundary cells (stamp_dirichlet_bc, mg_kernels.rs:35-61)
+__global__ void __launch_bounds__(kNT) k_mg_stamp(double *__restrict__ phi, const double *__restrict__ bc,
+                                                  int nx, int ny, int nz) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= boundary_count(nx, ny, nz)) return;
+    int i, j, k;
+    boundary_cell(t, nx, ny, nz, i, j, k);
+    const int64_t id = idx3(i, j, k, nx, ny);
+    phi[id] = bc ? bc[id] : 0.0;
+}
+
+// NaN/Inf guard of solve_with_timing (:2018-2033): the first 2 nx*ny values
+__global__ void __launch_bounds__(kNT) k_mg_nan(const double *__restrict__ phi, int64_t n,
+                                                double *__restrict__ flag) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && !isfinite(phi[t])) *flag = 1.0;
+}
+
+// max |rhs| over interior cells of the planes [k_lo, k_lo + nk)
+__global__ void __launch_bounds__(kNT) k_mg_rhs_absmax(const double *__restrict__ rhs, int nx, int ny, int k_lo,
+                                                       int nk, double *__restrict__ partial) {
+    const int64_t plane = (int64_t)(nx - 2) * (ny - 2);
+    const int64_t n = plane * nk;
+    double mx = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int k = k_lo + (int)(t / plane);
+        const int64_t r = t % plane;
+        const int j = 1 + (int)(r / (nx - 2));
+        const int i = 1 + (int)(r % (nx - 2));
+        mx = fmax(mx, fabs(rhs[idx3(i, j, k, nx, ny)]));
+    }
+    mx = block_max(mx);
+    if (threadIdx.x == 0) partial[blockIdx.x] = mx;
+}
+
+__global__ void __launch_bounds__(kNT) k_mg_final_max(const double *__restrict__ partial, int n,
+                                                      double *__restrict__ out) {
+    double mx = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) mx = fmax(mx, partial[q]);
+    mx = block_max(mx);
+    if (threadIdx.x == 0) *out = mx;
+}
+
+// extract_gradient_on_magnet_layer (mg_kernels.rs:450-507) as B_demag (overwrite)
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_mg_gradient(const double *__restrict__ phi, int px, int py, int pz,
+                                                     double dx, double dy, double dz, int offx, int offy,
+                                                     int kmag, int nxm, int nym, const T *__restrict__ m,
+                                                     T *__restrict__ bd) {
+    const int64_t n = (int64_t)nxm * nym;
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int i = (int)(id % nxm), j = (int)(id / nxm);
+    const double v0 = (double)m[id], v1 = (double)m[n + id], v2 = (double)m[2 * n + id];
+    if (!(v0 * v0 + v1 * v1 + v2 * v2 > 1e-30)) {
+        bd[id] = T(0);
+        bd[n + id] = T(0);
+        bd[2 * n + id] = T(0);
+        return;
+    }
+    const int pi = offx + i, pj = offy + j, k = kmag;
+    const double pc = phi[idx3(pi, pj, k, px, py)];
+    double gx = 0.0, wx = 0.0;
+    if (pi + 1 < px) { gx += (phi[idx3(pi + 1, pj, k, px, py)] - pc) / dx; wx += 1.0; }
+    if (pi > 0) { gx += (pc - phi[idx3(pi - 1, pj, k, px, py)]) / dx; wx += 1.0; }
+    if (wx > 0.0) gx /= wx;
+    double gy = 0.0, wy = 0.0;
+    if (pj + 1 < py) { gy += (phi[idx3(pi, pj + 1, k, px, py)] - pc) / dy; wy += 1.0; }
+    if (pj > 0) { gy += (pc - phi[idx3(pi, pj - 1, k, px, py)]) / dy; wy += 1.0; }
+    if (wy > 0.0) gy /= wy;
+    double gz = 0.0, wz = 0.0;
+    if (k + 1 < pz) { gz += (phi[idx3(pi, pj, k + 1, px, py)] - pc) / dz; wz += 1.0; }
+    if (k > 0) { gz += (pc - phi[idx3(pi, pj, k - 1, px, py)]) / dz; wz += 1.0; }
+    if (wz > 0.0) gz /= wz;
+    bd[id] = T(-kMu0 * gx);
+    bd[n + id] = T(-kMu0 * gy);
+    bd[2 * n + id] = T(-kMu0 * gz);
+}
+
+// ---------------------------------------------------------------------------
+// Treecode (mg_treecode.rs): charges, moments, boundary evaluation
+// ---------------------------------------------------------------------------
+// Candidate cells: interior cells of the planes [k_lo, k_lo + nk) in raster
+// order (the only cells the RHS can make nonzero).
+__device__ __forceinline__ int64_t cand_cell(int64_t t, int nx, int ny, int k_lo) {
+    const int64_t plane = (int64_t)(nx - 2) * (ny - 2);
+    const int k = k_lo + (int)(t / plane);
+    const int64_t r = t % plane;
+    return idx3(1 + (int)(r % (nx - 2)), 1 + (int)(r / (nx - 2)), k, nx, ny);
+}
+
+__global__ void __launch_bounds__(kNT) k_tc_count(const double *__restrict__ rhs, int nx, int ny, int k_lo,
+                                                  int64_t ncand, int *__restrict__ counts) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int f = 0;
+    if (t < ncand) f = fabs(rhs[cand_cell(t, nx, ny, k_lo)]) < 1e-40 ? 0 : 1;
+    const int c = __syncthreads_count(f);
+    if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// Exclusive scan of the block counts (one block); meta[1] = total.
+__global__ void __launch_bounds__(1024) k_tc_scan(const int *__restrict__ counts, int nb, int *__restrict__ offsets,
+                                                  int *__restrict__ meta) {
+    __shared__ int part[1024];
+    const int per = (nb + 1023) / 1024;
+    const int lo = threadIdx.x * per, hi = min(nb, lo + per);
+    int s = 0;
+    for (int q = lo; q < hi; ++q) s += counts[q];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int q = 0; q < 1024; ++q) {
+            const int v = part[q];
+            part[q] = acc;
+            acc += v;
+        }
+        meta[1] = acc;
+    }
+    __syncthreads();
+    int acc = part[threadIdx.x];
+    for (int q = lo; q < hi; ++q) {
+        offsets[q] = acc;
+        acc += counts[q];
+    }
+}
+
+// Compacted charges in raster order: cell index and q = rhs dV
+// (build_charges_from_rhs, mg_treecode.rs:273-300); meta[0] = 1 when the
+// list differs from the cached one.
+__global__ void __launch_bounds__(kNT) k_tc_write(const double *__restrict__ rhs, int nx, int ny, int k_lo,
+                                                  int64_t ncand, const int *__restrict__ offsets, double dvol,
+                                                  int *__restrict__ idx_out, double *__restrict__ q_out,
+                                                  const int *__restrict__ cached, int cached_n,
+                                                  int *__restrict__ meta) {
+    __shared__ int wsum[kNT / 64];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t cell = 0;
+    double rho = 0.0;
+    bool f = false;
+    if (t < ncand) {
+        cell = cand_cell(t, nx, ny, k_lo);
+        rho = rhs[cell];
+        f = !(fabs(rho) < 1e-40);
+    }
+    const unsigned long long bal = __ballot(f);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(bal);
+    __syncthreads();
+    int base = offsets[blockIdx.x];
+    for (int q = 0; q < w; ++q) base += wsum[q];
+    if (f) {
+        const int pos = base + before;
+        idx_out[pos] = (int)cell;
+        q_out[pos] = rho * dvol;
+        if (pos >= cached_n || cached[pos] != (int)cell) meta[0] = 1;
+    }
+}
+
+__global__ void __launch_bounds__(kNT) k_tc_gather(const double *__restrict__ q, const int *__restrict__ ord, int n,
+                                                   double *__restrict__ qp) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) qp[t] = q[ord[t]];
+}
+
+// compute_moments_rec (mg_treecode.rs:158-190), leaves
+__global__ void __launch_bounds__(kNT) k_tc_leaf_moments(const TNode *__restrict__ nodes,
+                                                         const int *__restrict__ leaves, int nleaves,
+                                                         const double *__restrict__ pos,
+                                                         const double *__restrict__ qp,
+                                                         double *__restrict__ mom) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nleaves) return;
+    const int id = leaves[t];
+    const TNode nd = nodes[id];
+    double q = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    for (int a = nd.start; a < nd.start + nd.count; ++a) {
+        const double cq = qp[a];
+        q += cq;
+        p0 += cq * (pos[3 * (int64_t)a] - nd.cx);
+        p1 += cq * (pos[3 * (int64_t)a + 1] - nd.cy);
+        p2 += cq * (pos[3 * (int64_t)a + 2] - nd.cz);
+    }
+    mom[4 * (int64_t)id] = q;
+    mom[4 * (int64_t)id + 1] = p0;
+    mom[4 * (int64_t)id + 2] = p1;
+    mom[4 * (int64_t)id + 3] = p2;
+}
+
+// compute_moments_rec, internal nodes of one depth (children done before)
+__global__ void __launch_bounds__(kNT) k_tc_node_moments(const TNode *__restrict__ nodes,
+                                                         const int *__restrict__ ids, int n,
+                                                         double *__restrict__ mom) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int id = ids[t];
+    const TNode nd = nodes[id];
+    double q = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    for (int c = id + 1; c < nd.skip; c = nodes[c].skip) {
+        const TNode cn = nodes[c];
+        const double cq = mom[4 * (int64_t)c];
+        q += cq;
+        p0 += mom[4 * (int64_t)c + 1] + cq * (cn.cx - nd.cx);
+        p1 += mom[4 * (int64_t)c + 2] + cq * (cn.cy - nd.cy);
+        p2 += mom[4 * (int64_t)c + 3] + cq * (cn.cz - nd.cz);
+    }
+    mom[4 * (int64_t)id] = q;
+    mom[4 * (int64_t)id + 1] = p0;
+    mom[4 * (int64_t)id + 2] = p1;
+    mom[4 * (int64_t)id + 3] = p2;
+}
+
+struct BoxGeom {
+    int nx, ny, nz;
+    double dx, dy, dz, cx, cy, cz;
+};
+
+__device__ __forceinline__ void cell_pos(const BoxGeom &g, int i, int j, int k, double &x, double &y, double &z) {
+    x = ((double)i + 0.5 - g.cx) * g.dx;
+    y = ((double)j + 0.5 - g.cy) * g.dy;
+    z = ((double)k + 0.5 - g.cz) * g.dz;
+}
+
+// eval_phi (mg_treecode.rs:192-259) at every boundary cell; writes bc and
+// stamps phi.  The recursion is walked in pre-order through skip links; the
+// open ancestors' partial sums sit in part[depth], so each node's value is
+// added to its parent's sum in the reference's order.
+__global__ void __launch_bounds__(kNT) k_tc_eval(const TNode *__restrict__ nodes, int nnodes,
+                                                 const double *__restrict__ mom, const double *__restrict__ pos,
+                                                 const double *__restrict__ qp, BoxGeom g, double theta,
+                                                 double *__restrict__ bc, double *__restrict__ phi) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= boundary_count(g.nx, g.ny, g.nz)) return;
+    int i, j, k;
+    boundary_cell(t, g.nx, g.ny, g.nz, i, j, k);
+    double tx, ty, tz;
+    cell_pos(g, i, j, k, tx, ty, tz);
+    double part[kMaxDepth + 2];
+    double total = 0.0;
+    int n = 0;
+    for (;;) {
+        const TNode nd = nodes[n];
+        const int d = nd.depth;
+        double v = 0.0;
+        bool accept;
+        if (nd.count > 0) {
+            accept = true;
+            for (int a = nd.start; a < nd.start + nd.count; ++a) {
+                const double ex = tx - pos[3 * (int64_t)a], ey = ty - pos[3 * (int64_t)a + 1],
+                             ez = tz - pos[3 * (int64_t)a + 2];
+                const double rr2 = ex * ex + ey * ey + ez * ez;
+                if (rr2 > 0.0) v += qp[a] / sqrt(rr2);
+            }
+        } else {
+            const double rx = tx - nd.cx, ry = ty - nd.cy, rz = tz - nd.cz;
+            const double ddx = fmax(fabs(rx) - nd.half, 0.0), ddy = fmax(fabs(ry) - nd.half, 0.0),
+                         ddz = fmax(fabs(rz) - nd.half, 0.0);
+            const double dd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            const double size = nd.half * 2.0;
+            accept = dd > 0.0 ? (size / dd < theta) : false;
+            if (accept) {
+                const double r = sqrt(rx * rx + ry * ry + rz * rz);
+                const double inv_r = 1.0 / r;
+                const double inv_r3 = inv_r * inv_r * inv_r;
+                const double pr = mom[4 * (int64_t)n + 1] * rx + mom[4 * (int64_t)n + 2] * ry +
+                                  mom[4 * (int64_t)n + 3] * rz;
+                v = mom[4 * (int64_t)n] * inv_r + pr * inv_r3;
+            }
+        }
+        if (!accept) {
+            part[d] = 0.0;
+            ++n;
+            continue;
+        }
+        if (d == 0) {
+            total = v;
+            break;
+        }
+        part[d - 1] += v;
+        const int next = nd.skip;
+        const int nd_next = next < nnodes ? nodes[next].depth : 0;
+        bool done = false;
+        for (int l = d - 1; l >= nd_next; --l) {
+            if (l == 0) {
+                total = part[0];
+                done = true;
+                break;
+            }
+            part[l - 1] += part[l];
+        }
+        if (done) break;
+        n = next;
+    }
+    const int64_t id = idx3(i, j, k, g.nx, g.ny);
+    const double val = -total / (4.0 * kPi);
+    bc[id] = val;
+    phi[id] = val;
+}
+
+// evaluate_dipole_bc (mg_treecode.rs:371-448): moments per block, then one
+// thread sums the partials in block order.
+__global__ void __launch_bounds__(kNT) k_dip_partial(const double *__restrict__ rhs, BoxGeom g, int k_lo, int nk,
+                                                     double dvol, double *__restrict__ partial) {
+    __shared__ double red[4][kNT];
+    const int64_t plane = (int64_t)(g.nx - 2) * (g.ny - 2);
+    const int64_t n = plane * nk;
+    double q = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int k = k_lo + (int)(t / plane);
+        const int64_t r = t % plane;
+        const int j = 1 + (int)(r / (g.nx - 2));
+        const int i = 1 + (int)(r % (g.nx - 2));
+        double x, y, z;
+        cell_pos(g, i, j, k, x, y, z);
+        const double w = rhs[idx3(i, j, k, g.nx, g.ny)] * dvol;
+        q += w;
+        px += w * x;
+        py += w * y;
+        pz += w * z;
+    }
+    red[0][threadIdx.x] = q;
+    red[1][threadIdx.x] = px;
+    red[2][threadIdx.x] = py;
+    red[3][threadIdx.x] = pz;
+    __syncthreads();
+    for (int s = kNT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) partial[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_dip_final(const double *__restrict__ partial, int nb, double *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < nb; ++b)
+        for (int c = 0; c < 4; ++c) s[c] += partial[4 * b + c];
+    for (int c = 0; c < 4; ++c) out[c] = s[c];
+}
+
+__global__ void __launch_bounds__(kNT) k_dip_eval(const double *__restrict__ mom, BoxGeom g,
+                                                  double *__restrict__ bc, double *__restrict__ phi) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= boundary_count(g.nx, g.ny, g.nz)) return;
+    int i, j, k;
+    boundary_cell(t, g.nx, g.ny, g.nz, i, j, k);
+    double x, y, z;
+    cell_pos(g, i, j, k, x, y, z);
+    const double q = mom[0], pxm = mom[1], pym = mom[2], pzm = mom[3];
+    const double inv4pi = 1.0 / (4.0 * kPi);
+    const double r2 = x * x + y * y + z * z;
+    double v = 0.0;
+    if (r2 > 0.0) {
+        const double r = sqrt(r2);
+        const double pr = pxm * x + pym * y + pzm * z;
+        v = -inv4pi * (q / r + pr / (r * r * r));
+    }
+    const int64_t id = idx3(i, j, k, g.nx, g.ny);
+    bc[id] = v;
+    phi[id] = v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host: configuration
+// ---------------------------------------------------------------------------
+void mg_config_default(llgmi_mg_config &c) {
+    std::memset(&c, 0, sizeof(c));
+    c.pad_xy = 6;
+    c.n_vac_z = 16;
+    c.v_cycles = 4;
+    c.v_cycles_max = 64;
+    c.has_tol_abs = 0;
+    c.tol_abs = 0.0;
+    c.has_tol_rel = 1;
+    c.tol_rel = 1e-6;
+    c.pre_smooth = 2;
+    c.post_smooth = 2;
+    c.smoother = 0;
+    c.omega = 2.0 / 3.0;
+    c.sor_omega = 1.0;
+    c.warm_start = 1;
+    c.bc = 2;
+    c.tree_theta = 0.6;
+    c.tree_leaf = 64;
+    c.tree_max_depth = 20;
+    c.stencil = 2;
+    c.prolong = 1;
+    c.coarse_op = 1;
+    c.iso27_alpha = 0.0;
+}
+
+namespace {
+std::string trim(const std::string &s) {
+    const auto b = s.find_first_not_of(" \t\r\n\v\f");
+    if (b == std::string::npos) return std::string();
+    const auto e = s.find_last_not_of(" \t\r\n\v\f");
+    return s.substr(b, e - b + 1);
+}
+std::string lower(std::string s) {
+    for (auto &ch : s) ch = (char)std::tolower((unsigned char)ch);
+    return s;
+}
+bool env_str(const char *name, std::string &out) {
+    const char *v = std::getenv(name);
+    if (!v) return false;
+    out = v;
+    return true;
+}
+// str::trim().parse::<usize>()
+bool env_usize(const char *name, int64_t &out) {
+    std::string s;
+    if (!env_str(name, s)) return false;
+    s = trim(s);
+    size_t p = 0;
+    if (!s.empty() && s[0] == '+') p = 1;
+    if (p >= s.size()) return false;
+    uint64_t v = 0;
+    for (; p < s.size(); ++p) {
+        if (s[p] < '0' || s[p] > '9') return false;
+        if (v > (UINT64_MAX - 9) / 10) return false;
+        v = v * 10 + (uint64_t)(s[p] - '0');
+    }
+    if (v > (uint64_t)INT64_MAX) return false;
+    out = (int64_t)v;
+    return true;
+}
+// str::trim().parse::<f64>()
+bool env_f64(const char *name, double &out) {
+    std::string s;
+    if (!env_str(name, s)) return false;
+    s = trim(s);
+    if (s.empty()) return false;
+    char *end = nullptr;
+    const double v = std::strtod(s.c_str(), &end);
+    if (end != s.c_str() + s.size()) return false;
+    out = v;
+    return true;
+}
+double clampf(double v, double lo, double hi) {  // f64::clamp (NaN stays NaN)
+    if (std::isnan(v)) return v;
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+}  // namespace
+
+// DemagPoissonMGConfig::from_env (:188-267) + MGOperatorSettings::from_env (:741-794)
+void mg_config_from_env(llgmi_mg_config &c) {
+    mg_config_default(c);
+    int64_t u;
+    double f;
+    std::string s;
+    if (env_usize("LLG_DEMAG_MG_PAD_XY", u) || env_usize("LLG_DEMAG_MG_PAD_FACTOR_XY", u)) c.pad_xy = std::max<int64_t>(u, 1);
+    if (env_usize("LLG_DEMAG_MG_NVAC_Z", u)) c.n_vac_z = std::max<int64_t>(u, 1);
+    if (env_usize("LLG_DEMAG_MG_VCYCLES", u)) c.v_cycles = std::max<int64_t>(u, 1);
+    if (env_usize("LLG_DEMAG_MG_VCYCLES_MAX", u)) c.v_cycles_max = std::max<int64_t>(u, 1);
+    if (env_f64("LLG_DEMAG_MG_TOL_ABS", f)) { c.has_tol_abs = 1; c.tol_abs = std::fmax(f, 0.0); }
+    if (env_f64("LLG_DEMAG_MG_TOL_REL", f)) { c.has_tol_rel = 1; c.tol_rel = std::fmin(std::fmax(f, 0.0), 1.0); }
+    if (env_usize("LLG_DEMAG_MG_PRE_SMOOTH", u)) c.pre_smooth = std::max<int64_t>(u, 1);
+    if (env_usize("LLG_DEMAG_MG_POST_SMOOTH", u)) c.post_smooth = std::max<int64_t>(u, 1);
+    if (env_f64("LLG_DEMAG_MG_OMEGA", f)) c.omega = clampf(f, 0.05, 1.0);
+    if (env_f64("LLG_DEMAG_MG_SOR_OMEGA", f)) c.sor_omega = clampf(f, 0.2, 1.95);
+    if (env_str("LLG_DEMAG_MG_SMOOTHER", s)) {   // MGSmoother::from_str (:97-112)
+        const std::string t = lower(trim(s));
+        if (t == "jacobi" || t == "wj" || t == "weighted_jacobi") c.smoother = 0;
+        else if (t == "rbgs" || t == "redblack" || t == "red_black" || t == "red_black_sor" || t == "sor") c.smoother = 1;
+    }
+    if (env_str("LLG_DEMAG_MG_WARM_START", s))
+        c.warm_start = (s == "1" || s == "true" || s == "yes" || s == "on") ? 1 : 0;
+    if (env_str("LLG_DEMAG_MG_BC", s)) {
+        const std::string t = lower(trim(s));
+        if (t == "0" || t == "zero" || t == "dirichlet0" || t == "dirichlet_zero") c.bc = 0;
+        else if (t == "dipole" || t == "dirichlet_dipole" || t == "dirichletdipole") c.bc = 1;
+        else if (t == "tree" || t == "treecode" || t == "bh" || t == "fmm") c.bc = 2;
+    }
+    if (env_f64("LLG_DEMAG_MG_TREE_THETA", f)) c.tree_theta = clampf(f, 0.2, 1.5);
+    if (env_usize("LLG_DEMAG_MG_TREE_LEAF", u)) c.tree_leaf = std::min<int64_t>(std::max<int64_t>(u, 8), 4096);
+    if (env_usize("LLG_DEMAG_MG_TREE_MAX_DEPTH", u)) c.tree_max_depth = std::min<int64_t>(std::max<int64_t>(u, 4), 64);
+    if (env_str("LLG_DEMAG_MG_STENCIL", s)) {
+        const std::string t = lower(trim(s));
+        if (t == "7" || t == "7pt" || t == "seven" || t == "sevenpoint") c.stencil = 0;
+        else if (t == "iso9" || t == "9" || t == "9pt" || t == "mehrstellen9") c.stencil = 1;
+        else c.stencil = 2;
+    }
+    if (env_str("LLG_DEMAG_MG_PROLONG", s)) {
+        const std::string t = lower(trim(s));
+        c.prolong = (t == "inject" || t == "injection" || t == "pc" || t == "piecewiseconstant") ? 0 : 1;
+    }
+    if (env_str("LLG_DEMAG_MG_COARSE_OP", s)) {
+        const std::string t = lower(trim(s));
+        c.coarse_op = (t == "rediscretize" || t == "re" || t == "rd") ? 0 : 1;
+    }
+    if (env_f64("LLG_DEMAG_MG_ISO27_FLUX_ALPHA", f) && std::isfinite(f) && f > 0.0) c.iso27_alpha = f;
+}
+
+// rb_allowed_for_op / sanitize_cfg_for_op (:1400-1417)
+void mg_sanitize(llgmi_mg_config &c) {
+    const bool rb_ok = c.stencil == 0 && c.coarse_op == 0 && c.prolong == 0;
+    if (c.smoother == 1 && !rb_ok) c.smoother = 0;
+}
+
+void mg_validate(const llgmi_mg_config &c) {
+    auto bad = [](const std::string &m) { throw Error(1, "llgmi_mg_config: " + m); };
+    if (c.pad_xy < 0 || c.n_vac_z < 0 || c.v_cycles < 0 || c.v_cycles_max < 0 || c.pre_smooth < 0 ||
+        c.post_smooth < 0)
+        bad("counts must be >= 0");
+    if (c.smoother != 0 && c.smoother != 1) bad("smoother must be 0 or 1");
+    if (c.bc < 0 || c.bc > 2) bad("bc must be 0, 1 or 2");
+    if (c.stencil < 0 || c.stencil > 2) bad("stencil must be 0, 1 or 2");
+    if (c.prolong != 0 && c.prolong != 1) bad("prolong must be 0 or 1");
+    if (c.coarse_op != 0 && c.coarse_op != 1) bad("coarse_op must be 0 or 1");
+    if (c.tree_leaf < 1) bad("tree_leaf must be >= 1");
+    if (c.tree_max_depth < 0 || c.tree_max_depth > kMaxDepth) bad("tree_max_depth must be in [0, 64]");
+    if (!std::isfinite(c.omega) || !std::isfinite(c.sor_omega) || !std::isfinite(c.tree_theta))
+        bad("omega, sor_omega and tree_theta must be finite");
+    if (c.pad_xy > (1 << 20) || c.n_vac_z > (1 << 20)) bad("padding too large");
+}
+
+// ---------------------------------------------------------------------------
+// Host: stencils (demag_poisson_mg.rs:840-1189)
+// ---------------------------------------------------------------------------
+namespace {
+void push(MGStencil &s, int a, int b, int c, double w) {
+    s.offs.push_back({a, b, c});
+    s.coeffs.push_back(w);
+}
+
+MGStencil seven_point(double dx, double dy, double dz) {
+    const double sx = 1.0 / (dx * dx), sy = 1.0 / (dy * dy), sz = 1.0 / (dz * dz);
+    MGStencil s;
+    s.center = -2.0 * (sx + sy + sz);
+    push(s, 1, 0, 0, sx);
+    push(s, -1, 0, 0, sx);
+    push(s, 0, 1, 0, sy);
+    push(s, 0, -1, 0, sy);
+    push(s, 0, 0, 1, sz);
+    push(s, 0, 0, -1, sz);
+    s.diag = -s.center;
+    return s;
+}
+
+MGStencil iso9_plus_z(double dx, double dy, double dz) {
+    const double rel = std::fabs(dx - dy) / std::fmax(std::fmax(dx, dy), 1e-30);
+    if (rel > 1e-6) return seven_point(dx, dy, dz);
+    const double h = 0.5 * (dx + dy);
+    const double inv_h2 = 1.0 / (h * h);
+    const double cz = 1.0 / (dz * dz);
+    const double c_axis = (2.0 / 3.0) * inv_h2;
+    const double c_diag = (1.0 / 6.0) * inv_h2;
+    MGStencil s;
+    s.center = -(10.0 / 3.0) * inv_h2 - 2.0 * cz;
+    push(s, 1, 0, 0, c_axis);
+    push(s, -1, 0, 0, c_axis);
+    push(s, 0, 1, 0, c_axis);
+    push(s, 0, -1, 0, c_axis);
+    push(s, 1, 1, 0, c_diag);
+    push(s, 1, -1, 0, c_diag);
+    push(s, -1, 1, 0, c_diag);
+    push(s, -1, -1, 0, c_diag);
+    push(s, 0, 0, 1, cz);
+    push(s, 0, 0, -1, cz);
+    s.diag = -s.center;
+    return s;
+}
+
+MGStencil iso27(double dx, double dy, double dz, double alpha_in) {
+    double sx1 = 0.0, sy1 = 0.0, sz1 = 0.0;
+    for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj)
+            for (int dk = -1; dk <= 1; ++dk) {
+                if (di == 0 && dj == 0 && dk == 0) continue;
+                if (std::abs(di) + std::abs(dj) + std::abs(dk) == 1) continue;
+                const double ddx = (double)di * dx, ddy = (double)dj * dy, ddz = (double)dk * dz;
+                const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (d2 <= 0.0) continue;
+                const double w1 = 1.0 / d2;
+                sx1 += w1 * ddx * ddx;
+                sy1 += w1 * ddy * ddy;
+                sz1 += w1 * ddz * ddz;
+            }
+    const bool have = std::isfinite(alpha_in) && alpha_in > 0.0;
+    double amax = INFINITY;
+    if (sx1 > 0.0) amax = std::fmin(amax, 2.0 / sx1);
+    if (sy1 > 0.0) amax = std::fmin(amax, 2.0 / sy1);
+    if (sz1 > 0.0) amax = std::fmin(amax, 2.0 / sz1);
+    if (std::isfinite(amax)) amax *= 0.999;
+    const double adef = std::isfinite(amax) ? std::fmax(0.99 * amax, 0.0) : 0.0;
+    double alpha = have ? alpha_in : adef;
+    if (std::isfinite(amax) && alpha > amax) alpha = std::fmax(amax, 0.0);
+    const double wfx = (2.0 - alpha * sx1) / (2.0 * dx * dx);
+    const double wfy = (2.0 - alpha * sy1) / (2.0 * dy * dy);
+    const double wfz = (2.0 - alpha * sz1) / (2.0 * dz * dz);
+    MGStencil s;
+    push(s, 1, 0, 0, wfx);
+    push(s, -1, 0, 0, wfx);
+    push(s, 0, 1, 0, wfy);
+    push(s, 0, -1, 0, wfy);
+    push(s, 0, 0, 1, wfz);
+    push(s, 0, 0, -1, wfz);
+    for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj)
+            for (int dk = -1; dk <= 1; ++dk) {
+                if (di == 0 && dj == 0 && dk == 0) continue;
+                if (std::abs(di) + std::abs(dj) + std::abs(dk) <= 1) continue;
+                const double ddx = (double)di * dx, ddy = (double)dj * dy, ddz = (double)dk * dz;
+                const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (d2 <= 0.0) continue;
+                push(s, di, dj, dk, alpha / d2);
+            }
+    double sum = 0.0;
+    for (double c : s.coeffs) sum += c;
+    s.center = -sum;
+    s.diag = sum;
+    return s;
+}
+
+inline int64_t hidx(int64_t i, int64_t j, int64_t k, int64_t nx, int64_t ny) { return (k * ny + j) * nx + i; }
+
+double apply_at(const MGStencil &s, const std::vector<double> &phi, int nx, int ny, int nz, int i, int j,
+                int k) {
+    double sum = 0.0;
+    for (size_t q = 0; q < s.coeffs.size(); ++q) {
+        const int ii = std::min(std::max(i + s.offs[q][0], 0), nx - 1);
+        const int jj = std::min(std::max(j + s.offs[q][1], 0), ny - 1);
+        const int kk = std::min(std::max(k + s.offs[q][2], 0), nz - 1);
+        sum += s.coeffs[q] * phi[hidx(ii, jj, kk, nx, ny)];
+    }
+    return s.center * phi[hidx(i, j, k, nx, ny)] + sum;
+}
+
+void interp_1d_h(int ic, int ri, int nc, int r, int &i0o, int &i1o, double &w0, double &w1) {
+    const int i0 = std::min(ic, nc - 1);
+    if (r == 1) { i0o = i1o = i0; w0 = 1.0; w1 = 0.0; return; }
+    const int i1 = ri == 0 ? (i0 > 0 ? i0 - 1 : 0) : std::min(i0 + 1, nc - 1);
+    if (i1 == i0) { i0o = i1o = i0; w0 = 1.0; w1 = 0.0; }
+    else { i0o = i0; i1o = i1; w0 = 0.75; w1 = 0.25; }
+}
+}  // namespace
+
+MGStencil mg_stencil_from_kind(int kind, double dx, double dy, double dz, double alpha) {
+    if (kind == 0) return seven_point(dx, dy, dz);
+    if (kind == 1) return iso9_plus_z(dx, dy, dz);
+    return iso27(dx, dy, dz, alpha);
+}
+
+// Stencil3D::galerkin_coarsen (:1087-1189).  Only the fine cells of the probe
+// box that restrict into the centre coarse cell are evaluated; each value is
+// computed exactly as in the full-box evaluation.
+MGStencil mg_galerkin(const MGStencil &fine, int rx, int ry, int rz, bool trilinear) {
+    const int nc = 9, c0 = nc / 2;
+    const int nfx = nc * rx, nfy = nc * ry, nfz = nc * rz;
+    std::vector<double> phic((size_t)nc * nc * nc), phif((size_t)nfx * nfy * nfz);
+    std::map<std::tuple<int, int, int>, double> map;
+    const double norm = 1.0 / (double)(rx * ry * rz);
+    for (int kz = 0; kz < nc; ++kz)
+        for (int jy = 0; jy < nc; ++jy)
+            for (int ix = 0; ix < nc; ++ix) {
+                std::fill(phic.begin(), phic.end(), 0.0);
+                phic[hidx(ix, jy, kz, nc, nc)] = 1.0;
+                std::fill(phif.begin(), phif.end(), 0.0);
+                if (!trilinear) {
+                    for (int fk = 0; fk < rz; ++fk)
+                        for (int fj = 0; fj < ry; ++fj)
+                            for (int fi = 0; fi < rx; ++fi)
+                                phif[hidx(ix * rx + fi, jy * ry + fj, kz * rz + fk, nfx, nfy)] = 1.0;
+                } else {
+                    for (int k = 0; k < nfz; ++k) {
+                        int k0, k1; double wk0, wk1;
+                        interp_1d_h(k / rz, k % rz, nc, rz, k0, k1, wk0, wk1);
+                        for (int j = 0; j < nfy; ++j) {
+                            int j0, j1; double wj0, wj1;
+                            interp_1d_h(j / ry, j % ry, nc, ry, j0, j1, wj0, wj1);
+                            for (int i = 0; i < nfx; ++i) {
+                                int i0, i1; double wi0, wi1;
+                                interp_1d_h(i / rx, i % rx, nc, rx, i0, i1, wi0, wi1);
+                                auto c = [&](int a, int b, int d) { return phic[hidx(a, b, d, nc, nc)]; };
+                                double v = 0.0;
+                                v += wi0 * wj0 * wk0 * c(i0, j0, k0);
+                                v += wi1 * wj0 * wk0 * c(i1, j0, k0);
+                                v += wi0 * wj1 * wk0 * c(i0, j1, k0);
+                                v += wi1 * wj1 * wk0 * c(i1, j1, k0);
+                                v += wi0 * wj0 * wk1 * c(i0, j0, k1);
+                                v += wi1 * wj0 * wk1 * c(i1, j0, k1);
+                                v += wi0 * wj1 * wk1 * c(i0, j1, k1);
+                                v += wi1 * wj1 * wk1 * c(i1, j1, k1);
+                                phif[hidx(i, j, k, nfx, nfy)] = v;
+                            }
+                        }
+                    }
+                }
+                double sum = 0.0;
+                for (int fk = 0; fk < rz; ++fk)
+                    for (int fj = 0; fj < ry; ++fj)
+                        for (int fi = 0; fi < rx; ++fi)
+                            sum += apply_at(fine, phif, nfx, nfy, nfz, c0 * rx + fi, c0 * ry + fj, c0 * rz + fk);
+                const double coeff = norm * sum;
+                if (std::fabs(coeff) > 1e-14) map[std::make_tuple(ix - c0, jy - c0, kz - c0)] = coeff;
+            }
+    MGStencil out;
+    for (const auto &kv : map) {
+        const auto &key = kv.first;
+        if (key == std::make_tuple(0, 0, 0)) out.center = kv.second;
+        else push(out, std::get<0>(key), std::get<1>(key), std::get<2>(key), kv.second);
+    }
+    out.diag = -out.center;
+    double abs_sum = 0.0;
+    size_t neg = 0;
+    for (double c : out.coeffs) {
+        abs_sum += std::fabs(c);
+        if (c < 0.0) ++neg;
+    }
+    const double rho = out.diag > 0.0 ? abs_sum / out.diag : INFINITY;
+    if (out.diag <= 0.0 || rho > 2.0 || neg > out.coeffs.size() / 2) return MGStencil{};
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Host: treecode octree (BarnesHutTree, mg_treecode.rs:71-156)
+// ---------------------------------------------------------------------------
+// The structure that sequential insertion produces depends only on the charge
+// set: a node below max_depth is split iff it holds more than leaf_size
+// charges, children exist for non-empty octants, and a leaf lists its charges
+// in increasing charge order.  The build below partitions stably level by
+// level, which yields exactly that tree, and lays it out in pre-order with
+// children in octant order (the evaluation order of eval_node).
+struct MGSolver::Tree {
+    int64_t n = 0;
+    int leaf = -1, max_depth = -1;
+    bool valid = false;
+    int nnodes = 0;
+    // compaction
+    int *d_counts = nullptr, *d_offsets = nullptr, *d_meta = nullptr, *h_meta = nullptr;
+    int64_t cap_blocks = 0;
+    int *d_idx = nullptr;
+    double *d_q = nullptr;
+    int64_t cap_cand = 0;
+    int *d_cached = nullptr;
+    int64_t cap_cached = 0;
+    // tree
+    TNode *d_nodes = nullptr;
+    double *d_mom = nullptr, *d_pos = nullptr, *d_qp = nullptr;
+    int *d_ord = nullptr, *d_leaves = nullptr, *d_internal = nullptr;
+    int nleaves = 0;
+    std::vector<std::pair<int, int>> internal_by_depth;  // (offset, count), depth order
+
+    void free_tree() {
+        for (void *p : {(void *)d_nodes, (void *)d_mom, (void *)d_pos, (void *)d_qp, (void *)d_ord,
+                        (void *)d_leaves, (void *)d_internal})
+            if (p) (void)hipFree(p);
+        d_nodes = nullptr; d_mom = d_pos = d_qp = nullptr; d_ord = d_leaves = d_internal = nullptr;
+        valid = false;
+    }
+    ~Tree() {
+        free_tree();
+        for (void *p : {(void *)d_counts, (void *)d_offsets, (void *)d_meta, (void *)d_idx, (void *)d_q,
+                        (void *)d_cached})
+            if (p) (void)hipFree(p);
+        if (h_meta) (void)hipHostFree(h_meta);
+    }
+};
+
+namespace {
+struct HostTreeBuilder {
+    const std::vector<double> &pos;   // 3 per charge (raster rank)
+    int leaf, max_depth;
+    std::vector<int> ord, tmp;
+    std::vector<TNode> nodes;
+    std::vector<int> leaves;
+    std::vector<std::vector<int>> internal;   // by depth
+
+    HostTreeBuilder(const std::vector<double> &p, int l, int md) : pos(p), leaf(l), max_depth(md) {}
+
+    static int octant(const double c[3], const double *p) {
+        int o = 0;
+        if (p[0] >= c[0]) o |= 1;
+        if (p[1] >= c[1]) o |= 2;
+        if (p[2] >= c[2]) o |= 4;
+        return o;
+    }
+
+    void emit(int lo, int hi, const double c[3], double half, int depth) {
+        const int id = (int)nodes.size();
+        nodes.push_back(TNode{c[0], c[1], c[2], half, -1, depth, 0, 0});
+        if (depth >= max_depth || hi - lo <= leaf) {
+            nodes[id].start = lo;
+            nodes[id].count = hi - lo;
+            leaves.push_back(id);
+        } else {
+            int cnt[8] = {0}, at[8];
+            for (int r = lo; r < hi; ++r) cnt[octant(c, &pos[3 * (size_t)ord[r]])]++;
+            at[0] = lo;
+            for (int o = 1; o < 8; ++o) at[o] = at[o - 1] + cnt[o - 1];
+            for (int r = lo; r < hi; ++r) tmp[at[octant(c, &pos[3 * (size_t)ord[r]])]++] = ord[r];
+            std::copy(tmp.begin() + lo, tmp.begin() + hi, ord.begin() + lo);
+            if ((int)internal.size() <= depth) internal.resize(depth + 1);
+            internal[depth].push_back(id);
+            const double ch = half * 0.5;
+            int b = lo;
+            for (int o = 0; o < 8; ++o) {
+                if (cnt[o] == 0) continue;
+                const double cc[3] = {c[0] + ((o & 1) ? ch : -ch), c[1] + ((o & 2) ? ch : -ch),
+                                      c[2] + ((o & 4) ? ch : -ch)};
+                emit(b, b + cnt[o], cc, ch, depth + 1);
+                b += cnt[o];
+            }
+        }
+        nodes[id].skip = (int)nodes.size();
+    }
+};
+
+template <typename P>
+void dev_alloc(P *&p, size_t count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    LLGMI_HIP(hipMalloc((void **)&p, sizeof(P) * std::max<size_t>(count, 1)));
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host: solver
+// ---------------------------------------------------------------------------
+MGSolver::MGSolver(const llgmi_grid &g, const llgmi_mg_config &cfg_in, hipStream_t stream, Profiler *prof)
+    : g_(g), cfg_(cfg_in), stream_(stream), prof_(prof) {
+    mg_validate(cfg_);
+    mg_sanitize(cfg_);
+    try {
+        // new_with_operator (:1425-1577)
+        const int64_t nx = std::max<int64_t>(g.nx, 1), ny = std::max<int64_t>(g.ny, 1);
+        const int64_t pad = std::max<int64_t>(cfg_.pad_xy, 1);
+        int64_t px = nx + 2 * pad, py = ny + 2 * pad;
+        if (px % 2 == 1) ++px;
+        if (py % 2 == 1) ++py;
+        const int64_t nvac = std::max<int64_t>(cfg_.n_vac_z, 1);
+        int64_t pz = 1 + 2 * nvac;
+        if (pz % 2 == 1) ++pz;
+        if (pz % 4 != 0) pz = ((pz + 3) / 4) * 4;
+        if (px * py * pz >= ((int64_t)1 << 31)) throw Error(3, "multigrid box beyond 2^31 cells");
+        px_ = (int)px; py_ = (int)py; pz_ = (int)pz;
+        offx_ = (int)((px - nx) / 2);
+        offy_ = (int)((py - ny) / 2);
+        offz_ = (int)nvac;
+        int lx = px_, ly = py_, lz = pz_;
+        double dx = g.dx, dy = g.dy, dz = g.dz;
+        lev_.push_back(Level{lx, ly, lz, dx, dy, dz});
+        for (;;) {
+            const bool cxy = lx >= 8 && ly >= 8 && lx % 2 == 0 && ly % 2 == 0;
+            const bool cz = lz >= 8 && lz % 2 == 0;
+            if (!cxy && !cz) break;
+            if (cxy) { lx /= 2; ly /= 2; dx *= 2.0; dy *= 2.0; }
+            if (cz) { lz /= 2; dz *= 2.0; }
+            lev_.push_back(Level{lx, ly, lz, dx, dy, dz});
+            if (lev_.size() > 32) break;
+        }
+        lev_[0].st = mg_stencil_from_kind(cfg_.stencil, lev_[0].dx, lev_[0].dy, lev_[0].dz, cfg_.iso27_alpha);
+        for (size_t l = 1; l < lev_.size(); ++l) {
+            Level &F = lev_[l - 1], &C = lev_[l];
+            if (cfg_.coarse_op == 0) {
+                C.st = mg_stencil_from_kind(cfg_.stencil, C.dx, C.dy, C.dz, cfg_.iso27_alpha);
+            } else {
+                C.st = mg_galerkin(F.st, F.nx / C.nx, F.ny / C.ny, F.nz / C.nz, cfg_.prolong == 1);
+                if (C.st.diag <= 0.0 || C.st.offs.empty())
+                    C.st = mg_stencil_from_kind(cfg_.stencil, C.dx, C.dy, C.dz, cfg_.iso27_alpha);
+            }
+        }
+        for (auto &L : lev_) {
+            if (L.st.coeffs.size() > (size_t)kMaxSt) throw Error(3, "multigrid stencil too wide");
+            const size_t n = (size_t)L.nx * L.ny * L.nz;
+            LLGMI_HIP(hipMalloc((void **)&L.phi[0], sizeof(double) * n));
+            LLGMI_HIP(hipMalloc((void **)&L.rhs, sizeof(double) * n));
+            LLGMI_HIP(hipMemsetAsync(L.phi[0], 0, sizeof(double) * n, stream_));
+            LLGMI_HIP(hipMemsetAsync(L.rhs, 0, sizeof(double) * n, stream_));
+            DevStencil h{};
+            h.center = L.st.center;
+            h.diag = L.st.diag;
+            h.n = (int)L.st.coeffs.size();
+            for (int q = 0; q < h.n; ++q) {
+                h.c[q] = L.st.coeffs[q];
+                for (int a = 0; a < 3; ++a) h.off[q][a] = L.st.offs[q][a];
+            }
+            LLGMI_HIP(hipMalloc(&L.d_st, sizeof(DevStencil)));
+            LLGMI_HIP(hipMemcpy(L.d_st, &h, sizeof(DevStencil), hipMemcpyHostToDevice));
+        }
+        const size_t n0 = (size_t)px_ * py_ * pz_;
+        LLGMI_HIP(hipMalloc((void **)&bc_, sizeof(double) * n0));
+        LLGMI_HIP(hipMemsetAsync(bc_, 0, sizeof(double) * n0, stream_));
+        LLGMI_HIP(hipMalloc((void **)&d_partial_, sizeof(double) * 4 * kRedBlocks));
+        LLGMI_HIP(hipMalloc((void **)&d_scalars_, sizeof(double) * 8));
+        LLGMI_HIP(hipHostMalloc((void **)&h_scalars_, sizeof(double) * 8, hipHostMallocDefault));
+        tree_ = new Tree();
+        LLGMI_HIP(hipHostMalloc((void **)&tree_->h_meta, sizeof(int) * 2, hipHostMallocDefault));
+        LLGMI_HIP(hipMalloc((void **)&tree_->d_meta, sizeof(int) * 2));
+        LLGMI_HIP(hipStreamSynchronize(stream_));
+    } catch (...) {
+        release();
+        throw;
+    }
+    stats_.levels = (int32_t)lev_.size();
+    stats_.box[0] = px_;
+    stats_.box[1] = py_;
+    stats_.box[2] = pz_;
+    stats_.last_residual = -1.0;
+}
+
+MGSolver::~MGSolver() { release(); }
+
+void MGSolver::release() {
+    for (auto &L : lev_) {
+        for (double *p : {L.phi[0], L.phi[1], L.rhs})
+            if (p) (void)hipFree(p);
+        if (L.d_st) (void)hipFree(L.d_st);
+        L.phi[0] = L.phi[1] = L.rhs = nullptr;
+        L.d_st = nullptr;
+    }
+    lev_.clear();
+    if (bc_) (void)hipFree(bc_);
+    if (d_partial_) (void)hipFree(d_partial_);
+    if (d_scalars_) (void)hipFree(d_scalars_);
+    if (h_scalars_) (void)hipHostFree(h_scalars_);
+    bc_ = d_partial_ = d_scalars_ = h_scalars_ = nullptr;
+    delete tree_;
+    tree_ = nullptr;
+}
+
+// same_structure (:1585-1593): grid, padding and operator
+bool MGSolver::same_structure(const llgmi_grid &g, const llgmi_mg_config &c) const {
+    const bool alpha_same = cfg_.iso27_alpha == c.iso27_alpha || (!(cfg_.iso27_alpha > 0.0) && !(c.iso27_alpha > 0.0));
+    return g_.nx == g.nx && g_.ny == g.ny && g_.dx == g.dx && g_.dy == g.dy && g_.dz == g.dz &&
+           cfg_.pad_xy == c.pad_xy && cfg_.n_vac_z == c.n_vac_z && cfg_.stencil == c.stencil &&
+           cfg_.prolong == c.prolong && cfg_.coarse_op == c.coarse_op && alpha_same;
+}
+
+// apply_cfg (:1579-1583): runtime knobs; the operator stays as built
+void MGSolver::apply_cfg(const llgmi_mg_config &c) {
+    mg_validate(c);
+    llgmi_mg_config k = c;
+    k.stencil = cfg_.stencil;
+    k.prolong = cfg_.prolong;
+    k.coarse_op = cfg_.coarse_op;
+    k.iso27_alpha = cfg_.iso27_alpha;
+    mg_sanitize(k);
+    cfg_ = k;
+}
+
+void MGSolver::build_rhs(const void *m, bool f32, double ms) {
+    RhsGeom rg{px_, py_, pz_, offx_, offy_, offz_, (int)g_.nx, (int)g_.ny, lev_[0].dx, lev_[0].dy, lev_[0].dz, ms};
+    const int k_lo = std::max(1, offz_ - 1), k_hi = std::min(pz_ - 2, offz_ + 1);
+    const int nk = k_hi - k_lo + 1;
+    const int64_t n = (int64_t)(px_ - 2) * (py_ - 2) * nk;
+    const int64_t nm = g_.nx * g_.ny;
+    if (f32)
+        hipLaunchKernelGGL(k_mg_rhs<float>, dim3(nblocks(n)), dim3(kNT), 0, stream_, (const float *)m, nm, rg,
+                           k_lo, nk, lev_[0].rhs);
+    else
+        hipLaunchKernelGGL(k_mg_rhs<double>, dim3(nblocks(n)), dim3(kNT), 0, stream_, (const double *)m, nm, rg,
+                           k_lo, nk, lev_[0].rhs);
+    LLGMI_LAUNCH_CHECK();
+}
+
+void MGSolver::stamp(int l) {
+    Level &L = lev_[l];
+    hipLaunchKernelGGL(k_mg_stamp, dim3(nblocks(boundary_count(L.nx, L.ny, L.nz))), dim3(kNT), 0, stream_,
+                       L.phi[L.cur], l == 0 ? bc_ : nullptr, L.nx, L.ny, L.nz);
+    LLGMI_LAUNCH_CHECK();
+}
+
+void MGSolver::treecode_bc() {
+    Tree &T = *tree_;
+    Level &L = lev_[0];
+    const int k_lo = std::max(1, offz_ - 1), k_hi = std::min(pz_ - 2, offz_ + 1);
+    const int64_t ncand = (int64_t)(px_ - 2) * (py_ - 2) * (k_hi - k_lo + 1);
+    const int64_t nb = nblocks(ncand);
+    if (nb > T.cap_blocks) {
+        dev_alloc(T.d_counts, (size_t)nb);
+        dev_alloc(T.d_offsets, (size_t)nb);
+        T.cap_blocks = nb;
+    }
+    if (ncand > T.cap_cand) {
+        dev_alloc(T.d_idx, (size_t)ncand);
+        dev_alloc(T.d_q, (size_t)ncand);
+        T.cap_cand = ncand;
+    }
+    const double dvol = L.dx * L.dy * L.dz;
+    hipLaunchKernelGGL(k_tc_count, dim3((unsigned)nb), dim3(kNT), 0, stream_, L.rhs, px_, py_, k_lo, ncand,
+                       T.d_counts);
+    LLGMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tc_scan, dim3(1), dim3(1024), 0, stream_, T.d_counts, (int)nb, T.d_offsets, T.d_meta);
+    LLGMI_LAUNCH_CHECK();
+    LLGMI_HIP(hipMemsetAsync(T.d_meta, 0, sizeof(int), stream_));
+    hipLaunchKernelGGL(k_tc_write, dim3((unsigned)nb), dim3(kNT), 0, stream_, L.rhs, px_, py_, k_lo, ncand,
+                       T.d_offsets, dvol, T.d_idx, T.d_q, T.d_cached, T.valid ? (int)T.n : 0, T.d_meta);
+    LLGMI_LAUNCH_CHECK();
+    LLGMI_HIP(hipMemcpyAsync(T.h_meta, T.d_meta, sizeof(int) * 2, hipMemcpyDeviceToHost, stream_));
+    LLGMI_HIP(hipStreamSynchronize(stream_));
+    const int64_t n = T.h_meta[1];
+    stats_.n_charges = (int32_t)n;
+    stats_.tree_rebuilt = 0;
+    if (n == 0) {   // evaluate_treecode_bc returns with bc = 0
+        LLGMI_HIP(hipMemsetAsync(bc_, 0, sizeof(double) * (size_t)px_ * py_ * pz_, stream_));
+        stamp(0);
+        return;
+    }
+    const bool rebuild = !T.valid || T.h_meta[0] != 0 || n != T.n || T.leaf != (int)cfg_.tree_leaf ||
+                         T.max_depth != (int)cfg_.tree_max_depth;
+    if (rebuild) {
+        std::vector<int> idx((size_t)n);
+        LLGMI_HIP(hipMemcpy(idx.data(), T.d_idx, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+        // charge positions, build_charges_from_rhs (mg_treecode.rs:280-296)
+        const double cx = (double)px_ * 0.5, cy = (double)py_ * 0.5, cz = (double)pz_ * 0.5;
+        std::vector<double> pos(3 * (size_t)n);
+        for (int64_t a = 0; a < n; ++a) {
+            const int64_t cell = idx[(size_t)a];
+            const int64_t i = cell % px_, j = (cell / px_) % py_, k = cell / ((int64_t)px_ * py_);
+            pos[3 * a] = ((double)i + 0.5 - cx) * L.dx;
+            pos[3 * a + 1] = ((double)j + 0.5 - cy) * L.dy;
+            pos[3 * a + 2] = ((double)k + 0.5 - cz) * L.dz;
+        }
+        const double lx = (double)px_ * L.dx, ly = (double)py_ * L.dy, lz = (double)pz_ * L.dz;
+        const double root_half = 0.5 * std::fmax(std::fmax(lx, ly), lz) + 1e-12;
+        HostTreeBuilder b(pos, (int)std::min<int64_t>(cfg_.tree_leaf, INT32_MAX), (int)cfg_.tree_max_depth);
+        b.ord.resize((size_t)n);
+        b.tmp.resize((size_t)n);
+        for (int64_t a = 0; a < n; ++a) b.ord[(size_t)a] = (int)a;
+        const double rc[3] = {0.0, 0.0, 0.0};
+        b.emit(0, (int)n, rc, root_half, 0);
+        std::vector<double> pos_leaf(3 * (size_t)n);
+        for (int64_t a = 0; a < n; ++a)
+            for (int c = 0; c < 3; ++c) pos_leaf[3 * a + c] = pos[3 * (size_t)b.ord[(size_t)a] + c];
+        std::vector<int> internal;
+        T.internal_by_depth.clear();
+        for (auto &lvl : b.internal) {
+            T.internal_by_depth.push_back({(int)internal.size(), (int)lvl.size()});
+            internal.insert(internal.end(), lvl.begin(), lvl.end());
+        }
+        T.free_tree();
+        T.nnodes = (int)b.nodes.size();
+        T.nleaves = (int)b.leaves.size();
+        dev_alloc(T.d_nodes, b.nodes.size());
+        dev_alloc(T.d_mom, 4 * b.nodes.size());
+        dev_alloc(T.d_pos, pos_leaf.size());
+        dev_alloc(T.d_qp, (size_t)n);
+        dev_alloc(T.d_ord, (size_t)n);
+        dev_alloc(T.d_leaves, b.leaves.size());
+        dev_alloc(T.d_internal, internal.size());
+        LLGMI_HIP(hipMemcpy(T.d_nodes, b.nodes.data(), sizeof(TNode) * b.nodes.size(), hipMemcpyHostToDevice));
+        LLGMI_HIP(hipMemcpy(T.d_pos, pos_leaf.data(), sizeof(double) * pos_leaf.size(), hipMemcpyHostToDevice));
+        LLGMI_HIP(hipMemcpy(T.d_ord, b.ord.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+        LLGMI_HIP(hipMemcpy(T.d_leaves, b.leaves.data(), sizeof(int) * b.leaves.size(), hipMemcpyHostToDevice));
+        if (!internal.empty())
+            LLGMI_HIP(hipMemcpy(T.d_internal, internal.data(), sizeof(int) * internal.size(), hipMemcpyHostToDevice));
+        if (n > T.cap_cached) {
+            dev_alloc(T.d_cached, (size_t)n);
+            T.cap_cached = n;
+        }
+        LLGMI_HIP(hipMemcpyAsync(T.d_cached, T.d_idx, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, stream_));
+        T.n = n;
+        T.leaf = (int)cfg_.tree_leaf;
+        T.max_depth = (int)cfg_.tree_max_depth;
+        T.valid = true;
+        stats_.tree_rebuilt = 1;
+    }
+    hipLaunchKernelGGL(k_tc_gather, dim3(nblocks(n)), dim3(kNT), 0, stream_, T.d_q, T.d_ord, (int)n, T.d_qp);
+    LLGMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tc_leaf_moments, dim3(nblocks(T.nleaves)), dim3(kNT), 0, stream_, T.d_nodes, T.d_leaves,
+                       T.nleaves, T.d_pos, T.d_qp, T.d_mom);
+    LLGMI_LAUNCH_CHECK();
+    for (int d = (int)T.internal_by_depth.size() - 1; d >= 0; --d) {
+        const auto od = T.internal_by_depth[(size_t)d];
+        if (od.second == 0) continue;
+        hipLaunchKernelGGL(k_tc_node_moments, dim3(nblocks(od.second)), dim3(kNT), 0, stream_, T.d_nodes,
+                           T.d_internal + od.first, od.second, T.d_mom);
+        LLGMI_LAUNCH_CHECK();
+    }
+    BoxGeom bg{px_, py_, pz_, L.dx, L.dy, L.dz, (double)px_ * 0.5, (double)py_ * 0.5, (double)pz_ * 0.5};
+    hipLaunchKernelGGL(k_tc_eval, dim3(nblocks(boundary_count(px_, py_, pz_))), dim3(kNT), 0, stream_, T.d_nodes,
+                       T.nnodes, T.d_mom, T.d_pos, T.d_qp, bg, cfg_.tree_theta, bc_, L.phi[L.cur]);
+    LLGMI_LAUNCH_CHECK();
+}
+
+void MGSolver::dipole_bc() {
+    Level &L = lev_[0];
+    const int k_lo = std::max(1, offz_ - 1), k_hi = std::min(pz_ - 2, offz_ + 1);
+    BoxGeom bg{px_, py_, pz_, L.dx, L.dy, L.dz, (double)px_ * 0.5, (double)py_ * 0.5, (double)pz_ * 0.5};
+    const double dvol = L.dx * L.dy * L.dz;
+    hipLaunchKernelGGL(k_dip_partial, dim3(kRedBlocks), dim3(kNT), 0, stream_, L.rhs, bg, k_lo, k_hi - k_lo + 1,
+                       dvol, d_partial_);
+    LLGMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dip_final, dim3(1), dim3(64), 0, stream_, d_partial_, kRedBlocks, d_scalars_ + 2);
+    LLGMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dip_eval, dim3(nblocks(boundary_count(px_, py_, pz_))), dim3(kNT), 0, stream_,
+                       d_scalars_ + 2, bg, bc_, L.phi[L.cur]);
+    LLGMI_LAUNCH_CHECK();
+}
+
+// update_finest_boundary_bc (:1804-1833): boundary values, then the stamp
+void MGSolver::update_bc() {
+    ProfScope ps(prof_, LLGMI_K_MG_BC, stream_);
+    stats_.n_charges = 0;
+    stats_.tree_rebuilt = 0;
+    if (cfg_.bc == 0) {
+        LLGMI_HIP(hipMemsetAsync(bc_, 0, sizeof(double) * (size_t)px_ * py_ * pz_, stream_));
+        stamp(0);
+    } else if (cfg_.bc == 1) {
+        dipole_bc();
+    } else {
+        treecode_bc();
+    }
+}
+
+void MGSolver::smooth(int l, int64_t iters) {
+    Level &L = lev_[l];
+    if (cfg_.smoother == 1) {
+        const double sx = 1.0 / (L.dx * L.dx), sy = 1.0 / (L.dy * L.dy), sz = 1.0 / (L.dz * L.dz);
+        const double denom = 2.0 * (sx + sy + sz);
+        const int64_t n = (int64_t)(L.ny - 2) * (L.nz - 2) * ((L.nx - 1) / 2);
+        for (int64_t it = 0; it < iters; ++it)
+            for (int color = 0; color < 2; ++color) {
+                hipLaunchKernelGGL(k_mg_rb, dim3(nblocks(n)), dim3(kNT), 0, stream_, L.phi[L.cur], L.rhs, L.nx,
+                                   L.ny, L.nz, sx, sy, sz, denom, cfg_.sor_omega, color);
+                LLGMI_LAUNCH_CHECK();
+            }
+        return;
+    }
+    if (!L.phi[1]) {
+        const size_t n = (size_t)L.nx * L.ny * L.nz;
+        LLGMI_HIP(hipMalloc((void **)&L.phi[1], sizeof(double) * n));
+        LLGMI_HIP(hipMemsetAsync(L.phi[1], 0, sizeof(double) * n, stream_));
+    }
+    const int64_t n = (int64_t)L.nx * L.ny * L.nz;
+    for (int64_t it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL(k_mg_wj, dim3(nblocks(n)), dim3(kNT), 0, stream_, L.phi[L.cur], L.phi[1 - L.cur], L.rhs,
+                           l == 0 ? bc_ : nullptr, (const DevStencil *)L.d_st, cfg_.omega, L.nx, L.ny, L.nz);
+        LLGMI_LAUNCH_CHECK();
+        L.cur = 1 - L.cur;
+    }
+}
+
+// v_cycle (:1877-1945)
+void MGSolver::v_cycle(int l) {
+    if (l == (int)lev_.size() - 1) {
+        smooth(l, 80);
+        return;
+    }
+    smooth(l, cfg_.pre_smooth);
+    Level &F = lev_[l], &C = lev_[l + 1];
+    const int rx = F.nx / C.nx, ry = F.ny / C.ny, rz = F.nz / C.nz;
+    const int64_t nc = (int64_t)(C.nx - 2) * (C.ny - 2) * (C.nz - 2);
+    hipLaunchKernelGGL(k_mg_resid_restrict, dim3(nblocks(nc)), dim3(kNT), 0, stream_, F.phi[F.cur], F.rhs,
+                       (const DevStencil *)F.d_st, F.nx, F.ny, F.nz, C.rhs, C.phi[C.cur], C.nx, C.ny, C.nz, rx, ry,
+                       rz, 1.0 / (double)(rx * ry * rz));
+    LLGMI_LAUNCH_CHECK();
+    v_cycle(l + 1);
+    const int64_t nf = (int64_t)(F.nx - 2) * (F.ny - 2) * (F.nz - 2);
+    hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(nf)), dim3(kNT), 0, stream_, C.phi[C.cur], C.nx, C.ny, C.nz,
+                       F.phi[F.cur], F.nx, F.ny, F.nz, cfg_.prolong == 1 ? 1 : 0);
+    LLGMI_LAUNCH_CHECK();
+    smooth(l, cfg_.post_smooth);
+}
+
+double MGSolver::finest_rhs_max() {
+    const int k_lo = std::max(1, offz_ - 1), k_hi = std::min(pz_ - 2, offz_ + 1);
+    hipLaunchKernelGGL(k_mg_rhs_absmax, dim3(kRedBlocks), dim3(kNT), 0, stream_, lev_[0].rhs, px_, py_, k_lo,
+                       k_hi - k_lo + 1, d_partial_);
+    LLGMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mg_final_max, dim3(1), dim3(kNT), 0, stream_, d_partial_, kRedBlocks, d_scalars_);
+    LLGMI_LAUNCH_CHECK();
+    LLGMI_HIP(hipMemcpyAsync(h_scalars_, d_scalars_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LLGMI_HIP(hipStreamSynchronize(stream_));
+    return h_scalars_[0];
+}
+
+// solve_with_timing (:1947-2057)
+void MGSolver::solve() {
+    Level &L = lev_[0];
+    const size_t n0 = (size_t)L.nx * L.ny * L.nz;
+    if (!cfg_.warm_start) LLGMI_HIP(hipMemsetAsync(L.phi[L.cur], 0, sizeof(double) * n0, stream_));
+    update_bc();   // ends with the boundary stamped (enforce_dirichlet)
+    ProfScope ps(prof_, LLGMI_K_MG, stream_);
+    const bool use_tol = cfg_.has_tol_abs || cfg_.has_tol_rel;
+    const double rhs_max = (use_tol && cfg_.has_tol_rel) ? finest_rhs_max() : 0.0;
+    double tol_target = 0.0;
+    if (use_tol) {
+        if (cfg_.has_tol_abs) tol_target = std::fmax(tol_target, std::fmax(cfg_.tol_abs, 0.0));
+        if (cfg_.has_tol_rel) tol_target = std::fmax(tol_target, std::fmax(std::fmax(cfg_.tol_rel, 0.0) * rhs_max, 0.0));
+    }
+    const int64_t min_cycles = use_tol ? std::max<int64_t>(cfg_.v_cycles, 1) : 0;
+    const int64_t max_cycles = use_tol ? std::max<int64_t>(std::max<int64_t>(cfg_.v_cycles_max, min_cycles), 1)
+                                       : std::max<int64_t>(cfg_.v_cycles, 1);
+    stats_.rhs_max = rhs_max;
+    stats_.tol_target = tol_target;
+    stats_.last_residual = -1.0;
+    stats_.diverged = 0;
+    stats_.cycles = 0;
+    const int64_t n_nan = std::min<int64_t>(2 * (int64_t)L.nx * L.ny, (int64_t)n0);
+    for (int64_t iter = 0; iter < max_cycles; ++iter) {
+        v_cycle(0);
+        // enforce_dirichlet: the smoothers leave the boundary cells at bc
+        stats_.cycles = iter + 1;
+        const bool nan_check = iter % 2 == 1 || iter + 1 == max_cycles;
+        const bool res_check = use_tol && iter + 1 >= min_cycles;
+        if (!nan_check && !res_check) continue;
+        if (nan_check) {
+            LLGMI_HIP(hipMemsetAsync(d_scalars_ + 1, 0, sizeof(double), stream_));
+            hipLaunchKernelGGL(k_mg_nan, dim3(nblocks(n_nan)), dim3(kNT), 0, stream_, L.phi[L.cur], n_nan,
+                               d_scalars_ + 1);
+            LLGMI_LAUNCH_CHECK();
+        }
+        if (res_check) {
+            hipLaunchKernelGGL(k_mg_resid_max, dim3(kRedBlocks), dim3(kNT), 0, stream_, L.phi[L.cur], L.rhs,
+                               (const DevStencil *)L.d_st, L.nx, L.ny, L.nz, d_partial_);
+            LLGMI_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_mg_final_max, dim3(1), dim3(kNT), 0, stream_, d_partial_, kRedBlocks, d_scalars_);
+            LLGMI_LAUNCH_CHECK();
+        }
+        LLGMI_HIP(hipMemcpyAsync(h_scalars_, d_scalars_, sizeof(double) * 2, hipMemcpyDeviceToHost, stream_));
+        LLGMI_HIP(hipStreamSynchronize(stream_));
+        if (nan_check && h_scalars_[1] != 0.0) {
+            LLGMI_HIP(hipMemsetAsync(L.phi[L.cur], 0, sizeof(double) * n0, stream_));
+            stamp(0);
+            stats_.diverged = 1;
+            break;
+        }
+        if (res_check) {
+            stats_.last_residual = h_scalars_[0];
+            if (h_scalars_[0] <= tol_target) break;
+        }
+    }
+}
+
+template <typename T>
+void MGSolver::gradient(const T *m, T *bd) {
+    const Level &L = lev_[0];
+    const int64_t n = g_.nx * g_.ny;
+    hipLaunchKernelGGL(k_mg_gradient<T>, dim3(nblocks(n)), dim3(kNT), 0, stream_, L.phi[L.cur], px_, py_, pz_, L.dx,
+                       L.dy, L.dz, offx_, offy_, offz_, (int)g_.nx, (int)g_.ny, m, bd);
+    LLGMI_LAUNCH_CHECK();
+}
+
+// DemagPoissonMG::add_field (:2094-2098): rhs, solve, gradient
+template <typename T>
+void MGSolver::demag_field(const T *m, double ms, T *bd) {
+    {
+        ProfScope ps(prof_, LLGMI_K_MG, stream_);
+        build_rhs(m, std::is_same<T, float>::value, ms);
+    }
+    solve();
+    ProfScope ps(prof_, LLGMI_K_MG, stream_);
+    gradient(m, bd);
+}
+
+template void MGSolver::demag_field<float>(const float *, double, float *);
+template void MGSolver::demag_field<double>(const double *, double, double *);
+
+}  // namespace llgmi

@@ -72,6 +72,8 @@ struct Engine : EngineBase {
 
     DemagFFT<T> fft;
     std::unique_ptr<MGSolver> mg;
+    bool has_mg_cfg = false;       // llgmi_ctx_set_mg_config; otherwise the environment per call
+    llgmi_mg_config mg_cfg{};
 
     // RK45 scratch (RK45Scratch, llg.rs:126-189): k1..k7, stage state, m7.
     T *d_k[8] = {nullptr};
@@ -108,8 +110,15 @@ struct Engine : EngineBase {
         if (!fft.built) fft.build(g.nx, g.ny, g.dx, g.dy, g.dz, stream);
     }
 
+    // add_demag_field_poisson_mg (demag_poisson_mg.rs:3612-3632): configuration
+    // per call, cached solver rebuilt on a structure change, runtime knobs applied.
     MGSolver &ensure_mg() {
-        if (!mg) mg.reset(new MGSolver(g.nx, g.ny, g.dx, g.dy, g.dz, stream, prof));
+        llgmi_mg_config c;
+        if (has_mg_cfg) c = mg_cfg;
+        else mg_config_from_env(c);
+        if (mg && !mg->same_structure(g, c)) mg.reset();
+        if (!mg) mg.reset(new MGSolver(g, c, stream, prof));
+        mg->apply_cfg(c);
         return *mg;
     }
 
@@ -197,11 +206,11 @@ struct Engine : EngineBase {
     }
 
     void host_add_demag(const double *m, double *b, const llgmi_material &mat, uint64_t pbc_x,
-                        uint64_t pbc_y) {
+                        uint64_t pbc_y, int method = -1) {
         if (!mat.demag) return;
         ensure_io();
         upload(m, d_m);
-        demag_into_bd(d_m, mat, resolved_demag_method(mat), pbc_x, pbc_y);
+        demag_into_bd(d_m, mat, method >= 0 ? method : resolved_demag_method(mat), pbc_x, pbc_y);
         h_tmp.resize((size_t)(3 * n));
         download(d_bd, h_tmp.data());
         for (int64_t i = 0; i < 3 * n; ++i) b[i] += h_tmp[(size_t)i];
@@ -614,6 +623,57 @@ int llgmi_dev_download(llgmi_ctx *ctx, const void *dev_soa, double *host_aos) {
             using T = std::remove_pointer_t<decltype(eng.d_m)>;
             eng.ensure_io();
             eng.download((const T *)dev_soa, host_aos);
+        });
+    });
+}
+
+int llgmi_mg_config_default(llgmi_mg_config *cfg) {
+    if (!cfg) return fail(LLGMI_ERR_INVALID, "cfg is NULL");
+    mg_config_default(*cfg);
+    return LLGMI_OK;
+}
+
+int llgmi_mg_config_from_env(llgmi_mg_config *cfg) {
+    if (!cfg) return fail(LLGMI_ERR_INVALID, "cfg is NULL");
+    mg_config_from_env(*cfg);
+    return LLGMI_OK;
+}
+
+int llgmi_ctx_set_mg_config(llgmi_ctx *ctx, const llgmi_mg_config *cfg) {
+    return guarded(ctx, [&] {
+        if (cfg) mg_validate(*cfg);
+        with_engine(ctx, [&](auto &eng) {
+            eng.has_mg_cfg = cfg != nullptr;
+            if (cfg) eng.mg_cfg = *cfg;
+        });
+    });
+}
+
+int llgmi_add_demag_field_poisson_mg(llgmi_ctx *ctx, const double *m, double *b_eff,
+                                     const llgmi_material *mat) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!m || !b_eff) return fail(LLGMI_ERR_INVALID, "NULL field");
+    std::string e = check_mat(mat);
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) { eng.host_add_demag(m, b_eff, *mat, 0, 0, LLGMI_DEMAG_MG); });
+    });
+}
+
+int llgmi_compute_demag_field_poisson_mg(llgmi_ctx *ctx, const double *m, double *out,
+                                         const llgmi_material *mat) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!out) return fail(LLGMI_ERR_INVALID, "NULL field");
+    std::memset(out, 0, sizeof(double) * 3 * (size_t)(ctx->grid.nx * ctx->grid.ny));
+    return llgmi_add_demag_field_poisson_mg(ctx, m, out, mat);
+}
+
+int llgmi_mg_last_stats(llgmi_ctx *ctx, llgmi_mg_stats *out) {
+    if (!out) return fail(LLGMI_ERR_INVALID, "out is NULL");
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            if (!eng.mg) throw Error(LLGMI_ERR_INVALID, "no multigrid solve has run on this context");
+            *out = eng.mg->stats();
         });
     });
 }

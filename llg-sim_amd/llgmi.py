@@ -32,7 +32,7 @@ GAMMA_E_RAD_PER_S_T = 1.76085963023e11
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_ALLOC = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
 
-K_FFT_ROW_FWD, K_FFT_COL, K_FFT_ROW_INV, K_LOCAL_FIELD, K_RK45_STAGE, K_RK45_ERR, K_MG = range(7)
+K_FFT_ROW_FWD, K_FFT_COL, K_FFT_ROW_INV, K_LOCAL_FIELD, K_RK45_STAGE, K_RK45_ERR, K_MG, K_MG_BC = range(8)
 KERNEL_CLASS_NAMES = {
     K_FFT_ROW_FWD: "fft_row_fwd",
     K_FFT_COL: "fft_col_mul",
@@ -41,6 +41,7 @@ KERNEL_CLASS_NAMES = {
     K_RK45_STAGE: "rk45_stage",
     K_RK45_ERR: "rk45_err",
     K_MG: "mg",
+    K_MG_BC: "mg_bc",
 }
 
 
@@ -139,8 +140,63 @@ EXPORTED = [
     "llgmi_rk45_step", "llgmi_rk45_step_adaptive", "llgmi_rk45_reset",
     "llgmi_dev_build_h_eff", "llgmi_dev_add_demag_field_pbc", "llgmi_dev_rk45_step",
     "llgmi_dev_upload", "llgmi_dev_download", "llgmi_profile_enable", "llgmi_profile_reset",
-    "llgmi_profile_read",
+    "llgmi_profile_read", "llgmi_mg_config_default", "llgmi_mg_config_from_env",
+    "llgmi_ctx_set_mg_config", "llgmi_add_demag_field_poisson_mg",
+    "llgmi_compute_demag_field_poisson_mg", "llgmi_mg_last_stats",
 ]
+
+
+class MGConfig(C.Structure):
+    """llgmi_mg_config: DemagPoissonMGConfig (demag_poisson_mg.rs:115-185) with the
+    operator choice (MGOperatorSettings, :727-804)."""
+    _fields_ = [("pad_xy", C.c_int64), ("n_vac_z", C.c_int64), ("v_cycles", C.c_int64),
+                ("v_cycles_max", C.c_int64), ("has_tol_abs", C.c_int32), ("has_tol_rel", C.c_int32),
+                ("tol_abs", C.c_double), ("tol_rel", C.c_double), ("pre_smooth", C.c_int64),
+                ("post_smooth", C.c_int64), ("smoother", C.c_int32), ("warm_start", C.c_int32),
+                ("omega", C.c_double), ("sor_omega", C.c_double), ("bc", C.c_int32),
+                ("stencil", C.c_int32), ("tree_theta", C.c_double), ("tree_leaf", C.c_int64),
+                ("tree_max_depth", C.c_int64), ("prolong", C.c_int32), ("coarse_op", C.c_int32),
+                ("iso27_alpha", C.c_double)]
+
+
+class MGStats(C.Structure):
+    _fields_ = [("cycles", C.c_int64), ("rhs_max", C.c_double), ("tol_target", C.c_double),
+                ("last_residual", C.c_double), ("diverged", C.c_int32), ("n_charges", C.c_int32),
+                ("tree_rebuilt", C.c_int32), ("levels", C.c_int32), ("box", C.c_int64 * 3)]
+
+    def as_dict(self) -> dict:
+        return {"cycles": self.cycles, "rhs_max": self.rhs_max, "tol_target": self.tol_target,
+                "last_residual": self.last_residual, "diverged": bool(self.diverged),
+                "n_charges": self.n_charges, "tree_rebuilt": bool(self.tree_rebuilt),
+                "levels": self.levels, "box": tuple(self.box)}
+
+
+SMOOTHER_WJ, SMOOTHER_RB = 0, 1
+BC_ZERO, BC_DIPOLE, BC_TREECODE = 0, 1, 2
+STENCIL_7, STENCIL_ISO9, STENCIL_ISO27 = 0, 1, 2
+PROLONG_INJECT, PROLONG_TRILINEAR = 0, 1
+COARSE_REDISCRETIZE, COARSE_GALERKIN = 0, 1
+
+
+def mg_config(**kw) -> MGConfig:
+    """DemagPoissonMGConfig::default() with the default operator, fields from kw.
+    tol_abs / tol_rel = None switch that tolerance off."""
+    c = MGConfig()
+    _check(lib().llgmi_mg_config_default(C.byref(c)))
+    for k, v in kw.items():
+        if k in ("tol_abs", "tol_rel"):
+            setattr(c, "has_" + k, 0 if v is None else 1)
+            setattr(c, k, 0.0 if v is None else float(v))
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def mg_config_from_env() -> MGConfig:
+    """DemagPoissonMGConfig::from_env (:188-267) + MGOperatorSettings::from_env."""
+    c = MGConfig()
+    _check(lib().llgmi_mg_config_from_env(C.byref(c)))
+    return c
 
 
 def lib():
@@ -179,6 +235,12 @@ def lib():
     L.llgmi_profile_enable.argtypes = [vp, C.c_int]
     L.llgmi_profile_reset.argtypes = [vp]
     L.llgmi_profile_read.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_int64)]
+    L.llgmi_mg_config_default.argtypes = [C.POINTER(MGConfig)]
+    L.llgmi_mg_config_from_env.argtypes = [C.POINTER(MGConfig)]
+    L.llgmi_ctx_set_mg_config.argtypes = [vp, C.POINTER(MGConfig)]
+    L.llgmi_add_demag_field_poisson_mg.argtypes = [vp, dp, dp, M]
+    L.llgmi_compute_demag_field_poisson_mg.argtypes = [vp, dp, dp, M]
+    L.llgmi_mg_last_stats.argtypes = [vp, C.POINTER(MGStats)]
     _lib = L
     return L
 
@@ -278,6 +340,29 @@ class Context:
         b = np.zeros_like(m)
         self.add_demag_field_pbc(m, b, mat, 0, 0)
         return b
+
+    # ---- multigrid Poisson demag (demag_poisson_mg.rs) ----
+    def set_mg_config(self, cfg: Optional[MGConfig]) -> None:
+        """Pins the MG configuration; None reads LLG_DEMAG_MG_* at every call."""
+        _check(lib().llgmi_ctx_set_mg_config(self._h, C.byref(cfg) if cfg is not None else None))
+
+    def add_demag_field_poisson_mg(self, m, b: np.ndarray, mat: Material) -> None:
+        m = _f64(m).reshape(-1, 3)
+        assert b.dtype == np.float64 and b.flags.c_contiguous and b.size == m.size
+        cm = mat._c()
+        _check(lib().llgmi_add_demag_field_poisson_mg(self._h, _dp(m), _dp(b), C.byref(cm)))
+
+    def compute_demag_field_poisson_mg(self, m, mat: Material) -> np.ndarray:
+        m = _f64(m).reshape(-1, 3)
+        b = np.zeros_like(m)
+        cm = mat._c()
+        _check(lib().llgmi_compute_demag_field_poisson_mg(self._h, _dp(m), _dp(b), C.byref(cm)))
+        return b
+
+    def mg_last_stats(self) -> dict:
+        st = MGStats()
+        _check(lib().llgmi_mg_last_stats(self._h, C.byref(st)))
+        return st.as_dict()
 
     def step_llg_rk45_recompute_field_masked_adaptive(self, m: np.ndarray, params: LLGParams,
                                                       mat: Material, mask: int, max_err: float,

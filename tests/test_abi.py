@@ -61,3 +61,62 @@ def test_context_creation_reports_missing_device():
     with pytest.raises(llgmi.LlgmiError) as e:
         llgmi.Context(llgmi.Grid2D(4, 4, 1e-9, 1e-9, 1e-9))
     assert e.value.code == llgmi.ERR_HIP
+
+
+def test_mg_struct_layouts_and_defaults():
+    import llgmi
+    assert C.sizeof(llgmi.MGConfig) == 144
+    assert C.sizeof(llgmi.MGStats) == 72
+    c = llgmi.mg_config()
+    # DemagPoissonMGConfig::default() (demag_poisson_mg.rs:166-185) and the
+    # default operator iso27 / trilinear / Galerkin (:741-794)
+    assert (c.pad_xy, c.n_vac_z, c.v_cycles, c.v_cycles_max) == (6, 16, 4, 64)
+    assert (c.has_tol_abs, c.has_tol_rel, c.tol_rel) == (0, 1, 1e-6)
+    assert (c.pre_smooth, c.post_smooth, c.smoother, c.warm_start) == (2, 2, 0, 1)
+    assert c.omega == 2.0 / 3.0 and c.sor_omega == 1.0
+    assert (c.bc, c.tree_theta, c.tree_leaf, c.tree_max_depth) == (2, 0.6, 64, 20)
+    assert (c.stencil, c.prolong, c.coarse_op, c.iso27_alpha) == (2, 1, 1, 0.0)
+
+
+_MG_ENV = [k for k in ("PAD_XY", "PAD_FACTOR_XY", "NVAC_Z", "VCYCLES", "VCYCLES_MAX", "TOL_ABS", "TOL_REL",
+                       "PRE_SMOOTH", "POST_SMOOTH", "OMEGA", "SOR_OMEGA", "SMOOTHER", "WARM_START", "BC",
+                       "TREE_THETA", "TREE_LEAF", "TREE_MAX_DEPTH", "STENCIL", "PROLONG", "COARSE_OP",
+                       "ISO27_FLUX_ALPHA")]
+
+
+def test_mg_config_from_env_follows_reference_parsing(monkeypatch):
+    """DemagPoissonMGConfig::from_env (demag_poisson_mg.rs:188-267) and
+    MGOperatorSettings::from_env (:741-794): aliases, clamps, fallbacks."""
+    import llgmi
+    for k in _MG_ENV:
+        monkeypatch.delenv("LLG_DEMAG_MG_" + k, raising=False)
+    d = llgmi.mg_config_from_env()
+    assert bytes(d) == bytes(llgmi.mg_config())
+    env = {"PAD_FACTOR_XY": " 0 ", "NVAC_Z": "3", "VCYCLES": "0", "VCYCLES_MAX": "200",
+           "TOL_ABS": "-5", "TOL_REL": "3.5", "PRE_SMOOTH": "0", "POST_SMOOTH": "4",
+           "OMEGA": "9", "SOR_OMEGA": "0.01", "SMOOTHER": " RBGS ", "WARM_START": "on",
+           "BC": "Dipole", "TREE_THETA": "0.1", "TREE_LEAF": "2", "TREE_MAX_DEPTH": "100",
+           "STENCIL": "7pt", "PROLONG": "injection", "COARSE_OP": "rd", "ISO27_FLUX_ALPHA": "-1"}
+    for k, v in env.items():
+        monkeypatch.setenv("LLG_DEMAG_MG_" + k, v)
+    c = llgmi.mg_config_from_env()
+    assert (c.pad_xy, c.n_vac_z, c.v_cycles, c.v_cycles_max) == (1, 3, 1, 200)
+    assert (c.has_tol_abs, c.tol_abs, c.has_tol_rel, c.tol_rel) == (1, 0.0, 1, 1.0)
+    assert (c.pre_smooth, c.post_smooth) == (1, 4)
+    assert (c.omega, c.sor_omega) == (1.0, 0.2)
+    assert (c.smoother, c.warm_start, c.bc) == (1, 1, 1)
+    assert (c.tree_theta, c.tree_leaf, c.tree_max_depth) == (0.2, 8, 64)
+    assert (c.stencil, c.prolong, c.coarse_op, c.iso27_alpha) == (0, 0, 0, 0.0)
+    # PAD_XY wins over the legacy alias; unknown words fall back as the reference does
+    monkeypatch.setenv("LLG_DEMAG_MG_PAD_XY", "9")
+    monkeypatch.setenv("LLG_DEMAG_MG_SMOOTHER", "gauss")     # unknown: keeps the default
+    monkeypatch.setenv("LLG_DEMAG_MG_WARM_START", "TRUE")    # case-sensitive match: off
+    monkeypatch.setenv("LLG_DEMAG_MG_BC", "nonsense")        # unknown: keeps the default
+    monkeypatch.setenv("LLG_DEMAG_MG_STENCIL", "13pt")       # unknown: iso27
+    monkeypatch.setenv("LLG_DEMAG_MG_PROLONG", "cubic")      # unknown: trilinear
+    monkeypatch.setenv("LLG_DEMAG_MG_COARSE_OP", "x")        # unknown: Galerkin
+    monkeypatch.setenv("LLG_DEMAG_MG_ISO27_FLUX_ALPHA", "0.5")
+    monkeypatch.setenv("LLG_DEMAG_MG_VCYCLES", "abc")         # unparsable: default
+    c = llgmi.mg_config_from_env()
+    assert (c.pad_xy, c.smoother, c.warm_start, c.bc) == (9, 0, 0, 2)
+    assert (c.stencil, c.prolong, c.coarse_op, c.iso27_alpha, c.v_cycles) == (2, 1, 1, 0.5, 4)
