@@ -18,7 +18,14 @@ The JSON line also carries:
                 evaluations after an untimed warm-up that builds its kernel;
   rel_err_vs_fp64  FP32 against the FP64 GPU path on the same m, per term;
   kernels_us    per-kernel average durations from a separate event-timed pass;
-  rk45          accepted RK45 steps/s (FSAL, 6 field evaluations per step).
+  rk45          accepted RK45 steps/s (FSAL, 6 field evaluations per step);
+  host_aos      the same evaluation through the host entry point (AoS f64 in and
+                out, PCIe transfers included);
+  mg            the multigrid Poisson demag on a 512 x 512 film with a square
+                lattice of holes (3 nm cells, 7-point operator, red-black SOR,
+                injection, rediscretised coarse levels, treecode boundary,
+                tol_rel 1e-8): cold and warm-started solves on the GPU and the
+                CPU oracle's time for the same solves.
 """
 import argparse
 import json
@@ -40,20 +47,26 @@ BEXT = (-24.6e-3, 4.3e-3, 0.0)
 ALPHA = 0.3
 
 
+# HBM bytes per launch of the column pass at this workload, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE counters (profiles/field_eval_1024_fp32_pmc_traffic.md),
+# FETCH_SIZE scaled by 2.0 (a plain 256 MiB copy reads as 128 MiB).
+COL_TRAFFIC_BYTES = None
+
+
 def col_pass_bytes(nx, ny, elem):
     """Algorithmic bytes of one column-pass launch: read + write the half
-    spectrum (3 components x (px/2+1) columns x ny rows, complex) and read the
-    real quadrant-symmetric kernel ((px/2+1) x (py/2+1) x 4 reals)."""
+    spectrum (3 components x px/2 complex columns x ny rows: the real DC and
+    Nyquist columns share one complex column) and read the real
+    quadrant-symmetric kernel ((px/2+1) x (py/2+1) x 4 reals)."""
     px, py = 2 * nx, 2 * ny
     hx, hy = px // 2 + 1, py // 2 + 1
-    return 2 * 3 * hx * ny * 2 * elem + hx * hy * 4 * elem
+    return 2 * 3 * (px // 2) * ny * 2 * elem + hx * hy * 4 * elem
 
 
 def field_eval_bytes(nx, ny, elem):
     """Algorithmic bytes of the whole pipeline per field evaluation."""
     n = nx * ny
-    hx = nx + 1
-    spec = 3 * hx * ny * 2 * elem
+    spec = 3 * nx * ny * 2 * elem
     row_fwd = 3 * n * elem + spec
     row_inv = spec + 3 * n * elem
     local = 3 * 3 * n * elem
@@ -113,6 +126,82 @@ def cpu_baseline(m_host, n_eval=2):
                       f"one untimed warm-up that built the kernel ({t_build:.1f} s)"}
 
 
+MG_N = 512
+MG_CELL = (3e-9, 3e-9, 3e-9)
+MG_MS = 8.6e5
+
+
+def mg_workload():
+    """512 x 512 film, 3 nm cells, holes of radius 24 nm on a 96 nm square
+    lattice (m = 0 there), seeded smoothed-random m elsewhere."""
+    from initial_states import hole_lattice_mask, smoothed_random
+    mask = hole_lattice_mask(MG_N, MG_N, MG_CELL[0], MG_CELL[1], pitch=96e-9, radius=24e-9)
+    m = smoothed_random(MG_N, MG_N, seed=11).reshape(-1, 3)
+    m[~mask.reshape(-1)] = 0.0
+    return m
+
+
+MG_CFG = dict(stencil=0, prolong=0, coarse_op=0, smoother=1, bc=2, tol_rel=1e-8, v_cycles_max=300)
+
+
+def mg_bench(L, cpu: bool, reps: int = 5):
+    """Cold (phi = 0) and warm-started solves of the multigrid demag."""
+    m = mg_workload()
+    ctx = L.Context(L.Grid2D(MG_N, MG_N, *MG_CELL), precision=L.F64)
+    mat = L.Material(ms=MG_MS, demag=True)
+    out = {"workload": "512x512 film, 3 nm cells, hole lattice (pitch 96 nm, r 24 nm), box 524x524x36, "
+                       "7-point / red-black SOR / injection / rediscretised / treecode, tol_rel 1e-8"}
+    for warm in (0, 1):
+        ctx.set_mg_config(L.mg_config(warm_start=warm, **MG_CFG))
+        ctx.compute_demag_field_poisson_mg(m, mat)          # untimed: hierarchy, tree, first solve
+        ctx.profile_reset()
+        ctx.profile_enable([L.K_MG, L.K_MG_BC])
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.compute_demag_field_poisson_mg(m, mat)
+        dt = (time.perf_counter() - t0) / reps
+        ctx.profile_enable([])
+        st = ctx.mg_last_stats()
+        bc_ms, bc_n = ctx.profile_read(L.K_MG_BC)
+        key = "warm" if warm else "cold"
+        out[key] = {"solve_ms": round(1e3 * dt, 3), "v_cycles": st["cycles"],
+                    "boundary_ms": round(bc_ms / max(bc_n, 1), 3),
+                    "ms_per_v_cycle": round((1e3 * dt - bc_ms / max(bc_n, 1)) / max(st["cycles"], 1), 3),
+                    "final_residual_over_tol": round(st["last_residual"] / st["tol_target"], 4)}
+    ctx.close()
+    if cpu:
+        from oracle import oracle as O
+        g = O.Grid(MG_N, MG_N, *MG_CELL)
+        mg = O.MG(g, O.mg_config(warm_start=1, **MG_CFG))
+        b = np.zeros_like(m)
+        t0 = time.perf_counter()
+        mg.add_field(m, b, MG_MS)
+        t_cold = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        mg.add_field(m, b, MG_MS)
+        t_warm = time.perf_counter() - t0
+        out["cpu_oracle"] = {"cold_solve_s": round(t_cold, 3), "warm_solve_s": round(t_warm, 3),
+                             "cores": O.get_threads(), "kind": "port",
+                             "sample": "one cold and one warm-started solve of the same state"}
+        out["speedup_vs_cpu"] = {"cold": round(t_cold / (out["cold"]["solve_ms"] * 1e-3), 1),
+                                 "warm": round(t_warm / (out["warm"]["solve_ms"] * 1e-3), 1)}
+    return out
+
+
+def host_aos_rate(L, m_host, n=20):
+    """Field evaluations/s through the host entry point (AoS f64 in and out)."""
+    ctx = L.Context(L.Grid2D(NX, NY, *CELL), precision=L.F32)
+    p = L.LLGParams(alpha=ALPHA, b_ext=BEXT)
+    mat = L.Material(**MAT)
+    ctx.build_h_eff(m_host, p, mat)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        ctx.build_h_eff(m_host, p, mat)
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return round(n / dt, 1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +211,9 @@ def main():
     ap.add_argument("--rk45-steps", type=int, default=20)
     ap.add_argument("--quick", action="store_true",
                     help="field-evaluation loop only (no accuracy check, CPU leg or RK45); for profilers")
+    ap.add_argument("--no-mg", action="store_true", help="skip the multigrid leg")
+    ap.add_argument("--mg-only", action="store_true",
+                    help="run only the multigrid leg (GPU solves, no CPU oracle); for profilers")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 256 MiB (x10) to scale HBM byte counters")
     args = ap.parse_args()
@@ -141,6 +233,7 @@ def main():
         return
     if args.quick:
         args.no_cpu = True
+        args.no_mg = True
         args.rk45_steps = 0
 
     global np
@@ -148,6 +241,10 @@ def main():
     import torch
     import llgmi as L
     from initial_states import voronoi_domains
+
+    if args.mg_only:
+        print(json.dumps({"mg": mg_bench(L, cpu=False)}), flush=True)
+        return
 
     assert torch.cuda.is_available(), "no GPU visible to torch"
     m_host = voronoi_domains(NX, NY, seed=2048, n_points=2048, noise=0.1).reshape(-1, 3)
@@ -220,9 +317,15 @@ def main():
               "max_err": 1e-5, "last_dt": rp.dt}
 
     acc_rep = None
+    host_rate = None
     if not args.quick:
         acc_rep = accuracy_report(L, m_host)
         print(f"[bench] accuracy done: {acc_rep}", file=sys.stderr, flush=True)
+        host_rate = host_aos_rate(L, m_host)
+    mg = None
+    if not args.no_mg:
+        mg = mg_bench(L, cpu=not args.no_cpu)
+        print(f"[bench] multigrid: {mg}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(m_host)
@@ -245,7 +348,7 @@ def main():
                                "interfacial DMI+Zeeman+FFT demag (padded 2048^2), Voronoi +-z m0",
                    "nx": NX, "ny": NY, "precision": "fp32", "field_mask": "Full"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": COL_TRAFFIC_BYTES,
                      "kernel": "k_col_demag (column FFT x tensor x inverse column FFT)",
                      "bytes_per_launch": col_bytes, "avg_us": round(col_us, 2)},
         "cpu_baseline": cpu,
@@ -254,6 +357,8 @@ def main():
         "kernels_us": kernels,
         "rel_err_vs_fp64": acc_rep,
         "rk45": rk,
+        "host_aos_evals_per_s": host_rate,
+        "mg": mg,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

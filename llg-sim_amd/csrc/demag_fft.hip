@@ -146,7 +146,7 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
 template <typename T, int LOG2L, bool HALF, int RQ>
 __global__ void __launch_bounds__((ColCfg<T, LOG2L, RQ>::NT))
 k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
-            const quad4<T> *__restrict__ kern, T scale) {
+            const T *__restrict__ kern, T scale) {
     using C = ColCfg<T, LOG2L, RQ>;
     constexpr int L = C::L, SLEN = C::SLEN, TPS = C::TPS;
     constexpr int R0 = C::P::R0;
@@ -171,39 +171,42 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
 #pragma unroll
     for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
     __syncthreads();
+    // kernel planes [Kxx, Kxy, Kyy, Kzz][hxs + 1][HY]: the (x, y) blocks read
+    // three planes, the z blocks only Kzz
+    const int64_t plane = (int64_t)(hxs + 1) * HY;
     if (kx != 0) {
-        const quad4<T> *kc = kern + (int64_t)(active ? kx : 0) * HY;
+        const int64_t base = (int64_t)(active ? kx : 0) * HY;
         const cpx<T> *other = buf + (sidx ^ 1) * SLEN;
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int ky = t + r * TPS;
             const int kq = ky < HY ? ky : L - ky;
-            const quad4<T> K = kc[kq];
-            const T kxy = ky < HY ? K.xy : -K.xy;
             const cpx<T> own = v[r];
-            if (comp == 0) {
-                const cpx<T> my = other[lds_pad(ky)];
-                v[r] = {scale * (K.xx * own.x + kxy * my.x), scale * (K.xx * own.y + kxy * my.y)};
-            } else if (comp == 1) {
-                const cpx<T> mx = other[lds_pad(ky)];
-                v[r] = {scale * (kxy * mx.x + K.yy * own.x), scale * (kxy * mx.y + K.yy * own.y)};
+            if (comp == 2) {
+                const T kzz = kern[3 * plane + base + kq];
+                v[r] = {scale * (kzz * own.x), scale * (kzz * own.y)};
             } else {
-                v[r] = {scale * (K.zz * own.x), scale * (K.zz * own.y)};
+                const T kd = kern[(comp == 0 ? 0 : 2 * plane) + base + kq];
+                const T k01 = kern[plane + base + kq];
+                const T kxy = ky < HY ? k01 : -k01;
+                const cpx<T> oth = other[lds_pad(ky)];
+                if (comp == 0)
+                    v[r] = {scale * (kd * own.x + kxy * oth.x), scale * (kd * own.y + kxy * oth.y)};
+                else
+                    v[r] = {scale * (kxy * oth.x + kd * own.x), scale * (kxy * oth.y + kd * own.y)};
             }
         }
     } else {
         // packed column: Z = A + iB with A, B the transforms of the real kx = 0
         // and kx = px/2 columns; Kxy vanishes on both, so per component
         // Y = Ka A + i Kb B = ((Ka+Kb)/2) Z + ((Ka-Kb)/2) conj(Z[-ky])
-        const quad4<T> *ka = kern;
-        const quad4<T> *kb = kern + (int64_t)hxs * HY;
+        const T *kp = kern + (comp == 0 ? 0 : (comp == 1 ? 2 : 3)) * plane;
+        const T *kb = kp + (int64_t)hxs * HY;
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int ky = t + r * TPS;
             const int kq = ky < HY ? ky : L - ky;
-            const quad4<T> A = ka[kq], B = kb[kq];
-            const T a = comp == 0 ? A.xx : (comp == 1 ? A.yy : A.zz);
-            const T b = comp == 0 ? B.xx : (comp == 1 ? B.yy : B.zz);
+            const T a = kp[kq], b = kb[kq];
             const T c1 = T(0.5) * (a + b), c2 = T(0.5) * (a - b);
             const cpx<T> z = v[r];
             const cpx<T> zp = seq[lds_pad((L - ky) & (L - 1))];
@@ -301,10 +304,11 @@ k_col_extract(const cpx<double> *__restrict__ Sk, int hxs, const cpx<double> *__
     for (int r = 0; r < R0; ++r) {
         const int ky = t + r * TPS;
         if (ky < HY) {
-            kern[((int64_t)kx * HY + ky) * 4 + comp] = Tout(v[r].x);
+            const int64_t plane = (int64_t)(hxs + 1) * HY;
+            kern[comp * plane + (int64_t)kx * HY + ky] = Tout(v[r].x);
             // packed column 0: the imaginary part is the transform of the kx = px/2
             // column (both transforms are real for the even/odd-odd kernels)
-            if (kx == 0) kern[((int64_t)hxs * HY + ky) * 4 + comp] = Tout(v[r].y);
+            if (kx == 0) kern[comp * plane + (int64_t)hxs * HY + ky] = Tout(v[r].y);
         }
     }
 }
@@ -568,13 +572,13 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
             LLGMI_LAUNCH_CHECK();
         });
         // 4) column FFT (FP64), real quadrant -> kern (T)
-        LLGMI_HIP(hipMalloc(&kern, sizeof(quad4<T>) * hx * hy));
+        LLGMI_HIP(hipMalloc(&kern, sizeof(T) * 4 * hx * hy));
         dispatch_log2<1, kExtractMax>(ly, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
             using C = ExtractCfg<LG>;
             allow_lds(k_col_extract<T, LG>, C::LDS);
             hipLaunchKernelGGL((k_col_extract<T, LG>), dim3((unsigned)hxs, 4), dim3(C::NT), C::LDS,
-                               stream, sk, (int)hxs, twy64, reinterpret_cast<T *>(kern));
+                               stream, sk, (int)hxs, twy64, kern);
             LLGMI_LAUNCH_CHECK();
         });
         LLGMI_HIP(hipStreamSynchronize(stream));
@@ -717,27 +721,35 @@ void DemagFFT<T>::autotune(hipStream_t stream, int forced) {
         LLGMI_HIP(hipEventCreate(&e1));
         auto time = [&](auto &&launch) {
             LLGMI_HIP(hipEventRecord(e0, stream));
-            for (int i = 0; i < 8; ++i) launch();
+            for (int i = 0; i < 4; ++i) launch();
             LLGMI_HIP(hipEventRecord(e1, stream));
             LLGMI_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
             LLGMI_HIP(hipEventElapsedTime(&ms, e0, e1));
             return ms;
         };
-        // variants alternate over three rounds (the first round also warms
-        // caches and code); the minimum per variant decides
-        auto pick = [&](auto &&launch_rq) {
-            float best[2] = {1e30f, 1e30f};
-            for (int round = 0; round < 3; ++round)
-                for (int v = 0; v < 2; ++v) {
-                    float t = time([&] { launch_rq(3 + v); });
-                    if (t < best[v]) best[v] = t;
-                }
-            return best[0] < best[1] ? 3 : 4;
-        };
-        rq_row = pick([&](int rq) { launch_row_fwd(m, T(1), stream, rq); });
-        rq_col = pick([&](int rq) { launch_col(stream, rq); });
-        rq_row_inv = pick([&](int rq) { launch_row_inv(bd, stream, rq); });
+        // The three passes are timed together for all eight radix combinations
+        // (a pass alone sees other cache contents than inside the pipeline);
+        // combinations alternate over three rounds, the minimum per
+        // combination decides.
+        float best[8];
+        for (float &b : best) b = 1e30f;
+        for (int round = 0; round < 3; ++round)
+            for (int c = 0; c < 8; ++c) {
+                const int rr = 3 + (c & 1), rc = 3 + ((c >> 1) & 1), ri = 3 + ((c >> 2) & 1);
+                const float t = time([&] {
+                    launch_row_fwd(m, T(1), stream, rr);
+                    launch_col(stream, rc);
+                    launch_row_inv(bd, stream, ri);
+                });
+                if (t < best[c]) best[c] = t;
+            }
+        int bc = 0;
+        for (int c = 1; c < 8; ++c)
+            if (best[c] < best[bc]) bc = c;
+        rq_row = 3 + (bc & 1);
+        rq_col = 3 + ((bc >> 1) & 1);
+        rq_row_inv = 3 + ((bc >> 2) & 1);
     } catch (...) {
         cleanup();
         throw;
@@ -763,15 +775,12 @@ void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *p
 
 template <typename T>
 void DemagFFT<T>::read_kernel(std::vector<double> &out) const {
-    std::vector<quad4<T>> h((size_t)(hx * hy));
-    LLGMI_HIP(hipMemcpy(h.data(), kern, sizeof(quad4<T>) * h.size(), hipMemcpyDeviceToHost));
-    out.resize(h.size() * 4);
-    for (size_t i = 0; i < h.size(); ++i) {
-        out[4 * i + 0] = (double)h[i].xx;
-        out[4 * i + 1] = (double)h[i].xy;
-        out[4 * i + 2] = (double)h[i].yy;
-        out[4 * i + 3] = (double)h[i].zz;
-    }
+    const size_t np = (size_t)(hx * hy);
+    std::vector<T> h(4 * np);
+    LLGMI_HIP(hipMemcpy(h.data(), kern, sizeof(T) * h.size(), hipMemcpyDeviceToHost));
+    out.resize(4 * np);
+    for (size_t i = 0; i < np; ++i)
+        for (size_t c = 0; c < 4; ++c) out[4 * i + c] = (double)h[c * np + i];
 }
 
 template struct DemagFFT<float>;

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
@@ -55,6 +56,7 @@ struct DevStencil {
 struct TNode {            // octree node in pre-order; leaf iff count > 0
     double cx, cy, cz, half;
     int skip, depth, start, count;
+    int skip_depth, pad_;    // depth of node `skip` (0 past the end)
 };
 
 inline unsigned nblocks(int64_t n, int nt = kNT) {
@@ -80,6 +82,21 @@ __device__ __forceinline__ double offdiag(const double *__restrict__ phi, int nx
         const int kk = clampi(k + st->off[q][2], 0, nz - 1);
         sum += st->c[q] * phi[idx3(ii, jj, kk, nx, ny)];
     }
+    return sum;
+}
+
+// Off-diagonal sum of the 7-point stencil (offsets +x, -x, +y, -y, +z, -z, the
+// order of Stencil3D::seven_point) at an interior cell: the clamp of
+// offdiag_sum_at never acts there, so this is the same sum in the same order.
+__device__ __forceinline__ double offdiag7(const double *__restrict__ phi, int64_t id, int nx, int64_t plane,
+                                           const DevStencil *__restrict__ st) {
+    double sum = 0.0;
+    sum += st->c[0] * phi[id + 1];
+    sum += st->c[1] * phi[id - 1];
+    sum += st->c[2] * phi[id + nx];
+    sum += st->c[3] * phi[id - nx];
+    sum += st->c[4] * phi[id + plane];
+    sum += st->c[5] * phi[id - plane];
     return sum;
 }
 
@@ -203,6 +220,156 @@ __global__ void __launch_bounds__(kNT) k_mg_rb(double *__restrict__ phi, const d
     phi[id] = po + omega * (pn - po);
 }
 
+// One full red-black SOR sweep (colour 0, then colour 1) from A into B.
+// The colour-1 update reads the new colour-0 values and the old own value,
+// the colour-0 update only old values, so with separate input and output
+// buffers the sweep is the reference's in-place pair of half-sweeps exactly.
+// A block owns an RBX x RBY tile of interior (i, j) columns and a chunk of
+// RBZ planes and marches up k; the new colour-0 values of planes k-1, k, k+1
+// on the tile plus a one-cell halo sit in LDS (halo cells and the plane below
+// the chunk recomputed, boundary cells keep their value).  rbz = planes per
+// chunk (z-chunks add blocks at the cost of one recomputed plane each).
+constexpr int RBX = 32, RBY = 8, RBW = RBX + 2, RBH = RBY + 2;
+__global__ void __launch_bounds__(RBX * RBY) k_mg_rb_sweep(const double *__restrict__ A, double *__restrict__ B,
+                                                       const double *__restrict__ rhs, int nx, int ny,
+                                                       int nz, double sx, double sy, double sz, double denom,
+                                                       double omega, int rbz) {
+    __shared__ double red[3][RBH][RBW];
+    const int tid = threadIdx.x;
+    const int i0 = 1 + (int)blockIdx.x * RBX, j0 = 1 + (int)blockIdx.y * RBY;
+    const int kb = 1 + (int)blockIdx.z * rbz, ke = min(nz - 2, kb + rbz - 1);
+    const int64_t plane = (int64_t)nx * ny;
+    // new colour-0 value at an even cell (i, j, k), or the unchanged value of a
+    // boundary cell
+    auto red_new = [&](int i, int j, int k) -> double {
+        const int64_t id = idx3(i, j, k, nx, ny);
+        if (i <= 0 || i >= nx - 1 || j <= 0 || j >= ny - 1 || k <= 0 || k >= nz - 1) return A[id];
+        const double off = sx * (A[id - 1] + A[id + 1]) + sy * (A[id - nx] + A[id + nx]) +
+                           sz * (A[id - plane] + A[id + plane]);
+        const double pn = (off - rhs[id]) / denom;
+        const double po = A[id];
+        return po + omega * (pn - po);
+    };
+    // only even-parity positions are ever read by the colour-1 update: the
+    // in-plane and vertical neighbours of an odd cell are even
+    auto fill = [&](int slot, int k) {
+        for (int q = tid; q < (RBW / 2) * RBH; q += RBX * RBY) {
+            const int hj = q / (RBW / 2);
+            const int hi = 2 * (q % (RBW / 2)) + ((i0 + j0 + hj + k) & 1);
+            const int i = i0 - 1 + hi, j = j0 - 1 + hj;
+            red[slot][hj][hi] = (i < nx && j < ny) ? red_new(i, j, k) : 0.0;
+        }
+    };
+    fill((kb - 1) % 3, kb - 1);
+    fill(kb % 3, kb);
+    const int ti = tid % RBX, tj = tid / RBX;
+    const int i = i0 + ti, j = j0 + tj;
+    const bool inside = i <= nx - 2 && j <= ny - 2;
+    // cells that never change: boundary columns and rows next to this tile
+    // (the planes of this chunk, plus the bottom / top plane for the first /
+    // last chunk) and the bottom / top planes under the tile
+    if (inside) {
+        const int klo = kb == 1 ? 0 : kb, khi = ke == nz - 2 ? nz - 1 : ke;
+        auto keep = [&](int a, int c) {
+            for (int k = klo; k <= khi; ++k) B[idx3(a, c, k, nx, ny)] = A[idx3(a, c, k, nx, ny)];
+        };
+        if (i == 1) keep(0, j);
+        if (i == nx - 2) keep(nx - 1, j);
+        if (j == 1) keep(i, 0);
+        if (j == ny - 2) keep(i, ny - 1);
+        if (i == 1 && j == 1) keep(0, 0);
+        if (i == nx - 2 && j == 1) keep(nx - 1, 0);
+        if (i == 1 && j == ny - 2) keep(0, ny - 1);
+        if (i == nx - 2 && j == ny - 2) keep(nx - 1, ny - 1);
+        if (kb == 1) B[idx3(i, j, 0, nx, ny)] = A[idx3(i, j, 0, nx, ny)];
+        if (ke == nz - 2) B[idx3(i, j, nz - 1, nx, ny)] = A[idx3(i, j, nz - 1, nx, ny)];
+    }
+    for (int k = kb; k <= ke; ++k) {
+        const int sm = (k - 1) % 3, sc = k % 3, sp = (k + 1) % 3;
+        fill(sp, k + 1);
+        __syncthreads();
+        if (inside) {
+            const int64_t id = idx3(i, j, k, nx, ny);
+            const int hi = ti + 1, hj = tj + 1;
+            if (((i + j + k) & 1) == 0) {
+                B[id] = red[sc][hj][hi];
+            } else {
+                const double off = sx * (red[sc][hj][hi - 1] + red[sc][hj][hi + 1]) +
+                                   sy * (red[sc][hj - 1][hi] + red[sc][hj + 1][hi]) +
+                                   sz * (red[sm][hj][hi] + red[sp][hj][hi]);
+                const double pn = (off - rhs[id]) / denom;
+                const double po = A[id];
+                B[id] = po + omega * (pn - po);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Up to TBH red-black half-sweeps (colours alternating from `color`) in one
+// launch, for small levels (the coarsest level's 80 sweeps).  A block loads
+// an (TBT + 2 TBH)^2 x nz box of A and of rhs into LDS, updates the A box in
+// place half-sweep by half-sweep (each half-sweep leaves one more outer ring
+// stale), and writes the central TBT x TBT columns to B; after h <= TBH
+// half-sweeps those are exactly the values the in-place half-sweeps produce.
+constexpr int TBT = 16, TBH = 8, TBW = TBT + 2 * TBH, TB_MAXZ = 9, TB_NT = 1024;
+__global__ void __launch_bounds__(TB_NT) k_mg_rb_tb(const double *__restrict__ A, double *__restrict__ B,
+                                                    const double *__restrict__ rhs, int nx, int ny, int nz,
+                                                    double sx, double sy, double sz, double denom,
+                                                    double omega, int color, int nhalf) {
+    extern __shared__ unsigned char lds_raw[];
+    const int nbox = nz * TBW * TBW;
+    double *box = reinterpret_cast<double *>(lds_raw);   // [nz][TBW][TBW]
+    double *rb = box + nbox;                               // rhs, same layout
+    const int tid = threadIdx.x;
+    const int i0 = 1 + (int)blockIdx.x * TBT, j0 = 1 + (int)blockIdx.y * TBT;
+    const int ib = i0 - TBH, jb = j0 - TBH;   // global index of box column 0
+    for (int q = tid; q < nbox; q += TB_NT) {
+        const int hi = q % TBW, hj = (q / TBW) % TBW, k = q / (TBW * TBW);
+        const int gi = ib + hi, gj = jb + hj;
+        const bool in = gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+        box[q] = in ? A[idx3(gi, gj, k, nx, ny)] : 0.0;
+        rb[q] = in ? rhs[idx3(gi, gj, k, nx, ny)] : 0.0;
+    }
+    __syncthreads();
+    constexpr int PER_ROW = (TBW - 2) / 2;   // cells of one colour per box row
+    const int rows = (nz - 2) * (TBW - 2);
+    for (int s = 0; s < nhalf; ++s) {
+        const int c = (color + s) & 1;
+        for (int q = tid; q < rows * PER_ROW; q += TB_NT) {
+            const int row = q / PER_ROW, r = q % PER_ROW;
+            const int hj = 1 + row % (TBW - 2), k = 1 + row / (TBW - 2);
+            const int gj = jb + hj;
+            const int first = 1 + (((ib + 1 + gj + k) & 1) == c ? 0 : 1);
+            const int hi = first + 2 * r;
+            const int gi = ib + hi;
+            if (gi < 1 || gi > nx - 2 || gj < 1 || gj > ny - 2) continue;
+            const int id = (k * TBW + hj) * TBW + hi;
+            const double off = sx * (box[id - 1] + box[id + 1]) + sy * (box[id - TBW] + box[id + TBW]) +
+                               sz * (box[id - TBW * TBW] + box[id + TBW * TBW]);
+            const double pn = (off - rb[id]) / denom;
+            const double po = box[id];
+            box[id] = po + omega * (pn - po);
+        }
+        __syncthreads();
+    }
+    // central columns (and the untouched boundary cells next to them) to B
+    for (int q = tid; q < nz * TBT * TBT; q += TB_NT) {
+        const int ci = q % TBT, cj = (q / TBT) % TBT, k = q / (TBT * TBT);
+        const int gi = i0 + ci, gj = j0 + cj;
+        if (gi > nx - 2 || gj > ny - 2) continue;
+        B[idx3(gi, gj, k, nx, ny)] = box[(k * TBW + cj + TBH) * TBW + ci + TBH];
+        if (gi == 1) B[idx3(0, gj, k, nx, ny)] = A[idx3(0, gj, k, nx, ny)];
+        if (gi == nx - 2) B[idx3(nx - 1, gj, k, nx, ny)] = A[idx3(nx - 1, gj, k, nx, ny)];
+        if (gj == 1) B[idx3(gi, 0, k, nx, ny)] = A[idx3(gi, 0, k, nx, ny)];
+        if (gj == ny - 2) B[idx3(gi, ny - 1, k, nx, ny)] = A[idx3(gi, ny - 1, k, nx, ny)];
+        if (gi == 1 && gj == 1) B[idx3(0, 0, k, nx, ny)] = A[idx3(0, 0, k, nx, ny)];
+        if (gi == nx - 2 && gj == 1) B[idx3(nx - 1, 0, k, nx, ny)] = A[idx3(nx - 1, 0, k, nx, ny)];
+        if (gi == 1 && gj == ny - 2) B[idx3(0, ny - 1, k, nx, ny)] = A[idx3(0, ny - 1, k, nx, ny)];
+        if (gi == nx - 2 && gj == ny - 2) B[idx3(nx - 1, ny - 1, k, nx, ny)] = A[idx3(nx - 1, ny - 1, k, nx, ny)];
+    }
+}
+
 // One sweep of smooth_weighted_jacobi from src into dst; boundary cells of dst
 // take the Dirichlet values (bc, or 0 when bc is null).
 __global__ void __launch_bounds__(kNT) k_mg_wj(const double *__restrict__ src, double *__restrict__ dst,
@@ -243,8 +410,9 @@ __device__ __forceinline__ double block_max(double v) {
 __global__ void __launch_bounds__(kNT) k_mg_resid_max(const double *__restrict__ phi,
                                                       const double *__restrict__ rhs,
                                                       const DevStencil *__restrict__ st, int nx, int ny,
-                                                      int nz, double *__restrict__ partial) {
+                                                      int nz, int seven, double *__restrict__ partial) {
     const int64_t n = (int64_t)(nx - 2) * (ny - 2) * (nz - 2);
+    const int64_t plane = (int64_t)nx * ny;
     double mx = 0.0;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         const int i = 1 + (int)(t % (nx - 2));
@@ -252,7 +420,8 @@ __global__ void __launch_bounds__(kNT) k_mg_resid_max(const double *__restrict__
         const int j = 1 + (int)(r % (ny - 2));
         const int k = 1 + (int)(r / (ny - 2));
         const int64_t id = idx3(i, j, k, nx, ny);
-        const double a = st->center * phi[id] + offdiag(phi, nx, ny, nz, i, j, k, st);
+        const double a = st->center * phi[id] +
+                         (seven ? offdiag7(phi, id, nx, plane, st) : offdiag(phi, nx, ny, nz, i, j, k, st));
         mx = fmax(mx, fabs(rhs[id] - a));
     }
     mx = block_max(mx);
@@ -266,7 +435,8 @@ __global__ void __launch_bounds__(kNT) k_mg_resid_restrict(const double *__restr
                                                            const DevStencil *__restrict__ st, int fnx,
                                                            int fny, int fnz, double *__restrict__ crhs,
                                                            double *__restrict__ cphi, int cnx, int cny,
-                                                           int cnz, int rx, int ry, int rz, double w) {
+                                                           int cnz, int rx, int ry, int rz, double w,
+                                                           int seven) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)(cnx - 2) * (cny - 2) * (cnz - 2);
     if (t >= n) return;
@@ -280,7 +450,9 @@ __global__ void __launch_bounds__(kNT) k_mg_resid_restrict(const double *__restr
             for (int di = 0; di < rx; ++di) {
                 const int i = rx * ic + di, j = ry * jc + dj, k = rz * kc + dk;
                 const int64_t id = idx3(i, j, k, fnx, fny);
-                const double a = st->center * phi[id] + offdiag(phi, fnx, fny, fnz, i, j, k, st);
+                const double a = st->center * phi[id] +
+                                 (seven ? offdiag7(phi, id, fnx, (int64_t)fnx * fny, st)
+                                        : offdiag(phi, fnx, fny, fnz, i, j, k, st));
                 sum += rhs[id] - a;
             }
     const int64_t cid = idx3(ic, jc, kc, cnx, cny);
@@ -570,75 +742,87 @@ __device__ __forceinline__ void cell_pos(const BoxGeom &g, int i, int j, int k, 
 }
 
 // eval_phi (mg_treecode.rs:192-259) at every boundary cell; writes bc and
-// stamps phi.  The recursion is walked in pre-order through skip links; the
-// open ancestors' partial sums sit in part[depth], so each node's value is
-// added to its parent's sum in the reference's order.
+// stamps phi.
+//
+// The 64 lanes of a wave hold 64 neighbouring boundary cells and walk the
+// pre-order node array together: node n is visited by the lanes whose own
+// traversal reaches it (lane.next == n).  A lane that opens n continues at
+// n + 1, a lane that accepts n (or sums the leaf n) continues at skip[n].
+// Lanes waiting further on accepted an ancestor, whose skip is >= skip[n], so
+// the wave's next node is n + 1 if any lane opened n and skip[n] otherwise.
+// Node, moment and charge loads are wave-uniform.  Each lane sees exactly its
+// own traversal in its own order, and keeps one partial sum per open depth,
+// so its value is the reference's nested recursive sum bit for bit.
 __global__ void __launch_bounds__(kNT) k_tc_eval(const TNode *__restrict__ nodes, int nnodes,
                                                  const double *__restrict__ mom, const double *__restrict__ pos,
                                                  const double *__restrict__ qp, BoxGeom g, double theta,
                                                  double *__restrict__ bc, double *__restrict__ phi) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= boundary_count(g.nx, g.ny, g.nz)) return;
-    int i, j, k;
-    boundary_cell(t, g.nx, g.ny, g.nz, i, j, k);
+    const bool valid = t < boundary_count(g.nx, g.ny, g.nz);
+    int i = 0, j = 0, k = 0;
+    if (valid) boundary_cell(t, g.nx, g.ny, g.nz, i, j, k);
     double tx, ty, tz;
     cell_pos(g, i, j, k, tx, ty, tz);
     double part[kMaxDepth + 2];
     double total = 0.0;
-    int n = 0;
-    for (;;) {
+    bool done = !valid;
+    int next = 0;       // this lane's next node
+    int n = 0;          // the wave's node (uniform)
+    while (n < nnodes) {
+        n = __builtin_amdgcn_readfirstlane(n);
         const TNode nd = nodes[n];
-        const int d = nd.depth;
-        double v = 0.0;
-        bool accept;
-        if (nd.count > 0) {
-            accept = true;
-            for (int a = nd.start; a < nd.start + nd.count; ++a) {
-                const double ex = tx - pos[3 * (int64_t)a], ey = ty - pos[3 * (int64_t)a + 1],
-                             ez = tz - pos[3 * (int64_t)a + 2];
-                const double rr2 = ex * ex + ey * ey + ez * ez;
-                if (rr2 > 0.0) v += qp[a] / sqrt(rr2);
+        bool opened = false;
+        if (!done && next == n) {
+            const int d = nd.depth;
+            double v = 0.0;
+            bool accept;
+            if (nd.count > 0) {
+                accept = true;
+                for (int a = nd.start; a < nd.start + nd.count; ++a) {
+                    const double ex = tx - pos[3 * (int64_t)a], ey = ty - pos[3 * (int64_t)a + 1],
+                                 ez = tz - pos[3 * (int64_t)a + 2];
+                    const double rr2 = ex * ex + ey * ey + ez * ez;
+                    if (rr2 > 0.0) v += qp[a] / sqrt(rr2);
+                }
+            } else {
+                const double rx = tx - nd.cx, ry = ty - nd.cy, rz = tz - nd.cz;
+                const double ddx = fmax(fabs(rx) - nd.half, 0.0), ddy = fmax(fabs(ry) - nd.half, 0.0),
+                             ddz = fmax(fabs(rz) - nd.half, 0.0);
+                const double dd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                const double size = nd.half * 2.0;
+                accept = dd > 0.0 ? (size / dd < theta) : false;
+                if (accept) {
+                    const double r = sqrt(rx * rx + ry * ry + rz * rz);
+                    const double inv_r = 1.0 / r;
+                    const double inv_r3 = inv_r * inv_r * inv_r;
+                    const double pr = mom[4 * (int64_t)n + 1] * rx + mom[4 * (int64_t)n + 2] * ry +
+                                      mom[4 * (int64_t)n + 3] * rz;
+                    v = mom[4 * (int64_t)n] * inv_r + pr * inv_r3;
+                }
             }
-        } else {
-            const double rx = tx - nd.cx, ry = ty - nd.cy, rz = tz - nd.cz;
-            const double ddx = fmax(fabs(rx) - nd.half, 0.0), ddy = fmax(fabs(ry) - nd.half, 0.0),
-                         ddz = fmax(fabs(rz) - nd.half, 0.0);
-            const double dd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-            const double size = nd.half * 2.0;
-            accept = dd > 0.0 ? (size / dd < theta) : false;
-            if (accept) {
-                const double r = sqrt(rx * rx + ry * ry + rz * rz);
-                const double inv_r = 1.0 / r;
-                const double inv_r3 = inv_r * inv_r * inv_r;
-                const double pr = mom[4 * (int64_t)n + 1] * rx + mom[4 * (int64_t)n + 2] * ry +
-                                  mom[4 * (int64_t)n + 3] * rz;
-                v = mom[4 * (int64_t)n] * inv_r + pr * inv_r3;
-            }
-        }
-        if (!accept) {
-            part[d] = 0.0;
-            ++n;
-            continue;
-        }
-        if (d == 0) {
-            total = v;
-            break;
-        }
-        part[d - 1] += v;
-        const int next = nd.skip;
-        const int nd_next = next < nnodes ? nodes[next].depth : 0;
-        bool done = false;
-        for (int l = d - 1; l >= nd_next; --l) {
-            if (l == 0) {
-                total = part[0];
+            if (!accept) {
+                part[d] = 0.0;
+                next = n + 1;
+                opened = true;
+            } else if (d == 0) {
+                total = v;
                 done = true;
-                break;
+            } else {
+                part[d - 1] += v;
+                for (int l = d - 1; l >= nd.skip_depth; --l) {
+                    if (l == 0) {
+                        total = part[0];
+                        done = true;
+                        break;
+                    }
+                    part[l - 1] += part[l];
+                }
+                next = nd.skip;
             }
-            part[l - 1] += part[l];
         }
-        if (done) break;
-        n = next;
+        n = __ballot(opened) ? n + 1 : nd.skip;
     }
+    if (!valid) return;
     const int64_t id = idx3(i, j, k, g.nx, g.ny);
     const double val = -total / (4.0 * kPi);
     bc[id] = val;
@@ -1129,7 +1313,7 @@ struct HostTreeBuilder {
 
     void emit(int lo, int hi, const double c[3], double half, int depth) {
         const int id = (int)nodes.size();
-        nodes.push_back(TNode{c[0], c[1], c[2], half, -1, depth, 0, 0});
+        nodes.push_back(TNode{c[0], c[1], c[2], half, -1, depth, 0, 0, 0, 0});
         if (depth >= max_depth || hi - lo <= leaf) {
             nodes[id].start = lo;
             nodes[id].count = hi - lo;
@@ -1172,6 +1356,9 @@ MGSolver::MGSolver(const llgmi_grid &g, const llgmi_mg_config &cfg_in, hipStream
     : g_(g), cfg_(cfg_in), stream_(stream), prof_(prof) {
     mg_validate(cfg_);
     mg_sanitize(cfg_);
+    if (const char *e = std::getenv("LLGMI_MG_GRAPH")) use_graphs_ = std::string(e) != "0";
+    if (const char *e = std::getenv("LLGMI_MG_RB_FUSED")) rb_fused_ = std::string(e) != "0";
+    if (const char *e = std::getenv("LLGMI_MG_RBZ")) rbz_ = std::max(1, std::atoi(e));
     try {
         // new_with_operator (:1425-1577)
         const int64_t nx = std::max<int64_t>(g.nx, 1), ny = std::max<int64_t>(g.ny, 1);
@@ -1218,6 +1405,10 @@ MGSolver::MGSolver(const llgmi_grid &g, const llgmi_mg_config &cfg_in, hipStream
             LLGMI_HIP(hipMalloc((void **)&L.rhs, sizeof(double) * n));
             LLGMI_HIP(hipMemsetAsync(L.phi[0], 0, sizeof(double) * n, stream_));
             LLGMI_HIP(hipMemsetAsync(L.rhs, 0, sizeof(double) * n, stream_));
+            static const int o7[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+            L.seven = L.st.coeffs.size() == 6;
+            for (size_t q = 0; L.seven && q < 6; ++q)
+                for (int a = 0; a < 3; ++a) L.seven = L.seven && L.st.offs[q][a] == o7[q][a];
             DevStencil h{};
             h.center = L.st.center;
             h.diag = L.st.diag;
@@ -1229,6 +1420,8 @@ MGSolver::MGSolver(const llgmi_grid &g, const llgmi_mg_config &cfg_in, hipStream
             LLGMI_HIP(hipMalloc(&L.d_st, sizeof(DevStencil)));
             LLGMI_HIP(hipMemcpy(L.d_st, &h, sizeof(DevStencil), hipMemcpyHostToDevice));
         }
+        LLGMI_HIP(hipFuncSetAttribute((const void *)k_mg_rb_tb, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(2 * sizeof(double) * TB_MAXZ * TBW * TBW)));
         const size_t n0 = (size_t)px_ * py_ * pz_;
         LLGMI_HIP(hipMalloc((void **)&bc_, sizeof(double) * n0));
         LLGMI_HIP(hipMemsetAsync(bc_, 0, sizeof(double) * n0, stream_));
@@ -1253,6 +1446,9 @@ MGSolver::MGSolver(const llgmi_grid &g, const llgmi_mg_config &cfg_in, hipStream
 MGSolver::~MGSolver() { release(); }
 
 void MGSolver::release() {
+    for (auto &kv : graphs_)
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    graphs_.clear();
     for (auto &L : lev_) {
         for (double *p : {L.phi[0], L.phi[1], L.rhs})
             if (p) (void)hipFree(p);
@@ -1371,6 +1567,7 @@ void MGSolver::treecode_bc() {
         for (int64_t a = 0; a < n; ++a) b.ord[(size_t)a] = (int)a;
         const double rc[3] = {0.0, 0.0, 0.0};
         b.emit(0, (int)n, rc, root_half, 0);
+        for (auto &nd : b.nodes) nd.skip_depth = nd.skip < (int)b.nodes.size() ? b.nodes[(size_t)nd.skip].depth : 0;
         std::vector<double> pos_leaf(3 * (size_t)n);
         for (int64_t a = 0; a < n; ++a)
             for (int c = 0; c < 3; ++c) pos_leaf[3 * a + c] = pos[3 * (size_t)b.ord[(size_t)a] + c];
@@ -1455,11 +1652,54 @@ void MGSolver::update_bc() {
     }
 }
 
+// second phi buffer of a level (Jacobi / fused red-black sweeps write into it)
+void MGSolver::ensure_pair(Level &L) {
+    if (L.phi[1]) return;
+    const size_t n = (size_t)L.nx * L.ny * L.nz;
+    LLGMI_HIP(hipMalloc((void **)&L.phi[1], sizeof(double) * n));
+    LLGMI_HIP(hipMemsetAsync(L.phi[1], 0, sizeof(double) * n, stream_));
+}
+
 void MGSolver::smooth(int l, int64_t iters) {
     Level &L = lev_[l];
     if (cfg_.smoother == 1) {
         const double sx = 1.0 / (L.dx * L.dx), sy = 1.0 / (L.dy * L.dy), sz = 1.0 / (L.dz * L.dz);
         const double denom = 2.0 * (sx + sy + sz);
+        const bool coarsest = l == (int)lev_.size() - 1;
+        if (rb_fused_ && l == 0) {
+            // finest level: one fused kernel per sweep
+            ensure_pair(L);
+            const int rbz = std::max(1, std::min(rbz_, L.nz - 2));
+            const dim3 grid((unsigned)((L.nx - 2 + RBX - 1) / RBX), (unsigned)((L.ny - 2 + RBY - 1) / RBY),
+                            (unsigned)((L.nz - 2 + rbz - 1) / rbz));
+            for (int64_t it = 0; it < iters; ++it) {
+                hipLaunchKernelGGL(k_mg_rb_sweep, grid, dim3(RBX * RBY), 0, stream_, L.phi[L.cur],
+                                   L.phi[1 - L.cur], L.rhs, L.nx, L.ny, L.nz, sx, sy, sz, denom, cfg_.sor_omega,
+                                   rbz);
+                LLGMI_LAUNCH_CHECK();
+                L.cur = 1 - L.cur;
+            }
+            return;
+        }
+        if (rb_fused_ && coarsest && l > 0 && L.nz <= TB_MAXZ) {
+            // small coarsest level: TBH half-sweeps per launch in LDS
+            ensure_pair(L);
+            const dim3 grid((unsigned)((L.nx - 2 + TBT - 1) / TBT), (unsigned)((L.ny - 2 + TBT - 1) / TBT));
+            const size_t lds = 2 * sizeof(double) * (size_t)L.nz * TBW * TBW;
+            int64_t left = 2 * iters;
+            int color = 0;
+            while (left > 0) {
+                const int h = (int)std::min<int64_t>(left, TBH);
+                hipLaunchKernelGGL(k_mg_rb_tb, grid, dim3(TB_NT), lds, stream_, L.phi[L.cur], L.phi[1 - L.cur],
+                                   L.rhs, L.nx, L.ny, L.nz, sx, sy, sz, denom, cfg_.sor_omega, color, h);
+                LLGMI_LAUNCH_CHECK();
+                L.cur = 1 - L.cur;
+                color = (color + h) & 1;
+                left -= h;
+            }
+            return;
+        }
+        // other levels: the two half-sweep kernels, in place
         const int64_t n = (int64_t)(L.ny - 2) * (L.nz - 2) * ((L.nx - 1) / 2);
         for (int64_t it = 0; it < iters; ++it)
             for (int color = 0; color < 2; ++color) {
@@ -1469,11 +1709,7 @@ void MGSolver::smooth(int l, int64_t iters) {
             }
         return;
     }
-    if (!L.phi[1]) {
-        const size_t n = (size_t)L.nx * L.ny * L.nz;
-        LLGMI_HIP(hipMalloc((void **)&L.phi[1], sizeof(double) * n));
-        LLGMI_HIP(hipMemsetAsync(L.phi[1], 0, sizeof(double) * n, stream_));
-    }
+    ensure_pair(L);
     const int64_t n = (int64_t)L.nx * L.ny * L.nz;
     for (int64_t it = 0; it < iters; ++it) {
         hipLaunchKernelGGL(k_mg_wj, dim3(nblocks(n)), dim3(kNT), 0, stream_, L.phi[L.cur], L.phi[1 - L.cur], L.rhs,
@@ -1495,7 +1731,7 @@ void MGSolver::v_cycle(int l) {
     const int64_t nc = (int64_t)(C.nx - 2) * (C.ny - 2) * (C.nz - 2);
     hipLaunchKernelGGL(k_mg_resid_restrict, dim3(nblocks(nc)), dim3(kNT), 0, stream_, F.phi[F.cur], F.rhs,
                        (const DevStencil *)F.d_st, F.nx, F.ny, F.nz, C.rhs, C.phi[C.cur], C.nx, C.ny, C.nz, rx, ry,
-                       rz, 1.0 / (double)(rx * ry * rz));
+                       rz, 1.0 / (double)(rx * ry * rz), F.seven ? 1 : 0);
     LLGMI_LAUNCH_CHECK();
     v_cycle(l + 1);
     const int64_t nf = (int64_t)(F.nx - 2) * (F.ny - 2) * (F.nz - 2);
@@ -1503,6 +1739,51 @@ void MGSolver::v_cycle(int l) {
                        F.phi[F.cur], F.nx, F.ny, F.nz, cfg_.prolong == 1 ? 1 : 0);
     LLGMI_LAUNCH_CHECK();
     smooth(l, cfg_.post_smooth);
+}
+
+// One V-cycle from a HIP graph: the cycle's ~180 launches (80 coarsest
+// sweeps) are captured once per smoothing configuration and replayed, which
+// removes the host launch cost between the short coarse-level kernels.  The
+// captured work has no memset/memcpy nodes; every buffer it touches exists
+// before the capture (the Jacobi target buffers are allocated first).
+void MGSolver::v_cycle_graph() {
+    if (!use_graphs_) {
+        v_cycle(0);
+        return;
+    }
+    if (cfg_.smoother == 0 || rb_fused_)
+        for (auto &L : lev_) ensure_pair(L);
+    std::string key = std::to_string(cfg_.smoother) + "/" + std::to_string(cfg_.pre_smooth) + "/" +
+                      std::to_string(cfg_.post_smooth) + "/";
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%a/%a/", cfg_.omega, cfg_.sor_omega);
+    key += buf;
+    for (auto &L : lev_) key += (char)('0' + L.cur);
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+        std::vector<int> cur_before;
+        for (auto &L : lev_) cur_before.push_back(L.cur);
+        hipGraph_t graph = nullptr;
+        LLGMI_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            v_cycle(0);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream_, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            for (size_t l = 0; l < lev_.size(); ++l) lev_[l].cur = cur_before[l];
+            throw;
+        }
+        LLGMI_HIP(hipStreamEndCapture(stream_, &graph));
+        Graph g;
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        LLGMI_HIP(e);
+        for (auto &L : lev_) g.cur_after.push_back(L.cur);
+        for (size_t l = 0; l < lev_.size(); ++l) lev_[l].cur = cur_before[l];
+        it = graphs_.emplace(key, std::move(g)).first;
+    }
+    LLGMI_HIP(hipGraphLaunch(it->second.exec, stream_));
+    for (size_t l = 0; l < lev_.size(); ++l) lev_[l].cur = it->second.cur_after[l];
 }
 
 double MGSolver::finest_rhs_max() {
@@ -1541,7 +1822,7 @@ void MGSolver::solve() {
     stats_.cycles = 0;
     const int64_t n_nan = std::min<int64_t>(2 * (int64_t)L.nx * L.ny, (int64_t)n0);
     for (int64_t iter = 0; iter < max_cycles; ++iter) {
-        v_cycle(0);
+        v_cycle_graph();
         // enforce_dirichlet: the smoothers leave the boundary cells at bc
         stats_.cycles = iter + 1;
         const bool nan_check = iter % 2 == 1 || iter + 1 == max_cycles;
@@ -1555,7 +1836,7 @@ void MGSolver::solve() {
         }
         if (res_check) {
             hipLaunchKernelGGL(k_mg_resid_max, dim3(kRedBlocks), dim3(kNT), 0, stream_, L.phi[L.cur], L.rhs,
-                               (const DevStencil *)L.d_st, L.nx, L.ny, L.nz, d_partial_);
+                               (const DevStencil *)L.d_st, L.nx, L.ny, L.nz, L.seven ? 1 : 0, d_partial_);
             LLGMI_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_mg_final_max, dim3(1), dim3(kNT), 0, stream_, d_partial_, kRedBlocks, d_scalars_);
             LLGMI_LAUNCH_CHECK();

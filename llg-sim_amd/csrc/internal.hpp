@@ -83,7 +83,7 @@ struct DemagFFT {
     int lx = 0, ly = 0;
     double dx = 0, dy = 0, dz = 0;
     cpx<T> *tw_x = nullptr, *tw_y = nullptr;
-    quad4<T> *kern = nullptr;  // [hx][hy]
+    T *kern = nullptr;  // 4 planes [Kxx, Kxy, Kyy, Kzz], each [hx][hy]
     cpx<T> *spec = nullptr;    // [3][hx][ny]
     int rq_row = 4, rq_col = 4, rq_row_inv = 4;  // log2 first/last radix, picked by autotune()
 

@@ -13,6 +13,8 @@
 #include "../../include/llgmi.h"
 
 #include <array>
+#include <map>
+#include <string>
 #include <vector>
 
 namespace llgmi {
@@ -54,6 +56,7 @@ public:
         int cur = 0;
         double *rhs = nullptr;
         void *d_st = nullptr;                 // device copy of the stencil
+        bool seven = false;                   // stencil is the 7-point layout (fast residual path)
     };
     struct Tree;
 
@@ -65,7 +68,9 @@ private:
     void dipole_bc();
     void stamp(int l);
     void smooth(int l, int64_t iters);
+    void ensure_pair(Level &L);
     void v_cycle(int l);
+    void v_cycle_graph();
     void solve();
     double finest_rhs_max();
     template <typename T>
@@ -83,6 +88,16 @@ private:
     double *h_scalars_ = nullptr;              // pinned mirror
     Tree *tree_ = nullptr;
     llgmi_mg_stats stats_{};
+    // V-cycles captured as HIP graphs, keyed by the smoothing settings and
+    // the Jacobi buffer parity of every level at entry
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<int> cur_after;
+    };
+    std::map<std::string, Graph> graphs_;
+    bool use_graphs_ = true;   // LLGMI_MG_GRAPH=0: launch the V-cycle kernels directly
+    bool rb_fused_ = true;     // LLGMI_MG_RB_FUSED=0: two half-sweep launches per sweep
+    int rbz_ = 1 << 20;        // LLGMI_MG_RBZ: planes per z-chunk of the fused finest sweep
 };
 
 }  // namespace llgmi
