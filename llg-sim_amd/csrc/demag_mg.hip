@@ -312,24 +312,43 @@ __global__ void __launch_bounds__(RBX * RBY) k_mg_rb_sweep(const double *__restr
 // place half-sweep by half-sweep (each half-sweep leaves one more outer ring
 // stale), and writes the central TBT x TBT columns to B; after h <= TBH
 // half-sweeps those are exactly the values the in-place half-sweeps produce.
-constexpr int TBT = 16, TBH = 8, TBW = TBT + 2 * TBH, TB_MAXZ = 9, TB_NT = 1024;
+// LDS rows are stored de-interleaved (even x, then odd x), so the cells of one
+// colour and their x, y, z neighbours are contiguous across the lanes.
+constexpr int TBT = 16, TBH = 8, TBW = TBT + 2 * TBH, TBHW = TBW / 2, TB_MAXZ = 9, TB_NT = 1024;
+__device__ __forceinline__ int tb_at(int k, int hj, int hi) { return ((k * TBW + hj) * 2 + (hi & 1)) * TBHW + (hi >> 1); }
 __global__ void __launch_bounds__(TB_NT) k_mg_rb_tb(const double *__restrict__ A, double *__restrict__ B,
                                                     const double *__restrict__ rhs, int nx, int ny, int nz,
                                                     double sx, double sy, double sz, double denom,
                                                     double omega, int color, int nhalf) {
     extern __shared__ unsigned char lds_raw[];
     const int nbox = nz * TBW * TBW;
-    double *box = reinterpret_cast<double *>(lds_raw);   // [nz][TBW][TBW]
+    double *box = reinterpret_cast<double *>(lds_raw);   // A, [nz][TBW][2][TBHW]
     double *rb = box + nbox;                               // rhs, same layout
     const int tid = threadIdx.x;
     const int i0 = 1 + (int)blockIdx.x * TBT, j0 = 1 + (int)blockIdx.y * TBT;
     const int ib = i0 - TBH, jb = j0 - TBH;   // global index of box column 0
-    for (int q = tid; q < nbox; q += TB_NT) {
-        const int hi = q % TBW, hj = (q / TBW) % TBW, k = q / (TBW * TBW);
-        const int gi = ib + hi, gj = jb + hj;
-        const bool in = gi >= 0 && gi < nx && gj >= 0 && gj < ny;
-        box[q] = in ? A[idx3(gi, gj, k, nx, ny)] : 0.0;
-        rb[q] = in ? rhs[idx3(gi, gj, k, nx, ny)] : 0.0;
+    {
+        // all global loads of this thread first, then the LDS stores
+        constexpr int PER = (TB_MAXZ * TBW * TBW + TB_NT - 1) / TB_NT;
+        double va[PER], vr[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + u * TB_NT;
+            const int hi = q % TBW, hj = (q / TBW) % TBW, k = q / (TBW * TBW);
+            const int gi = ib + hi, gj = jb + hj;
+            const bool in = q < nbox && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+            va[u] = in ? A[idx3(gi, gj, k, nx, ny)] : 0.0;
+            vr[u] = in ? rhs[idx3(gi, gj, k, nx, ny)] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + u * TB_NT;
+            if (q < nbox) {
+                const int hi = q % TBW, hj = (q / TBW) % TBW, k = q / (TBW * TBW);
+                box[tb_at(k, hj, hi)] = va[u];
+                rb[tb_at(k, hj, hi)] = vr[u];
+            }
+        }
     }
     __syncthreads();
     constexpr int PER_ROW = (TBW - 2) / 2;   // cells of one colour per box row
@@ -344,9 +363,10 @@ __global__ void __launch_bounds__(TB_NT) k_mg_rb_tb(const double *__restrict__ A
             const int hi = first + 2 * r;
             const int gi = ib + hi;
             if (gi < 1 || gi > nx - 2 || gj < 1 || gj > ny - 2) continue;
-            const int id = (k * TBW + hj) * TBW + hi;
-            const double off = sx * (box[id - 1] + box[id + 1]) + sy * (box[id - TBW] + box[id + TBW]) +
-                               sz * (box[id - TBW * TBW] + box[id + TBW * TBW]);
+            const int id = tb_at(k, hj, hi);
+            const double off = sx * (box[tb_at(k, hj, hi - 1)] + box[tb_at(k, hj, hi + 1)]) +
+                               sy * (box[tb_at(k, hj - 1, hi)] + box[tb_at(k, hj + 1, hi)]) +
+                               sz * (box[tb_at(k - 1, hj, hi)] + box[tb_at(k + 1, hj, hi)]);
             const double pn = (off - rb[id]) / denom;
             const double po = box[id];
             box[id] = po + omega * (pn - po);
@@ -358,7 +378,7 @@ __global__ void __launch_bounds__(TB_NT) k_mg_rb_tb(const double *__restrict__ A
         const int ci = q % TBT, cj = (q / TBT) % TBT, k = q / (TBT * TBT);
         const int gi = i0 + ci, gj = j0 + cj;
         if (gi > nx - 2 || gj > ny - 2) continue;
-        B[idx3(gi, gj, k, nx, ny)] = box[(k * TBW + cj + TBH) * TBW + ci + TBH];
+        B[idx3(gi, gj, k, nx, ny)] = box[tb_at(k, cj + TBH, ci + TBH)];
         if (gi == 1) B[idx3(0, gj, k, nx, ny)] = A[idx3(0, gj, k, nx, ny)];
         if (gi == nx - 2) B[idx3(nx - 1, gj, k, nx, ny)] = A[idx3(nx - 1, gj, k, nx, ny)];
         if (gj == 1) B[idx3(gi, 0, k, nx, ny)] = A[idx3(gi, 0, k, nx, ny)];
@@ -742,7 +762,8 @@ __device__ __forceinline__ void cell_pos(const BoxGeom &g, int i, int j, int k, 
 }
 
 // eval_phi (mg_treecode.rs:192-259) at every boundary cell; writes bc and
-// stamps phi.
+// stamps phi.  The boundary cells come in 8 x 8 patches of one face per wave
+// (targets, -1 = idle lane), so a wave's cells are close together.
 //
 // The 64 lanes of a wave hold 64 neighbouring boundary cells and walk the
 // pre-order node array together: node n is visited by the lanes whose own
@@ -756,11 +777,13 @@ __device__ __forceinline__ void cell_pos(const BoxGeom &g, int i, int j, int k, 
 __global__ void __launch_bounds__(kNT) k_tc_eval(const TNode *__restrict__ nodes, int nnodes,
                                                  const double *__restrict__ mom, const double *__restrict__ pos,
                                                  const double *__restrict__ qp, BoxGeom g, double theta,
+                                                 const int *__restrict__ targets, int ntargets,
                                                  double *__restrict__ bc, double *__restrict__ phi) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = t < boundary_count(g.nx, g.ny, g.nz);
-    int i = 0, j = 0, k = 0;
-    if (valid) boundary_cell(t, g.nx, g.ny, g.nz, i, j, k);
+    const int cell = t < ntargets ? targets[t] : -1;
+    const bool valid = cell >= 0;
+    const int i = valid ? cell % g.nx : 0, j = valid ? (cell / g.nx) % g.ny : 0,
+              k = valid ? cell / (g.nx * g.ny) : 0;
     double tx, ty, tz;
     cell_pos(g, i, j, k, tx, ty, tz);
     double part[kMaxDepth + 2];
@@ -788,9 +811,20 @@ __global__ void __launch_bounds__(kNT) k_tc_eval(const TNode *__restrict__ nodes
                 const double rx = tx - nd.cx, ry = ty - nd.cy, rz = tz - nd.cz;
                 const double ddx = fmax(fabs(rx) - nd.half, 0.0), ddy = fmax(fabs(ry) - nd.half, 0.0),
                              ddz = fmax(fabs(rz) - nd.half, 0.0);
-                const double dd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
                 const double size = nd.half * 2.0;
-                accept = dd > 0.0 ? (size / dd < theta) : false;
+                // the reference's test is size / sqrt(d2) < theta (sqrt(d2) > 0
+                // iff d2 > 0); outside a relative band of 1e-12 around equality
+                // the squared comparison decides it the same way (its rounding
+                // is below 1e-15), inside the band the exact test runs
+                if (d2 > 0.0) {
+                    const double lhs = size * size, rhs2 = theta * theta * d2;
+                    if (lhs < rhs2 * (1.0 - 1e-12)) accept = true;
+                    else if (lhs > rhs2 * (1.0 + 1e-12)) accept = false;
+                    else accept = size / sqrt(d2) < theta;
+                } else {
+                    accept = false;
+                }
                 if (accept) {
                     const double r = sqrt(rx * rx + ry * ry + rz * rz);
                     const double inv_r = 1.0 / r;
@@ -1269,6 +1303,8 @@ struct MGSolver::Tree {
     int64_t cap_cand = 0;
     int *d_cached = nullptr;
     int64_t cap_cached = 0;
+    int *d_targets = nullptr;   // boundary cells in face patches (-1 = idle lane)
+    int ntargets = 0;
     // tree
     TNode *d_nodes = nullptr;
     double *d_mom = nullptr, *d_pos = nullptr, *d_qp = nullptr;
@@ -1286,7 +1322,7 @@ struct MGSolver::Tree {
     ~Tree() {
         free_tree();
         for (void *p : {(void *)d_counts, (void *)d_offsets, (void *)d_meta, (void *)d_idx, (void *)d_q,
-                        (void *)d_cached})
+                        (void *)d_cached, (void *)d_targets})
             if (p) (void)hipFree(p);
         if (h_meta) (void)hipHostFree(h_meta);
     }
@@ -1617,8 +1653,32 @@ void MGSolver::treecode_bc() {
         LLGMI_LAUNCH_CHECK();
     }
     BoxGeom bg{px_, py_, pz_, L.dx, L.dy, L.dz, (double)px_ * 0.5, (double)py_ * 0.5, (double)pz_ * 0.5};
-    hipLaunchKernelGGL(k_tc_eval, dim3(nblocks(boundary_count(px_, py_, pz_))), dim3(kNT), 0, stream_, T.d_nodes,
-                       T.nnodes, T.d_mom, T.d_pos, T.d_qp, bg, cfg_.tree_theta, bc_, L.phi[L.cur]);
+    if (!T.d_targets) {
+        // boundary cells of the finest box in 8 x 8 patches of each face,
+        // each patch padded to 64 entries (one wave)
+        std::vector<int> tg;
+        auto face = [&](int nu, int nv, auto &&cell_of) {
+            for (int v0 = 0; v0 < nv; v0 += 8)
+                for (int u0 = 0; u0 < nu; u0 += 8)
+                    for (int l = 0; l < 64; ++l) {
+                        const int u = u0 + l % 8, v = v0 + l / 8;
+                        tg.push_back(u < nu && v < nv ? cell_of(u, v) : -1);
+                    }
+        };
+        const int nx = px_, ny = py_, nz = pz_;
+        auto at = [&](int i, int j, int k) { return (int)(((int64_t)k * ny + j) * nx + i); };
+        face(nx, ny, [&](int u, int v) { return at(u, v, 0); });
+        face(nx, ny, [&](int u, int v) { return at(u, v, nz - 1); });
+        face(nx, nz - 2, [&](int u, int v) { return at(u, 0, v + 1); });
+        face(nx, nz - 2, [&](int u, int v) { return at(u, ny - 1, v + 1); });
+        face(ny - 2, nz - 2, [&](int u, int v) { return at(0, u + 1, v + 1); });
+        face(ny - 2, nz - 2, [&](int u, int v) { return at(nx - 1, u + 1, v + 1); });
+        T.ntargets = (int)tg.size();
+        dev_alloc(T.d_targets, tg.size());
+        LLGMI_HIP(hipMemcpy(T.d_targets, tg.data(), sizeof(int) * tg.size(), hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(k_tc_eval, dim3(nblocks(T.ntargets)), dim3(kNT), 0, stream_, T.d_nodes, T.nnodes, T.d_mom,
+                       T.d_pos, T.d_qp, bg, cfg_.tree_theta, T.d_targets, T.ntargets, bc_, L.phi[L.cur]);
     LLGMI_LAUNCH_CHECK();
 }
 
