@@ -30,6 +30,7 @@
 
 #include <cstdlib>
 #include "internal.hpp"
+#include "local_cell.hpp"
 
 #include <cmath>
 #include <vector>
@@ -276,6 +277,89 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hxs, int nx, int ny, const cpx<T> *_
             if (ja < ny) o[(int64_t)ja * nx + i] = v[r].x;
             if (ja + 1 < ny) o[(int64_t)(ja + 1) * nx + i] = v[r].y;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3 fused with the local field: a block takes PAIRS pairs of rows for the
+// three components, inverse-transforms them, parks the real B_demag rows in
+// LDS and evaluates the local field of those rows (Zeeman, exchange,
+// anisotropy, DMI, + B_demag, optional torque) with the same per-cell code as
+// the stand-alone local-field kernel, so B_demag never goes through HBM.
+// ---------------------------------------------------------------------------
+template <typename T, int LOG2L, int RQ>
+struct FusedCfg {
+    using P = RegPlan<LOG2L, RQ>;
+    static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
+    static constexpr size_t kLdsCU = 160 * 1024;
+    static constexpr int PAIRS = (3 * 2 * TPS <= 1024 && sizeof(cpx<T>) * 3 * 2 * SLEN <= kLdsCU) ? 2 : 1;
+    static constexpr int NT = 3 * PAIRS * TPS;
+    static constexpr size_t LDS = sizeof(cpx<T>) * 3 * PAIRS * SLEN;
+    static constexpr bool OK = NT <= 1024 && LDS <= kLdsCU;
+};
+
+template <typename T, int LG, int RQ>
+constexpr bool fused_ok() { return FusedCfg<T, LG, RQ>::OK; }
+
+template <typename T, int LOG2L, bool HALF, int RQ, bool GEOM, bool TORQUE>
+__global__ void __launch_bounds__((FusedCfg<T, LOG2L, RQ>::NT))
+k_row_c2r_local(const cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw, LocalArgs<T> a,
+                const T *__restrict__ m, const uint8_t *__restrict__ geom, T *__restrict__ b,
+                T *__restrict__ kout) {
+    using C = FusedCfg<T, LOG2L, RQ>;
+    constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
+    constexpr int R0 = C::P::R0;
+    constexpr int NYQ = L >> 1;
+    extern __shared__ unsigned char lds_raw[];
+    cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
+    const int nx = a.nx;
+    const int j0 = xcd_remap(blockIdx.x, gridDim.x) * PAIRS * 2;
+    for (int idx = threadIdx.x; idx < NYQ * 3 * PAIRS; idx += NT) {
+        const int p = idx % PAIRS, rem = idx / PAIRS;
+        const int comp = rem % 3, k = rem / 3;
+        const cpx<T> *s = S + (int64_t)comp * hxs * ny;
+        const int ja = j0 + 2 * p, jb = ja + 1;
+        const cpx<T> xa = ja < ny ? s[(int64_t)k * ny + ja] : cpx<T>{T(0), T(0)};
+        const cpx<T> xb = jb < ny ? s[(int64_t)k * ny + jb] : cpx<T>{T(0), T(0)};
+        cpx<T> *row = buf + (comp * PAIRS + p) * SLEN;
+        if (k == 0) {
+            row[lds_pad(0)] = {xa.x, xb.x};
+            row[lds_pad(NYQ)] = {xa.y, xb.y};
+        } else {
+            row[lds_pad(k)] = {xa.x - xb.y, xa.y + xb.x};
+            row[lds_pad(L - k)] = {xa.x + xb.y, xb.x - xa.y};
+        }
+    }
+    __syncthreads();
+    const int sq = threadIdx.x / TPS, t = threadIdx.x - sq * TPS;
+    cpx<T> *seq = buf + sq * SLEN;
+    cpx<T> v[R0];
+#pragma unroll
+    for (int r = 0; r < R0; ++r) v[r] = seq[lds_pad(t + r * TPS)];
+    reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
+    __syncthreads();   // every sequence is out of LDS before the rows go in
+    T *bdl = reinterpret_cast<T *>(lds_raw);   // [3][2 PAIRS][NYQ]
+    {
+        const int comp = sq / PAIRS, p = sq - comp * PAIRS;
+        T *ra = bdl + (comp * 2 * PAIRS + 2 * p) * NYQ;
+        T *rb = ra + NYQ;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int i = t + r * TPS;
+            if (HALF ? r < R0 / 2 : i < nx) {
+                ra[i] = v[r].x;
+                rb[i] = v[r].y;
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < 2 * PAIRS * nx; q += NT) {
+        const int rr = q / nx, i = q - rr * nx;
+        const int j = j0 + rr;
+        if (j >= ny) continue;
+        const T bd3[3] = {bdl[(0 * 2 * PAIRS + rr) * NYQ + i], bdl[(1 * 2 * PAIRS + rr) * NYQ + i],
+                          bdl[(2 * 2 * PAIRS + rr) * NYQ + i]};
+        local_cell<T, GEOM, true, TORQUE>(a, m, geom, i, j, bd3, b, kout);
     }
 }
 
@@ -611,6 +695,25 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         allow_lds(k_row_c2r<T, LG, true, 4>, RowCfg<T, LG, 4>::LDS);
         allow_lds(k_row_c2r<T, LG, false, 4>, RowCfg<T, LG, 4>::LDS);
     });
+    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        auto allow = [&](auto rqc) {
+            constexpr int RQ = decltype(rqc)::value;
+            if constexpr (fused_ok<T, LG, RQ>()) {
+                constexpr size_t lds = FusedCfg<T, LG, RQ>::LDS;
+                allow_lds(k_row_c2r_local<T, LG, true, RQ, false, false>, lds);
+                allow_lds(k_row_c2r_local<T, LG, true, RQ, false, true>, lds);
+                allow_lds(k_row_c2r_local<T, LG, true, RQ, true, false>, lds);
+                allow_lds(k_row_c2r_local<T, LG, true, RQ, true, true>, lds);
+                allow_lds(k_row_c2r_local<T, LG, false, RQ, false, false>, lds);
+                allow_lds(k_row_c2r_local<T, LG, false, RQ, false, true>, lds);
+                allow_lds(k_row_c2r_local<T, LG, false, RQ, true, false>, lds);
+                allow_lds(k_row_c2r_local<T, LG, false, RQ, true, true>, lds);
+            }
+        };
+        allow(std::integral_constant<int, 3>{});
+        allow(std::integral_constant<int, 4>{});
+    });
     dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         if constexpr (col_rq_ok<LG, 3>()) {
@@ -771,6 +874,109 @@ void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *p
         ProfScope ps(prof, 2, stream);
         launch_row_inv(bd, stream, rq_row_inv);
     }
+}
+
+template <typename T>
+void DemagFFT<T>::launch_row_inv_local(const LocalArgs<T> &a, const T *m, const uint8_t *geom, T *b, T *kout,
+                                       hipStream_t stream, int rq) {
+    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        auto go = [&](auto rqc) {
+            constexpr int RQ = decltype(rqc)::value;
+            if constexpr (fused_ok<T, LG, RQ>()) {
+                using C = FusedCfg<T, LG, RQ>;
+                const dim3 grid((unsigned)((ny + 2 * C::PAIRS - 1) / (2 * C::PAIRS)));
+                const bool half = 2 * nx == px, g = geom != nullptr, q = kout != nullptr;
+#define LLGMI_FUSED(H, G, Q)                                                                          \
+    hipLaunchKernelGGL((k_row_c2r_local<T, LG, H, RQ, G, Q>), grid, dim3(C::NT), C::LDS, stream, spec, \
+                       (int)(px / 2), (int)ny, tw_x, a, m, geom, b, kout)
+                if (half) {
+                    if (g) { if (q) LLGMI_FUSED(true, true, true); else LLGMI_FUSED(true, true, false); }
+                    else { if (q) LLGMI_FUSED(true, false, true); else LLGMI_FUSED(true, false, false); }
+                } else {
+                    if (g) { if (q) LLGMI_FUSED(false, true, true); else LLGMI_FUSED(false, true, false); }
+                    else { if (q) LLGMI_FUSED(false, false, true); else LLGMI_FUSED(false, false, false); }
+                }
+#undef LLGMI_FUSED
+                LLGMI_LAUNCH_CHECK();
+            } else {
+                throw Error(3, "fused inverse row pass: block too large for this radix");
+            }
+        };
+        if (rq == 3) go(std::integral_constant<int, 3>{});
+        else go(std::integral_constant<int, 4>{});
+    });
+}
+
+// First call on a grid: the fused pass (radix 8 or 16, where the block fits)
+// and the unfused pair (inverse row pass + local-field kernel) are timed on
+// the real arguments and the fastest kept.  LLGMI_FFT_FUSED=0 / 1 forces the
+// unfused / fused path (fused with the radix of the inverse row pass).
+template <typename T>
+void DemagFFT<T>::field_local(const T *m, T ms, const LocalArgs<T> &a, const uint8_t *geom, T *b, T *kout,
+                              T *bd_scratch, hipStream_t stream, Profiler *prof) {
+    {
+        ProfScope ps(prof, 0, stream);
+        launch_row_fwd(m, ms, stream, rq_row);
+    }
+    {
+        ProfScope ps(prof, 1, stream);
+        launch_col(stream, rq_col);
+    }
+    auto fits = [&](int rq) {
+        bool ok = false;
+        dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+            constexpr int LG = decltype(lg)::value;
+            ok = rq == 3 ? fused_ok<T, LG, 3>() : fused_ok<T, LG, 4>();
+        });
+        return ok;
+    };
+    auto unfused = [&] {
+        launch_row_inv(bd_scratch, stream, rq_row_inv);
+        launch_local_field<T>(a, m, bd_scratch, geom, b, kout, stream);
+    };
+    if (rq_fused == 0) {
+        const char *f = std::getenv("LLGMI_FFT_FUSED");
+        const std::string fv = f ? std::string(f) : std::string();
+        if (fv == "0") {
+            rq_fused = -1;
+        } else if (fv == "1") {
+            rq_fused = fits(rq_row_inv) ? rq_row_inv : (fits(3) ? 3 : (fits(4) ? 4 : -1));
+        } else if (px * py < 256 * 256) {
+            rq_fused = fits(4) ? 4 : (fits(3) ? 3 : -1);
+        } else {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            try {
+                LLGMI_HIP(hipEventCreate(&e0));
+                LLGMI_HIP(hipEventCreate(&e1));
+                auto time = [&](auto &&launch) {
+                    LLGMI_HIP(hipEventRecord(e0, stream));
+                    for (int i = 0; i < 4; ++i) launch();
+                    LLGMI_HIP(hipEventRecord(e1, stream));
+                    LLGMI_HIP(hipEventSynchronize(e1));
+                    float ms_ = 0.f;
+                    LLGMI_HIP(hipEventElapsedTime(&ms_, e0, e1));
+                    return ms_;
+                };
+                float best[3] = {1e30f, 1e30f, 1e30f};   // unfused, fused radix 8, fused radix 16
+                for (int round = 0; round < 3; ++round) {
+                    best[0] = std::min(best[0], time(unfused));
+                    if (fits(3)) best[1] = std::min(best[1], time([&] { launch_row_inv_local(a, m, geom, b, kout, stream, 3); }));
+                    if (fits(4)) best[2] = std::min(best[2], time([&] { launch_row_inv_local(a, m, geom, b, kout, stream, 4); }));
+                }
+                rq_fused = best[0] <= best[1] && best[0] <= best[2] ? -1 : (best[1] <= best[2] ? 3 : 4);
+            } catch (...) {
+                if (e0) (void)hipEventDestroy(e0);
+                if (e1) (void)hipEventDestroy(e1);
+                throw;
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+    }
+    ProfScope ps(prof, 2, stream);
+    if (rq_fused < 0) unfused();
+    else launch_row_inv_local(a, m, geom, b, kout, stream, rq_fused);
 }
 
 template <typename T>

@@ -312,17 +312,15 @@ __global__ void __launch_bounds__(RBX * RBY) k_mg_rb_sweep(const double *__restr
 // place half-sweep by half-sweep (each half-sweep leaves one more outer ring
 // stale), and writes the central TBT x TBT columns to B; after h <= TBH
 // half-sweeps those are exactly the values the in-place half-sweeps produce.
-// LDS rows are stored de-interleaved (even x, then odd x), so the cells of one
-// colour and their x, y, z neighbours are contiguous across the lanes.
-constexpr int TBT = 16, TBH = 8, TBW = TBT + 2 * TBH, TBHW = TBW / 2, TB_MAXZ = 9, TB_NT = 1024;
-__device__ __forceinline__ int tb_at(int k, int hj, int hi) { return ((k * TBW + hj) * 2 + (hi & 1)) * TBHW + (hi >> 1); }
+constexpr int TBT = 16, TBH = 8, TBW = TBT + 2 * TBH, TB_MAXZ = 9, TB_NT = 1024;
+__device__ __forceinline__ int tb_at(int k, int hj, int hi) { return (k * TBW + hj) * TBW + hi; }
 __global__ void __launch_bounds__(TB_NT) k_mg_rb_tb(const double *__restrict__ A, double *__restrict__ B,
                                                     const double *__restrict__ rhs, int nx, int ny, int nz,
                                                     double sx, double sy, double sz, double denom,
                                                     double omega, int color, int nhalf) {
     extern __shared__ unsigned char lds_raw[];
     const int nbox = nz * TBW * TBW;
-    double *box = reinterpret_cast<double *>(lds_raw);   // A, [nz][TBW][2][TBHW]
+    double *box = reinterpret_cast<double *>(lds_raw);   // A, [nz][TBW][TBW]
     double *rb = box + nbox;                               // rhs, same layout
     const int tid = threadIdx.x;
     const int i0 = 1 + (int)blockIdx.x * TBT, j0 = 1 + (int)blockIdx.y * TBT;

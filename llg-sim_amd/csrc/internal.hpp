@@ -77,6 +77,9 @@ struct ProfScope {
 // FFT demag operator for one grid (Ms independent), precision T.
 // ---------------------------------------------------------------------------
 template <typename T>
+struct LocalArgs;
+
+template <typename T>
 struct DemagFFT {
     bool built = false;
     int64_t nx = 0, ny = 0, px = 0, py = 0, hx = 0, hy = 0;
@@ -86,6 +89,7 @@ struct DemagFFT {
     T *kern = nullptr;  // 4 planes [Kxx, Kxy, Kyy, Kzz], each [hx][hy]
     cpx<T> *spec = nullptr;    // [3][hx][ny]
     int rq_row = 4, rq_col = 4, rq_row_inv = 4;  // log2 first/last radix, picked by autotune()
+    int rq_fused = 0;   // inverse row pass fused with the local field: log2 radix, -1 = unfused, 0 = untuned
 
     ~DemagFFT();
     void release();
@@ -97,6 +101,13 @@ struct DemagFFT {
     void launch_row_fwd(const T *m, T ms, hipStream_t stream, int rq);
     void launch_col(hipStream_t stream, int rq);
     void launch_row_inv(T *bd, hipStream_t stream, int rq);
+    // b (overwrite) = local field terms + B_demag of M = ms * m, with the inverse
+    // row FFT and the local field in one kernel (kout: torque, or null);
+    // bd_scratch (3 n) serves the unfused variant
+    void field_local(const T *m, T ms, const LocalArgs<T> &a, const uint8_t *geom, T *b, T *kout,
+                     T *bd_scratch, hipStream_t stream, Profiler *prof);
+    void launch_row_inv_local(const LocalArgs<T> &a, const T *m, const uint8_t *geom, T *b, T *kout,
+                              hipStream_t stream, int rq);
     void read_kernel(std::vector<double> &out) const;
 };
 

@@ -169,8 +169,15 @@ struct Engine : EngineBase {
     void field(const T *m, T *b, const llgmi_params &p, const llgmi_material &mat, int mask,
                const uint8_t *geom, T *kout) {
         const bool demag = mask == LLGMI_MASK_FULL && mat.demag;
-        if (demag) demag_into_bd(m, mat, resolved_demag_method(mat), 0, 0);
         LocalArgs<T> a = local_args(p, mat, mask, kout != nullptr);
+        if (demag && resolved_demag_method(mat) == LLGMI_DEMAG_FFT) {
+            // FFT demag: inverse row pass and local field share one kernel
+            ensure_fft();
+            alloc(d_bd, 3 * n);
+            fft.field_local(m, T(mat.ms), a, geom, b, kout, d_bd, stream, prof);
+            return;
+        }
+        if (demag) demag_into_bd(m, mat, resolved_demag_method(mat), 0, 0);
         ProfScope ps(prof, LLGMI_K_LOCAL_FIELD, stream);
         launch_local_field<T>(a, m, demag ? d_bd : nullptr, geom, b, kout, stream);
     }

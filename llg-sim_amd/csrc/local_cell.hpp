@@ -1,0 +1,147 @@
+// Per-cell local effective field (Zeeman, exchange, uniaxial anisotropy,
+// interfacial DMI, optional demag addend, optional LLG torque), shared by the
+// stand-alone local-field kernel and the kernel that fuses it with the
+// inverse row FFT of the demag field.
+//
+// Terms follow llg-sim's effective_field/ (zeeman.rs:6-11,
+// exchange.rs:45-222, anisotropy.rs:10-25, dmi.rs:67-209), summed in the
+// order of build_h_eff_masked (mod.rs:48-75); the torque is llg_rhs
+// (llg.rs:20-32).
+#pragma once
+
+#include "internal.hpp"
+
+namespace llgmi {
+
+template <typename T>
+__device__ __forceinline__ void ghost_x(const T *m, T n_x, T eta, T dx, T *o) {
+    o[0] = m[0] + n_x * dx * (-eta * m[2]);
+    o[1] = m[1];
+    o[2] = m[2] + n_x * dx * (eta * m[0]);
+}
+template <typename T>
+__device__ __forceinline__ void ghost_y(const T *m, T n_y, T eta, T dy, T *o) {
+    o[0] = m[0];
+    o[1] = m[1] + n_y * dy * (-eta * m[2]);
+    o[2] = m[2] + n_y * dy * (eta * m[1]);
+}
+
+// Cell (i, j) of an nx x ny SoA field; bd3 = the cell's demag field (DEMAG).
+template <typename T, bool GEOM, bool DEMAG, bool TORQUE>
+__device__ __forceinline__ void local_cell(const LocalArgs<T> &a, const T *__restrict__ m,
+                                           const uint8_t *__restrict__ geom, int i, int j, const T *bd3,
+                                           T *__restrict__ b, T *__restrict__ kout) {
+    const int nx = a.nx, ny = a.ny;
+    const int64_t n = (int64_t)nx * ny;
+    const int64_t c = (int64_t)j * nx + i;
+    const T *mx = m, *my = m + n, *mz = m + 2 * n;
+    T m0[3] = {mx[c], my[c], mz[c]};
+    T bv[3] = {a.bext[0], a.bext[1], a.bext[2]};  // b = 0 + B_ext
+    const bool inside = !GEOM || geom[c] != 0;
+
+    if ((a.exch_on || a.dmi_on) && inside) {
+        // raw neighbours (valid = exists in the domain and is material)
+        T nl[3], nr[3], nd[3], nu[3];
+        bool vl = false, vr = false, vd = false, vu = false;
+        bool el = nx > 1 && i == 0, er = nx > 1 && i == nx - 1;   // domain edges
+        bool ed = ny > 1 && j == 0, eu = ny > 1 && j == ny - 1;
+        if (nx > 1 && i > 0) {
+            int64_t q = c - 1;
+            vl = !GEOM || geom[q] != 0;
+            if (vl) { nl[0] = mx[q]; nl[1] = my[q]; nl[2] = mz[q]; }
+        }
+        if (nx > 1 && i < nx - 1) {
+            int64_t q = c + 1;
+            vr = !GEOM || geom[q] != 0;
+            if (vr) { nr[0] = mx[q]; nr[1] = my[q]; nr[2] = mz[q]; }
+        }
+        if (ny > 1 && j > 0) {
+            int64_t q = c - nx;
+            vd = !GEOM || geom[q] != 0;
+            if (vd) { nd[0] = mx[q]; nd[1] = my[q]; nd[2] = mz[q]; }
+        }
+        if (ny > 1 && j < ny - 1) {
+            int64_t q = c + nx;
+            vu = !GEOM || geom[q] != 0;
+            if (vu) { nu[0] = mx[q]; nu[1] = my[q]; nu[2] = mz[q]; }
+        }
+        if (a.exch_on) {
+            // exchange.rs:96-204: domain edge or masked neighbour -> chiral ghost
+            // when D != 0, else the centre value; nx == 1 -> centre value.
+            T xm[3], xp[3], ym[3], yp[3];
+            if (vl) { xm[0] = nl[0]; xm[1] = nl[1]; xm[2] = nl[2]; }
+            else if (a.has_eta && nx > 1) ghost_x(m0, T(-1), a.eta, a.dx, xm);
+            else { xm[0] = m0[0]; xm[1] = m0[1]; xm[2] = m0[2]; }
+            if (vr) { xp[0] = nr[0]; xp[1] = nr[1]; xp[2] = nr[2]; }
+            else if (a.has_eta && nx > 1) ghost_x(m0, T(1), a.eta, a.dx, xp);
+            else { xp[0] = m0[0]; xp[1] = m0[1]; xp[2] = m0[2]; }
+            if (vd) { ym[0] = nd[0]; ym[1] = nd[1]; ym[2] = nd[2]; }
+            else if (a.has_eta && ny > 1) ghost_y(m0, T(-1), a.eta, a.dy, ym);
+            else { ym[0] = m0[0]; ym[1] = m0[1]; ym[2] = m0[2]; }
+            if (vu) { yp[0] = nu[0]; yp[1] = nu[1]; yp[2] = nu[2]; }
+            else if (a.has_eta && ny > 1) ghost_y(m0, T(1), a.eta, a.dy, yp);
+            else { yp[0] = m0[0]; yp[1] = m0[1]; yp[2] = m0[2]; }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                T d2x = nx > 1 ? (xp[q] - T(2) * m0[q] + xm[q]) * a.inv_dx2 : T(0);
+                T d2y = ny > 1 ? (yp[q] - T(2) * m0[q] + ym[q]) * a.inv_dy2 : T(0);
+                bv[q] += a.exch_coeff * (d2x + d2y);
+            }
+        }
+        if (a.anis_on) {
+            T mdotu = m0[0] * a.u[0] + m0[1] * a.u[1] + m0[2] * a.u[2];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) bv[q] += a.anis_coeff * mdotu * a.u[q];
+        }
+        if (a.dmi_on) {
+            // dmi.rs:123-181: domain edge -> chiral ghost; masked neighbour -> centre.
+            T xm[3], xp[3], ym[3], yp[3];
+            if (el) ghost_x(m0, T(-1), a.eta, a.dx, xm);
+            else if (vl) { xm[0] = nl[0]; xm[1] = nl[1]; xm[2] = nl[2]; }
+            else { xm[0] = m0[0]; xm[1] = m0[1]; xm[2] = m0[2]; }
+            if (er) ghost_x(m0, T(1), a.eta, a.dx, xp);
+            else if (vr) { xp[0] = nr[0]; xp[1] = nr[1]; xp[2] = nr[2]; }
+            else { xp[0] = m0[0]; xp[1] = m0[1]; xp[2] = m0[2]; }
+            if (ed) ghost_y(m0, T(-1), a.eta, a.dy, ym);
+            else if (vd) { ym[0] = nd[0]; ym[1] = nd[1]; ym[2] = nd[2]; }
+            else { ym[0] = m0[0]; ym[1] = m0[1]; ym[2] = m0[2]; }
+            if (eu) ghost_y(m0, T(1), a.eta, a.dy, yp);
+            else if (vu) { yp[0] = nu[0]; yp[1] = nu[1]; yp[2] = nu[2]; }
+            else { yp[0] = m0[0]; yp[1] = m0[1]; yp[2] = m0[2]; }
+            T dmz_dx = nx > 1 ? (xp[2] - xm[2]) * a.inv_2dx : T(0);
+            T dmz_dy = ny > 1 ? (yp[2] - ym[2]) * a.inv_2dy : T(0);
+            T dmx_dx = nx > 1 ? (xp[0] - xm[0]) * a.inv_2dx : T(0);
+            T dmy_dy = ny > 1 ? (yp[1] - ym[1]) * a.inv_2dy : T(0);
+            bv[0] += a.dmi_pref * dmz_dx;
+            bv[1] += a.dmi_pref * dmz_dy;
+            bv[2] += -a.dmi_pref * (dmx_dx + dmy_dy);
+        }
+    } else if (a.anis_on) {
+        // anisotropy is not masked (mod.rs:91)
+        T mdotu = m0[0] * a.u[0] + m0[1] * a.u[1] + m0[2] * a.u[2];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) bv[q] += a.anis_coeff * mdotu * a.u[q];
+    }
+    if (DEMAG) {
+        bv[0] += bd3[0];
+        bv[1] += bd3[1];
+        bv[2] += bd3[2];
+    }
+    b[c] = bv[0];
+    b[n + c] = bv[1];
+    b[2 * n + c] = bv[2];
+    if (TORQUE) {
+        // llg_rhs: -gamma/(1+alpha^2) [m x B + alpha m x (m x B)]
+        T cx = m0[1] * bv[2] - m0[2] * bv[1];
+        T cy = m0[2] * bv[0] - m0[0] * bv[2];
+        T cz = m0[0] * bv[1] - m0[1] * bv[0];
+        T ccx = m0[1] * cz - m0[2] * cy;
+        T ccy = m0[2] * cx - m0[0] * cz;
+        T ccz = m0[0] * cy - m0[1] * cx;
+        kout[c] = a.rhs_pref * (cx + a.alpha * ccx);
+        kout[n + c] = a.rhs_pref * (cy + a.alpha * ccy);
+        kout[2 * n + c] = a.rhs_pref * (cz + a.alpha * ccz);
+    }
+}
+
+}  // namespace llgmi
