@@ -50,7 +50,7 @@ ALPHA = 0.3
 # HBM bytes per launch of the column pass at this workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE counters (profiles/field_eval_1024_fp32_pmc_traffic.md),
 # FETCH_SIZE scaled by 2.0 (a plain 256 MiB copy reads as 128 MiB).
-COL_TRAFFIC_BYTES = None
+COL_TRAFFIC_BYTES = 67_780_000
 
 
 def col_pass_bytes(nx, ny, elem):
