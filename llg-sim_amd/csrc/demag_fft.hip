@@ -44,12 +44,20 @@ namespace llgmi {
 __host__ __device__ constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 __host__ __device__ constexpr int mini(int a, int b) { return a < b ? a : b; }
 
+// Row blocks: as many row pairs as fit the LDS / thread budget (more pairs per
+// block = longer contiguous runs in the column-major spectrum).
+#ifndef LLGMI_ROW_LDS_KB
+#define LLGMI_ROW_LDS_KB 80
+#endif
+#ifndef LLGMI_ROW_NT
+#define LLGMI_ROW_NT 512
+#endif
 template <typename T, int LOG2L, int RQ = 0>
 struct RowCfg {
     using P = RegPlan<LOG2L, RQ>;
     static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
-    static constexpr int PAIRS =
-        clampi(mini((int)(80 * 1024 / (sizeof(cpx<T>) * SLEN)), 512 / TPS), 1, 64);
+    static constexpr int PAIRS = clampi(mini((int)(LLGMI_ROW_LDS_KB * 1024 / (sizeof(cpx<T>) * SLEN)),
+                                             LLGMI_ROW_NT / TPS), 1, 64);
     static constexpr int NT = PAIRS * TPS;
     static constexpr size_t LDS = sizeof(cpx<T>) * PAIRS * SLEN;
 };
