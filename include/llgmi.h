@@ -228,6 +228,99 @@ int llgmi_compute_demag_field_poisson_mg(llgmi_ctx *ctx, const double *m, double
                                          const llgmi_material *mat);
 int llgmi_mg_last_stats(llgmi_ctx *ctx, llgmi_mg_stats *out);
 
+/* ---- relaxation (llg.rs:38-50, 196-436; energy.rs; relax.rs) -------------------- */
+/* TorqueMetric (relax.rs:32-38) */
+#define LLGMI_TORQUE_MAX 0
+#define LLGMI_TORQUE_MEAN 1
+#define LLGMI_TORQUE_RMS 2
+#define LLGMI_TORQUE_MEANSQ 3
+/* RelaxStopReason (relax.rs:40-46) */
+#define LLGMI_STOP_MAX_ACCEPTED_STEPS 0
+#define LLGMI_STOP_PHASE2_DISABLED 1
+#define LLGMI_STOP_TIGHTEN_FLOOR_REACHED 2
+#define LLGMI_STOP_TORQUE_GATE_NOT_SATISFIED 3
+/* RelaxStageStop (relax.rs:48-53); NONE = None */
+#define LLGMI_STAGE_NONE (-1)
+#define LLGMI_STAGE_BELOW_THRESHOLD 0
+#define LLGMI_STAGE_PLATEAU 1
+#define LLGMI_STAGE_MAX_ACCEPTED_STEPS 2
+
+/* RelaxSettings (relax.rs:73-105); has_* = 0 is None for the Option fields.
+ * 176 bytes, no padding. */
+typedef struct llgmi_relax_settings {
+    double max_err, headroom, dt_min, dt_max;
+    int32_t phase1_enabled, phase2_enabled;
+    int64_t energy_stride;
+    double rel_energy_tol;
+    int32_t torque_metric;            /* LLGMI_TORQUE_* */
+    int32_t has_torque_threshold;
+    double torque_threshold;
+    int64_t torque_check_stride;
+    double torque_plateau_rel, torque_plateau_abs;
+    int64_t torque_plateau_checks, torque_plateau_min_checks;
+    double tighten_factor, tighten_floor;
+    int64_t max_accepted_steps;
+    int32_t has_final_torque_gate, has_final_torque_gate_max;
+    double final_torque_gate, final_torque_gate_max;
+    int64_t gate_max_extra_accepted_steps, gate_plateau_fails;
+} llgmi_relax_settings;
+
+/* RelaxReport (relax.rs:55-71).  80 bytes. */
+typedef struct llgmi_relax_report {
+    int64_t accepted_steps, rejected_steps, torque_checks, torque_field_rebuilds;
+    int32_t stop_reason;              /* LLGMI_STOP_* */
+    int32_t last_stage_stop;          /* LLGMI_STAGE_* */
+    int32_t has_final_torque, has_final_torque_max;
+    double final_torque, final_torque_max, final_max_err, final_dt;
+} llgmi_relax_report;
+
+/* EnergyBreakdown (energy.rs:21-34), joules. */
+typedef struct llgmi_energy {
+    double exchange, anisotropy, zeeman, dmi, demag;
+} llgmi_energy;
+
+/* RelaxSettings::default() (relax.rs:107-139). */
+int llgmi_relax_settings_default(llgmi_relax_settings *s);
+/* mumax_like_relax_settings (relax.rs:789-808). */
+int llgmi_relax_settings_mumax_like(llgmi_relax_settings *s);
+
+/* step_llg_rk23_recompute_field_masked_relax_adaptive (llg.rs:278-436):
+ * damping-only Bogacki-Shampine RK23 step with FSAL.  m (AoS f64) advances in
+ * place when accepted; params->dt becomes the next step size.  The RK23
+ * scratch and its FSAL / last-field state (RK23Scratch, llg.rs:196-268) live
+ * in the context, apart from the RK45 state. */
+int llgmi_rk23_relax_step(llgmi_ctx *ctx, double *m, llgmi_params *params,
+                          const llgmi_material *mat, int field_mask, double max_err,
+                          double headroom, double dt_min, double dt_max, double *eps,
+                          int *accepted, double *dt_used);
+/* RK23Scratch::invalidate_last_b_eff (llg.rs:264-268). */
+int llgmi_rk23_invalidate(llgmi_ctx *ctx);
+/* RK23Scratch::last_b_eff (llg.rs:252-258): *valid = 1 and b_out (AoS f64)
+ * filled when the field of the accepted state is held; *valid = 0 otherwise. */
+int llgmi_rk23_last_b_eff(llgmi_ctx *ctx, double *b_out, int *valid);
+/* compute_energy_geom (energy.rs:80-283); geom_mask NULL = None.  The demag
+ * term evaluates the demag field through the same dispatch as
+ * llgmi_add_demag_field. */
+int llgmi_compute_energy_geom(llgmi_ctx *ctx, const double *m, const llgmi_material *mat,
+                              const double b_ext[3], const uint8_t *geom_mask,
+                              llgmi_energy *out);
+/* relax_with_report (relax.rs:334-774): m (AoS f64) and params->dt in place;
+ * settings->max_err is written back as the reference does. */
+int llgmi_relax_with_report(llgmi_ctx *ctx, double *m, llgmi_params *params,
+                            const llgmi_material *mat, int field_mask,
+                            llgmi_relax_settings *settings, llgmi_relax_report *report);
+/* Device-resident forms (SoA, context precision; geom_mask_dev on the device). */
+int llgmi_dev_rk23_relax_step(llgmi_ctx *ctx, void *m_dev, llgmi_params *params,
+                              const llgmi_material *mat, int field_mask, double max_err,
+                              double headroom, double dt_min, double dt_max, double *eps,
+                              int *accepted, double *dt_used);
+int llgmi_dev_compute_energy_geom(llgmi_ctx *ctx, const void *m_dev, const llgmi_material *mat,
+                                  const double b_ext[3], const uint8_t *geom_mask_dev,
+                                  llgmi_energy *out);
+int llgmi_dev_relax_with_report(llgmi_ctx *ctx, void *m_dev, llgmi_params *params,
+                                const llgmi_material *mat, int field_mask,
+                                llgmi_relax_settings *settings, llgmi_relax_report *report);
+
 /* ---- per-kernel timing (HIP events on the context stream) ---------------------- */
 /* Kernel classes timed when profiling is on. */
 #define LLGMI_K_FFT_ROW_FWD 0   /* row R2C of Ms*m (zero padded) */
@@ -238,7 +331,8 @@ int llgmi_mg_last_stats(llgmi_ctx *ctx, llgmi_mg_stats *out);
 #define LLGMI_K_RK45_ERR 5      /* error-norm reduction */
 #define LLGMI_K_MG 6            /* multigrid RHS, smoothing, transfers, residuals, gradient */
 #define LLGMI_K_MG_BC 7         /* multigrid boundary values (treecode / dipole) */
-#define LLGMI_K_COUNT 8
+#define LLGMI_K_RELAX 8         /* RK23 stage states, error norm, energy and torque reductions */
+#define LLGMI_K_COUNT 9
 /* Times the kernel classes whose bit (1 << class) is set in class_mask; 0 stops. */
 int llgmi_profile_enable(llgmi_ctx *ctx, int class_mask);
 int llgmi_profile_reset(llgmi_ctx *ctx);

@@ -18,8 +18,9 @@ struct Profiler {
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
-    double total_ms[8] = {0};
-    int64_t count[8] = {0};
+    static constexpr int kClasses = 16;   // >= LLGMI_K_COUNT
+    double total_ms[kClasses] = {0};
+    int64_t count[kClasses] = {0};
 
     hipEvent_t get() {
         if (!pool.empty()) {
@@ -45,7 +46,7 @@ struct Profiler {
     }
     void reset() {
         drain();
-        for (int i = 0; i < 8; ++i) { total_ms[i] = 0; count[i] = 0; }
+        for (int i = 0; i < kClasses; ++i) { total_ms[i] = 0; count[i] = 0; }
     }
     ~Profiler() {
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -129,8 +130,8 @@ struct LocalArgs {
     int dmi_on;            // mask includes DMI and D != 0, A != 0, Ms != 0
     T dmi_pref;            // 2D/Ms
     T inv_2dx, inv_2dy;
-    int torque;            // also write k = llg_rhs(m, B)
-    T rhs_pref;            // -gamma/(1+alpha^2)
+    int torque;            // also write k: 1 = llg_rhs(m, B), 2 = llg_rhs_relax(m, B)
+    T rhs_pref;            // -gamma/(1+alpha^2), or -gamma alpha/(1+alpha^2) for relax
     T alpha;
 };
 
@@ -165,5 +166,40 @@ template <typename T>
 void launch_rk_err(const T *k1, const T *k3, const T *k4, const T *k5, const T *k6,
                    const T *k7, int64_t n, bool combined, double *partial, double *out,
                    hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Relaxation path (relax.hip)
+// ---------------------------------------------------------------------------
+static constexpr int kRelaxRedBlocks = 1024;   // partial rows of the reductions below
+
+// dt-scaled Bogacki-Shampine weights: b = third order (k1..k3), bh = embedded
+// second order (k1..k4)
+struct Rk23ErrCoef {
+    double b[3];
+    double bh[4];
+};
+// eps = max |m4 - m2nd| (llg.rs:399-420), written to out[0]; partial holds
+// kRelaxRedBlocks doubles.
+template <typename T>
+void launch_rk23_err(const T *m, const T *k1, const T *k2, const T *k3, const T *k4, const T *m4,
+                     const Rk23ErrCoef &c, int64_t n, double *partial, double *out, hipStream_t s);
+
+// compute_energy_geom (energy.rs:80-283) coefficients
+struct EnergyArgs {
+    int nx, ny;
+    double dx, dy, v;      // v = cell volume
+    double aex, ku, ms;
+    double u[3], bext[3];
+    int dmi_on;            // Some(D != 0) and A != 0
+    double pref, eta;      // 2D/Ms, D/(2A)
+};
+// out[0..4] = exchange, anisotropy, zeeman, dmi, demag; partial: 5 kRelaxRedBlocks doubles;
+// bd (demag field) and geom may be null.
+template <typename T>
+void launch_energy(const EnergyArgs &a, const T *m, const T *bd, const uint8_t *geom,
+                   double *partial, double *out, hipStream_t s);
+// out[0..2] = max |m x B|, sum |m x B|, sum |m x B|^2; partial: 3 kRelaxRedBlocks doubles
+template <typename T>
+void launch_torque(const T *m, const T *b, int64_t n, double *partial, double *out, hipStream_t s);
 
 }  // namespace llgmi

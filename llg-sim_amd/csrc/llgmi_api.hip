@@ -84,15 +84,30 @@ struct Engine : EngineBase {
     int last_mask = -1;
     double last_bext[3] = {0, 0, 0};
 
+    // RK23 relax scratch (RK23Scratch, llg.rs:196-268): k1..k4, b4 and m4 of
+    // the last step, FSAL / last-field state.  Stage states 2 and 3 share d_stage.
+    T *d_r23k[5] = {nullptr};
+    T *d_r23b4 = nullptr, *d_r23m4 = nullptr;
+    bool r23_b4_valid = false;   // last_b_eff_valid
+    bool r23_has_last = false;   // last_mask / last_b_ext are Some
+    int r23_last_mask = -1;
+    double r23_last_bext[3] = {0, 0, 0};
+    // reduction rows and results of the energy / torque / RK23 error kernels
+    double *d_red = nullptr, *d_redout = nullptr, *h_red = nullptr;
+
     Engine(const llgmi_grid &g_, hipStream_t s, Profiler *p) : g(g_), n(g_.nx * g_.ny), stream(s), prof(p) {}
 
     ~Engine() override {
-        void *ps[] = {d_m, d_b, d_bd, d_aos, d_geom, d_stage, d_m7, d_partial, d_err};
+        void *ps[] = {d_m, d_b, d_bd, d_aos, d_geom, d_stage, d_m7, d_partial, d_err,
+                      d_r23b4, d_r23m4, d_red, d_redout};
         for (void *q : ps)
             if (q) (void)hipFree(q);
         for (int i = 0; i < 8; ++i)
             if (d_k[i]) (void)hipFree(d_k[i]);
+        for (int i = 0; i < 5; ++i)
+            if (d_r23k[i]) (void)hipFree(d_r23k[i]);
         if (h_err) (void)hipHostFree(h_err);
+        if (h_red) (void)hipHostFree(h_red);
     }
 
     template <typename U>
@@ -137,7 +152,8 @@ struct Engine : EngineBase {
         }
     }
 
-    LocalArgs<T> local_args(const llgmi_params &p, const llgmi_material &mat, int mask, bool torque) const {
+    LocalArgs<T> local_args(const llgmi_params &p, const llgmi_material &mat, int mask, bool torque,
+                            bool relax = false) const {
         LocalArgs<T> a{};
         a.nx = (int)g.nx;
         a.ny = (int)g.ny;
@@ -159,17 +175,19 @@ struct Engine : EngineBase {
         a.dmi_pref = T(mat.ms != 0.0 ? 2.0 * mat.dmi / mat.ms : 0.0);
         a.inv_2dx = T(1.0 / (2.0 * g.dx));
         a.inv_2dy = T(1.0 / (2.0 * g.dy));
-        a.torque = torque ? 1 : 0;
-        a.rhs_pref = T(-p.gamma / (1.0 + p.alpha * p.alpha));
+        a.torque = torque ? (relax ? 2 : 1) : 0;
+        a.rhs_pref = relax ? T(-p.gamma * p.alpha / (1.0 + p.alpha * p.alpha))
+                           : T(-p.gamma / (1.0 + p.alpha * p.alpha));
         a.alpha = T(p.alpha);
         return a;
     }
 
     // build_h_eff_masked[_geom] on device buffers; optional torque output.
+    // relax: kout = llg_rhs_relax instead of llg_rhs.
     void field(const T *m, T *b, const llgmi_params &p, const llgmi_material &mat, int mask,
-               const uint8_t *geom, T *kout) {
+               const uint8_t *geom, T *kout, bool relax = false) {
         const bool demag = mask == LLGMI_MASK_FULL && mat.demag;
-        LocalArgs<T> a = local_args(p, mat, mask, kout != nullptr);
+        LocalArgs<T> a = local_args(p, mat, mask, kout != nullptr, relax);
         if (demag && resolved_demag_method(mat) == LLGMI_DEMAG_FFT) {
             // FFT demag: inverse row pass and local field share one kernel
             ensure_fft();
@@ -303,7 +321,311 @@ struct Engine : EngineBase {
         acc_out = accept ? 1 : 0;
         dtu_out = dt0;
     }
+
+    // ---- relaxation ------------------------------------------------------------
+    void ensure_red() {
+        alloc(d_red, (size_t)5 * kRelaxRedBlocks);
+        alloc(d_redout, 8);
+        if (!h_red) LLGMI_HIP(hipHostMalloc((void **)&h_red, 8 * sizeof(double), hipHostMallocDefault));
+    }
+    void read_red(int count) {
+        LLGMI_HIP(hipMemcpyAsync(h_red, d_redout, sizeof(double) * count, hipMemcpyDeviceToHost, stream));
+        LLGMI_HIP(hipStreamSynchronize(stream));
+    }
+    void ensure_rk23() {
+        for (int s = 1; s <= 4; ++s) alloc(d_r23k[s], 3 * n);
+        alloc(d_r23b4, 3 * n);
+        alloc(d_r23m4, 3 * n);
+        alloc(d_stage, 3 * n);
+        alloc(d_b, 3 * n);
+        ensure_red();
+    }
+    void rk23_invalidate() {
+        r23_b4_valid = false;
+        r23_has_last = false;
+        r23_last_mask = -1;
+    }
+
+    // step_llg_rk23_recompute_field_masked_relax_adaptive (llg.rs:278-436) on device state m.
+    void rk23(T *m, llgmi_params &p, const llgmi_material &mat, int mask, double max_err,
+              double headroom, double dt_min, double dt_max, double &eps_out, int &acc_out,
+              double &dtu_out) {
+        const double A21 = 1.0 / 2.0, A32 = 3.0 / 4.0;
+        const double B1 = 2.0 / 9.0, B2 = 1.0 / 3.0, B3 = 4.0 / 9.0;
+        const double BH1 = 7.0 / 24.0, BH2 = 1.0 / 4.0, BH3 = 1.0 / 3.0, BH4 = 1.0 / 8.0;
+        ensure_rk23();
+        T **K = d_r23k;
+        const double dt0 = p.dt;
+        const bool fsal_ok = r23_b4_valid && r23_has_last && r23_last_mask == mask &&
+                             r23_last_bext[0] == p.b_ext[0] && r23_last_bext[1] == p.b_ext[1] &&
+                             r23_last_bext[2] == p.b_ext[2];
+        // FSAL: k1 <- k4 of the accepted state (b1 is not read again)
+        if (fsal_ok) std::swap(K[1], K[4]);
+        else field(m, d_b, p, mat, mask, nullptr, K[1], true);
+        auto stage = [&](std::initializer_list<std::pair<const T *, double>> terms, T *out) {
+            StageArgs<T> sa{};
+            sa.nk = 0;
+            for (auto &t : terms) {
+                sa.k[sa.nk] = t.first;
+                sa.c[sa.nk] = T(dt0 * t.second);
+                ++sa.nk;
+            }
+            ProfScope ps(prof, LLGMI_K_RELAX, stream);
+            launch_rk_stage<T>(m, sa, out, n, stream);
+        };
+        stage({{K[1], A21}}, d_stage);
+        field(d_stage, d_b, p, mat, mask, nullptr, K[2], true);
+        stage({{K[2], A32}}, d_stage);
+        field(d_stage, d_b, p, mat, mask, nullptr, K[3], true);
+        stage({{K[1], B1}, {K[2], B2}, {K[3], B3}}, d_r23m4);
+        field(d_r23m4, d_r23b4, p, mat, mask, nullptr, K[4], true);
+        {
+            Rk23ErrCoef c{{dt0 * B1, dt0 * B2, dt0 * B3}, {dt0 * BH1, dt0 * BH2, dt0 * BH3, dt0 * BH4}};
+            ProfScope ps(prof, LLGMI_K_RELAX, stream);
+            launch_rk23_err<T>(m, K[1], K[2], K[3], K[4], d_r23m4, c, n, d_red, d_redout, stream);
+        }
+        read_red(1);
+        const double eps = h_red[0];
+        // controller (llg.rs:422-435), FP64 on the host
+        double dt_next = eps > 0.0 ? dt0 * headroom * std::pow(max_err / eps, 1.0 / 3.0) : dt0 * 2.0;
+        if (dt_next < dt_min) dt_next = dt_min;
+        if (dt_next > dt_max) dt_next = dt_max;
+        const bool accept = (eps <= max_err) || (dt0 <= dt_min * 1.0000000001);
+        if (accept) {
+            LLGMI_HIP(hipMemcpyAsync(m, d_r23m4, sizeof(T) * 3 * n, hipMemcpyDeviceToDevice, stream));
+            r23_b4_valid = true;
+            r23_has_last = true;
+            r23_last_mask = mask;
+            for (int q = 0; q < 3; ++q) r23_last_bext[q] = p.b_ext[q];
+        } else {
+            rk23_invalidate();
+        }
+        p.dt = dt_next;
+        eps_out = eps;
+        acc_out = accept ? 1 : 0;
+        dtu_out = dt0;
+    }
+
+    // compute_energy_geom (energy.rs:80-283); geom on the device or null.
+    void energy(const T *m, const llgmi_material &mat, const double bext[3], const uint8_t *geom,
+                double out[5]) {
+        ensure_red();
+        const T *bd = nullptr;
+        if (mat.demag) {
+            demag_into_bd(m, mat, resolved_demag_method(mat), 0, 0);
+            bd = d_bd;
+        }
+        EnergyArgs a{};
+        a.nx = (int)g.nx;
+        a.ny = (int)g.ny;
+        a.dx = g.dx;
+        a.dy = g.dy;
+        a.v = g.dx * g.dy * g.dz;
+        a.aex = mat.a_ex;
+        a.ku = mat.k_u;
+        a.ms = mat.ms;
+        for (int q = 0; q < 3; ++q) { a.u[q] = mat.easy_axis[q]; a.bext[q] = bext[q]; }
+        a.dmi_on = (mat.has_dmi && mat.dmi != 0.0 && mat.a_ex != 0.0) ? 1 : 0;
+        a.pref = mat.ms != 0.0 ? 2.0 * mat.dmi / mat.ms : 0.0;
+        a.eta = a.dmi_on ? mat.dmi / (2.0 * mat.a_ex) : 0.0;
+        {
+            ProfScope ps(prof, LLGMI_K_RELAX, stream);
+            launch_energy<T>(a, m, bd, geom, d_red, d_redout, stream);
+        }
+        read_red(5);
+        for (int q = 0; q < 5; ++q) out[q] = h_red[q];
+    }
+    double energy_total(const T *m, const llgmi_material &mat, const double bext[3]) {
+        double e[5];
+        energy(m, mat, bext, nullptr, e);
+        return e[0] + e[1] + e[2] + e[3] + e[4];   // EnergyBreakdown::total (energy.rs:30-34)
+    }
+
+    // torque_metric_from_field (relax.rs:144-190)
+    double torque_metric(const T *m, const T *b, int metric) {
+        ensure_red();
+        {
+            ProfScope ps(prof, LLGMI_K_RELAX, stream);
+            launch_torque<T>(m, b, n, d_red, d_redout, stream);
+        }
+        read_red(3);
+        const double nf = std::max((double)n, 1.0);
+        switch (metric) {
+            case LLGMI_TORQUE_MAX: return h_red[0];
+            case LLGMI_TORQUE_MEAN: return h_red[1] / nf;
+            case LLGMI_TORQUE_RMS: return std::sqrt(h_red[2] / nf);
+            default: return h_red[2] / nf;
+        }
+    }
+
+    // relax_with_report (relax.rs:334-774) on device state m.
+    void relax(T *m, llgmi_params &p, const llgmi_material &mat, int mask, llgmi_relax_settings &st,
+               llgmi_relax_report &rep);
 };
+
+// relax_with_report (relax.rs:334-774).  The controller is host logic over
+// device steps; each step and each torque / energy check synchronises once to
+// read its scalar.
+template <typename T>
+void Engine<T>::relax(T *m, llgmi_params &p, const llgmi_material &mat, int mask,
+                      llgmi_relax_settings &st, llgmi_relax_report &rep) {
+    const bool phase1 = st.phase1_enabled != 0, phase2 = st.phase2_enabled != 0;
+    const int64_t energy_stride = std::max<int64_t>(st.energy_stride, 1);
+    const int metric = st.torque_metric;
+    const bool has_thr = st.has_torque_threshold != 0;
+    const int64_t check_stride = std::max<int64_t>(st.torque_check_stride, 1);
+    const double plateau_rel = st.torque_plateau_rel, plateau_abs = st.torque_plateau_abs;
+    const int64_t plateau_checks = st.torque_plateau_checks;
+    const int64_t plateau_min_checks = std::max<int64_t>(st.torque_plateau_min_checks, 1);
+    const int64_t max_acc = st.max_accepted_steps;
+    const bool has_gate = st.has_final_torque_gate != 0, has_gate_max = st.has_final_torque_gate_max != 0;
+    const double headroom = st.headroom, dt_min = st.dt_min, dt_max = st.dt_max;
+    double max_err = st.max_err;
+
+    p.dt = std::min(std::max(p.dt, dt_min), dt_max);
+    rk23_invalidate();
+
+    int64_t accepted = 0, rejected = 0, checks = 0, rebuilds = 0;
+    int last_stage = LLGMI_STAGE_NONE;
+    bool has_ft = false, has_ftmax = false;
+    double ft = 0.0, ftmax = 0.0;
+
+    auto finish = [&](int stop, int stage) {
+        st.max_err = max_err;
+        rep.accepted_steps = accepted;
+        rep.rejected_steps = rejected;
+        rep.torque_checks = checks;
+        rep.torque_field_rebuilds = rebuilds;
+        rep.stop_reason = stop;
+        rep.last_stage_stop = stage;
+        rep.has_final_torque = has_ft ? 1 : 0;
+        rep.final_torque = has_ft ? ft : 0.0;
+        rep.has_final_torque_max = has_ftmax ? 1 : 0;
+        rep.final_torque_max = has_ftmax ? ftmax : 0.0;
+        rep.final_max_err = max_err;
+        rep.final_dt = p.dt;
+    };
+    // torque_now (relax.rs:206-230)
+    auto torque_now = [&](int which) {
+        ++checks;
+        if (r23_b4_valid) return torque_metric(m, d_r23b4, which);
+        ++rebuilds;
+        alloc(d_b, 3 * n);
+        field(m, d_b, p, mat, mask, nullptr, nullptr);
+        return torque_metric(m, d_b, which);
+    };
+    // advance_accepted (relax.rs:247-277)
+    auto advance = [&](int64_t n_accept) {
+        const int64_t target = accepted + std::max<int64_t>(n_accept, 1);
+        while (accepted < target) {
+            if (accepted >= max_acc) return false;
+            double eps, dtu;
+            int ok;
+            rk23(m, p, mat, mask, max_err, headroom, dt_min, dt_max, eps, ok, dtu);
+            if (ok) ++accepted;
+            else ++rejected;
+        }
+        return true;
+    };
+
+    if (phase1) {
+        double e0 = energy_total(m, mat, p.b_ext);
+        for (;;) {
+            if (!advance(energy_stride)) return finish(LLGMI_STOP_MAX_ACCEPTED_STEPS, last_stage);
+            const double e1 = energy_total(m, mat, p.b_ext);
+            const double tol = st.rel_energy_tol * std::max(std::fabs(e0), 1e-30);
+            if (e1 < e0 - tol) {
+                e0 = e1;
+                continue;
+            }
+            break;
+        }
+    }
+    if (!phase2) return finish(LLGMI_STOP_PHASE2_DISABLED, last_stage);
+
+    const bool plateau_enabled = !has_thr && plateau_checks > 0;
+    const int64_t need_fails = std::max<int64_t>(plateau_checks, 1);
+    // MuMax's float32 sum-of-squares plateau (relax.rs:472-488)
+    const bool mumax_f32 = plateau_enabled && metric == LLGMI_TORQUE_MEANSQ && plateau_rel == 0.0 &&
+                           plateau_abs == 0.0 && plateau_checks == 1 && plateau_min_checks == 1;
+    const double n_f = (double)n;
+    for (;;) {
+        double t_prev = torque_now(metric);
+        float t_prev_f32 = mumax_f32 ? (float)(t_prev * n_f) : (float)t_prev;
+        int64_t fails = 0, comparisons = 0;
+        for (;;) {
+            if (has_thr && t_prev <= st.torque_threshold) {
+                last_stage = LLGMI_STAGE_BELOW_THRESHOLD;
+                break;
+            }
+            if (plateau_enabled && fails >= need_fails) {
+                last_stage = LLGMI_STAGE_PLATEAU;
+                break;
+            }
+            if (accepted >= max_acc || !advance(check_stride)) {
+                has_ft = true;
+                ft = t_prev;
+                return finish(LLGMI_STOP_MAX_ACCEPTED_STEPS, LLGMI_STAGE_MAX_ACCEPTED_STEPS);
+            }
+            const double t_new = torque_now(metric);
+            if (plateau_enabled) {
+                ++comparisons;
+                if (comparisons < plateau_min_checks) {
+                    fails = 0;
+                } else if (mumax_f32) {
+                    const float t_new_f32 = (float)(t_new * n_f);
+                    const bool improved = t_new_f32 < t_prev_f32;
+                    t_prev_f32 = t_new_f32;
+                    fails = improved ? 0 : fails + 1;
+                } else {
+                    const double need = std::max(plateau_rel * std::max(std::fabs(t_prev), 1e-30), plateau_abs);
+                    fails = (t_prev - t_new) > need ? 0 : fails + 1;
+                }
+            }
+            t_prev = t_new;
+        }
+        has_ft = true;
+        ft = t_prev;
+        // tighten (MaxErr /= sqrt 2)
+        if (max_err <= st.tighten_floor) break;
+        max_err *= st.tighten_factor;
+        if (max_err < st.tighten_floor) max_err = st.tighten_floor;
+    }
+    has_ftmax = true;
+    ftmax = torque_now(LLGMI_TORQUE_MAX);
+
+    if ((has_gate || has_gate_max) && st.gate_max_extra_accepted_steps > 0) {
+        const int64_t start = accepted;
+        double last_t = has_ft ? ft : INFINITY;
+        int64_t gate_fails = 0;
+        for (;;) {
+            // gates_ok (relax.rs:279-332)
+            const double t_metric = torque_now(metric);
+            const bool need_max = has_gate_max || metric == LLGMI_TORQUE_MAX;
+            double t_max = NAN;
+            if (need_max) t_max = metric == LLGMI_TORQUE_MAX ? t_metric : torque_now(LLGMI_TORQUE_MAX);
+            const bool ok = (!has_gate || t_metric <= st.final_torque_gate) &&
+                            (!has_gate_max || t_max <= st.final_torque_gate_max);
+            has_ft = true;
+            ft = t_metric;
+            if (std::isfinite(t_max)) { has_ftmax = true; ftmax = t_max; }
+            if (ok) break;
+            const int64_t used = accepted > start ? accepted - start : 0;
+            if (used >= st.gate_max_extra_accepted_steps)
+                return finish(LLGMI_STOP_TORQUE_GATE_NOT_SATISFIED, last_stage);
+            if (st.gate_plateau_fails > 0) {
+                const double need = std::max(plateau_rel * std::max(std::fabs(last_t), 1e-30), plateau_abs);
+                gate_fails = (last_t - t_metric) > need ? 0 : gate_fails + 1;
+                if (gate_fails >= st.gate_plateau_fails)
+                    return finish(LLGMI_STOP_TORQUE_GATE_NOT_SATISFIED, last_stage);
+                last_t = t_metric;
+            }
+            if (!advance(check_stride))
+                return finish(LLGMI_STOP_MAX_ACCEPTED_STEPS, LLGMI_STAGE_MAX_ACCEPTED_STEPS);
+        }
+    }
+    finish(LLGMI_STOP_TIGHTEN_FLOOR_REACHED, last_stage);
+}
 
 struct llgmi_ctx {
     llgmi_grid grid{};
@@ -681,6 +1003,212 @@ int llgmi_mg_last_stats(llgmi_ctx *ctx, llgmi_mg_stats *out) {
         with_engine(ctx, [&](auto &eng) {
             if (!eng.mg) throw Error(LLGMI_ERR_INVALID, "no multigrid solve has run on this context");
             *out = eng.mg->stats();
+        });
+    });
+}
+
+// ---- relaxation ----------------------------------------------------------------
+namespace {
+
+void relax_settings_default(llgmi_relax_settings &s) {
+    std::memset(&s, 0, sizeof s);
+    s.max_err = 1e-5;
+    s.headroom = 0.8;
+    s.dt_min = 1e-18;
+    s.dt_max = 1e-11;
+    s.phase1_enabled = 1;
+    s.phase2_enabled = 1;
+    s.energy_stride = 3;
+    s.rel_energy_tol = 1e-12;
+    s.torque_metric = LLGMI_TORQUE_MAX;
+    s.has_torque_threshold = 1;
+    s.torque_threshold = 1e-4;
+    s.torque_check_stride = 1;
+    s.torque_plateau_rel = 1e-3;
+    s.torque_plateau_abs = 0.0;
+    s.torque_plateau_checks = 0;
+    s.torque_plateau_min_checks = 5;
+    s.tighten_factor = 0.70710678118654752440;   // FRAC_1_SQRT_2
+    s.tighten_floor = 1e-9;
+    s.max_accepted_steps = 2000000;
+}
+
+std::string check_relax_settings(const llgmi_relax_settings *s) {
+    if (!s) return "settings is NULL";
+    if (s->torque_metric < 0 || s->torque_metric > 3) return "unknown torque_metric";
+    if (s->dt_min > s->dt_max) return "dt_min > dt_max";
+    if (s->max_accepted_steps < 0) return "max_accepted_steps must be >= 0";
+    if (!(s->tighten_factor > 0.0 && s->tighten_factor < 1.0) && s->tighten_floor > 0.0 &&
+        s->max_err > s->tighten_floor)
+        return "tighten_factor must lie in (0, 1) for max_err to reach tighten_floor";
+    if (!(s->tighten_floor > 0.0) && s->phase2_enabled)
+        return "tighten_floor must be positive";
+    return "";
+}
+
+std::string check_step_args(const llgmi_params *params, const double *eps, const int *accepted,
+                            const double *dt_used, double dt_min, double dt_max) {
+    if (!params || !eps || !accepted || !dt_used) return "NULL argument";
+    if (dt_min > dt_max) return "dt_min > dt_max";
+    return "";
+}
+
+}  // namespace
+
+int llgmi_relax_settings_default(llgmi_relax_settings *s) {
+    if (!s) return fail(LLGMI_ERR_INVALID, "settings is NULL");
+    relax_settings_default(*s);
+    return LLGMI_OK;
+}
+
+int llgmi_relax_settings_mumax_like(llgmi_relax_settings *s) {
+    if (!s) return fail(LLGMI_ERR_INVALID, "settings is NULL");
+    relax_settings_default(*s);
+    s->phase1_enabled = 1;
+    s->phase2_enabled = 1;
+    s->energy_stride = 3;
+    s->rel_energy_tol = 0.0;
+    s->torque_metric = LLGMI_TORQUE_MEANSQ;
+    s->has_torque_threshold = 0;
+    s->torque_threshold = 0.0;
+    s->torque_plateau_checks = 1;
+    s->torque_plateau_min_checks = 1;
+    s->torque_plateau_rel = 0.0;
+    s->torque_plateau_abs = 0.0;
+    s->tighten_factor = 0.70710678118654752440;
+    s->tighten_floor = 1e-9;
+    return LLGMI_OK;
+}
+
+int llgmi_rk23_relax_step(llgmi_ctx *ctx, double *m, llgmi_params *params, const llgmi_material *mat,
+                          int field_mask, double max_err, double headroom, double dt_min,
+                          double dt_max, double *eps, int *accepted, double *dt_used) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!m) return fail(LLGMI_ERR_INVALID, "m is NULL");
+    std::string e = check_step_args(params, eps, accepted, dt_used, dt_min, dt_max);
+    if (e.empty()) e = check_mat(mat);
+    if (e.empty()) e = check_mask(field_mask);
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            eng.ensure_io();
+            eng.upload(m, eng.d_m);
+            eng.rk23(eng.d_m, *params, *mat, field_mask, max_err, headroom, dt_min, dt_max, *eps,
+                     *accepted, *dt_used);
+            if (*accepted) eng.download(eng.d_m, m);
+        });
+    });
+}
+
+int llgmi_dev_rk23_relax_step(llgmi_ctx *ctx, void *m_dev, llgmi_params *params,
+                              const llgmi_material *mat, int field_mask, double max_err,
+                              double headroom, double dt_min, double dt_max, double *eps,
+                              int *accepted, double *dt_used) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    std::string e = check_step_args(params, eps, accepted, dt_used, dt_min, dt_max);
+    if (e.empty()) e = check_mat(mat);
+    if (e.empty()) e = check_mask(field_mask);
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    if (e.empty()) e = check_dev(m_dev, 3 * n * elem_size(ctx), "m");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.rk23((T *)m_dev, *params, *mat, field_mask, max_err, headroom, dt_min, dt_max, *eps,
+                     *accepted, *dt_used);
+        });
+    });
+}
+
+int llgmi_rk23_invalidate(llgmi_ctx *ctx) {
+    return guarded(ctx, [&] { with_engine(ctx, [&](auto &eng) { eng.rk23_invalidate(); }); });
+}
+
+int llgmi_rk23_last_b_eff(llgmi_ctx *ctx, double *b_out, int *valid) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!b_out || !valid) return fail(LLGMI_ERR_INVALID, "NULL argument");
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            *valid = eng.r23_b4_valid ? 1 : 0;
+            if (eng.r23_b4_valid) {
+                eng.ensure_io();
+                eng.download(eng.d_r23b4, b_out);
+            }
+        });
+    });
+}
+
+int llgmi_compute_energy_geom(llgmi_ctx *ctx, const double *m, const llgmi_material *mat,
+                              const double b_ext[3], const uint8_t *geom_mask, llgmi_energy *out) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!m || !b_ext || !out) return fail(LLGMI_ERR_INVALID, "NULL argument");
+    std::string e = check_mat(mat);
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            eng.ensure_io();
+            eng.upload(m, eng.d_m);
+            const uint8_t *dg = eng.upload_geom(geom_mask);
+            double r[5];
+            eng.energy(eng.d_m, *mat, b_ext, dg, r);
+            *out = llgmi_energy{r[0], r[1], r[2], r[3], r[4]};
+        });
+    });
+}
+
+int llgmi_dev_compute_energy_geom(llgmi_ctx *ctx, const void *m_dev, const llgmi_material *mat,
+                                  const double b_ext[3], const uint8_t *geom_mask_dev,
+                                  llgmi_energy *out) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!b_ext || !out) return fail(LLGMI_ERR_INVALID, "NULL argument");
+    std::string e = check_mat(mat);
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    if (e.empty()) e = check_dev(m_dev, 3 * n * elem_size(ctx), "m");
+    if (e.empty() && geom_mask_dev) e = check_dev(geom_mask_dev, n, "geom_mask");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            double r[5];
+            eng.energy((const T *)m_dev, *mat, b_ext, geom_mask_dev, r);
+            *out = llgmi_energy{r[0], r[1], r[2], r[3], r[4]};
+        });
+    });
+}
+
+int llgmi_relax_with_report(llgmi_ctx *ctx, double *m, llgmi_params *params, const llgmi_material *mat,
+                            int field_mask, llgmi_relax_settings *settings, llgmi_relax_report *report) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!m || !params || !report) return fail(LLGMI_ERR_INVALID, "NULL argument");
+    std::string e = check_relax_settings(settings);
+    if (e.empty()) e = check_mat(mat);
+    if (e.empty()) e = check_mask(field_mask);
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            eng.ensure_io();
+            eng.upload(m, eng.d_m);
+            eng.relax(eng.d_m, *params, *mat, field_mask, *settings, *report);
+            eng.download(eng.d_m, m);
+        });
+    });
+}
+
+int llgmi_dev_relax_with_report(llgmi_ctx *ctx, void *m_dev, llgmi_params *params,
+                                const llgmi_material *mat, int field_mask,
+                                llgmi_relax_settings *settings, llgmi_relax_report *report) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!params || !report) return fail(LLGMI_ERR_INVALID, "NULL argument");
+    std::string e = check_relax_settings(settings);
+    if (e.empty()) e = check_mat(mat);
+    if (e.empty()) e = check_mask(field_mask);
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    if (e.empty()) e = check_dev(m_dev, 3 * n * elem_size(ctx), "m");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.relax((T *)m_dev, *params, *mat, field_mask, *settings, *report);
         });
     });
 }

@@ -6,7 +6,7 @@
 // Terms follow llg-sim's effective_field/ (zeeman.rs:6-11,
 // exchange.rs:45-222, anisotropy.rs:10-25, dmi.rs:67-209), summed in the
 // order of build_h_eff_masked (mod.rs:48-75); the torque is llg_rhs
-// (llg.rs:20-32).
+// (llg.rs:20-32) or, for relaxation, llg_rhs_relax (llg.rs:38-50).
 #pragma once
 
 #include "internal.hpp"
@@ -138,9 +138,16 @@ __device__ __forceinline__ void local_cell(const LocalArgs<T> &a, const T *__res
         T ccx = m0[1] * cz - m0[2] * cy;
         T ccy = m0[2] * cx - m0[0] * cz;
         T ccz = m0[0] * cy - m0[1] * cx;
-        kout[c] = a.rhs_pref * (cx + a.alpha * ccx);
-        kout[n + c] = a.rhs_pref * (cy + a.alpha * ccy);
-        kout[2 * n + c] = a.rhs_pref * (cz + a.alpha * ccz);
+        if (a.torque == 2) {
+            // llg_rhs_relax (llg.rs:38-50): -gamma alpha/(1+alpha^2) m x (m x B)
+            kout[c] = a.rhs_pref * ccx;
+            kout[n + c] = a.rhs_pref * ccy;
+            kout[2 * n + c] = a.rhs_pref * ccz;
+        } else {
+            kout[c] = a.rhs_pref * (cx + a.alpha * ccx);
+            kout[n + c] = a.rhs_pref * (cy + a.alpha * ccy);
+            kout[2 * n + c] = a.rhs_pref * (cz + a.alpha * ccz);
+        }
     }
 }
 

@@ -7,7 +7,11 @@
 //   effective_field::build_h_eff_masked[_geom] / build_h_eff (mod.rs:48-112),
 //   effective_field::demag::{add,compute}_demag_field[_pbc] (demag.rs:47-156),
 //   llg::RK45Scratch (llg.rs:126-189),
-//   llg::step_llg_rk45_recompute_field[_masked]_adaptive (llg.rs:795-1050).
+//   llg::step_llg_rk45_recompute_field[_masked]_adaptive (llg.rs:795-1050),
+//   llg::RK23Scratch / step_llg_rk23_recompute_field_masked_relax_adaptive
+//   (llg.rs:196-436), energy::compute_energy[_geom] / compute_total_energy
+//   (energy.rs:21-305), relax::RelaxSettings / relax_with_report / relax /
+//   mumax_like_relax_settings (relax.rs:32-808).
 //
 // Field builders use a process-wide cache of GPU contexts keyed by grid and
 // precision (the reference keys its demag operator cache the same way,
@@ -231,5 +235,182 @@ inline std::tuple<double, bool, double> step_llg_rk45_recompute_field_adaptive(
                                                           max_err, headroom, dt_min, dt_max);
 }
 
+// RK23Scratch::new(grid) (llg.rs:223-247): relax-mode stage storage, FSAL and
+// last-field state live on the GPU.
+class RK23Scratch {
+public:
+    explicit RK23Scratch(Grid2D grid, int precision = LLGMI_F64) : grid_(grid), ctx_(grid, precision) {}
+    llgmi_ctx *ctx() const { return ctx_.get(); }
+    const Grid2D &grid() const { return grid_; }
+    // last_b_eff (llg.rs:252-258): the field of the accepted state, if held.
+    std::optional<VectorField2D> last_b_eff() const {
+        VectorField2D b(grid_);
+        int valid = 0;
+        check(llgmi_rk23_last_b_eff(ctx_.get(), b.ptr(), &valid), "last_b_eff");
+        if (!valid) return std::nullopt;
+        return b;
+    }
+    // invalidate_last_b_eff (llg.rs:264-268)
+    void invalidate_last_b_eff() { check(llgmi_rk23_invalidate(ctx_.get()), "invalidate_last_b_eff"); }
+
+private:
+    Grid2D grid_;
+    Context ctx_;
+};
+
+// step_llg_rk23_recompute_field_masked_relax_adaptive (llg.rs:278-436).
+inline std::tuple<double, bool, double> step_llg_rk23_recompute_field_masked_relax_adaptive(
+    VectorField2D &m, LLGParams &params, const Material &material, RK23Scratch &scratch,
+    FieldMask mask, double max_err, double headroom, double dt_min, double dt_max) {
+    if (m.data.size() != scratch.grid().n_cells()) throw std::runtime_error("grid mismatch");
+    llgmi_params p = params.c();
+    llgmi_material cm = material.c();
+    double eps = 0, dt_used = 0;
+    int accepted = 0;
+    check(llgmi_rk23_relax_step(scratch.ctx(), m.ptr(), &p, &cm, (int)mask, max_err, headroom, dt_min,
+                                dt_max, &eps, &accepted, &dt_used),
+          "step_llg_rk23_recompute_field_masked_relax_adaptive");
+    params.dt = p.dt;
+    return {eps, accepted != 0, dt_used};
+}
+
 }  // namespace llg
+
+// energy.rs:21-305
+namespace energy {
+
+struct EnergyBreakdown {
+    double exchange = 0, anisotropy = 0, zeeman = 0, dmi = 0, demag = 0;
+    double total() const { return exchange + anisotropy + zeeman + dmi + demag; }
+};
+
+inline EnergyBreakdown compute_energy_geom(const Grid2D &grid, const VectorField2D &m, const Material &material,
+                                           const Vec3 &b_ext, const std::vector<uint8_t> *geom_mask) {
+    if (m.data.size() != grid.n_cells()) throw std::runtime_error("field length must match grid.n_cells()");
+    if (geom_mask && geom_mask->size() != grid.n_cells())
+        throw std::runtime_error("mask length must match grid.n_cells()");
+    auto &ctx = detail::shared_context(grid, detail::default_precision());
+    llgmi_material cm = material.c();
+    llgmi_energy e{};
+    check(llgmi_compute_energy_geom(ctx.get(), m.ptr(), &cm, b_ext.data(),
+                                    geom_mask ? geom_mask->data() : nullptr, &e),
+          "compute_energy_geom");
+    return EnergyBreakdown{e.exchange, e.anisotropy, e.zeeman, e.dmi, e.demag};
+}
+inline EnergyBreakdown compute_energy(const Grid2D &grid, const VectorField2D &m, const Material &material,
+                                      const Vec3 &b_ext) {
+    return compute_energy_geom(grid, m, material, b_ext, nullptr);
+}
+inline double compute_total_energy(const Grid2D &grid, const VectorField2D &m, const Material &material,
+                                   const Vec3 &b_ext) {
+    return compute_energy(grid, m, material, b_ext).total();
+}
+
+}  // namespace energy
+
+// relax.rs:32-808
+namespace relax {
+
+enum class TorqueMetric { Max = LLGMI_TORQUE_MAX, Mean = LLGMI_TORQUE_MEAN, Rms = LLGMI_TORQUE_RMS,
+                          MeanSq = LLGMI_TORQUE_MEANSQ };
+enum class RelaxStopReason { MaxAcceptedSteps = LLGMI_STOP_MAX_ACCEPTED_STEPS,
+                             Phase2Disabled = LLGMI_STOP_PHASE2_DISABLED,
+                             TightenFloorReached = LLGMI_STOP_TIGHTEN_FLOOR_REACHED,
+                             TorqueGateNotSatisfied = LLGMI_STOP_TORQUE_GATE_NOT_SATISFIED };
+enum class RelaxStageStop { BelowThreshold = LLGMI_STAGE_BELOW_THRESHOLD, Plateau = LLGMI_STAGE_PLATEAU,
+                            MaxAcceptedSteps = LLGMI_STAGE_MAX_ACCEPTED_STEPS };
+
+struct RelaxReport {
+    size_t accepted_steps = 0, rejected_steps = 0, torque_checks = 0, torque_field_rebuilds = 0;
+    RelaxStopReason stop_reason = RelaxStopReason::MaxAcceptedSteps;
+    std::optional<RelaxStageStop> last_stage_stop;
+    std::optional<double> final_torque, final_torque_max;
+    double final_max_err = 0, final_dt = 0;
+};
+
+struct RelaxSettings {
+    double max_err = 1e-5, headroom = 0.8, dt_min = 1e-18, dt_max = 1e-11;
+    bool phase1_enabled = true, phase2_enabled = true;
+    size_t energy_stride = 3;
+    double rel_energy_tol = 1e-12;
+    TorqueMetric torque_metric = TorqueMetric::Max;
+    std::optional<double> torque_threshold = 1e-4;
+    size_t torque_check_stride = 1;
+    double torque_plateau_rel = 1e-3, torque_plateau_abs = 0.0;
+    size_t torque_plateau_checks = 0, torque_plateau_min_checks = 5;
+    double tighten_factor = 0.70710678118654752440, tighten_floor = 1e-9;
+    size_t max_accepted_steps = 2000000;
+    std::optional<double> final_torque_gate, final_torque_gate_max;
+    size_t gate_max_extra_accepted_steps = 0, gate_plateau_fails = 0;
+
+    llgmi_relax_settings c() const {
+        llgmi_relax_settings s{};
+        s.max_err = max_err; s.headroom = headroom; s.dt_min = dt_min; s.dt_max = dt_max;
+        s.phase1_enabled = phase1_enabled; s.phase2_enabled = phase2_enabled;
+        s.energy_stride = (int64_t)energy_stride; s.rel_energy_tol = rel_energy_tol;
+        s.torque_metric = (int32_t)torque_metric;
+        s.has_torque_threshold = torque_threshold.has_value();
+        s.torque_threshold = torque_threshold.value_or(0.0);
+        s.torque_check_stride = (int64_t)torque_check_stride;
+        s.torque_plateau_rel = torque_plateau_rel; s.torque_plateau_abs = torque_plateau_abs;
+        s.torque_plateau_checks = (int64_t)torque_plateau_checks;
+        s.torque_plateau_min_checks = (int64_t)torque_plateau_min_checks;
+        s.tighten_factor = tighten_factor; s.tighten_floor = tighten_floor;
+        s.max_accepted_steps = (int64_t)max_accepted_steps;
+        s.has_final_torque_gate = final_torque_gate.has_value();
+        s.has_final_torque_gate_max = final_torque_gate_max.has_value();
+        s.final_torque_gate = final_torque_gate.value_or(0.0);
+        s.final_torque_gate_max = final_torque_gate_max.value_or(0.0);
+        s.gate_max_extra_accepted_steps = (int64_t)gate_max_extra_accepted_steps;
+        s.gate_plateau_fails = (int64_t)gate_plateau_fails;
+        return s;
+    }
+};
+
+// mumax_like_relax_settings (relax.rs:789-808)
+inline RelaxSettings mumax_like_relax_settings() {
+    RelaxSettings s;
+    s.rel_energy_tol = 0.0;
+    s.torque_metric = TorqueMetric::MeanSq;
+    s.torque_threshold.reset();
+    s.torque_plateau_checks = 1;
+    s.torque_plateau_min_checks = 1;
+    s.torque_plateau_rel = 0.0;
+    s.torque_plateau_abs = 0.0;
+    return s;
+}
+
+// relax_with_report (relax.rs:334-774): settings.max_err is written back.
+inline RelaxReport relax_with_report(const Grid2D &grid, VectorField2D &m, LLGParams &params,
+                                     const Material &material, llg::RK23Scratch &scratch, FieldMask mask,
+                                     RelaxSettings &settings) {
+    if (m.data.size() != grid.n_cells() || scratch.grid().n_cells() != grid.n_cells())
+        throw std::runtime_error("grid mismatch");
+    llgmi_params p = params.c();
+    llgmi_material cm = material.c();
+    llgmi_relax_settings cs = settings.c();
+    llgmi_relax_report r{};
+    check(llgmi_relax_with_report(scratch.ctx(), m.ptr(), &p, &cm, (int)mask, &cs, &r), "relax_with_report");
+    params.dt = p.dt;
+    settings.max_err = cs.max_err;
+    RelaxReport out;
+    out.accepted_steps = (size_t)r.accepted_steps;
+    out.rejected_steps = (size_t)r.rejected_steps;
+    out.torque_checks = (size_t)r.torque_checks;
+    out.torque_field_rebuilds = (size_t)r.torque_field_rebuilds;
+    out.stop_reason = (RelaxStopReason)r.stop_reason;
+    if (r.last_stage_stop != LLGMI_STAGE_NONE) out.last_stage_stop = (RelaxStageStop)r.last_stage_stop;
+    if (r.has_final_torque) out.final_torque = r.final_torque;
+    if (r.has_final_torque_max) out.final_torque_max = r.final_torque_max;
+    out.final_max_err = r.final_max_err;
+    out.final_dt = r.final_dt;
+    return out;
+}
+
+inline void relax(const Grid2D &grid, VectorField2D &m, LLGParams &params, const Material &material,
+                  llg::RK23Scratch &scratch, FieldMask mask, RelaxSettings &settings) {
+    (void)relax_with_report(grid, m, params, material, scratch, mask, settings);
+}
+
+}  // namespace relax
 }  // namespace llg_sim

@@ -32,7 +32,9 @@ GAMMA_E_RAD_PER_S_T = 1.76085963023e11
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_ALLOC = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
 
-K_FFT_ROW_FWD, K_FFT_COL, K_FFT_ROW_INV, K_LOCAL_FIELD, K_RK45_STAGE, K_RK45_ERR, K_MG, K_MG_BC = range(8)
+(K_FFT_ROW_FWD, K_FFT_COL, K_FFT_ROW_INV, K_LOCAL_FIELD, K_RK45_STAGE, K_RK45_ERR, K_MG, K_MG_BC,
+ K_RELAX) = range(9)
+K_COUNT = 9
 KERNEL_CLASS_NAMES = {
     K_FFT_ROW_FWD: "fft_row_fwd",
     K_FFT_COL: "fft_col_mul",
@@ -42,6 +44,7 @@ KERNEL_CLASS_NAMES = {
     K_RK45_ERR: "rk45_err",
     K_MG: "mg",
     K_MG_BC: "mg_bc",
+    K_RELAX: "relax",
 }
 
 
@@ -143,6 +146,10 @@ EXPORTED = [
     "llgmi_profile_read", "llgmi_mg_config_default", "llgmi_mg_config_from_env",
     "llgmi_ctx_set_mg_config", "llgmi_add_demag_field_poisson_mg",
     "llgmi_compute_demag_field_poisson_mg", "llgmi_mg_last_stats",
+    "llgmi_relax_settings_default", "llgmi_relax_settings_mumax_like", "llgmi_rk23_relax_step",
+    "llgmi_rk23_invalidate", "llgmi_rk23_last_b_eff", "llgmi_compute_energy_geom",
+    "llgmi_relax_with_report", "llgmi_dev_rk23_relax_step", "llgmi_dev_compute_energy_geom",
+    "llgmi_dev_relax_with_report",
 ]
 
 
@@ -199,6 +206,82 @@ def mg_config_from_env() -> MGConfig:
     return c
 
 
+# ---- relaxation (relax.rs, energy.rs, llg.rs:196-436) ----
+TORQUE_MAX, TORQUE_MEAN, TORQUE_RMS, TORQUE_MEANSQ = 0, 1, 2, 3
+(STOP_MAX_ACCEPTED_STEPS, STOP_PHASE2_DISABLED, STOP_TIGHTEN_FLOOR_REACHED,
+ STOP_TORQUE_GATE_NOT_SATISFIED) = 0, 1, 2, 3
+STAGE_NONE, STAGE_BELOW_THRESHOLD, STAGE_PLATEAU, STAGE_MAX_ACCEPTED_STEPS = -1, 0, 1, 2
+
+
+class RelaxSettings(C.Structure):
+    """llgmi_relax_settings: RelaxSettings (relax.rs:73-105); has_* = 0 is None."""
+    _fields_ = [("max_err", C.c_double), ("headroom", C.c_double), ("dt_min", C.c_double),
+                ("dt_max", C.c_double), ("phase1_enabled", C.c_int32),
+                ("phase2_enabled", C.c_int32), ("energy_stride", C.c_int64),
+                ("rel_energy_tol", C.c_double), ("torque_metric", C.c_int32),
+                ("has_torque_threshold", C.c_int32), ("torque_threshold", C.c_double),
+                ("torque_check_stride", C.c_int64), ("torque_plateau_rel", C.c_double),
+                ("torque_plateau_abs", C.c_double), ("torque_plateau_checks", C.c_int64),
+                ("torque_plateau_min_checks", C.c_int64), ("tighten_factor", C.c_double),
+                ("tighten_floor", C.c_double), ("max_accepted_steps", C.c_int64),
+                ("has_final_torque_gate", C.c_int32), ("has_final_torque_gate_max", C.c_int32),
+                ("final_torque_gate", C.c_double), ("final_torque_gate_max", C.c_double),
+                ("gate_max_extra_accepted_steps", C.c_int64), ("gate_plateau_fails", C.c_int64)]
+
+
+class RelaxReport(C.Structure):
+    """llgmi_relax_report: RelaxReport (relax.rs:55-71)."""
+    _fields_ = [("accepted_steps", C.c_int64), ("rejected_steps", C.c_int64),
+                ("torque_checks", C.c_int64), ("torque_field_rebuilds", C.c_int64),
+                ("stop_reason", C.c_int32), ("last_stage_stop", C.c_int32),
+                ("has_final_torque", C.c_int32), ("has_final_torque_max", C.c_int32),
+                ("final_torque", C.c_double), ("final_torque_max", C.c_double),
+                ("final_max_err", C.c_double), ("final_dt", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {"accepted_steps": self.accepted_steps, "rejected_steps": self.rejected_steps,
+                "torque_checks": self.torque_checks,
+                "torque_field_rebuilds": self.torque_field_rebuilds,
+                "stop_reason": self.stop_reason, "last_stage_stop": self.last_stage_stop,
+                "final_torque": self.final_torque if self.has_final_torque else None,
+                "final_torque_max": self.final_torque_max if self.has_final_torque_max else None,
+                "final_max_err": self.final_max_err, "final_dt": self.final_dt}
+
+
+class Energy(C.Structure):
+    """llgmi_energy: EnergyBreakdown (energy.rs:21-34), joules."""
+    _fields_ = [("exchange", C.c_double), ("anisotropy", C.c_double), ("zeeman", C.c_double),
+                ("dmi", C.c_double), ("demag", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    def total(self) -> float:
+        return self.exchange + self.anisotropy + self.zeeman + self.dmi + self.demag
+
+
+def _settings(fill, kw) -> RelaxSettings:
+    s = RelaxSettings()
+    _check(fill(C.byref(s)))
+    for k, v in kw.items():
+        if k in ("torque_threshold", "final_torque_gate", "final_torque_gate_max"):
+            setattr(s, "has_" + k, 0 if v is None else 1)
+            setattr(s, k, 0.0 if v is None else float(v))
+        else:
+            setattr(s, k, v)
+    return s
+
+
+def relax_settings(**kw) -> RelaxSettings:
+    """RelaxSettings::default() with fields from kw; None switches an Option field off."""
+    return _settings(lib().llgmi_relax_settings_default, kw)
+
+
+def mumax_like_relax_settings(**kw) -> RelaxSettings:
+    """mumax_like_relax_settings (relax.rs:789-808) with fields from kw."""
+    return _settings(lib().llgmi_relax_settings_mumax_like, kw)
+
+
 def lib():
     """Loads libllgmi.so; raises if it has not been built."""
     global _lib
@@ -241,6 +324,18 @@ def lib():
     L.llgmi_add_demag_field_poisson_mg.argtypes = [vp, dp, dp, M]
     L.llgmi_compute_demag_field_poisson_mg.argtypes = [vp, dp, dp, M]
     L.llgmi_mg_last_stats.argtypes = [vp, C.POINTER(MGStats)]
+    RS, RR = C.POINTER(RelaxSettings), C.POINTER(RelaxReport)
+    L.llgmi_relax_settings_default.argtypes = [RS]
+    L.llgmi_relax_settings_mumax_like.argtypes = [RS]
+    L.llgmi_rk23_relax_step.argtypes = step_args
+    L.llgmi_dev_rk23_relax_step.argtypes = [vp, vp, P, M, C.c_int, C.c_double, C.c_double,
+                                            C.c_double, C.c_double, dp, ip, dp]
+    L.llgmi_rk23_invalidate.argtypes = [vp]
+    L.llgmi_rk23_last_b_eff.argtypes = [vp, dp, ip]
+    L.llgmi_compute_energy_geom.argtypes = [vp, dp, M, dp, u8p, C.POINTER(Energy)]
+    L.llgmi_dev_compute_energy_geom.argtypes = [vp, vp, M, dp, vp, C.POINTER(Energy)]
+    L.llgmi_relax_with_report.argtypes = [vp, dp, P, M, C.c_int, RS, RR]
+    L.llgmi_dev_relax_with_report.argtypes = [vp, vp, P, M, C.c_int, RS, RR]
     _lib = L
     return L
 
@@ -387,6 +482,90 @@ class Context:
     def rk45_reset(self) -> None:
         _check(lib().llgmi_rk45_reset(self._h))
 
+    # ---- relaxation ----
+    def step_llg_rk23_recompute_field_masked_relax_adaptive(self, m: np.ndarray, params: LLGParams,
+                                                            mat: Material, mask: int,
+                                                            max_err: float, headroom: float,
+                                                            dt_min: float, dt_max: float):
+        """llg.rs:278-436: advances m (float64, C-contiguous) in place when accepted;
+        updates params.dt; returns (eps, accepted, dt_used)."""
+        assert m.dtype == np.float64 and m.flags.c_contiguous and m.size == 3 * self.n
+        p, cm = params._c(), mat._c()
+        eps, dtu, acc = C.c_double(), C.c_double(), C.c_int()
+        _check(lib().llgmi_rk23_relax_step(self._h, _dp(m), C.byref(p), C.byref(cm), mask,
+                                           max_err, headroom, dt_min, dt_max, C.byref(eps),
+                                           C.byref(acc), C.byref(dtu)))
+        params.dt = p.dt
+        return eps.value, bool(acc.value), dtu.value
+
+    def rk23_invalidate_last_b_eff(self) -> None:
+        _check(lib().llgmi_rk23_invalidate(self._h))
+
+    def rk23_last_b_eff(self) -> Optional[np.ndarray]:
+        b = np.zeros((self.n, 3))
+        v = C.c_int()
+        _check(lib().llgmi_rk23_last_b_eff(self._h, _dp(b), C.byref(v)))
+        return b if v.value else None
+
+    def compute_energy_geom(self, m, mat: Material, b_ext, geom=None) -> dict:
+        """energy.rs:80-283 -> {exchange, anisotropy, zeeman, dmi, demag} (J)."""
+        m = _f64(m).reshape(-1, 3)
+        bx = _f64(b_ext).reshape(3)
+        cm = mat._c()
+        e = Energy()
+        g = None if geom is None else np.ascontiguousarray(np.asarray(geom).astype(np.uint8).reshape(-1))
+        if g is not None:
+            assert g.size == self.n
+        _check(lib().llgmi_compute_energy_geom(self._h, _dp(m), C.byref(cm), _dp(bx),
+                                               None if g is None else g.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                               C.byref(e)))
+        return e.as_dict()
+
+    def compute_energy(self, m, mat: Material, b_ext) -> dict:
+        return self.compute_energy_geom(m, mat, b_ext, None)
+
+    def compute_total_energy(self, m, mat: Material, b_ext) -> float:
+        e = self.compute_energy(m, mat, b_ext)
+        return (((e["exchange"] + e["anisotropy"]) + e["zeeman"]) + e["dmi"]) + e["demag"]
+
+    def relax_with_report(self, m: np.ndarray, params: LLGParams, mat: Material, mask: int,
+                          settings: RelaxSettings) -> dict:
+        """relax.rs:334-774: m and params.dt in place, settings.max_err written back."""
+        assert m.dtype == np.float64 and m.flags.c_contiguous and m.size == 3 * self.n
+        p, cm = params._c(), mat._c()
+        rep = RelaxReport()
+        _check(lib().llgmi_relax_with_report(self._h, _dp(m), C.byref(p), C.byref(cm), mask,
+                                             C.byref(settings), C.byref(rep)))
+        params.dt = p.dt
+        return rep.as_dict()
+
+    def dev_rk23_relax_step(self, m_dev, params: LLGParams, mat: Material, mask: int,
+                            max_err: float, headroom: float, dt_min: float, dt_max: float):
+        p, cm = params._c(), mat._c()
+        eps, dtu, acc = C.c_double(), C.c_double(), C.c_int()
+        _check(lib().llgmi_dev_rk23_relax_step(self._h, _ptr(m_dev), C.byref(p), C.byref(cm), mask,
+                                               max_err, headroom, dt_min, dt_max, C.byref(eps),
+                                               C.byref(acc), C.byref(dtu)))
+        params.dt = p.dt
+        return eps.value, bool(acc.value), dtu.value
+
+    def dev_compute_energy_geom(self, m_dev, mat: Material, b_ext, geom_dev=None) -> dict:
+        bx = _f64(b_ext).reshape(3)
+        cm = mat._c()
+        e = Energy()
+        _check(lib().llgmi_dev_compute_energy_geom(self._h, _ptr(m_dev), C.byref(cm), _dp(bx),
+                                                   _ptr(geom_dev), C.byref(e)))
+        return e.as_dict()
+
+    def dev_relax_with_report(self, m_dev, params: LLGParams, mat: Material, mask: int,
+                              settings: RelaxSettings) -> dict:
+        p, cm = params._c(), mat._c()
+        rep = RelaxReport()
+        _check(lib().llgmi_dev_relax_with_report(self._h, _ptr(m_dev), C.byref(p), C.byref(cm),
+                                                 mask, C.byref(settings), C.byref(rep)))
+        params.dt = p.dt
+        return rep.as_dict()
+
     # ---- device (SoA, context precision) ----
     def dev_build_h_eff(self, m_dev, b_dev, params: LLGParams, mat: Material, mask: int,
                         geom_dev=None) -> None:
@@ -422,7 +601,7 @@ class Context:
     def profile_enable(self, classes=None) -> None:
         """Times the given kernel classes (iterable of K_*), all when None; [] stops."""
         if classes is None:
-            mask = (1 << 8) - 1
+            mask = (1 << K_COUNT) - 1
         else:
             mask = 0
             for c in classes:

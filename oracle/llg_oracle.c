@@ -657,6 +657,7 @@ void orc_ctx_free(orc_ctx *c) {
 }
 
 int64_t orc_ctx_field_builds(const orc_ctx *c) { return c->builds; }
+void orc_ctx_grid(const orc_ctx *c, orc_grid *g) { *g = c->g; }
 
 /* demag.rs:47-67, 109-142: FFT (demag_fft_uniform.rs:157-172, open BC), or
  * MG for demag_method mg / dst (add_demag_field_poisson_mg,

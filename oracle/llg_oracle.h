@@ -104,6 +104,8 @@ void orc_rk45_step(orc_ctx *c, double *m, orc_params *p, const orc_material *mat
                    int field_mask, double max_err, double headroom, double dt_min,
                    double dt_max, double *eps, int *accepted, double *dt_used);
 void orc_rk45_reset(orc_ctx *c);
+/* The context's grid. */
+void orc_ctx_grid(const orc_ctx *c, orc_grid *g);
 /* Number of field builds performed by this context (for FSAL accounting). */
 int64_t orc_ctx_field_builds(const orc_ctx *c);
 

@@ -83,6 +83,7 @@ def lib():
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         _bind_mg(L)
+        _bind_relax(L)
         _lib = L
     return _lib
 
@@ -597,3 +598,172 @@ class MG:
         return {"cycles": st.cycles, "rhs_max": st.rhs_max, "tol_target": st.tol_target,
                 "last_residual": st.last_residual, "diverged": bool(st.diverged),
                 "n_charges": st.n_charges}
+
+
+# --- relaxation: RK23 relax stepper, energy, relax controller ---------------
+# llg.rs:38-50, 196-436; energy.rs:36-305; relax.rs:32-808
+TQ_MAX, TQ_MEAN, TQ_RMS, TQ_MEANSQ = 0, 1, 2, 3
+STOP_MAX_ACCEPTED, STOP_PHASE2_DISABLED, STOP_TIGHTEN_FLOOR, STOP_GATE_NOT_SATISFIED = 0, 1, 2, 3
+STAGE_NONE, STAGE_BELOW_THRESHOLD, STAGE_PLATEAU, STAGE_MAX_ACCEPTED = -1, 0, 1, 2
+
+
+class RelaxSettings(C.Structure):
+    """RelaxSettings (relax.rs:73-105); has_* flags stand for Option::Some."""
+    _fields_ = [("max_err", C.c_double), ("headroom", C.c_double), ("dt_min", C.c_double),
+                ("dt_max", C.c_double), ("phase1_enabled", C.c_int32),
+                ("phase2_enabled", C.c_int32), ("energy_stride", C.c_int64),
+                ("rel_energy_tol", C.c_double), ("torque_metric", C.c_int32),
+                ("has_torque_threshold", C.c_int32), ("torque_threshold", C.c_double),
+                ("torque_check_stride", C.c_int64), ("torque_plateau_rel", C.c_double),
+                ("torque_plateau_abs", C.c_double), ("torque_plateau_checks", C.c_int64),
+                ("torque_plateau_min_checks", C.c_int64), ("tighten_factor", C.c_double),
+                ("tighten_floor", C.c_double), ("max_accepted_steps", C.c_int64),
+                ("has_final_torque_gate", C.c_int32), ("has_final_torque_gate_max", C.c_int32),
+                ("final_torque_gate", C.c_double), ("final_torque_gate_max", C.c_double),
+                ("gate_max_extra_accepted_steps", C.c_int64), ("gate_plateau_fails", C.c_int64)]
+
+
+class _RelaxReport(C.Structure):
+    _fields_ = [("accepted_steps", C.c_int64), ("rejected_steps", C.c_int64),
+                ("torque_checks", C.c_int64), ("torque_field_rebuilds", C.c_int64),
+                ("stop_reason", C.c_int32), ("last_stage_stop", C.c_int32),
+                ("has_final_torque", C.c_int32), ("has_final_torque_max", C.c_int32),
+                ("final_torque", C.c_double), ("final_torque_max", C.c_double),
+                ("final_max_err", C.c_double), ("final_dt", C.c_double)]
+
+
+def _report_dict(r: _RelaxReport) -> dict:
+    return {"accepted_steps": r.accepted_steps, "rejected_steps": r.rejected_steps,
+            "torque_checks": r.torque_checks, "torque_field_rebuilds": r.torque_field_rebuilds,
+            "stop_reason": r.stop_reason, "last_stage_stop": r.last_stage_stop,
+            "final_torque": r.final_torque if r.has_final_torque else None,
+            "final_torque_max": r.final_torque_max if r.has_final_torque_max else None,
+            "final_max_err": r.final_max_err, "final_dt": r.final_dt}
+
+
+def _bind_relax(L):
+    dp = C.POINTER(C.c_double)
+    u8p = C.POINTER(C.c_uint8)
+    vp = C.c_void_p
+    L.orc_llg_rhs_relax.argtypes = [C.c_int64, dp, dp, C.c_double, C.c_double, dp]
+    L.orc_rk23_new.argtypes = [C.POINTER(_Grid)]
+    L.orc_rk23_new.restype = vp
+    L.orc_rk23_free.argtypes = [vp]
+    L.orc_rk23_invalidate.argtypes = [vp]
+    L.orc_rk23_last_b_eff.argtypes = [vp]
+    L.orc_rk23_last_b_eff.restype = dp
+    L.orc_rk23_step.argtypes = [vp, vp, dp, C.POINTER(_Params), C.POINTER(_Material), C.c_int,
+                                C.c_double, C.c_double, C.c_double, C.c_double, dp,
+                                C.POINTER(C.c_int), dp]
+    L.orc_energy.argtypes = [vp, dp, C.POINTER(_Material), dp, u8p, dp]
+    L.orc_torque_metric.argtypes = [C.c_int64, dp, dp, C.c_int]
+    L.orc_torque_metric.restype = C.c_double
+    L.orc_relax_settings_default.argtypes = [C.POINTER(RelaxSettings)]
+    L.orc_relax_settings_mumax_like.argtypes = [C.POINTER(RelaxSettings)]
+    L.orc_relax.argtypes = [vp, vp, dp, C.POINTER(_Params), C.POINTER(_Material), C.c_int,
+                            C.POINTER(RelaxSettings), C.POINTER(_RelaxReport)]
+
+
+def llg_rhs_relax(m, b, gamma: float, alpha: float) -> np.ndarray:
+    m, b = _f(m), _f(b)
+    k = np.zeros_like(m)
+    lib().orc_llg_rhs_relax(m.size // 3, _dp(m), _dp(b), gamma, alpha, _dp(k))
+    return k
+
+
+def torque_metric(m, b, metric: int = TQ_MAX) -> float:
+    m, b = _f(m), _f(b)
+    return float(lib().orc_torque_metric(m.size // 3, _dp(m), _dp(b), int(metric)))
+
+
+def relax_settings(**kw) -> RelaxSettings:
+    """RelaxSettings::default() with fields from kw; torque_threshold /
+    final_torque_gate / final_torque_gate_max = None switch that option off."""
+    s = RelaxSettings()
+    lib().orc_relax_settings_default(C.byref(s))
+    return _apply_settings(s, kw)
+
+
+def mumax_like_relax_settings(**kw) -> RelaxSettings:
+    s = RelaxSettings()
+    lib().orc_relax_settings_mumax_like(C.byref(s))
+    return _apply_settings(s, kw)
+
+
+def _apply_settings(s: RelaxSettings, kw: dict) -> RelaxSettings:
+    for k, v in kw.items():
+        if k in ("torque_threshold", "final_torque_gate", "final_torque_gate_max"):
+            setattr(s, "has_" + k, 0 if v is None else 1)
+            setattr(s, k, 0.0 if v is None else float(v))
+        else:
+            setattr(s, k, v)
+    return s
+
+
+class RK23:
+    """RK23Scratch (llg.rs:196-268) for a grid, used with a Context."""
+
+    def __init__(self, grid: Grid):
+        g = grid.c()
+        self.grid = grid
+        self._h = lib().orc_rk23_new(C.byref(g))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_rk23_free(self._h)
+            self._h = None
+
+    def invalidate(self) -> None:
+        lib().orc_rk23_invalidate(self._h)
+
+    def last_b_eff(self) -> Optional[np.ndarray]:
+        p = lib().orc_rk23_last_b_eff(self._h)
+        if not p:
+            return None
+        return np.ctypeslib.as_array(p, shape=(self.grid.n, 3)).copy()
+
+    def step(self, ctx: "Context", m: np.ndarray, params: Params, mat: Material,
+             field_mask: int = FULL, max_err: float = 1e-5, headroom: float = 0.8,
+             dt_min: float = 1e-18, dt_max: float = 1e-11):
+        """step_llg_rk23_recompute_field_masked_relax_adaptive: advances m in
+        place, updates params.dt, returns (eps, accepted, dt_used)."""
+        assert m.dtype == np.float64 and m.flags.c_contiguous
+        p, cm = params.c(), mat.c()
+        eps, dtu, acc = C.c_double(), C.c_double(), C.c_int()
+        lib().orc_rk23_step(ctx._h, self._h, _dp(m), C.byref(p), C.byref(cm), field_mask,
+                            max_err, headroom, dt_min, dt_max, C.byref(eps), C.byref(acc),
+                            C.byref(dtu))
+        params.dt = p.dt
+        return eps.value, bool(acc.value), dtu.value
+
+
+ENERGY_TERMS = ("exchange", "anisotropy", "zeeman", "dmi", "demag")
+
+
+def compute_energy(ctx: "Context", m, mat: Material, b_ext, geom=None) -> dict:
+    """compute_energy_geom (energy.rs:80-283) -> {term: joules}."""
+    m = _f(m)
+    e = np.zeros(5)
+    bx = _f(b_ext)
+    mk = _mask(geom)
+    cm = mat.c()
+    lib().orc_energy(ctx._h, _dp(m), C.byref(cm), _dp(bx), _u8(mk), _dp(e))
+    return dict(zip(ENERGY_TERMS, e.tolist()))
+
+
+def compute_total_energy(ctx: "Context", m, mat: Material, b_ext, geom=None) -> float:
+    e = compute_energy(ctx, m, mat, b_ext, geom)
+    return (((e["exchange"] + e["anisotropy"]) + e["zeeman"]) + e["dmi"]) + e["demag"]
+
+
+def relax_with_report(ctx: "Context", rk: RK23, m: np.ndarray, params: Params, mat: Material,
+                      field_mask: int, settings: RelaxSettings) -> dict:
+    """relax_with_report (relax.rs:334-774): m and params.dt in place,
+    settings.max_err written back; returns the RelaxReport as a dict."""
+    assert m.dtype == np.float64 and m.flags.c_contiguous
+    p, cm = params.c(), mat.c()
+    rep = _RelaxReport()
+    lib().orc_relax(ctx._h, rk._h, _dp(m), C.byref(p), C.byref(cm), field_mask,
+                    C.byref(settings), C.byref(rep))
+    params.dt = p.dt
+    return _report_dict(rep)

@@ -33,3 +33,40 @@ def gpu_lib():
     if llgmi.device_count() < 1:
         pytest.fail("no HIP device visible to libllgmi.so")
     return llgmi
+
+
+class DeviceBuffer:
+    """Device memory from the HIP runtime libllgmi.so is linked against
+    (hipMalloc through ctypes), for the llgmi_dev_* entry points; freed on
+    close() or garbage collection."""
+
+    _hip = None
+
+    def __init__(self, nbytes: int):
+        import ctypes as C
+        if DeviceBuffer._hip is None:
+            import llgmi
+            llgmi.lib()                       # loads libamdhip64.so.7 with the library
+            DeviceBuffer._hip = C.CDLL("libamdhip64.so.7")
+            DeviceBuffer._hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            DeviceBuffer._hip.hipFree.argtypes = [C.c_void_p]
+        p = C.c_void_p()
+        rc = DeviceBuffer._hip.hipMalloc(C.byref(p), nbytes)
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc({nbytes}) failed: {rc}")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def close(self) -> None:
+        if self.ptr:
+            DeviceBuffer._hip.hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

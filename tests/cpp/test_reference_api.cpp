@@ -163,6 +163,68 @@ static void dispatcher_fft_matches_builder() {
     CHECK(d < 1e-12, "max diff %g", d);
 }
 
+// tests/validation.rs:222-318: dE = -Ms V B_eff . dm at an interior cell of a
+// Neel wall (init_bloch_wall -> init_neel_wall_x, vector_field.rs:103-126, 162-164).
+static void energy_gradient_consistency_exchange_anisotropy() {
+    Grid2D grid(16, 16, 5e-9, 5e-9, 5e-9);
+    VectorField2D m(grid);
+    const double x0 = 0.5 * (double)grid.nx * grid.dx, width = 5.0 * grid.dx;
+    for (size_t j = 0; j < grid.ny; ++j)
+        for (size_t i = 0; i < grid.nx; ++i) {
+            double u = (((double)i + 0.5) * grid.dx - x0) / width;
+            double mz = std::tanh(u), mx = 1.0 / std::cosh(u);
+            double nn = std::sqrt(mx * mx + mz * mz);
+            m.data[m.idx(i, j)] = {mx / nn, 0.0, mz / nn};
+        }
+    LLGParams params{GAMMA_E_RAD_PER_S_T, 0.02, 1e-13, {0.0, 0.0, 0.0}};
+    Material material;
+    material.ms = 8.0e5; material.a_ex = 13e-12; material.k_u = 500.0;
+    VectorField2D b_eff(grid);
+    effective_field::build_h_eff(grid, m, b_eff, params, material);
+    const double e0 = energy::compute_total_energy(grid, m, material, params.b_ext);
+    const size_t idx = grid.idx(grid.nx / 2, grid.ny / 2);
+    const Vec3 m0 = m.data[idx];
+    const double a[3] = {0.37, -0.24, 0.91};
+    const double mdota = m0[0] * a[0] + m0[1] * a[1] + m0[2] * a[2];
+    double dir[3] = {a[0] - mdota * m0[0], a[1] - mdota * m0[1], a[2] - mdota * m0[2]};
+    const double dn = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    for (double &d : dir) d /= dn;
+    const double eps = 1e-6;
+    double v[3] = {m0[0] + eps * dir[0], m0[1] + eps * dir[1], m0[2] + eps * dir[2]};
+    const double vn = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const Vec3 m1{v[0] / vn, v[1] / vn, v[2] / vn};
+    VectorField2D m_pert = m;
+    m_pert.data[idx] = m1;
+    const double e1 = energy::compute_total_energy(grid, m_pert, material, params.b_ext);
+    const Vec3 b = b_eff.data[idx];
+    const double de_pred = -material.ms * grid.cell_volume() *
+                           (b[0] * (m1[0] - m0[0]) + b[1] * (m1[1] - m0[1]) + b[2] * (m1[2] - m0[2]));
+    const double bn = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    const double scale = std::fmax(material.ms * grid.cell_volume() * bn * eps, 1e-30);
+    CHECK(std::fabs((e1 - e0) - de_pred) < 5e-2 * scale, "num=%g pred=%g", e1 - e0, de_pred);
+}
+
+// validation.rs:88-122 (macrospin relaxes toward the easy axis), through the
+// relax controller (relax.rs:334-774) with the SP4-style threshold settings.
+static void macrospin_relax_controller_reaches_easy_axis() {
+    Grid2D grid(1, 1, 1.0, 1.0, 1.0);
+    VectorField2D m(grid);
+    m.set_uniform(0.6, 0.0, 0.8);
+    LLGParams params{GAMMA_E_RAD_PER_S_T, 0.5, 1e-13, {0.0, 0.0, 0.0}};
+    Material material;
+    material.ms = 8.0e5; material.k_u = 5e5;
+    llg::RK23Scratch scratch(grid);
+    relax::RelaxSettings settings;
+    auto rep = relax::relax_with_report(grid, m, params, material, scratch, FieldMask::ExchAnis, settings);
+    CHECK(rep.stop_reason == relax::RelaxStopReason::TightenFloorReached, "stop=%d", (int)rep.stop_reason);
+    CHECK(rep.final_torque && *rep.final_torque <= 1e-4, "torque");
+    CHECK(settings.max_err == 1e-9, "max_err=%g", settings.max_err);
+    CHECK(m.data[0][2] > 0.999999, "mz=%g", m.data[0][2]);
+    CHECK(scratch.last_b_eff().has_value(), "last_b_eff");
+    scratch.invalidate_last_b_eff();
+    CHECK(!scratch.last_b_eff().has_value(), "invalidated");
+}
+
 int main() {
     const std::pair<const char *, std::function<void()>> tests[] = {
         {"demag_single_cell_cube_is_close_to_minus_mu0_over_3", demag_single_cell_cube_is_close_to_minus_mu0_over_3},
@@ -172,6 +234,8 @@ int main() {
         {"dmi_field_flips_sign_with_d", dmi_field_flips_sign_with_d},
         {"macrospin_precession_quarter_turn_rk45_adaptive", macrospin_precession_quarter_turn_rk45_adaptive},
         {"dispatcher_fft_matches_builder", dispatcher_fft_matches_builder},
+        {"energy_gradient_consistency_exchange_anisotropy", energy_gradient_consistency_exchange_anisotropy},
+        {"macrospin_relax_controller_reaches_easy_axis", macrospin_relax_controller_reaches_easy_axis},
     };
     for (auto &t : tests) {
         std::printf("test %s\n", t.first);

@@ -120,3 +120,42 @@ def test_mg_config_from_env_follows_reference_parsing(monkeypatch):
     c = llgmi.mg_config_from_env()
     assert (c.pad_xy, c.smoother, c.warm_start, c.bc) == (9, 0, 0, 2)
     assert (c.stencil, c.prolong, c.coarse_op, c.iso27_alpha, c.v_cycles) == (2, 1, 1, 0.5, 4)
+
+
+def test_relax_struct_layouts_and_defaults():
+    """llgmi_relax_settings / _report / llgmi_energy sizes, and the defaults of
+    RelaxSettings::default() (relax.rs:107-139) and mumax_like_relax_settings
+    (relax.rs:789-808) as the library fills them, against the oracle's."""
+    import math
+    import llgmi
+    from oracle import oracle as O
+    assert C.sizeof(llgmi.RelaxSettings) == 176
+    assert C.sizeof(llgmi.RelaxReport) == 80
+    assert C.sizeof(llgmi.Energy) == 40
+    assert [f for f, _ in llgmi.RelaxSettings._fields_] == [f for f, _ in O.RelaxSettings._fields_]
+    for lib_fn, orc_fn in ((llgmi.relax_settings, O.relax_settings),
+                           (llgmi.mumax_like_relax_settings, O.mumax_like_relax_settings)):
+        a, b = lib_fn(), orc_fn()
+        for f, _ in llgmi.RelaxSettings._fields_:
+            assert getattr(a, f) == getattr(b, f), f
+    d = llgmi.relax_settings()
+    assert (d.max_err, d.headroom, d.dt_min, d.dt_max) == (1e-5, 0.8, 1e-18, 1e-11)
+    assert d.has_torque_threshold == 1 and d.torque_threshold == 1e-4
+    assert d.torque_metric == llgmi.TORQUE_MAX and d.energy_stride == 3
+    assert d.torque_plateau_checks == 0 and d.torque_plateau_min_checks == 5
+    assert d.tighten_factor == math.sqrt(0.5) and d.tighten_floor == 1e-9   # FRAC_1_SQRT_2
+    assert d.max_accepted_steps == 2_000_000 and d.has_final_torque_gate == 0
+    mm = llgmi.mumax_like_relax_settings()
+    assert mm.torque_metric == llgmi.TORQUE_MEANSQ and mm.has_torque_threshold == 0
+    assert mm.rel_energy_tol == 0.0 and mm.torque_plateau_checks == 1
+    assert mm.torque_plateau_min_checks == 1 and mm.torque_plateau_rel == 0.0
+
+
+def test_relax_arguments_rejected_without_device():
+    import llgmi
+    L = llgmi.lib()
+    s = llgmi.relax_settings(torque_metric=7)
+    assert L.llgmi_relax_with_report(None, None, None, None, 2, C.byref(s), None) == llgmi.ERR_INVALID
+    assert L.llgmi_rk23_relax_step(None, None, None, None, 2, 1e-5, 0.8, 1e-18, 1e-11,
+                                   None, None, None) == llgmi.ERR_INVALID
+    assert L.llgmi_relax_settings_default(None) == llgmi.ERR_INVALID
