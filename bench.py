@@ -19,6 +19,8 @@ The JSON line also carries:
   rel_err_vs_fp64  FP32 against the FP64 GPU path on the same m, per term;
   kernels_us    per-kernel average durations from a separate event-timed pass;
   rk45          accepted RK45 steps/s (FSAL, 6 field evaluations per step);
+  rk23_relax    accepted relax-mode RK23 steps/s (FSAL, 3 field evaluations per
+                step) and energy breakdowns/s;
   host_aos      the same evaluation through the host entry point (AoS f64 in and
                 out, PCIe transfers included);
   mg            the multigrid Poisson demag on a 512 x 512 film with a square
@@ -316,6 +318,30 @@ def main():
         rk = {"accepted_steps_per_s": round(acc / dtr, 1), "attempts": att, "accepted": acc,
               "max_err": 1e-5, "last_dt": rp.dt}
 
+    # relax-mode RK23 accepted steps/s (FSAL, 3 field evaluations per step)
+    # and energy breakdowns/s (one demag field + one reduction each)
+    r23 = {}
+    if args.rk45_steps > 0:
+        mr = m.clone()
+        rp = L.LLGParams(alpha=ALPHA, dt=1e-14, b_ext=BEXT)
+        for _ in range(3):
+            ctx.dev_rk23_relax_step(mr, rp, mat, L.FieldMask.Full, 1e-5, 0.8, 1e-18, 1e-12)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        acc = att = 0
+        while acc < args.rk45_steps and att < 4 * args.rk45_steps:
+            _, a, _ = ctx.dev_rk23_relax_step(mr, rp, mat, L.FieldMask.Full, 1e-5, 0.8, 1e-18, 1e-12)
+            acc += int(a)
+            att += 1
+        ctx.synchronize()
+        dtr = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(10):
+            ctx.dev_compute_energy_geom(mr, mat, BEXT)
+        dte = time.perf_counter() - t0
+        r23 = {"accepted_steps_per_s": round(acc / dtr, 1), "attempts": att, "accepted": acc,
+               "max_err": 1e-5, "last_dt": rp.dt, "energy_evals_per_s": round(10 / dte, 1)}
+
     acc_rep = None
     host_rate = None
     if not args.quick:
@@ -357,6 +383,7 @@ def main():
         "kernels_us": kernels,
         "rel_err_vs_fp64": acc_rep,
         "rk45": rk,
+        "rk23_relax": r23,
         "host_aos_evals_per_s": host_rate,
         "mg": mg,
     }

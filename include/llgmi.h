@@ -121,8 +121,11 @@ int llgmi_build_h_eff(llgmi_ctx *ctx, const double *m, double *b_eff,
 
 /* demag::add_demag_field_pbc (effective_field/demag.rs:70-143): accumulate.
  * Dispatch follows mat->demag_method (MG for open boundaries, FFT when
- * pbc_x or pbc_y > 0 or when the method is FFT).  Periodic images
- * (pbc > 0) return LLGMI_ERR_UNSUPPORTED. */
+ * pbc_x or pbc_y > 0 or when the method is FFT).  pbc > 0 makes that axis
+ * periodic with the MuMax-style image sums of build_kernel_realspace_pbc
+ * (demag_fft_uniform.rs:941-1005); the periodic operator is cached in the
+ * context beside the open one and rebuilt when pbc_x / pbc_y change.  Image
+ * ranges beyond 2^27 kernel displacements return LLGMI_ERR_UNSUPPORTED. */
 int llgmi_add_demag_field_pbc(llgmi_ctx *ctx, const double *m, double *b_eff,
                               const llgmi_material *mat, uint64_t pbc_x, uint64_t pbc_y);
 /* demag::compute_demag_field_pbc (effective_field/demag.rs:146-156): overwrite. */

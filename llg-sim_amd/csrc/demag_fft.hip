@@ -548,6 +548,46 @@ __global__ void k_reflect(const double *__restrict__ base, int nx, int ny, int p
     kr[3 * np + o] = k[3];
 }
 
+// Periodic images (build_kernel_realspace_pbc, demag_fft_uniform.rs:941-1005):
+// base(sx, sy) = sum of the kernel over every displacement that folds onto bin
+// (sx, sy) -- sx + k nx with |sx + k nx| <= ex - 1 on a periodic axis (ex =
+// nx pbc_x), sx alone on an open one -- summed y-major in ascending order as the
+// reference accumulates its bins.  ext holds the kernel on the non-negative
+// quadrant [0, ex) x [0, ey); a negative displacement takes the parity of each
+// component (xx, yy, zz even; xy odd in x and in y).  The folded bins keep that
+// parity, so the open-boundary reflection below applies unchanged.
+__global__ void k_fold_images(const double *__restrict__ ext, int ex, int ey, int nx, int ny, int pbx,
+                              int pby, double *__restrict__ base) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny) return;
+    const int sx = (int)(t % nx), sy = (int)(t / nx);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int kyl = pby > 0 ? -pby : 0, kyh = pby > 0 ? pby : 0;
+    const int kxl = pbx > 0 ? -pbx : 0, kxh = pbx > 0 ? pbx : 0;
+    for (int ky = kyl; ky <= kyh; ++ky) {
+        const int64_t vy = (int64_t)sy + (int64_t)ky * ny;
+        if (vy > ey - 1 || vy < -(int64_t)(ey - 1)) continue;
+        const int64_t ay = vy < 0 ? -vy : vy;
+        for (int kx = kxl; kx <= kxh; ++kx) {
+            const int64_t vx = (int64_t)sx + (int64_t)kx * nx;
+            if (vx > ex - 1 || vx < -(int64_t)(ex - 1)) continue;
+            const int64_t ax = vx < 0 ? -vx : vx;
+            const double *e = ext + (ay * ex + ax) * 4;
+            const double sg = (vx == 0 || vy == 0) ? 0.0 : ((vx > 0) == (vy > 0) ? 1.0 : -1.0);
+            acc[0] += e[0];
+            acc[1] += sg * e[1];
+            acc[2] += e[2];
+            acc[3] += e[3];
+        }
+    }
+    // base keeps the +x +y quadrant convention of k_quad (k_reflect signs kxy)
+    base[t * 4 + 0] = acc[0];
+    base[t * 4 + 1] = acc[1];
+    base[t * 4 + 2] = acc[2];
+    base[t * 4 + 3] = acc[3];
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -614,10 +654,16 @@ static int forced_radix_log2() {
 
 template <typename T>
 void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double dz_,
-                        hipStream_t stream) {
+                        hipStream_t stream, int64_t pbc_x_, int64_t pbc_y_) {
     const int forced = forced_radix_log2();
     release();
     nx = nx_; ny = ny_; dx = dx_; dy = dy_; dz = dz_;
+    pbc_x = pbc_x_; pbc_y = pbc_y_;
+    // kernel range per axis: |s| <= n pbc - 1 on a periodic axis, n - 1 on an open one
+    const int64_t ex = pbc_x > 0 ? nx * pbc_x : nx, ey = pbc_y > 0 ? ny * pbc_y : ny;
+    if (pbc_x < 0 || pbc_y < 0 || pbc_x > (1 << 20) || pbc_y > (1 << 20) || ex * ey > (int64_t(1) << 27))
+        throw Error(3, "FFT demag: periodic image range " + std::to_string(ex) + " x " + std::to_string(ey) +
+                           " cells is beyond the built limit of 2^27 kernel displacements");
     px = next_pow2(2 * nx);
     py = next_pow2(2 * ny);
     hx = px / 2 + 1;
@@ -630,17 +676,26 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     const double mu0 = 4.0 * M_PI * 1e-7;
     const double acc = 10.0;  // DEMAG_ACCURACY (demag_fft_uniform.rs:432)
 
-    double *base = nullptr, *kr = nullptr;
+    double *base = nullptr, *kr = nullptr, *ext = nullptr;
     cpx<double> *sk = nullptr, *twx64 = nullptr, *twy64 = nullptr;
     try {
-        // 1) base-quadrant quadrature (FP64)
+        // 1) base-quadrant quadrature (FP64); with periodic axes over the whole
+        //    image range, then folded onto the bins
         const int64_t nb = nx * ny;
         LLGMI_HIP(hipMalloc(&base, sizeof(double) * 4 * nb));
+        const bool periodic = pbc_x > 0 || pbc_y > 0;
+        if (periodic) LLGMI_HIP(hipMalloc(&ext, sizeof(double) * 4 * ex * ey));
         {
             int nt = 256;
-            int64_t nbk = (3 * nb + nt - 1) / nt;
-            hipLaunchKernelGGL(k_quad, dim3((unsigned)nbk), dim3(nt), 0, stream, base, (int)nx,
-                               (int)ny, dx, dy, dz, acc, mu0);
+            int64_t nq = periodic ? ex * ey : nb;
+            int64_t nbk = (3 * nq + nt - 1) / nt;
+            hipLaunchKernelGGL(k_quad, dim3((unsigned)nbk), dim3(nt), 0, stream, periodic ? ext : base,
+                               (int)(periodic ? ex : nx), (int)(periodic ? ey : ny), dx, dy, dz, acc, mu0);
+            LLGMI_LAUNCH_CHECK();
+        }
+        if (periodic) {
+            hipLaunchKernelGGL(k_fold_images, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, ext,
+                               (int)ex, (int)ey, (int)nx, (int)ny, (int)pbc_x, (int)pbc_y, base);
             LLGMI_LAUNCH_CHECK();
         }
         // 2) reflect into padded real arrays
@@ -676,6 +731,7 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         LLGMI_HIP(hipStreamSynchronize(stream));
     } catch (...) {
         if (base) (void)hipFree(base);
+        if (ext) (void)hipFree(ext);
         if (kr) (void)hipFree(kr);
         if (sk) (void)hipFree(sk);
         if (twx64) (void)hipFree(twx64);
@@ -684,6 +740,7 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         throw;
     }
     (void)hipFree(base);
+    if (ext) (void)hipFree(ext);
     (void)hipFree(kr);
     (void)hipFree(sk);
     (void)hipFree(twx64);

@@ -84,6 +84,7 @@ template <typename T>
 struct DemagFFT {
     bool built = false;
     int64_t nx = 0, ny = 0, px = 0, py = 0, hx = 0, hy = 0;
+    int64_t pbc_x = 0, pbc_y = 0;   // periodic image counts (0 = open axis)
     int lx = 0, ly = 0;
     double dx = 0, dy = 0, dz = 0;
     cpx<T> *tw_x = nullptr, *tw_y = nullptr;
@@ -94,7 +95,11 @@ struct DemagFFT {
 
     ~DemagFFT();
     void release();
-    void build(int64_t nx, int64_t ny, double dx, double dy, double dz, hipStream_t stream);
+    // pbc_x / pbc_y > 0: periodic axis with MuMax-style image sums; the
+    // convolution stays zero padded (a periodic kernel folded onto the
+    // displacements -(n-1)..n-1 gives the circular convolution of period n).
+    void build(int64_t nx, int64_t ny, double dx, double dy, double dz, hipStream_t stream,
+               int64_t pbc_x = 0, int64_t pbc_y = 0);
     // bd = B_demag (overwrite) for M = ms * m; m and bd are SoA with 3*nx*ny entries.
     void field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *prof);
     // Times the radix-8 and radix-16 variants of each pass on this grid and keeps the faster.

@@ -71,6 +71,7 @@ struct Engine : EngineBase {
     std::vector<double> h_tmp;
 
     DemagFFT<T> fft;
+    DemagFFT<T> fft_pbc;   // the last periodic operator (keyed by pbc_x, pbc_y)
     std::unique_ptr<MGSolver> mg;
     bool has_mg_cfg = false;       // llgmi_ctx_set_mg_config; otherwise the environment per call
     llgmi_mg_config mg_cfg{};
@@ -141,10 +142,15 @@ struct Engine : EngineBase {
     void demag_into_bd(const T *m, const llgmi_material &mat, int method, uint64_t pbc_x,
                        uint64_t pbc_y) {
         alloc(d_bd, 3 * n);
-        bool use_fft = method == LLGMI_DEMAG_FFT || pbc_x > 0 || pbc_y > 0;
-        if (use_fft) {
-            if (pbc_x > 0 || pbc_y > 0)
-                throw Error(LLGMI_ERR_UNSUPPORTED, "FFT demag with periodic images (pbc_x/pbc_y > 0) is not built");
+        // demag.rs:109-141: MG and DST fall back to FFT with periodic axes
+        if (pbc_x > 0 || pbc_y > 0) {
+            if (!fft_pbc.built || fft_pbc.pbc_x != (int64_t)pbc_x || fft_pbc.pbc_y != (int64_t)pbc_y) {
+                if (pbc_x > (uint64_t)(1 << 20) || pbc_y > (uint64_t)(1 << 20))
+                    throw Error(LLGMI_ERR_UNSUPPORTED, "periodic image count beyond 2^20");
+                fft_pbc.build(g.nx, g.ny, g.dx, g.dy, g.dz, stream, (int64_t)pbc_x, (int64_t)pbc_y);
+            }
+            fft_pbc.field(m, T(mat.ms), d_bd, stream, prof);
+        } else if (method == LLGMI_DEMAG_FFT) {
             ensure_fft();
             fft.field(m, T(mat.ms), d_bd, stream, prof);
         } else {

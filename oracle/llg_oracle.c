@@ -518,22 +518,68 @@ void orc_build_kernel_realspace(const orc_grid *g, int64_t px, int64_t py,
     free(base);
 }
 
+/* build_kernel_realspace_pbc (demag_fft_uniform.rs:941-1005): every
+ * displacement |sx| <= nx*pbc_x - 1 (nx - 1 for an open axis), likewise y,
+ * evaluated directly and accumulated into the wrapped bin (sy outer, sx inner,
+ * as the reference loops); then kxy = 0 on the bin axes. */
+void orc_build_kernel_realspace_pbc(const orc_grid *g, int64_t px, int64_t py, int64_t pbc_x,
+                                    int64_t pbc_y, double *kxx, double *kxy, double *kyy,
+                                    double *kzz) {
+    const int64_t nx = g->nx, ny = g->ny;
+    const int64_t sx_max = pbc_x == 0 ? nx - 1 : nx * pbc_x - 1;
+    const int64_t sy_max = pbc_y == 0 ? ny - 1 : ny * pbc_y - 1;
+    const int64_t wx = 2 * sx_max + 1, wy = 2 * sy_max + 1;
+    size_t np = (size_t)(px * py);
+    memset(kxx, 0, sizeof(double) * np);
+    memset(kxy, 0, sizeof(double) * np);
+    memset(kyy, 0, sizeof(double) * np);
+    memset(kzz, 0, sizeof(double) * np);
+    double *vals = (double *)malloc(sizeof(double) * 4 * (size_t)(wx * wy));
+    /* the values are independent of evaluation order; accumulation below is serial */
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t t = 0; t < wx * wy; ++t) {
+        int64_t sy = t / wx - sy_max, sx = t % wx - sx_max;
+        orc_kernel_components(g->dx, g->dy, g->dz, sx, sy, ORC_DEMAG_ACCURACY, vals + 4 * t);
+    }
+    for (int64_t sy = -sy_max; sy <= sy_max; ++sy) {
+        int64_t iy = wrap_index(sy, py);
+        for (int64_t sx = -sx_max; sx <= sx_max; ++sx) {
+            int64_t ix = wrap_index(sx, px);
+            int64_t idx = iy * px + ix;
+            const double *k = vals + 4 * ((sy + sy_max) * wx + (sx + sx_max));
+            kxx[idx] += k[0];
+            kxy[idx] += k[1];
+            kyy[idx] += k[2];
+            kzz[idx] += k[3];
+        }
+    }
+    for (int64_t sy = -sy_max; sy <= sy_max; ++sy) kxy[wrap_index(sy, py) * px + wrap_index(0, px)] = 0.0;
+    for (int64_t sx = -sx_max; sx <= sx_max; ++sx) kxy[wrap_index(0, py) * px + wrap_index(sx, px)] = 0.0;
+    free(vals);
+}
+
 /* ------------------------------------------------------------------------ */
-/* Demag2D (demag_fft_uniform.rs:434-769), open boundaries                    */
+/* Demag2D (demag_fft_uniform.rs:434-769)                                     */
 /* ------------------------------------------------------------------------ */
 struct orc_demag {
     orc_grid g;
+    int64_t pbc_x, pbc_y;
     int64_t px, py;
     fft_plan planx, plany;
     double *kxx, *kxy, *kyy, *kzz;   /* k-space, complex */
     double *mx, *my, *mz, *bx, *by, *bz, *tmp;
 };
 
-orc_demag *orc_demag_new(const orc_grid *g) {
+orc_demag *orc_demag_new(const orc_grid *g) { return orc_demag_new_pbc(g, 0, 0); }
+
+/* Demag2D::new (demag_fft_uniform.rs:470-634): a periodic axis is not padded. */
+orc_demag *orc_demag_new_pbc(const orc_grid *g, int64_t pbc_x, int64_t pbc_y) {
     orc_demag *d = (orc_demag *)calloc(1, sizeof(orc_demag));
     d->g = *g;
-    d->px = 2 * g->nx;
-    d->py = 2 * g->ny;
+    d->pbc_x = pbc_x;
+    d->pbc_y = pbc_y;
+    d->px = pbc_x > 0 ? g->nx : 2 * g->nx;
+    d->py = pbc_y > 0 ? g->ny : 2 * g->ny;
     size_t np = (size_t)(d->px * d->py);
     plan_init(&d->planx, d->px);
     plan_init(&d->plany, d->py);
@@ -541,7 +587,11 @@ orc_demag *orc_demag_new(const orc_grid *g) {
                          &d->bx, &d->by, &d->bz, &d->tmp};
     for (int i = 0; i < 11; ++i) *bufs[i] = (double *)calloc(2 * np, sizeof(double));
     double *re = (double *)malloc(sizeof(double) * 4 * np);
-    orc_build_kernel_realspace(g, d->px, d->py, re, re + np, re + 2 * np, re + 3 * np);
+    if (pbc_x == 0 && pbc_y == 0)
+        orc_build_kernel_realspace(g, d->px, d->py, re, re + np, re + 2 * np, re + 3 * np);
+    else
+        orc_build_kernel_realspace_pbc(g, d->px, d->py, pbc_x, pbc_y, re, re + np, re + 2 * np,
+                                       re + 3 * np);
     double *ks[4] = {d->kxx, d->kxy, d->kyy, d->kzz};
     for (int c = 0; c < 4; ++c) {
         for (size_t i = 0; i < np; ++i) { ks[c][2 * i] = re[c * np + i]; ks[c][2 * i + 1] = 0.0; }
@@ -623,6 +673,7 @@ void orc_demag_add_field(orc_demag *d, const double *m, double *b, double ms) {
 struct orc_ctx {
     orc_grid g;
     orc_demag *demag;
+    orc_demag *demag_pbc;   /* the last periodic operator (keyed by pbc_x, pbc_y) */
     orc_mg *mg;
     orc_mg_cfg mg_cfg;
     double *ms_[8];      /* m2..m7 at [2..7] */
@@ -651,6 +702,7 @@ orc_ctx *orc_ctx_new(const orc_grid *g) {
 void orc_ctx_free(orc_ctx *c) {
     if (!c) return;
     orc_demag_free(c->demag);
+    orc_demag_free(c->demag_pbc);
     orc_mg_free(c->mg);
     for (int s = 1; s <= 7; ++s) { free(c->bs[s]); free(c->ks[s]); free(c->ms_[s]); }
     free(c);
@@ -677,6 +729,23 @@ void orc_add_demag(orc_ctx *c, const double *m, double *b, const orc_material *m
     }
     if (!c->demag) c->demag = orc_demag_new(&c->g);
     orc_demag_add_field(c->demag, m, b, mat->ms);
+}
+
+/* add_demag_field_pbc (demag.rs:70-143): MG and DST fall back to FFT when
+ * either axis is periodic. */
+void orc_add_demag_pbc(orc_ctx *c, const double *m, double *b, const orc_material *mat,
+                       int64_t pbc_x, int64_t pbc_y) {
+    if (!mat->demag) return;
+    if (pbc_x == 0 && pbc_y == 0) {
+        orc_add_demag(c, m, b, mat);
+        return;
+    }
+    if (c->demag_pbc && (c->demag_pbc->pbc_x != pbc_x || c->demag_pbc->pbc_y != pbc_y)) {
+        orc_demag_free(c->demag_pbc);
+        c->demag_pbc = NULL;
+    }
+    if (!c->demag_pbc) c->demag_pbc = orc_demag_new_pbc(&c->g, pbc_x, pbc_y);
+    orc_demag_add_field(c->demag_pbc, m, b, mat->ms);
 }
 
 void orc_ctx_set_mg_cfg(orc_ctx *c, const orc_mg_cfg *cfg) { c->mg_cfg = *cfg; }

@@ -79,9 +79,16 @@ void orc_h_from_unit_m(int source_axis, const double r_center[3], const double c
 void orc_build_kernel_realspace(const orc_grid *g, int64_t px, int64_t py,
                                 double *kxx, double *kxy, double *kyy, double *kzz);
 
+/* Periodic real-space kernels (demag_fft_uniform.rs:941-1005): px = nx on a
+   periodic axis (pbc > 0), 2n on an open one. */
+void orc_build_kernel_realspace_pbc(const orc_grid *g, int64_t px, int64_t py, int64_t pbc_x,
+                                    int64_t pbc_y, double *kxx, double *kxy, double *kyy,
+                                    double *kzz);
+
 /* ---- demag operator ------------------------------------------------------ */
 typedef struct orc_demag orc_demag;
 orc_demag *orc_demag_new(const orc_grid *g);                                 /* Demag2D::new :470-634 (open BC) */
+orc_demag *orc_demag_new_pbc(const orc_grid *g, int64_t pbc_x, int64_t pbc_y);  /* Demag2D::new, any BC */
 void orc_demag_free(orc_demag *d);
 void orc_demag_add_field(orc_demag *d, const double *m, double *b, double ms);  /* :636-769 */
 /* Access to the k-space kernels (complex, px*py each) for tests. */
@@ -99,6 +106,9 @@ void orc_build_h_eff(orc_ctx *c, const double *m, double *b, const orc_params *p
                      const uint8_t *geom);
 /* demag::add_demag_field (demag.rs:47-67) through the cached FFT operator. */
 void orc_add_demag(orc_ctx *c, const double *m, double *b, const orc_material *mat);
+/* demag::add_demag_field_pbc (demag.rs:70-143). */
+void orc_add_demag_pbc(orc_ctx *c, const double *m, double *b, const orc_material *mat,
+                       int64_t pbc_x, int64_t pbc_y);
 /* step_llg_rk45_recompute_field_masked_adaptive (llg.rs:795-1026). */
 void orc_rk45_step(orc_ctx *c, double *m, orc_params *p, const orc_material *mat,
                    int field_mask, double max_err, double headroom, double dt_min,

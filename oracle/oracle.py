@@ -64,6 +64,11 @@ def lib():
         L.orc_build_kernel_realspace.argtypes = [C.POINTER(_Grid), C.c_int64, C.c_int64, dp, dp, dp, dp]
         L.orc_demag_new.argtypes = [C.POINTER(_Grid)]
         L.orc_demag_new.restype = vp
+        L.orc_demag_new_pbc.argtypes = [C.POINTER(_Grid), C.c_int64, C.c_int64]
+        L.orc_demag_new_pbc.restype = vp
+        L.orc_build_kernel_realspace_pbc.argtypes = [C.POINTER(_Grid), C.c_int64, C.c_int64,
+                                                     C.c_int64, C.c_int64, dp, dp, dp, dp]
+        L.orc_add_demag_pbc.argtypes = [vp, dp, dp, C.POINTER(_Material), C.c_int64, C.c_int64]
         L.orc_demag_free.argtypes = [vp]
         L.orc_demag_add_field.argtypes = [vp, dp, dp, C.c_double]
         L.orc_demag_dims.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -290,13 +295,22 @@ def kernel_realspace(grid: Grid, px: int, py: int) -> np.ndarray:
     return k
 
 
-class Demag:
-    """Demag2D (demag_fft_uniform.rs:434-769), open boundaries."""
+def kernel_realspace_pbc(grid: Grid, px: int, py: int, pbc_x: int, pbc_y: int) -> np.ndarray:
+    """Periodic-image kernels (demag_fft_uniform.rs:941-1005): (4, py, px)."""
+    k = np.zeros((4, py, px))
+    g = grid.c()
+    lib().orc_build_kernel_realspace_pbc(C.byref(g), px, py, pbc_x, pbc_y, _dp(k[0]), _dp(k[1]),
+                                         _dp(k[2]), _dp(k[3]))
+    return k
 
-    def __init__(self, grid: Grid):
+
+class Demag:
+    """Demag2D (demag_fft_uniform.rs:434-769); open boundaries unless pbc_x / pbc_y > 0."""
+
+    def __init__(self, grid: Grid, pbc_x: int = 0, pbc_y: int = 0):
         self.grid = grid
         g = grid.c()
-        self._h = lib().orc_demag_new(C.byref(g))
+        self._h = lib().orc_demag_new_pbc(C.byref(g), pbc_x, pbc_y)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -348,6 +362,12 @@ class Context:
         m = _f(m)
         cm = mat.c()
         lib().orc_add_demag(self._h, _dp(m), _dp(b), C.byref(cm))
+
+    def add_demag_pbc(self, m, b: np.ndarray, mat: Material, pbc_x: int, pbc_y: int) -> None:
+        """add_demag_field_pbc (demag.rs:70-143)."""
+        m = _f(m)
+        cm = mat.c()
+        lib().orc_add_demag_pbc(self._h, _dp(m), _dp(b), C.byref(cm), pbc_x, pbc_y)
 
     def rk45_step(self, m: np.ndarray, params: Params, mat: Material, field_mask: int = FULL,
                   max_err: float = 1e-5, headroom: float = 0.8, dt_min: float = 1e-18,

@@ -117,7 +117,7 @@ def test_invalid_arguments_are_rejected(gpu_lib):
         ctx.build_h_eff_masked(m, L.LLGParams(), L.Material(ms=1.0), 7)
     assert e.value.code == L.ERR_INVALID
     with pytest.raises(L.LlgmiError) as e:
-        ctx.add_demag_field_pbc(m, np.zeros_like(m), L.Material(ms=1.0, demag=True), 1, 0)
+        ctx.add_demag_field_pbc(m, np.zeros_like(m), L.Material(ms=1.0, demag=True), (1 << 20) + 1, 0)
     assert e.value.code == L.ERR_UNSUPPORTED
     with pytest.raises(L.LlgmiError):
         L.Context(L.Grid2D(0, 4, 1e-9, 1e-9, 1e-9))

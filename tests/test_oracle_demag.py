@@ -178,3 +178,134 @@ def test_demag_energy_is_nonnegative_for_uniform_state():
     assert e >= 0.0
     # thin film limit: Nzz -> 1 in the centre
     assert 0.8 < -b[16 * 32 + 16, 2] / MU0 < 1.0
+
+
+# ---------------------------------------------------------------------------
+# periodic images (build_kernel_realspace_pbc, demag_fft_uniform.rs:941-1005)
+# ---------------------------------------------------------------------------
+def _image_range(n, pbc):
+    return n - 1 if pbc == 0 else n * pbc - 1
+
+
+@pytest.mark.parametrize("shape,pbc", [((3, 4), (2, 0)), ((4, 3), (0, 2)), ((3, 3), (2, 3)), ((1, 2), (3, 1))])
+def test_pbc_kernel_is_the_folded_image_sum(shape, pbc):
+    """Each wrapped bin holds the sum of the kernel over every displacement
+    |s| <= n pbc - 1 that folds onto it (computed here from the per-displacement
+    quadrature with numpy), with kxy cleared on the bin axes."""
+    nx, ny = shape
+    g = O.Grid(nx, ny, 4e-9, 3e-9, 2e-9)
+    px = nx if pbc[0] else 2 * nx
+    py = ny if pbc[1] else 2 * ny
+    k = O.kernel_realspace_pbc(g, px, py, *pbc)
+    want = np.zeros((4, py, px))
+    rx, ry = _image_range(nx, pbc[0]), _image_range(ny, pbc[1])
+    for sy in range(-ry, ry + 1):
+        for sx in range(-rx, rx + 1):
+            want[:, sy % py, sx % px] += O.kernel_components(4e-9, 3e-9, 2e-9, sx, sy)
+    want[1, :, 0] = 0.0
+    want[1, 0, :] = 0.0
+    assert np.abs(k - want).max() <= 1e-13 * np.abs(want).max()
+    # parity of the folded bins: xx even, xy odd in each periodic direction
+    for iy in range(py):
+        for ix in range(px):
+            jx, jy = (-ix) % px, (-iy) % py
+            assert abs(k[0, iy, ix] - k[0, jy, jx]) <= 1e-12 * np.abs(k[0]).max()
+            assert abs(k[1, iy, ix] + k[1, iy, jx]) <= 1e-12 * np.abs(k[0]).max()
+
+
+def _brute_force_periodic(grid, m, ms, pbc):
+    """B_i = sum over displacements s (|s| <= n pbc - 1 on periodic axes,
+    sources at (i - s) mod n; open axes as in the open sum)."""
+    nx, ny = grid.nx, grid.ny
+    rx, ry = _image_range(nx, pbc[0]), _image_range(ny, pbc[1])
+    M = ms * m.reshape(ny, nx, 3)
+    B = np.zeros((ny, nx, 3))
+    kc = {}
+    for j in range(ny):
+        for i in range(nx):
+            for sy in range(-ry, ry + 1):
+                jj = j - sy
+                if pbc[1]:
+                    jj %= ny
+                elif not 0 <= jj < ny:
+                    continue
+                for sx in range(-rx, rx + 1):
+                    ii = i - sx
+                    if pbc[0]:
+                        ii %= nx
+                    elif not 0 <= ii < nx:
+                        continue
+                    if (sx, sy) not in kc:
+                        v = O.kernel_components(grid.dx, grid.dy, grid.dz, sx, sy)
+                        if sx % nx == 0 or sy % ny == 0:   # xy cleared on the bin axes
+                            v = v.copy()
+                            if (pbc[0] and sx % nx == 0) or sx == 0 or (pbc[1] and sy % ny == 0) or sy == 0:
+                                v[1] = 0.0
+                        kc[(sx, sy)] = v
+                    kxx, kxy, kyy, kzz = kc[(sx, sy)]
+                    mm = M[jj, ii]
+                    B[j, i, 0] += kxx * mm[0] + kxy * mm[1]
+                    B[j, i, 1] += kxy * mm[0] + kyy * mm[1]
+                    B[j, i, 2] += kzz * mm[2]
+    return B.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("shape,pbc", [((3, 4), (2, 0)), ((4, 3), (0, 2)), ((3, 3), (1, 2)), ((2, 1), (3, 0))])
+def test_pbc_convolution_equals_direct_periodic_sum(shape, pbc):
+    """The periodic operator (FFT over exactly n cells on a periodic axis)
+    equals the real-space sum over every image displacement in range."""
+    nx, ny = shape
+    g = O.Grid(nx, ny, 3e-9, 2e-9, 1e-9)
+    m = np.random.default_rng(nx * 5 + ny).standard_normal((nx * ny, 3))
+    b = np.zeros_like(m)
+    O.Demag(g, *pbc).add_field(m, b, 8e5)
+    ref = _brute_force_periodic(g, m, 8e5, pbc)
+    assert np.abs(b - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def test_pbc_translation_invariance():
+    """A cyclic shift of m along a periodic axis shifts B the same way."""
+    g = O.Grid(10, 6, 4e-9, 4e-9, 3e-9)
+    m = np.random.default_rng(3).standard_normal((60, 3))
+    d = O.Demag(g, 3, 0)
+    b = np.zeros_like(m)
+    d.add_field(m, b, 8e5)
+    ms = np.roll(m.reshape(6, 10, 3), 4, axis=1).reshape(-1, 3)
+    bs = np.zeros_like(m)
+    d.add_field(ms, bs, 8e5)
+    np.testing.assert_allclose(bs, np.roll(b.reshape(6, 10, 3), 4, axis=1).reshape(-1, 3),
+                               rtol=0, atol=1e-12 * np.abs(b).max())
+
+
+def test_pbc_thin_film_limits():
+    """Periodic in x and y with many images: an out-of-plane uniform film
+    approaches Bz = -mu0 Ms (Nzz -> 1), an in-plane one Bx -> 0."""
+    g = O.Grid(4, 4, 5e-9, 5e-9, 2e-9)
+    ms = 8e5
+    d = O.Demag(g, 24, 24)
+    mz = np.tile([0.0, 0.0, 1.0], (16, 1))
+    b = np.zeros_like(mz)
+    d.add_field(mz, b, ms)
+    assert np.ptp(b[:, 2]) <= 1e-12 * abs(b[0, 2])
+    assert abs(b[0, 2] / (-MU0 * ms) - 1.0) < 0.01
+    mx = np.tile([1.0, 0.0, 0.0], (16, 1))
+    b = np.zeros_like(mx)
+    d.add_field(mx, b, ms)
+    assert abs(b[0, 0]) < 0.01 * MU0 * ms
+    # open boundaries for the same cell pattern are far from that limit
+    bo = np.zeros_like(mz)
+    O.Demag(g).add_field(mz, bo, ms)
+    assert abs(bo[0, 2]) < 0.9 * MU0 * ms
+
+
+def test_ctx_pbc_dispatch_falls_back_to_fft():
+    """demag.rs:109-141: mg / dst with periodic axes use the FFT path."""
+    g = O.Grid(6, 4, 4e-9, 4e-9, 3e-9)
+    m = np.random.default_rng(8).standard_normal((24, 3))
+    want = np.zeros_like(m)
+    O.Demag(g, 2, 1).add_field(m, want, 8e5)
+    for method in (0, 1, 2):
+        ctx = O.Context(g)
+        b = np.zeros_like(m)
+        ctx.add_demag_pbc(m, b, O.Material(ms=8e5, demag=True, demag_method=method), 2, 1)
+        np.testing.assert_array_equal(b, want)
