@@ -324,6 +324,51 @@ int llgmi_dev_relax_with_report(llgmi_ctx *ctx, void *m_dev, llgmi_params *param
                                 const llgmi_material *mat, int field_mask,
                                 llgmi_relax_settings *settings, llgmi_relax_report *report);
 
+/* ---- on-disk formats (ovf.rs; demag_fft_uniform.rs:206-429) ----------------------- */
+/* OvfMeta (ovf.rs:17-46).  NULL fields (and a NULL meta) take
+ * OvfMeta::magnetization(): title "m", labels m_x m_y m_z, units 1. */
+typedef struct llgmi_ovf_meta {
+    const char *title;
+    const char *const *desc_lines;   /* n_desc lines, each written as "# Desc: <line>" */
+    int32_t n_desc;
+    int32_t reserved;
+    const char *valuelabels[3];
+    const char *valueunits[3];
+} llgmi_ovf_meta;
+
+/* write_ovf2_rectangular_text (ovf.rs:54-159): OVF 2.0 text, the reference's
+ * header lines and Rust number formatting; parent directories are created.
+ * m is AoS f64 (nx*ny cells). */
+int llgmi_write_ovf2_text(const char *path, const llgmi_grid *grid, const double *m,
+                          const llgmi_ovf_meta *meta);
+/* write_ovf2_rectangular_binary4 (ovf.rs:161-262): OVF 2.0 binary4 (f32 LE,
+ * check value 1234567.0). */
+int llgmi_write_ovf2_binary4(const char *path, const llgmi_grid *grid, const double *m,
+                             const llgmi_ovf_meta *meta);
+/* Either writer on a device-resident field (SoA, context precision). */
+int llgmi_dev_write_ovf2(llgmi_ctx *ctx, const void *m_dev, const char *path,
+                         const llgmi_ovf_meta *meta, int binary4);
+/* OvfMeta::with_total_sim_time (ovf.rs:36-41): the desc line for time t_s,
+ * NUL-terminated into buf (len bytes). */
+int llgmi_ovf_total_sim_time_line(double t_s, char *buf, size_t len);
+
+/* demag_cache_path (demag_fft_uniform.rs:262-280): the reference's cache file
+ * name, relative to the working directory, into buf (len bytes). */
+int llgmi_demag_cache_path(const llgmi_grid *grid, uint64_t pbc_x, uint64_t pbc_y, char *buf,
+                           size_t len);
+/* write_kernel_kspace (demag_fft_uniform.rs:282-320): writes the context's FFT
+ * demag kernel (built first if needed) as an LLGDMAG v2 file at the reference's
+ * FFT sizes; path NULL = llgmi_demag_cache_path. */
+int llgmi_demag_cache_save(llgmi_ctx *ctx, uint64_t pbc_x, uint64_t pbc_y, const char *path);
+/* try_load_kernel_kspace (demag_fft_uniform.rs:322-429): builds the context's
+ * FFT demag operator from an LLGDMAG v2 file (no quadrature).  *loaded = 0 and
+ * nothing changes when the file is missing or its header does not match
+ * (grid, pbc, sizes, accuracy), as in the reference.  With LLGMI_DEMAG_CACHE=1
+ * in the environment every operator build loads and saves the reference's
+ * cache file itself, as Demag2D::new does. */
+int llgmi_demag_cache_load(llgmi_ctx *ctx, uint64_t pbc_x, uint64_t pbc_y, const char *path,
+                           int *loaded);
+
 /* ---- per-kernel timing (HIP events on the context stream) ---------------------- */
 /* Kernel classes timed when profiling is on. */
 #define LLGMI_K_FFT_ROW_FWD 0   /* row R2C of Ms*m (zero padded) */

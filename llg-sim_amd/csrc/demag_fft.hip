@@ -654,7 +654,8 @@ static int forced_radix_log2() {
 
 template <typename T>
 void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double dz_,
-                        hipStream_t stream, int64_t pbc_x_, int64_t pbc_y_) {
+                        hipStream_t stream, int64_t pbc_x_, int64_t pbc_y_,
+                        const std::vector<double> *base_in) {
     const int forced = forced_radix_log2();
     release();
     nx = nx_; ny = ny_; dx = dx_; dy = dy_; dz = dz_;
@@ -683,9 +684,13 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         //    image range, then folded onto the bins
         const int64_t nb = nx * ny;
         LLGMI_HIP(hipMalloc(&base, sizeof(double) * 4 * nb));
-        const bool periodic = pbc_x > 0 || pbc_y > 0;
+        const bool periodic = (pbc_x > 0 || pbc_y > 0) && !base_in;
+        if (base_in) {
+            if ((int64_t)base_in->size() != 4 * nb) throw Error(1, "kernel cache: base quadrant of the wrong size");
+            LLGMI_HIP(hipMemcpyAsync(base, base_in->data(), sizeof(double) * 4 * nb, hipMemcpyHostToDevice, stream));
+        }
         if (periodic) LLGMI_HIP(hipMalloc(&ext, sizeof(double) * 4 * ex * ey));
-        {
+        if (!base_in) {
             int nt = 256;
             int64_t nq = periodic ? ex * ey : nb;
             int64_t nbk = (3 * nq + nt - 1) / nt;
@@ -698,6 +703,8 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
                                (int)ex, (int)ey, (int)nx, (int)ny, (int)pbc_x, (int)pbc_y, base);
             LLGMI_LAUNCH_CHECK();
         }
+        base_host.resize((size_t)(4 * nb));
+        LLGMI_HIP(hipMemcpyAsync(base_host.data(), base, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, stream));
         // 2) reflect into padded real arrays
         const int64_t np = px * py;
         LLGMI_HIP(hipMalloc(&kr, sizeof(double) * 4 * np));

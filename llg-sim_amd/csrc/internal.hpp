@@ -92,14 +92,18 @@ struct DemagFFT {
     cpx<T> *spec = nullptr;    // [3][hx][ny]
     int rq_row = 4, rq_col = 4, rq_row_inv = 4;  // log2 first/last radix, picked by autotune()
     int rq_fused = 0;   // inverse row pass fused with the local field: log2 radix, -1 = unfused, 0 = untuned
+    // real-space base quadrant of the kernel, [(sy*nx + sx)*4 + {xx, xy, yy, zz}]
+    // (periodic axes: folded bins), kept on the host for the kernel cache
+    std::vector<double> base_host;
 
     ~DemagFFT();
     void release();
     // pbc_x / pbc_y > 0: periodic axis with MuMax-style image sums; the
     // convolution stays zero padded (a periodic kernel folded onto the
     // displacements -(n-1)..n-1 gives the circular convolution of period n).
+    // base_in: a base quadrant from the kernel cache instead of the quadrature.
     void build(int64_t nx, int64_t ny, double dx, double dy, double dz, hipStream_t stream,
-               int64_t pbc_x = 0, int64_t pbc_y = 0);
+               int64_t pbc_x = 0, int64_t pbc_y = 0, const std::vector<double> *base_in = nullptr);
     // bd = B_demag (overwrite) for M = ms * m; m and bd are SoA with 3*nx*ny entries.
     void field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *prof);
     // Times the radix-8 and radix-16 variants of each pass on this grid and keeps the faster.

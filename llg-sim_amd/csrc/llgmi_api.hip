@@ -3,10 +3,12 @@
 #include "../../include/llgmi.h"
 #include "common.hpp"
 #include "internal.hpp"
+#include "formats.hpp"
 #include "mg.hpp"
 
 #include <algorithm>
 #include <cctype>
+#include <cstdio>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -122,8 +124,39 @@ struct Engine : EngineBase {
         alloc(d_aos, 3 * n);
     }
 
+    // LLGMI_DEMAG_CACHE=1: load / save the reference's kernel cache file around
+    // each operator build (Demag2D::new, demag_fft_uniform.rs:497-598); write
+    // failures only warn, as there.
+    static bool auto_cache() {
+        const char *e = std::getenv("LLGMI_DEMAG_CACHE");
+        return e && std::string(e) == "1";
+    }
+    void build_fft(DemagFFT<T> &op, int64_t pbc_x, int64_t pbc_y) {
+        std::vector<double> base;
+        const std::string path = kernel_cache_path(g, pbc_x, pbc_y);
+        const bool cached = auto_cache() && read_kernel_cache(path, g, pbc_x, pbc_y, base);
+        op.build(g.nx, g.ny, g.dx, g.dy, g.dz, stream, pbc_x, pbc_y, cached ? &base : nullptr);
+        if (auto_cache() && !cached) {
+            try {
+                write_kernel_cache(path, g, pbc_x, pbc_y, op.base_host);
+            } catch (const std::exception &e) {
+                std::fprintf(stderr, "[llgmi] WARN: demag kernel cache not written (%s)\n", e.what());
+            }
+        }
+    }
     void ensure_fft() {
-        if (!fft.built) fft.build(g.nx, g.ny, g.dx, g.dy, g.dz, stream);
+        if (!fft.built) build_fft(fft, 0, 0);
+    }
+    // the operator for (pbc_x, pbc_y), built if needed
+    DemagFFT<T> &fft_for(int64_t pbc_x, int64_t pbc_y) {
+        if (pbc_x == 0 && pbc_y == 0) {
+            ensure_fft();
+            return fft;
+        }
+        if (pbc_x > (1 << 20) || pbc_y > (1 << 20))
+            throw Error(LLGMI_ERR_UNSUPPORTED, "periodic image count beyond 2^20");
+        if (!fft_pbc.built || fft_pbc.pbc_x != pbc_x || fft_pbc.pbc_y != pbc_y) build_fft(fft_pbc, pbc_x, pbc_y);
+        return fft_pbc;
     }
 
     // add_demag_field_poisson_mg (demag_poisson_mg.rs:3612-3632): configuration
@@ -144,12 +177,9 @@ struct Engine : EngineBase {
         alloc(d_bd, 3 * n);
         // demag.rs:109-141: MG and DST fall back to FFT with periodic axes
         if (pbc_x > 0 || pbc_y > 0) {
-            if (!fft_pbc.built || fft_pbc.pbc_x != (int64_t)pbc_x || fft_pbc.pbc_y != (int64_t)pbc_y) {
-                if (pbc_x > (uint64_t)(1 << 20) || pbc_y > (uint64_t)(1 << 20))
-                    throw Error(LLGMI_ERR_UNSUPPORTED, "periodic image count beyond 2^20");
-                fft_pbc.build(g.nx, g.ny, g.dx, g.dy, g.dz, stream, (int64_t)pbc_x, (int64_t)pbc_y);
-            }
-            fft_pbc.field(m, T(mat.ms), d_bd, stream, prof);
+            if (pbc_x > (uint64_t)(1 << 20) || pbc_y > (uint64_t)(1 << 20))
+                throw Error(LLGMI_ERR_UNSUPPORTED, "periodic image count beyond 2^20");
+            fft_for((int64_t)pbc_x, (int64_t)pbc_y).field(m, T(mat.ms), d_bd, stream, prof);
         } else if (method == LLGMI_DEMAG_FFT) {
             ensure_fft();
             fft.field(m, T(mat.ms), d_bd, stream, prof);
@@ -1215,6 +1245,109 @@ int llgmi_dev_relax_with_report(llgmi_ctx *ctx, void *m_dev, llgmi_params *param
         with_engine(ctx, [&](auto &eng) {
             using T = std::remove_pointer_t<decltype(eng.d_m)>;
             eng.relax((T *)m_dev, *params, *mat, field_mask, *settings, *report);
+        });
+    });
+}
+
+// ---- on-disk formats --------------------------------------------------------------
+namespace {
+std::string check_path(const char *path) {
+    if (!path || !*path) return "path is NULL or empty";
+    return "";
+}
+int copy_string(const std::string &s, char *buf, size_t len) {
+    if (!buf) return fail(LLGMI_ERR_INVALID, "buffer is NULL");
+    if (s.size() + 1 > len) return fail(LLGMI_ERR_INVALID, "buffer too small (" + std::to_string(s.size() + 1) + " bytes needed)");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return LLGMI_OK;
+}
+extern "C++" template <typename F>
+int guarded_host(F &&f) {
+    try {
+        f();
+        return LLGMI_OK;
+    } catch (const Error &e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(LLGMI_ERR_ALLOC, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(LLGMI_ERR_INVALID, e.what());
+    }
+}
+}  // namespace
+
+int llgmi_write_ovf2_text(const char *path, const llgmi_grid *grid, const double *m, const llgmi_ovf_meta *meta) {
+    std::string e = check_path(path);
+    if (e.empty()) e = check_grid(grid);
+    if (e.empty() && !m) e = "m is NULL";
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded_host([&] { write_ovf2_text(path, *grid, m, meta); });
+}
+
+int llgmi_write_ovf2_binary4(const char *path, const llgmi_grid *grid, const double *m,
+                             const llgmi_ovf_meta *meta) {
+    std::string e = check_path(path);
+    if (e.empty()) e = check_grid(grid);
+    if (e.empty() && !m) e = "m is NULL";
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded_host([&] { write_ovf2_binary4(path, *grid, m, meta); });
+}
+
+int llgmi_dev_write_ovf2(llgmi_ctx *ctx, const void *m_dev, const char *path, const llgmi_ovf_meta *meta,
+                         int binary4) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    std::string e = check_path(path);
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    if (e.empty()) e = check_dev(m_dev, 3 * n * elem_size(ctx), "m");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.ensure_io();
+            std::vector<double> h((size_t)(3 * eng.n));
+            eng.download((const T *)m_dev, h.data());
+            if (binary4) write_ovf2_binary4(path, ctx->grid, h.data(), meta);
+            else write_ovf2_text(path, ctx->grid, h.data(), meta);
+        });
+    });
+}
+
+int llgmi_ovf_total_sim_time_line(double t_s, char *buf, size_t len) {
+    return copy_string(ovf_total_sim_time_line(t_s), buf, len);
+}
+
+int llgmi_demag_cache_path(const llgmi_grid *grid, uint64_t pbc_x, uint64_t pbc_y, char *buf, size_t len) {
+    std::string e = check_grid(grid);
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return copy_string(kernel_cache_path(*grid, (int64_t)pbc_x, (int64_t)pbc_y), buf, len);
+}
+
+int llgmi_demag_cache_save(llgmi_ctx *ctx, uint64_t pbc_x, uint64_t pbc_y, const char *path) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            auto &op = eng.fft_for((int64_t)pbc_x, (int64_t)pbc_y);
+            const std::string p = path ? std::string(path) : kernel_cache_path(ctx->grid, (int64_t)pbc_x, (int64_t)pbc_y);
+            write_kernel_cache(p, ctx->grid, (int64_t)pbc_x, (int64_t)pbc_y, op.base_host);
+        });
+    });
+}
+
+int llgmi_demag_cache_load(llgmi_ctx *ctx, uint64_t pbc_x, uint64_t pbc_y, const char *path, int *loaded) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!loaded) return fail(LLGMI_ERR_INVALID, "loaded is NULL");
+    if (pbc_x > (uint64_t)(1 << 20) || pbc_y > (uint64_t)(1 << 20))
+        return fail(LLGMI_ERR_UNSUPPORTED, "periodic image count beyond 2^20");
+    *loaded = 0;
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            const std::string p = path ? std::string(path) : kernel_cache_path(ctx->grid, (int64_t)pbc_x, (int64_t)pbc_y);
+            std::vector<double> base;
+            if (!read_kernel_cache(p, ctx->grid, (int64_t)pbc_x, (int64_t)pbc_y, base)) return;
+            auto &op = (pbc_x == 0 && pbc_y == 0) ? eng.fft : eng.fft_pbc;
+            op.build(ctx->grid.nx, ctx->grid.ny, ctx->grid.dx, ctx->grid.dy, ctx->grid.dz, eng.stream,
+                     (int64_t)pbc_x, (int64_t)pbc_y, &base);
+            *loaded = 1;
         });
     });
 }

@@ -11,7 +11,8 @@
 //   llg::RK23Scratch / step_llg_rk23_recompute_field_masked_relax_adaptive
 //   (llg.rs:196-436), energy::compute_energy[_geom] / compute_total_energy
 //   (energy.rs:21-305), relax::RelaxSettings / relax_with_report / relax /
-//   mumax_like_relax_settings (relax.rs:32-808).
+//   mumax_like_relax_settings (relax.rs:32-808), ovf::OvfMeta /
+//   write_ovf2_rectangular_{text,binary4} / OvfSeries (ovf.rs:17-316).
 //
 // Field builders use a process-wide cache of GPU contexts keyed by grid and
 // precision (the reference keys its demag operator cache the same way,
@@ -32,6 +33,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 namespace llg_sim {
@@ -413,4 +415,95 @@ inline void relax(const Grid2D &grid, VectorField2D &m, LLGParams &params, const
 }
 
 }  // namespace relax
+
+// ovf.rs:17-316
+namespace ovf {
+
+struct OvfMeta {
+    std::string title;
+    std::vector<std::string> desc_lines;
+    std::array<std::string, 3> valuelabels, valueunits;
+
+    static OvfMeta magnetization() {
+        OvfMeta m;
+        m.title = "m";
+        m.valuelabels = {"m_x", "m_y", "m_z"};
+        m.valueunits = {"1", "1", "1"};
+        return m;
+    }
+    OvfMeta with_total_sim_time(double t_s) && {
+        char buf[128];
+        check(llgmi_ovf_total_sim_time_line(t_s, buf, sizeof buf), "with_total_sim_time");
+        desc_lines.emplace_back(buf);
+        return std::move(*this);
+    }
+    void push_desc_line(std::string s) { desc_lines.push_back(std::move(s)); }
+};
+
+namespace detail {
+template <typename F>
+inline void with_c_meta(const OvfMeta &meta, F &&f) {
+    std::vector<const char *> lines;
+    for (auto &d : meta.desc_lines) lines.push_back(d.c_str());
+    llgmi_ovf_meta c{};
+    c.title = meta.title.c_str();
+    c.desc_lines = lines.empty() ? nullptr : lines.data();
+    c.n_desc = (int32_t)lines.size();
+    for (int q = 0; q < 3; ++q) {
+        c.valuelabels[q] = meta.valuelabels[q].c_str();
+        c.valueunits[q] = meta.valueunits[q].c_str();
+    }
+    f(c);
+}
+}  // namespace detail
+
+inline void write_ovf2_rectangular_text(const std::string &path, const Grid2D &grid, const VectorField2D &m,
+                                        const OvfMeta &meta) {
+    llgmi_grid g{(int64_t)grid.nx, (int64_t)grid.ny, grid.dx, grid.dy, grid.dz};
+    detail::with_c_meta(meta, [&](const llgmi_ovf_meta &c) {
+        check(llgmi_write_ovf2_text(path.c_str(), &g, m.ptr(), &c), "write_ovf2_rectangular_text");
+    });
+}
+
+inline void write_ovf2_rectangular_binary4(const std::string &path, const Grid2D &grid, const VectorField2D &m,
+                                           const OvfMeta &meta) {
+    llgmi_grid g{(int64_t)grid.nx, (int64_t)grid.ny, grid.dx, grid.dy, grid.dz};
+    detail::with_c_meta(meta, [&](const llgmi_ovf_meta &c) {
+        check(llgmi_write_ovf2_binary4(path.c_str(), &g, m.ptr(), &c), "write_ovf2_rectangular_binary4");
+    });
+}
+
+// OvfSeries (ovf.rs:264-316): binary4 dumps every `every` seconds.
+class OvfSeries {
+public:
+    std::string dir;
+    double every;
+    OvfSeries(std::string dir_, double every_s) : dir(std::move(dir_)), every(every_s > 0 ? every_s : 0.0) {}
+    template <typename MakePath>
+    void dump_initial(const Grid2D &grid, const VectorField2D &m, MakePath make_path, const OvfMeta &meta) {
+        next_dump_ = 0.0;
+        write_ovf2_rectangular_binary4(make_path(frame_, 0.0), grid, m, meta);
+        ++frame_;
+        next_dump_ = every;
+    }
+    template <typename MakePath, typename MetaBuilder>
+    void maybe_dump_due(const Grid2D &grid, const VectorField2D &m, double t_approx, MakePath make_path,
+                        MetaBuilder meta_builder) {
+        if (every <= 0.0) return;
+        const double eps = 1e-15;
+        while (t_approx + eps >= next_dump_) {
+            const double t = next_dump_;
+            write_ovf2_rectangular_binary4(make_path(frame_, t), grid, m, meta_builder(t));
+            ++frame_;
+            next_dump_ += every;
+        }
+    }
+    uint64_t frame() const { return frame_; }
+
+private:
+    double next_dump_ = 0.0;
+    uint64_t frame_ = 0;
+};
+
+}  // namespace ovf
 }  // namespace llg_sim

@@ -149,7 +149,9 @@ EXPORTED = [
     "llgmi_relax_settings_default", "llgmi_relax_settings_mumax_like", "llgmi_rk23_relax_step",
     "llgmi_rk23_invalidate", "llgmi_rk23_last_b_eff", "llgmi_compute_energy_geom",
     "llgmi_relax_with_report", "llgmi_dev_rk23_relax_step", "llgmi_dev_compute_energy_geom",
-    "llgmi_dev_relax_with_report",
+    "llgmi_dev_relax_with_report", "llgmi_write_ovf2_text", "llgmi_write_ovf2_binary4",
+    "llgmi_dev_write_ovf2", "llgmi_ovf_total_sim_time_line", "llgmi_demag_cache_path",
+    "llgmi_demag_cache_save", "llgmi_demag_cache_load",
 ]
 
 
@@ -282,6 +284,63 @@ def mumax_like_relax_settings(**kw) -> RelaxSettings:
     return _settings(lib().llgmi_relax_settings_mumax_like, kw)
 
 
+class OvfMeta(C.Structure):
+    """llgmi_ovf_meta: OvfMeta (ovf.rs:17-46); None fields take magnetization()."""
+    _fields_ = [("title", C.c_char_p), ("desc_lines", C.POINTER(C.c_char_p)), ("n_desc", C.c_int32),
+                ("reserved", C.c_int32), ("valuelabels", C.c_char_p * 3), ("valueunits", C.c_char_p * 3)]
+
+
+def ovf_meta(title=None, desc_lines=(), valuelabels=None, valueunits=None) -> OvfMeta:
+    m = OvfMeta()
+    m.title = None if title is None else title.encode()
+    lines = [d.encode() for d in desc_lines]
+    arr = (C.c_char_p * max(len(lines), 1))(*lines)
+    m._keep = arr                                    # keeps the strings alive with the struct
+    m.desc_lines = C.cast(arr, C.POINTER(C.c_char_p))
+    m.n_desc = len(lines)
+    for q in range(3):
+        m.valuelabels[q] = None if valuelabels is None else valuelabels[q].encode()
+        m.valueunits[q] = None if valueunits is None else valueunits[q].encode()
+    return m
+
+
+def ovf_total_sim_time_line(t_s: float) -> str:
+    """OvfMeta::with_total_sim_time's desc line (ovf.rs:36-41)."""
+    buf = C.create_string_buffer(128)
+    _check(lib().llgmi_ovf_total_sim_time_line(t_s, buf, 128))
+    return buf.value.decode()
+
+
+def _aos(grid: "Grid2D", m) -> np.ndarray:
+    a = _f64(m).reshape(-1, 3)
+    assert a.shape[0] == grid.n_cells()
+    return a
+
+
+def write_ovf2_rectangular_text(path: str, grid: "Grid2D", m, meta: Optional[OvfMeta] = None) -> None:
+    """ovf.rs:54-159."""
+    a = _aos(grid, m)
+    g = grid._c()
+    _check(lib().llgmi_write_ovf2_text(path.encode(), C.byref(g), _dp(a),
+                                       None if meta is None else C.byref(meta)))
+
+
+def write_ovf2_rectangular_binary4(path: str, grid: "Grid2D", m, meta: Optional[OvfMeta] = None) -> None:
+    """ovf.rs:161-262."""
+    a = _aos(grid, m)
+    g = grid._c()
+    _check(lib().llgmi_write_ovf2_binary4(path.encode(), C.byref(g), _dp(a),
+                                          None if meta is None else C.byref(meta)))
+
+
+def demag_cache_path(grid: "Grid2D", pbc_x: int = 0, pbc_y: int = 0) -> str:
+    """demag_cache_path (demag_fft_uniform.rs:262-280)."""
+    buf = C.create_string_buffer(512)
+    g = grid._c()
+    _check(lib().llgmi_demag_cache_path(C.byref(g), pbc_x, pbc_y, buf, 512))
+    return buf.value.decode()
+
+
 def lib():
     """Loads libllgmi.so; raises if it has not been built."""
     global _lib
@@ -336,6 +395,14 @@ def lib():
     L.llgmi_dev_compute_energy_geom.argtypes = [vp, vp, M, dp, vp, C.POINTER(Energy)]
     L.llgmi_relax_with_report.argtypes = [vp, dp, P, M, C.c_int, RS, RR]
     L.llgmi_dev_relax_with_report.argtypes = [vp, vp, P, M, C.c_int, RS, RR]
+    OM = C.POINTER(OvfMeta)
+    L.llgmi_write_ovf2_text.argtypes = [C.c_char_p, G, dp, OM]
+    L.llgmi_write_ovf2_binary4.argtypes = [C.c_char_p, G, dp, OM]
+    L.llgmi_dev_write_ovf2.argtypes = [vp, vp, C.c_char_p, OM, C.c_int]
+    L.llgmi_ovf_total_sim_time_line.argtypes = [C.c_double, C.c_char_p, C.c_size_t]
+    L.llgmi_demag_cache_path.argtypes = [G, C.c_uint64, C.c_uint64, C.c_char_p, C.c_size_t]
+    L.llgmi_demag_cache_save.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_char_p]
+    L.llgmi_demag_cache_load.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_char_p, ip]
     _lib = L
     return L
 
@@ -565,6 +632,22 @@ class Context:
                                                  mask, C.byref(settings), C.byref(rep)))
         params.dt = p.dt
         return rep.as_dict()
+
+    # ---- on-disk formats ----
+    def demag_cache_save(self, pbc_x: int = 0, pbc_y: int = 0, path: Optional[str] = None) -> None:
+        """write_kernel_kspace (demag_fft_uniform.rs:282-320) of this context's operator."""
+        _check(lib().llgmi_demag_cache_save(self._h, pbc_x, pbc_y, None if path is None else path.encode()))
+
+    def demag_cache_load(self, pbc_x: int = 0, pbc_y: int = 0, path: Optional[str] = None) -> bool:
+        """try_load_kernel_kspace (demag_fft_uniform.rs:322-429): True when loaded."""
+        ok = C.c_int()
+        _check(lib().llgmi_demag_cache_load(self._h, pbc_x, pbc_y, None if path is None else path.encode(),
+                                            C.byref(ok)))
+        return bool(ok.value)
+
+    def dev_write_ovf2(self, m_dev, path: str, meta: Optional[OvfMeta] = None, binary4: bool = True) -> None:
+        _check(lib().llgmi_dev_write_ovf2(self._h, _ptr(m_dev), path.encode(),
+                                          None if meta is None else C.byref(meta), int(binary4)))
 
     # ---- device (SoA, context precision) ----
     def dev_build_h_eff(self, m_dev, b_dev, params: LLGParams, mat: Material, mask: int,

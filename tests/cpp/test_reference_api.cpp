@@ -225,6 +225,27 @@ static void macrospin_relax_controller_reaches_easy_axis() {
     CHECK(!scratch.last_b_eff().has_value(), "invalidated");
 }
 
+// ovf.rs:264-316: OvfSeries dumps at t = 0, every, 2 every, ... as time passes.
+static void ovf_series_dumps_on_schedule() {
+    Grid2D grid(4, 3, 1e-9, 1e-9, 1e-9);
+    VectorField2D m(grid);
+    const std::string dir = "/tmp/llgmi_cpp_ovf";
+    ovf::OvfSeries series(dir, 1e-12);
+    std::vector<std::string> written;
+    auto path = [&](uint64_t frame, double) {
+        std::string p = dir + "/m" + std::to_string(frame) + ".ovf";
+        written.push_back(p);
+        return p;
+    };
+    series.dump_initial(grid, m, path, ovf::OvfMeta::magnetization());
+    series.maybe_dump_due(grid, m, 2.5e-12, path, [](double t) { return ovf::OvfMeta::magnetization().with_total_sim_time(t); });
+    CHECK(written.size() == 3, "frames=%zu", written.size());
+    CHECK(series.frame() == 3, "frame=%llu", (unsigned long long)series.frame());
+    std::FILE *f = std::fopen(written.back().c_str(), "rb");
+    CHECK(f != nullptr, "missing %s", written.back().c_str());
+    if (f) std::fclose(f);
+}
+
 int main() {
     const std::pair<const char *, std::function<void()>> tests[] = {
         {"demag_single_cell_cube_is_close_to_minus_mu0_over_3", demag_single_cell_cube_is_close_to_minus_mu0_over_3},
@@ -236,6 +257,7 @@ int main() {
         {"dispatcher_fft_matches_builder", dispatcher_fft_matches_builder},
         {"energy_gradient_consistency_exchange_anisotropy", energy_gradient_consistency_exchange_anisotropy},
         {"macrospin_relax_controller_reaches_easy_axis", macrospin_relax_controller_reaches_easy_axis},
+        {"ovf_series_dumps_on_schedule", ovf_series_dumps_on_schedule},
     };
     for (auto &t : tests) {
         std::printf("test %s\n", t.first);

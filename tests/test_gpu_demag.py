@@ -247,3 +247,80 @@ def test_demag_pbc_fp64_matches_oracle(gpu_lib, shape, pbc):
     b32 = np.zeros_like(m)
     c32.add_demag_field_pbc(m, b32, L.Material(ms=8e5, demag=True), *pbc)
     assert rel_err(b32, b) <= 1e-5
+
+
+def _write_reference_cache(path, nx, ny, cell, pbc, kspace):
+    """An LLGDMAG v2 file laid out as write_kernel_kspace writes it
+    (demag_fft_uniform.rs:282-320), holding the oracle's k-space kernels."""
+    import struct
+    px, py = kspace.shape[2], kspace.shape[1]
+    with open(path, "wb") as f:
+        f.write(b"LLGDMAG\0")
+        f.write(struct.pack("<7I", 2, nx, ny, pbc[0], pbc[1], px, py))
+        f.write(struct.pack("<4d", cell[0], cell[1], cell[2], 10.0))
+        for c in range(4):
+            f.write(np.ascontiguousarray(kspace[c]).astype("<c16").tobytes())
+
+
+def _read_cache(path):
+    import struct
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"LLGDMAG\0"
+    hdr = struct.unpack("<7I", raw[8:36]) + struct.unpack("<4d", raw[36:68])
+    px, py = hdr[5], hdr[6]
+    k = np.frombuffer(raw[68:], dtype="<c16")
+    assert k.size == 4 * px * py
+    return hdr, k.reshape(4, py, px)
+
+
+@pytest.mark.parametrize("shape,pbc", [((16, 8), (0, 0)), ((30, 7), (0, 0)), ((12, 10), (2, 0)), ((9, 6), (1, 2))])
+def test_kernel_cache_interchange_with_the_reference_layout(gpu_lib, tmp_path, shape, pbc):
+    """try_load_kernel_kspace / write_kernel_kspace (demag_fft_uniform.rs:282-429):
+    a cache file at the reference's FFT sizes (2n open, n periodic; not powers of
+    two here) written from the oracle's k-space kernels loads into a context and
+    gives the oracle's field; the context's own file holds the oracle's k-space
+    kernels."""
+    from oracle import oracle as O
+    L = gpu_lib
+    nx, ny = shape
+    cell = (4e-9, 3e-9, 2e-9)
+    print(f"cache {shape} {pbc}", flush=True)
+    od = O.Demag(O.Grid(nx, ny, *cell), *pbc)
+    ks = od.kspace()
+    src = str(tmp_path / "ref.bin")
+    _write_reference_cache(src, nx, ny, cell, pbc, ks)
+    m = _rand_m(nx * ny, 5 * nx + ny)
+    b_ref = np.zeros_like(m)
+    od.add_field(m, b_ref, 8e5)
+    ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    assert ctx.demag_cache_load(*pbc, path=src)
+    b = np.zeros_like(m)
+    ctx.add_demag_field_pbc(m, b, L.Material(ms=8e5, demag=True), *pbc)
+    assert rel_err(b, b_ref) <= 1e-12
+    # a fresh context writes the same kernels
+    out = str(tmp_path / "sub" / "mine.bin")
+    fresh = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F32)
+    fresh.demag_cache_save(*pbc, path=out)
+    hdr, k = _read_cache(out)
+    assert hdr[:7] == (2, nx, ny, pbc[0], pbc[1], ks.shape[2], ks.shape[1])
+    assert hdr[7:] == (cell[0], cell[1], cell[2], 10.0)
+    assert np.abs(k - ks).max() <= 1e-12 * np.abs(ks).max()
+    # a header for another grid is not loaded (nothing changes)
+    other = L.Context(L.Grid2D(nx + 1, ny, *cell), precision=L.F64)
+    assert not other.demag_cache_load(*pbc, path=src)
+    assert not other.demag_cache_load(*pbc, path=str(tmp_path / "absent.bin"))
+
+
+def test_kernel_cache_automatic_with_env(gpu_lib, tmp_path, monkeypatch):
+    """LLGMI_DEMAG_CACHE=1: the operator build reads and writes the reference's
+    cache file (out/demag_cache/... in the working directory)."""
+    L = gpu_lib
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("LLGMI_DEMAG_CACHE", "1")
+    g = L.Grid2D(20, 12, 4e-9, 4e-9, 3e-9)
+    m = _rand_m(240, 77)
+    a = L.Context(g, precision=L.F64).compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    path = tmp_path / L.demag_cache_path(g)
+    assert path.exists()
+    b = L.Context(g, precision=L.F64).compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(b, a) <= 1e-13
