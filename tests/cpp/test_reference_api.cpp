@@ -3,6 +3,7 @@
 // Exit status 0 when every test passes.
 #include "../../llg-sim_amd/host/llg_sim.hpp"
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <functional>
@@ -262,6 +263,7 @@ int main() {
     for (auto &t : tests) {
         std::printf("test %s\n", t.first);
         std::fflush(stdout);
+        const auto t0 = std::chrono::steady_clock::now();
         int before = g_fail;
         try {
             t.second();
@@ -269,7 +271,8 @@ int main() {
             std::printf("  FAIL exception: %s\n", e.what());
             ++g_fail;
         }
-        std::printf("  %s\n", g_fail == before ? "ok" : "FAILED");
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("  %s (%.2f s)\n", g_fail == before ? "ok" : "FAILED", sec);
     }
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "PASSED", g_fail);
     return g_fail ? 1 : 0;
