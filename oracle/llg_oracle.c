@@ -18,6 +18,7 @@
  */
 #include "llg_oracle.h"
 #include "mg_oracle.h"
+#include "hybrid_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -676,6 +677,11 @@ struct orc_ctx {
     orc_demag *demag_pbc;   /* the last periodic operator (keyed by pbc_x, pbc_y) */
     orc_mg *mg;
     orc_mg_cfg mg_cfg;
+    orc_hyb_cfg hyb;        /* hybrid MG correction (disabled by default) */
+    double *dk;             /* cached delta kernel, built for dk_cfg / dk_hyb / dk_r */
+    orc_mg_cfg dk_cfg;
+    orc_hyb_cfg dk_hyb;
+    int64_t dk_r;
     double *ms_[8];      /* m2..m7 at [2..7] */
     double *bs[8];       /* b1..b7 */
     double *ks[8];       /* k1..k7 */
@@ -696,6 +702,8 @@ orc_ctx *orc_ctx_new(const orc_grid *g) {
     }
     c->last_mask = -1;
     orc_mg_cfg_default(&c->mg_cfg);
+    orc_hyb_cfg_default(&c->hyb);
+    c->dk_r = -1;
     return c;
 }
 
@@ -704,6 +712,7 @@ void orc_ctx_free(orc_ctx *c) {
     orc_demag_free(c->demag);
     orc_demag_free(c->demag_pbc);
     orc_mg_free(c->mg);
+    free(c->dk);
     for (int s = 1; s <= 7; ++s) { free(c->bs[s]); free(c->ks[s]); free(c->ms_[s]); }
     free(c);
 }
@@ -724,7 +733,29 @@ void orc_add_demag(orc_ctx *c, const double *m, double *b, const orc_material *m
         }
         if (!c->mg) c->mg = orc_mg_new(&c->g, &c->mg_cfg);
         orc_mg_apply_cfg(c->mg, &c->mg_cfg);
-        orc_mg_add_field(c->mg, m, b, mat->ms);
+        if (!orc_hyb_enabled(&c->hyb)) {
+            orc_mg_add_field(c->mg, m, b, mat->ms);
+            return;
+        }
+        /* DemagPoissonMGHybrid::add_field with ensure_delta_kernel (:2412-2466):
+           the delta kernel is rebuilt when its key (MG settings, hybrid
+           settings, effective radius) changes */
+        const int64_t r = orc_hyb_effective_radius(&c->g, &c->hyb);
+        if (r > 0) {
+            orc_hyb_cfg hk = c->hyb;
+            hk.bscale = 0.0;
+            hk.bmargin = 0;
+            if (!c->dk || c->dk_r != r || memcmp(&c->dk_cfg, &c->mg_cfg, sizeof c->mg_cfg) != 0 ||
+                memcmp(&c->dk_hyb, &hk, sizeof hk) != 0) {
+                free(c->dk);
+                c->dk = (double *)malloc(sizeof(double) * 4 * (size_t)((2 * r + 1) * (2 * r + 1)));
+                orc_hyb_delta_kernel(&c->g, &c->mg_cfg, &c->hyb, r, mat->ms, c->dk);
+                c->dk_cfg = c->mg_cfg;
+                c->dk_hyb = hk;
+                c->dk_r = r;
+            }
+        }
+        orc_hyb_add_field(c->mg, &c->hyb, r, r > 0 ? c->dk : NULL, m, b, mat->ms, &c->g);
         return;
     }
     if (!c->demag) c->demag = orc_demag_new(&c->g);
@@ -877,3 +908,5 @@ void orc_rk45_step(orc_ctx *c, double *m, orc_params *p, const orc_material *mat
     *accepted_out = accept;
     *dt_used_out = dt0;
 }
+
+void orc_ctx_set_hyb_cfg(orc_ctx *c, const orc_hyb_cfg *h) { c->hyb = *h; }

@@ -1124,7 +1124,11 @@ void orc_mg_solve(orc_mg *s) {
 
 /* add_b_from_phi_on_magnet_layer -> extract_gradient_on_magnet_layer
  * (demag_poisson_mg.rs:2063-2092, mg_kernels.rs:450-507) */
-void orc_mg_add_b(const orc_mg *s, const double *m, double *b) {
+void orc_mg_add_b(const orc_mg *s, const double *m, double *b) { orc_mg_add_b_opt(s, m, b); }
+
+/* extract_gradient_on_magnet_layer with an optional magnet mask (m NULL: every
+ * film cell, add_b_from_phi_on_magnet_layer_all, demag_poisson_mg.rs:2072-2074) */
+void orc_mg_add_b_opt(const orc_mg *s, const double *m, double *b) {
     const mg_level *L = &s->lev[0];
     const double *phi = L->phi;
     const int64_t px = s->px, py = s->py, pz = s->pz, k = s->offz;
@@ -1133,8 +1137,10 @@ void orc_mg_add_b(const orc_mg *s, const double *m, double *b) {
     for (int64_t j = 0; j < s->g.ny; ++j) {
         int64_t pj = s->offy + j;
         for (int64_t i = 0; i < s->g.nx; ++i) {
-            const double *mv = m + 3 * (j * s->g.nx + i);
-            if (!(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2] > 1e-30)) continue;
+            if (m) {
+                const double *mv = m + 3 * (j * s->g.nx + i);
+                if (!(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2] > 1e-30)) continue;
+            }
             int64_t pi = s->offx + i;
             double pc = phi[IDX3(pi, pj, k, px, py)];
             double gx = 0.0, wx = 0.0;
@@ -1162,4 +1168,74 @@ void orc_mg_add_field(orc_mg *s, const double *m, double *b, double ms) {
     orc_mg_build_rhs(s, m, ms);
     orc_mg_solve(s);
     orc_mg_add_b(s, m, b);
+}
+
+/* gaussian_kernel_1d (mg_kernels.rs:580-599); returns r, fills w[2r+1] */
+int64_t orc_gaussian_kernel_1d(double sigma, double *w, int64_t cap) {
+    if (!(sigma > 0.0)) { if (cap > 0) w[0] = 1.0; return 0; }
+    int64_t r = (int64_t)ceil(3.0 * sigma);
+    if (r <= 0) { if (cap > 0) w[0] = 1.0; return 0; }
+    if (2 * r + 1 > cap) return -1;
+    double inv2s2 = 1.0 / (2.0 * sigma * sigma);
+    for (int64_t i = -r; i <= r; ++i) {
+        double x = (double)i;
+        w[i + r] = exp(-x * x * inv2s2);
+    }
+    double sum = 0.0;
+    for (int64_t i = 0; i < 2 * r + 1; ++i) sum += w[i];
+    if (sum > 0.0)
+        for (int64_t i = 0; i < 2 * r + 1; ++i) w[i] /= sum;
+    return r;
+}
+
+/* screen_rhs_gaussian_xy (mg_kernels.rs:604-680) on the finest level's rhs,
+ * its tmp array as scratch (demag_poisson_mg.rs:2714-2720) */
+void orc_mg_screen_rhs(orc_mg *s, double sigma) {
+    if (!(sigma > 0.0)) return;
+    double w[1024];
+    int64_t r = orc_gaussian_kernel_1d(sigma, w, 1024);
+    if (r <= 0) return;
+    mg_level *L = &s->lev[0];
+    const int64_t nx = L->nx, ny = L->ny, nz = L->nz;
+    if (nx < 3 || ny < 3 || nz < 3) return;
+    double *rhs = L->rhs, *tmp = L->tmp;
+    for (int64_t row = 0; row < ny * nz; ++row) {
+        int64_t k = row / ny, j = row - k * ny;
+        double *t = tmp + row * nx;
+        if (k == 0 || k + 1 == nz || j == 0 || j + 1 == ny) {
+            for (int64_t i = 0; i < nx; ++i) t[i] = 0.0;
+            continue;
+        }
+        const double *src = rhs + row * nx;
+        t[0] = 0.0;
+        t[nx - 1] = 0.0;
+        for (int64_t i = 1; i < nx - 1; ++i) {
+            double acc = 0.0;
+            for (int64_t q = -r; q <= r; ++q) {
+                int64_t x = i + q;
+                if (x < 0) x = 0; else if (x > nx - 1) x = nx - 1;
+                acc += w[q + r] * src[x];
+            }
+            t[i] = acc;
+        }
+    }
+    for (int64_t row = 0; row < ny * nz; ++row) {
+        int64_t k = row / ny, j = row - k * ny;
+        double *o = rhs + row * nx;
+        if (k == 0 || k + 1 == nz || j == 0 || j + 1 == ny) {
+            for (int64_t i = 0; i < nx; ++i) o[i] = 0.0;
+            continue;
+        }
+        o[0] = 0.0;
+        o[nx - 1] = 0.0;
+        for (int64_t i = 1; i < nx - 1; ++i) {
+            double acc = 0.0;
+            for (int64_t q = -r; q <= r; ++q) {
+                int64_t y = j + q;
+                if (y < 0) y = 0; else if (y > ny - 1) y = ny - 1;
+                acc += w[q + r] * tmp[(k * ny + y) * nx + i];
+            }
+            o[i] = acc;
+        }
+    }
 }

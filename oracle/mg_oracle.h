@@ -110,6 +110,9 @@ void orc_mg_level_stencil(const orc_mg *s, int level, orc_stencil *out);
 void orc_mg_build_rhs(orc_mg *s, const double *m, double ms);             /* :1595-1802 */
 void orc_mg_solve(orc_mg *s);                                             /* solve_with_timing :1947-2057 */
 void orc_mg_add_b(const orc_mg *s, const double *m, double *b);           /* :2063-2092, mg_kernels.rs:450-507 */
+void orc_mg_add_b_opt(const orc_mg *s, const double *m, double *b);       /* m NULL: every film cell (:2072-2074) */
+int64_t orc_gaussian_kernel_1d(double sigma, double *w, int64_t cap);     /* mg_kernels.rs:580-599 */
+void orc_mg_screen_rhs(orc_mg *s, double sigma);                          /* mg_kernels.rs:604-680 */
 void orc_mg_add_field(orc_mg *s, const double *m, double *b, double ms);  /* :2094-2098 */
 double *orc_mg_finest(orc_mg *s, int which);                              /* 0 phi, 1 rhs, 2 res, 3 bc */
 void orc_mg_stats_get(const orc_mg *s, orc_mg_stats *out);

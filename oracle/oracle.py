@@ -89,6 +89,7 @@ def lib():
         L.orc_get_threads.restype = C.c_int
         _bind_mg(L)
         _bind_relax(L)
+        _bind_hybrid(L)
         _lib = L
     return _lib
 
@@ -787,3 +788,82 @@ def relax_with_report(ctx: "Context", rk: RK23, m: np.ndarray, params: Params, m
                     C.byref(settings), C.byref(rep))
     params.dt = p.dt
     return _report_dict(rep)
+
+
+# --- hybrid MG (MG + delta-kernel correction), demag_poisson_mg.rs ---------
+class HybCfg(C.Structure):
+    """HybridConfig (demag_poisson_mg.rs:274-361) + BSCALE / BMARGIN."""
+    _fields_ = [("enabled", C.c_int32), ("reserved", C.c_int32), ("radius_xy", C.c_int64),
+                ("delta_v_cycles", C.c_int64), ("sigma_cells", C.c_double), ("bscale", C.c_double),
+                ("bmargin", C.c_int64)]
+
+
+def _bind_hybrid(L):
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    H = C.POINTER(HybCfg)
+    L.orc_hyb_cfg_default.argtypes = [H]
+    L.orc_hyb_effective_radius.argtypes = [C.POINTER(_Grid), H]
+    L.orc_hyb_effective_radius.restype = C.c_int64
+    L.orc_hyb_delta_kernel.argtypes = [C.POINTER(_Grid), C.POINTER(_MGCfg), H, C.c_int64, C.c_double, dp]
+    L.orc_hyb_add_correction.argtypes = [C.POINTER(_Grid), C.c_int64, dp, dp, dp, C.c_double, C.c_double,
+                                         C.c_int64]
+    L.orc_ctx_set_hyb_cfg.argtypes = [vp, H]
+    L.orc_gaussian_kernel_1d.argtypes = [C.c_double, dp, C.c_int64]
+    L.orc_gaussian_kernel_1d.restype = C.c_int64
+    L.orc_mg_screen_rhs.argtypes = [vp, C.c_double]
+    L.orc_mg_add_b_opt.argtypes = [vp, dp, dp]
+
+
+def hyb_config(**kw) -> HybCfg:
+    h = HybCfg()
+    lib().orc_hyb_cfg_default(C.byref(h))
+    for k, v in kw.items():
+        setattr(h, k, v)
+    return h
+
+
+def gaussian_kernel_1d(sigma: float) -> np.ndarray:
+    w = np.zeros(1024)
+    r = lib().orc_gaussian_kernel_1d(sigma, _dp(w), 1024)
+    return w[:2 * r + 1].copy()
+
+
+def hyb_effective_radius(grid: Grid, h: HybCfg) -> int:
+    g = grid.c()
+    return int(lib().orc_hyb_effective_radius(C.byref(g), C.byref(h)))
+
+
+def hyb_delta_kernel(grid: Grid, cfg, h: HybCfg, r: int, ms: float) -> np.ndarray:
+    """build_delta_kernel_impulse: (4, 2r+1, 2r+1) = xx, xy, yy, zz, [dy+r, dx+r]."""
+    dk = np.zeros((4, 2 * r + 1, 2 * r + 1))
+    g = grid.c()
+    lib().orc_hyb_delta_kernel(C.byref(g), C.byref(cfg), C.byref(h), r, ms, _dp(dk))
+    return dk
+
+
+def hyb_add_correction(grid: Grid, dk: np.ndarray, m, b: np.ndarray, ms: float, bscale=0.0, bmargin=2):
+    r = (dk.shape[1] - 1) // 2
+    dk = np.ascontiguousarray(dk, dtype=np.float64)
+    m = _f(m)
+    g = grid.c()
+    lib().orc_hyb_add_correction(C.byref(g), r, _dp(dk), _dp(m), _dp(b), ms, bscale, bmargin)
+
+
+def _mg_screen_rhs(self, sigma: float) -> None:
+    lib().orc_mg_screen_rhs(self._h, sigma)
+
+
+def _mg_add_b_all(self, b: np.ndarray) -> None:
+    lib().orc_mg_add_b_opt(self._h, None, _dp(b))
+
+
+MG.screen_rhs = _mg_screen_rhs
+MG.add_b_all = _mg_add_b_all
+
+
+def _ctx_set_hyb_config(self, h: HybCfg) -> None:
+    lib().orc_ctx_set_hyb_cfg(self._h, C.byref(h))
+
+
+Context.set_hyb_config = _ctx_set_hyb_config
