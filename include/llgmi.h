@@ -209,6 +209,30 @@ typedef struct llgmi_mg_stats {
     int64_t box[3];          /* finest padded box px, py, pz */
 } llgmi_mg_stats;
 
+/* HybridConfig (demag_poisson_mg.rs:274-361) with the boundary-scaling knobs
+ * LLG_DEMAG_MG_HYBRID_BSCALE / _BMARGIN (:2743-2760): the MG field plus a local
+ * correction stencil dK = K_FFT - K_MG of radius radius_xy built from impulse
+ * responses (:3159-3597), with the rhs screened by a Gaussian of sigma_cells.
+ * Active when enabled != 0 and radius_xy > 0.  48 bytes. */
+typedef struct llgmi_mg_hybrid_config {
+    int32_t enabled;
+    int32_t reserved;
+    int64_t radius_xy;        /* default 12 */
+    int64_t delta_v_cycles;   /* V-cycles of each impulse solve, default 60 */
+    double sigma_cells;       /* default 1.5; 0 = unscreened */
+    double bscale;            /* > 0: boundary scaling exponent; default 0 */
+    int64_t bmargin;          /* default 2 */
+} llgmi_mg_hybrid_config;
+
+/* HybridConfig::default() (disabled). */
+int llgmi_mg_hybrid_config_default(llgmi_mg_hybrid_config *cfg);
+/* HybridConfig::from_env (:296-356) + the BSCALE / BMARGIN variables. */
+int llgmi_mg_hybrid_config_from_env(llgmi_mg_hybrid_config *cfg);
+/* Pins the hybrid configuration of a context; NULL reads the environment at
+ * every MG call, as the reference does.  The correction stencil is cached in
+ * the context and rebuilt when the MG or hybrid settings change. */
+int llgmi_ctx_set_mg_hybrid_config(llgmi_ctx *ctx, const llgmi_mg_hybrid_config *cfg);
+
 /* DemagPoissonMGConfig::default() with the default operator (iso27 stencil,
  * trilinear prolongation, Galerkin coarse operators). */
 int llgmi_mg_config_default(llgmi_mg_config *cfg);

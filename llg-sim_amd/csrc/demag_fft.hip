@@ -28,6 +28,11 @@
 #include "common.hpp"
 #include "fft_reg.hpp"
 
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include <cstdlib>
 #include "internal.hpp"
 #include "local_cell.hpp"
@@ -419,6 +424,11 @@ __host__ __device__ inline double delta_cell_d(int64_t d) {
     if (a > 0.0) a -= 1.0;
     return a;
 }
+// (displacement, axis) sums longer than this go to k_quad_heavy
+static constexpr int64_t kQuadHeavy = int64_t(1) << 16;
+static constexpr int kQuadChunk = 256;
+static constexpr int kQuadHeavyScan = 256;   // heavy sums are looked for in [0, 256)^2
+
 __host__ __device__ inline int64_t mumax_n_d(double x) {
     return (int64_t)floor(fmax(x, 1.0) + 0.5);
 }
@@ -455,11 +465,34 @@ __host__ __device__ inline int64_t quad_points(double dx, double dy, double dz, 
     return t;
 }
 
-// Sum over sample points (ax, ay, az) for the pole pair at (pv, pw).
+// One sample point's contribution (hx, hy, hz) for the pole pair.
+__device__ __forceinline__ void quad_point(const QuadSetup &q, double rx, double ry, double rz,
+                                           const double pole_vw[3], double out[3]) {
+#pragma clang fp contract(off)
+    const double PI_D = 3.14159265358979323846;
+    double pole[3] = {pole_vw[0], pole_vw[1], pole_vw[2]};
+    pole[q.u] = q.pu1;
+    double r1x = rx - pole[0], r1y = ry - pole[1], r1z = rz - pole[2];
+    double r1 = sqrt(r1x * r1x + r1y * r1y + r1z * r1z);
+    double q1 = q.charge / (4.0 * PI_D * r1 * r1 * r1);
+    double hx1 = r1x * q1, hy1 = r1y * q1, hz1 = r1z * q1;
+    pole[q.u] = -q.pu1;
+    double r2x = rx - pole[0], r2y = ry - pole[1], r2z = rz - pole[2];
+    double r2 = sqrt(r2x * r2x + r2y * r2y + r2z * r2z);
+    double q2 = -q.charge / (4.0 * PI_D * r2 * r2 * r2);
+    out[0] = hx1 + r2x * q2;
+    out[1] = hy1 + r2y * q2;
+    out[2] = hz1 + r2z * q2;
+}
+
+// Sum over sample points (ax, ay, az) for the pole pair at (pv, pw), added to h
+// in the reference's order.  The contributions of QB consecutive z samples are
+// formed first (independent, so their square roots and divisions overlap in
+// one thread) and then added one by one: same values, same order.
 __device__ inline void quad_inner(const QuadSetup &q, const double rc[3], double pv, double pw,
                                   double h[3]) {
 #pragma clang fp contract(off)
-    const double PI_D = 3.14159265358979323846;
+    constexpr int QB = 4;
     double pole[3] = {0.0, 0.0, 0.0};
     pole[q.v] = pv;
     pole[q.w] = pw;
@@ -467,20 +500,29 @@ __device__ inline void quad_inner(const QuadSetup &q, const double rc[3], double
         double rx = rc[0] - 0.5 * q.cell[0] + q.cell[0] / (2.0 * (double)q.nx) + (double)ax * (q.cell[0] / (double)q.nx);
         for (int64_t ay = 0; ay < q.ny; ++ay) {
             double ry = rc[1] - 0.5 * q.cell[1] + q.cell[1] / (2.0 * (double)q.ny) + (double)ay * (q.cell[1] / (double)q.ny);
-            for (int64_t az = 0; az < q.nz; ++az) {
-                double rz = rc[2] - 0.5 * q.cell[2] + q.cell[2] / (2.0 * (double)q.nz) + (double)az * (q.cell[2] / (double)q.nz);
-                pole[q.u] = q.pu1;
-                double r1x = rx - pole[0], r1y = ry - pole[1], r1z = rz - pole[2];
-                double r1 = sqrt(r1x * r1x + r1y * r1y + r1z * r1z);
-                double q1 = q.charge / (4.0 * PI_D * r1 * r1 * r1);
-                double hx1 = r1x * q1, hy1 = r1y * q1, hz1 = r1z * q1;
-                pole[q.u] = -q.pu1;
-                double r2x = rx - pole[0], r2y = ry - pole[1], r2z = rz - pole[2];
-                double r2 = sqrt(r2x * r2x + r2y * r2y + r2z * r2z);
-                double q2 = -q.charge / (4.0 * PI_D * r2 * r2 * r2);
-                h[0] += hx1 + r2x * q2;
-                h[1] += hy1 + r2y * q2;
-                h[2] += hz1 + r2z * q2;
+            int64_t az0 = 0;
+            for (; az0 + QB <= q.nz; az0 += QB) {   // full chunks: straight-line code
+                double c[QB][3];
+#pragma unroll
+                for (int t = 0; t < QB; ++t) {
+                    const int64_t az = az0 + t;
+                    double rz = rc[2] - 0.5 * q.cell[2] + q.cell[2] / (2.0 * (double)q.nz) + (double)az * (q.cell[2] / (double)q.nz);
+                    quad_point(q, rx, ry, rz, pole, c[t]);
+                }
+#pragma unroll
+                for (int t = 0; t < QB; ++t) {
+                    h[0] += c[t][0];
+                    h[1] += c[t][1];
+                    h[2] += c[t][2];
+                }
+            }
+            for (; az0 < q.nz; ++az0) {
+                double c[3];
+                double rz = rc[2] - 0.5 * q.cell[2] + q.cell[2] / (2.0 * (double)q.nz) + (double)az0 * (q.cell[2] / (double)q.nz);
+                quad_point(q, rx, ry, rz, pole, c);
+                h[0] += c[0];
+                h[1] += c[1];
+                h[2] += c[2];
             }
         }
     }
@@ -508,6 +550,8 @@ k_quad(double *__restrict__ base, int nx, int ny, double dx, double dy, double d
     int64_t sx = t % nx, sy = t / nx;
     double rc[3] = {(double)sx * dx, (double)sy * dy, 0.0};
     QuadSetup q = quad_setup(u, dx, dy, dz, sx, sy, accuracy);
+    if (sx < kQuadHeavyScan && sy < kQuadHeavyScan && q.nv * q.nw * q.nx * q.ny * q.nz > kQuadHeavy)
+        return;   // k_quad_heavy takes it
     double h[3] = {0.0, 0.0, 0.0};
     for (int64_t i = 0; i < q.nv; ++i) {
         double pv = pole_pos(q.cell[q.v], q.nv, i);
@@ -517,6 +561,62 @@ k_quad(double *__restrict__ base, int nx, int ny, double dx, double dy, double d
     if (u == 0) base[t * 4 + 0] = mu0 * h[0];
     else if (u == 1) { base[t * 4 + 1] = mu0 * h[0]; base[t * 4 + 2] = mu0 * h[1]; }
     else base[t * 4 + 3] = mu0 * h[2];
+}
+
+// The near-field displacements whose sums exceed kQuadHeavy points: one block
+// per (displacement, axis).  Its threads evaluate kQuadChunk consecutive points
+// of the reference's loop order (pole v, pole w, x, y, z; z fastest) at once
+// and one thread adds them to the sum in that order, so the sum is the serial
+// one, bit for bit, while the square roots and divisions run 256 wide.
+__global__ void __launch_bounds__(256)
+k_quad_heavy(double *__restrict__ base, const int *__restrict__ items, int nx_base, double dx, double dy,
+             double dz, double accuracy, double mu0) {
+#pragma clang fp contract(off)
+    __shared__ double c[3][kQuadChunk];
+    const int item = items[blockIdx.x];
+    const int u = item % 3;
+    const int64_t t = item / 3;
+    const int64_t sx = t % nx_base, sy = t / nx_base;
+    const double rc[3] = {(double)sx * dx, (double)sy * dy, 0.0};
+    const QuadSetup q = quad_setup(u, dx, dy, dz, sx, sy, accuracy);
+    const int64_t total = q.nv * q.nw * q.nx * q.ny * q.nz;
+    double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+    for (int64_t c0 = 0; c0 < total; c0 += kQuadChunk) {
+        const int64_t f = c0 + threadIdx.x;
+        if (f < total) {
+            int64_t r = f;
+            const int64_t az = r % q.nz; r /= q.nz;
+            const int64_t ay = r % q.ny; r /= q.ny;
+            const int64_t ax = r % q.nx; r /= q.nx;
+            const int64_t j = r % q.nw, i = r / q.nw;
+            double pole[3] = {0.0, 0.0, 0.0};
+            pole[q.v] = pole_pos(q.cell[q.v], q.nv, i);
+            pole[q.w] = pole_pos(q.cell[q.w], q.nw, j);
+            const double rx = rc[0] - 0.5 * q.cell[0] + q.cell[0] / (2.0 * (double)q.nx) + (double)ax * (q.cell[0] / (double)q.nx);
+            const double ry = rc[1] - 0.5 * q.cell[1] + q.cell[1] / (2.0 * (double)q.ny) + (double)ay * (q.cell[1] / (double)q.ny);
+            const double rz = rc[2] - 0.5 * q.cell[2] + q.cell[2] / (2.0 * (double)q.nz) + (double)az * (q.cell[2] / (double)q.nz);
+            double o[3];
+            quad_point(q, rx, ry, rz, pole, o);
+            c[0][threadIdx.x] = o[0];
+            c[1][threadIdx.x] = o[1];
+            c[2][threadIdx.x] = o[2];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int cnt = (int)(total - c0 < kQuadChunk ? total - c0 : kQuadChunk);
+            for (int e = 0; e < cnt; ++e) {
+                h0 += c[0][e];
+                h1 += c[1][e];
+                h2 += c[2][e];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (u == 0) base[t * 4 + 0] = mu0 * h0;
+        else if (u == 1) { base[t * 4 + 1] = mu0 * h0; base[t * 4 + 2] = mu0 * h1; }
+        else base[t * 4 + 3] = mu0 * h2;
+    }
 }
 
 // Parity reflection into the padded real-space arrays (demag_fft_uniform.rs:1029-1071):
@@ -652,10 +752,43 @@ static int forced_radix_log2() {
     throw Error(1, "LLGMI_FFT_RADIX must be 8 or 16, got '" + v + "'");
 }
 
+// Process-wide cache of real-space base quadrants keyed by grid and periodic
+// images (the reference keeps its operators in a process-global cache keyed
+// the same way, demag_fft_uniform.rs:44-108): a second context, or the other
+// precision, on the same grid skips the quadrature.
+namespace {
+struct BaseKey {
+    int64_t nx, ny, pbc_x, pbc_y;
+    uint64_t dx, dy, dz;
+    bool operator<(const BaseKey &o) const {
+        return std::tie(nx, ny, pbc_x, pbc_y, dx, dy, dz) < std::tie(o.nx, o.ny, o.pbc_x, o.pbc_y, o.dx, o.dy, o.dz);
+    }
+};
+std::mutex g_base_mu;
+std::map<BaseKey, std::vector<double>> g_base_cache;
+BaseKey base_key(int64_t nx, int64_t ny, double dx, double dy, double dz, int64_t px, int64_t py) {
+    BaseKey k{nx, ny, px, py, 0, 0, 0};
+    std::memcpy(&k.dx, &dx, 8);
+    std::memcpy(&k.dy, &dy, 8);
+    std::memcpy(&k.dz, &dz, 8);
+    return k;
+}
+}  // namespace
+
 template <typename T>
 void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double dz_,
                         hipStream_t stream, int64_t pbc_x_, int64_t pbc_y_,
                         const std::vector<double> *base_in) {
+    std::vector<double> cached;
+    const BaseKey bkey = base_key(nx_, ny_, dx_, dy_, dz_, pbc_x_, pbc_y_);
+    if (!base_in) {
+        std::lock_guard<std::mutex> lock(g_base_mu);
+        auto it = g_base_cache.find(bkey);
+        if (it != g_base_cache.end()) {
+            cached = it->second;
+            base_in = &cached;
+        }
+    }
     const int forced = forced_radix_log2();
     release();
     nx = nx_; ny = ny_; dx = dx_; dy = dy_; dz = dz_;
@@ -694,9 +827,35 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
             int nt = 256;
             int64_t nq = periodic ? ex * ey : nb;
             int64_t nbk = (3 * nq + nt - 1) / nt;
-            hipLaunchKernelGGL(k_quad, dim3((unsigned)nbk), dim3(nt), 0, stream, periodic ? ext : base,
-                               (int)(periodic ? ex : nx), (int)(periodic ? ey : ny), dx, dy, dz, acc, mu0);
+            const int qnx = (int)(periodic ? ex : nx), qny = (int)(periodic ? ey : ny);
+            hipLaunchKernelGGL(k_quad, dim3((unsigned)nbk), dim3(nt), 0, stream, periodic ? ext : base, qnx, qny,
+                               dx, dy, dz, acc, mu0);
             LLGMI_LAUNCH_CHECK();
+            // heavy near-field sums: the point counts fall with distance, so they
+            // sit in a small corner of the quadrant; found on the host
+            std::vector<int> items;
+            const int lim = kQuadHeavyScan;
+            for (int sy = 0; sy < std::min(qny, lim); ++sy)
+                for (int sx = 0; sx < std::min(qnx, lim); ++sx)
+                    for (int u = 0; u < 3; ++u) {
+                        const QuadSetup q = quad_setup(u, dx, dy, dz, sx, sy, acc);
+                        if (q.nv * q.nw * q.nx * q.ny * q.nz > kQuadHeavy) items.push_back((sy * qnx + sx) * 3 + u);
+                    }
+            if (!items.empty()) {
+                int *d_items = nullptr;
+                LLGMI_HIP(hipMalloc(&d_items, sizeof(int) * items.size()));
+                hipError_t e1 = hipMemcpyAsync(d_items, items.data(), sizeof(int) * items.size(),
+                                               hipMemcpyHostToDevice, stream);
+                if (e1 == hipSuccess) {
+                    hipLaunchKernelGGL(k_quad_heavy, dim3((unsigned)items.size()), dim3(kQuadChunk), 0, stream,
+                                       periodic ? ext : base, d_items, qnx, dx, dy, dz, acc, mu0);
+                    e1 = hipGetLastError();
+                }
+                hipError_t e2 = hipStreamSynchronize(stream);
+                (void)hipFree(d_items);
+                LLGMI_HIP(e1);
+                LLGMI_HIP(e2);
+            }
         }
         if (periodic) {
             hipLaunchKernelGGL(k_fold_images, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, ext,
@@ -750,6 +909,11 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     if (ext) (void)hipFree(ext);
     (void)hipFree(kr);
     (void)hipFree(sk);
+    {
+        std::lock_guard<std::mutex> lock(g_base_mu);
+        if (g_base_cache.size() >= 16) g_base_cache.clear();
+        g_base_cache[bkey] = base_host;
+    }
     (void)hipFree(twx64);
     (void)hipFree(twy64);
 

@@ -588,12 +588,14 @@ __global__ void __launch_bounds__(kNT) k_mg_gradient(const double *__restrict__ 
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
     const int i = (int)(id % nxm), j = (int)(id / nxm);
-    const double v0 = (double)m[id], v1 = (double)m[n + id], v2 = (double)m[2 * n + id];
-    if (!(v0 * v0 + v1 * v1 + v2 * v2 > 1e-30)) {
-        bd[id] = T(0);
-        bd[n + id] = T(0);
-        bd[2 * n + id] = T(0);
-        return;
+    if (m) {   // m null: every film cell (add_b_from_phi_on_magnet_layer_all)
+        const double v0 = (double)m[id], v1 = (double)m[n + id], v2 = (double)m[2 * n + id];
+        if (!(v0 * v0 + v1 * v1 + v2 * v2 > 1e-30)) {
+            bd[id] = T(0);
+            bd[n + id] = T(0);
+            bd[2 * n + id] = T(0);
+            return;
+        }
     }
     const int pi = offx + i, pj = offy + j, k = kmag;
     const double pc = phi[idx3(pi, pj, k, px, py)];
@@ -612,6 +614,114 @@ __global__ void __launch_bounds__(kNT) k_mg_gradient(const double *__restrict__ 
     bd[id] = T(-kMu0 * gx);
     bd[n + id] = T(-kMu0 * gy);
     bd[2 * n + id] = T(-kMu0 * gz);
+}
+
+// screen_rhs_gaussian_xy (mg_kernels.rs:604-680): one pass of the separable
+// Gaussian along x (AXIS 0: src = rhs, dst = tmp) or y (AXIS 1: src = tmp,
+// dst = rhs); boundary rows and the row ends are zero, sources clamped.
+template <int AXIS>
+__global__ void __launch_bounds__(kNT) k_mg_screen(const double *__restrict__ src, double *__restrict__ dst,
+                                                   const double *__restrict__ w, int r, int nx, int ny, int nz) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (int64_t)nx * ny * nz) return;
+    const int64_t row = id / nx;
+    const int i = (int)(id - row * nx);
+    const int k = (int)(row / ny), j = (int)(row - (int64_t)k * ny);
+    if (k == 0 || k + 1 == nz || j == 0 || j + 1 == ny || i == 0 || i == nx - 1) {
+        dst[id] = 0.0;
+        return;
+    }
+    double acc = 0.0;
+    for (int q = -r; q <= r; ++q) {
+        if (AXIS == 0) {
+            int x = i + q;
+            x = x < 0 ? 0 : (x > nx - 1 ? nx - 1 : x);
+            acc += w[q + r] * src[row * nx + x];
+        } else {
+            int y = j + q;
+            y = y < 0 ? 0 : (y > ny - 1 ? ny - 1 : y);
+            acc += w[q + r] * src[((int64_t)k * ny + y) * nx + i];
+        }
+    }
+    dst[id] = acc;
+}
+
+// Boundary scaling of the hybrid correction (demag_poisson_mg.rs:424-501):
+// material cells within `margin` of vacuum (|m|^2 < 0.25) get
+// (material fraction in the (2r+1)^2 window)^alpha, every other cell 1.
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_hyb_scale(const T *__restrict__ m, int nx, int ny, int r, int margin,
+                                                   double alpha, double *__restrict__ scale) {
+    const int64_t n = (int64_t)nx * ny;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    auto vac = [&](int64_t q) {
+        const double a = (double)m[q], b = (double)m[n + q], d = (double)m[2 * n + q];
+        return (a * a + b * b + d * d) < 0.25;
+    };
+    const int i = (int)(c % nx), j = (int)(c / nx);
+    double s = 1.0;
+    if (!vac(c)) {
+        bool near = false;
+        for (int dj = -margin; dj <= margin && !near; ++dj)
+            for (int di = -margin; di <= margin; ++di) {
+                const int ni = i + di, nj = j + dj;
+                if (ni < 0 || ni >= nx || nj < 0 || nj >= ny) continue;
+                if (vac((int64_t)nj * nx + ni)) { near = true; break; }
+            }
+        if (near) {
+            unsigned n_mat = 0, n_tot = 0;
+            for (int dy = -r; dy <= r; ++dy) {
+                const int sj = j + dy;
+                if (sj < 0 || sj >= ny) continue;
+                for (int dx = -r; dx <= r; ++dx) {
+                    const int si = i + dx;
+                    if (si < 0 || si >= nx) continue;
+                    ++n_tot;
+                    if (!vac((int64_t)sj * nx + si)) ++n_mat;
+                }
+            }
+            s = pow((double)n_mat / (double)(n_tot > 1 ? n_tot : 1), alpha);
+        }
+    }
+    scale[c] = s;
+}
+
+// DeltaKernel2D::add_correction (demag_poisson_mg.rs:502-583): b += dK * (Ms m)
+// over the (2r+1)^2 stencil (sources outside the film skipped), summed in the
+// reference's order; dk = [xx | xy | yy | zz].
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_hyb_correct(const double *__restrict__ dk, int r, const T *__restrict__ m,
+                                                     T *__restrict__ b, double ms, const double *__restrict__ scale,
+                                                     int nx, int ny) {
+    const int64_t n = (int64_t)nx * ny;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int i = (int)(c % nx), j = (int)(c / nx), st = 2 * r + 1, nst = st * st;
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    for (int dy = -r; dy <= r; ++dy) {
+        const int sj = j - dy;
+        if (sj < 0 || sj >= ny) continue;
+        for (int dx = -r; dx <= r; ++dx) {
+            const int si = i - dx;
+            if (si < 0 || si >= nx) continue;
+            const int k = (dy + r) * st + (dx + r);
+            const int64_t q = (int64_t)sj * nx + si;
+            const double mx = ms * (double)m[q], my = ms * (double)m[n + q], mz = ms * (double)m[2 * n + q];
+            bx += dk[k] * mx + dk[nst + k] * my;
+            by += dk[nst + k] * mx + dk[2 * nst + k] * my;
+            bz += dk[3 * nst + k] * mz;
+        }
+    }
+    if (scale) {
+        const double sc = scale[c];
+        bx *= sc;
+        by *= sc;
+        bz *= sc;
+    }
+    b[c] = T((double)b[c] + bx);
+    b[n + c] = T((double)b[n + c] + by);
+    b[2 * n + c] = T((double)b[2 * n + c] + bz);
 }
 
 // ---------------------------------------------------------------------------
@@ -1008,6 +1118,59 @@ double clampf(double v, double lo, double hi) {  // f64::clamp (NaN stays NaN)
     return v < lo ? lo : (v > hi ? hi : v);
 }
 }  // namespace
+
+// HybridConfig::default (:283-293) and the boundary-scaling defaults (:2743-2760)
+void mg_hybrid_config_default(llgmi_mg_hybrid_config &h) {
+    std::memset(&h, 0, sizeof h);
+    h.enabled = 0;
+    h.radius_xy = 12;
+    h.delta_v_cycles = 60;
+    h.sigma_cells = 1.5;
+    h.bscale = 0.0;
+    h.bmargin = 2;
+}
+
+// HybridConfig::from_env (:296-356); LLG_DEMAG_MG_HYBRID_BSCALE / _BMARGIN are
+// parsed once per process without trimming, as the reference's OnceLock does.
+void mg_hybrid_config_from_env(llgmi_mg_hybrid_config &h) {
+    mg_hybrid_config_default(h);
+    std::string s;
+    if (env_str("LLG_DEMAG_MG_HYBRID_ENABLE", s)) {
+        const std::string t = lower(trim(s));
+        h.enabled = (t == "1" || t == "true" || t == "yes" || t == "on") ? 1 : 0;
+    }
+    if (h.enabled) {
+        int64_t u;
+        double f;
+        if (env_usize("LLG_DEMAG_MG_HYBRID_RADIUS", u)) h.radius_xy = u;
+        if (env_usize("LLG_DEMAG_MG_HYBRID_DELTA_VCYCLES", u)) h.delta_v_cycles = std::max<int64_t>(u, 1);
+        if (env_f64("LLG_DEMAG_MG_HYBRID_SIGMA", f)) {
+            f = std::isnan(f) ? 0.0 : std::max(f, 0.0);   // f64::max ignores NaN
+            h.sigma_cells = std::min(f, 32.0);
+        }
+    }
+    static const double bscale = [] {
+        const char *v = std::getenv("LLG_DEMAG_MG_HYBRID_BSCALE");
+        if (!v || !*v) return 0.0;
+        char *end = nullptr;
+        const double d = std::strtod(v, &end);
+        return (end && *end == '\0') ? d : 0.0;
+    }();
+    static const int64_t bmargin = [] {
+        const char *v = std::getenv("LLG_DEMAG_MG_HYBRID_BMARGIN");
+        if (!v || !*v) return (int64_t)2;
+        const char *p = v + (*v == '+' ? 1 : 0);
+        if (!*p) return (int64_t)2;
+        uint64_t x = 0;
+        for (; *p; ++p) {
+            if (*p < '0' || *p > '9' || x > (UINT64_MAX - 9) / 10) return (int64_t)2;
+            x = x * 10 + (uint64_t)(*p - '0');
+        }
+        return x > (uint64_t)INT64_MAX ? (int64_t)2 : (int64_t)x;
+    }();
+    h.bscale = bscale;
+    h.bmargin = bmargin;
+}
 
 // DemagPoissonMGConfig::from_env (:188-267) + MGOperatorSettings::from_env (:741-794)
 void mg_config_from_env(llgmi_mg_config &c) {
@@ -1496,6 +1659,9 @@ void MGSolver::release() {
     if (d_scalars_) (void)hipFree(d_scalars_);
     if (h_scalars_) (void)hipHostFree(h_scalars_);
     bc_ = d_partial_ = d_scalars_ = h_scalars_ = nullptr;
+    if (d_screen_w_) (void)hipFree(d_screen_w_);
+    d_screen_w_ = nullptr;
+    screen_cap_ = 0;
     delete tree_;
     tree_ = nullptr;
 }
@@ -1923,19 +2089,79 @@ void MGSolver::gradient(const T *m, T *bd) {
     LLGMI_LAUNCH_CHECK();
 }
 
-// DemagPoissonMG::add_field (:2094-2098): rhs, solve, gradient
+// DemagPoissonMG::add_field (:2094-2098): rhs, solve, gradient; with the
+// hybrid's Gaussian screening of the rhs (:2518-2520) when sigma > 0.
 template <typename T>
-void MGSolver::demag_field(const T *m, double ms, T *bd) {
+void MGSolver::demag_field(const T *m, double ms, T *bd, double sigma, bool masked) {
     {
         ProfScope ps(prof_, LLGMI_K_MG, stream_);
         build_rhs(m, std::is_same<T, float>::value, ms);
+        if (sigma > 0.0) screen_rhs(sigma);
     }
     solve();
     ProfScope ps(prof_, LLGMI_K_MG, stream_);
-    gradient(m, bd);
+    gradient(masked ? m : (const T *)nullptr, bd);
 }
 
-template void MGSolver::demag_field<float>(const float *, double, float *);
-template void MGSolver::demag_field<double>(const double *, double, double *);
+template void MGSolver::demag_field<float>(const float *, double, float *, double, bool);
+template void MGSolver::demag_field<double>(const double *, double, double *, double, bool);
+
+// gaussian_kernel_1d (mg_kernels.rs:580-599) on the host, the two passes on the
+// finest level with the second phi buffer as scratch (free between solves).
+void MGSolver::screen_rhs(double sigma) {
+    if (!(sigma > 0.0)) return;
+    const int64_t r = (int64_t)std::ceil(3.0 * sigma);
+    if (r <= 0) return;
+    Level &L = lev_[0];
+    if (L.nx < 3 || L.ny < 3 || L.nz < 3) return;
+    std::vector<double> &w = screen_w_;   // kept alive for the asynchronous upload
+    w.assign((size_t)(2 * r + 1), 0.0);
+    const double inv2s2 = 1.0 / (2.0 * sigma * sigma);
+    for (int64_t q = -r; q <= r; ++q) {
+        const double x = (double)q;
+        w[(size_t)(q + r)] = std::exp(-x * x * inv2s2);
+    }
+    double sum = 0.0;
+    for (double v : w) sum += v;
+    if (sum > 0.0)
+        for (double &v : w) v /= sum;
+    if (!d_screen_w_ || screen_cap_ < w.size()) {
+        if (d_screen_w_) (void)hipFree(d_screen_w_);
+        d_screen_w_ = nullptr;
+        LLGMI_HIP(hipMalloc(&d_screen_w_, sizeof(double) * w.size()));
+        screen_cap_ = w.size();
+    }
+    LLGMI_HIP(hipMemcpyAsync(d_screen_w_, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, stream_));
+    ensure_pair(L);
+    double *tmp = L.phi[1 - L.cur];
+    const int64_t n = (int64_t)L.nx * L.ny * L.nz;
+    hipLaunchKernelGGL(k_mg_screen<0>, dim3(nblocks(n)), dim3(kNT), 0, stream_, L.rhs, tmp, d_screen_w_, (int)r,
+                       L.nx, L.ny, L.nz);
+    LLGMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mg_screen<1>, dim3(nblocks(n)), dim3(kNT), 0, stream_, tmp, L.rhs, d_screen_w_, (int)r,
+                       L.nx, L.ny, L.nz);
+    LLGMI_LAUNCH_CHECK();
+    // the scratch buffer is the Jacobi / fused-sweep target: its contents are
+    // overwritten before they are read there
+}
+
+template <typename T>
+void launch_hybrid_correction(const double *dk, int r, const T *m, T *b, double ms, double bscale, int bmargin,
+                              double *scale_buf, int nx, int ny, hipStream_t s) {
+    const int64_t n = (int64_t)nx * ny;
+    const double *scale = nullptr;
+    if (bscale > 0.0) {
+        hipLaunchKernelGGL(k_hyb_scale<T>, dim3(nblocks(n)), dim3(kNT), 0, s, m, nx, ny, r, bmargin, bscale,
+                           scale_buf);
+        LLGMI_LAUNCH_CHECK();
+        scale = scale_buf;
+    }
+    hipLaunchKernelGGL(k_hyb_correct<T>, dim3(nblocks(n)), dim3(kNT), 0, s, dk, r, m, b, ms, scale, nx, ny);
+    LLGMI_LAUNCH_CHECK();
+}
+template void launch_hybrid_correction<float>(const double *, int, const float *, float *, double, double, int,
+                                              double *, int, int, hipStream_t);
+template void launch_hybrid_correction<double>(const double *, int, const double *, double *, double, double, int,
+                                               double *, int, int, hipStream_t);
 
 }  // namespace llgmi

@@ -77,6 +77,12 @@ struct Engine : EngineBase {
     std::unique_ptr<MGSolver> mg;
     bool has_mg_cfg = false;       // llgmi_ctx_set_mg_config; otherwise the environment per call
     llgmi_mg_config mg_cfg{};
+    bool has_hyb_cfg = false;      // llgmi_ctx_set_mg_hybrid_config; otherwise the environment per call
+    llgmi_mg_hybrid_config hyb_cfg{};
+    // hybrid correction stencil (DeltaKernel2D) and the settings it was built for
+    std::vector<double> dk_host;
+    double *d_dk = nullptr, *d_hyb_scale = nullptr;
+    std::string dk_key;
 
     // RK45 scratch (RK45Scratch, llg.rs:126-189): k1..k7, stage state, m7.
     T *d_k[8] = {nullptr};
@@ -102,7 +108,7 @@ struct Engine : EngineBase {
 
     ~Engine() override {
         void *ps[] = {d_m, d_b, d_bd, d_aos, d_geom, d_stage, d_m7, d_partial, d_err,
-                      d_r23b4, d_r23m4, d_red, d_redout};
+                      d_r23b4, d_r23m4, d_red, d_redout, d_dk, d_hyb_scale};
         for (void *q : ps)
             if (q) (void)hipFree(q);
         for (int i = 0; i < 8; ++i)
@@ -184,8 +190,145 @@ struct Engine : EngineBase {
             ensure_fft();
             fft.field(m, T(mat.ms), d_bd, stream, prof);
         } else {
-            ensure_mg().template demag_field<T>(m, mat.ms, d_bd);
+            MGSolver &mg = ensure_mg();
+            llgmi_mg_hybrid_config h;
+            if (has_hyb_cfg) h = hyb_cfg;
+            else mg_hybrid_config_from_env(h);
+            if (!(h.enabled && h.radius_xy > 0)) {
+                mg.template demag_field<T>(m, mat.ms, d_bd);
+                return;
+            }
+            // DemagPoissonMGHybrid::add_field (demag_poisson_mg.rs:2512-2528)
+            const int64_t r = std::min({h.radius_xy, std::max<int64_t>(g.nx - 2, 0) / 2,
+                                        std::max<int64_t>(g.ny - 2, 0) / 2});
+            if (r > 0) ensure_delta_kernel(h, r, mg.config(), mat);
+            mg.template demag_field<T>(m, mat.ms, d_bd, h.sigma_cells, true);
+            if (r > 0) {
+                if (h.bscale > 0.0) alloc(d_hyb_scale, n);
+                launch_hybrid_correction<T>(d_dk, (int)r, m, d_bd, mat.ms, h.bscale, (int)h.bmargin, d_hyb_scale,
+                                            (int)g.nx, (int)g.ny, stream);
+            }
         }
+    }
+
+    // ensure_delta_kernel (demag_poisson_mg.rs:2412-2466): rebuilt when the MG
+    // settings, the hybrid settings or the effective radius change.
+    void ensure_delta_kernel(const llgmi_mg_hybrid_config &h, int64_t r, const llgmi_mg_config &c,
+                             const llgmi_material &mat) {
+        llgmi_mg_hybrid_config hk = h;
+        hk.bscale = 0.0;
+        hk.bmargin = 0;
+        std::string key((const char *)&c, sizeof c);
+        key.append((const char *)&hk, sizeof hk);
+        key.append((const char *)&r, sizeof r);
+        if (d_dk && key == dk_key) return;
+        build_delta_kernel(h, r, c, mat);
+        dk_key = key;
+    }
+
+    // build_delta_kernel_impulse (demag_poisson_mg.rs:3159-3597), without its
+    // diagnostic branches: impulse responses of the FFT and of a fixed-cycle MG
+    // solve at the film centre, dK = (B_fft - B_mg) / Ms in the window, then the
+    // reference's in-place symmetrisation and uniform-bias fix on the host.
+    void build_delta_kernel(const llgmi_mg_hybrid_config &h, int64_t r, const llgmi_mg_config &c,
+                            const llgmi_material &mat) {
+        const int64_t nx = g.nx, ny = g.ny, cx = nx / 2, cy = ny / 2, st = 2 * r + 1, nst = st * st;
+        const double ms = mat.ms, ms_inv = std::fabs(ms) > 0.0 ? 1.0 / ms : 0.0;
+        const double geom_eps = 1e-14;
+        llgmi_mg_config ci = c;
+        ci.warm_start = 0;
+        ci.has_tol_abs = 0;
+        ci.has_tol_rel = 0;
+        ci.v_cycles = h.delta_v_cycles;
+        ci.v_cycles_max = std::max<int64_t>(h.delta_v_cycles, 1);
+        MGSolver imp(g, ci, stream, prof);
+        ensure_fft();
+        ensure_io();
+        alloc(d_bd, 3 * n);
+        T *tm = nullptr, *tb = nullptr;
+        LLGMI_HIP(hipMalloc(&tm, sizeof(T) * 3 * n));
+        std::vector<double> mh((size_t)(3 * n)), bf((size_t)(3 * n)), bm((size_t)(3 * n));
+        std::vector<double> xx((size_t)nst), xy((size_t)nst), yy((size_t)nst), zz((size_t)nst);
+        std::vector<double> xy_x((size_t)nst), xy_y((size_t)nst);
+        try {
+            LLGMI_HIP(hipMalloc(&tb, sizeof(T) * 3 * n));
+            for (int axis = 0; axis < 3; ++axis) {
+                for (int64_t q = 0; q < n; ++q) {
+                    mh[(size_t)(3 * q)] = geom_eps;
+                    mh[(size_t)(3 * q + 1)] = 0.0;
+                    mh[(size_t)(3 * q + 2)] = 0.0;
+                }
+                const size_t ctr = (size_t)(3 * (cy * nx + cx));
+                if (axis == 0) mh[ctr] = 1.0 + geom_eps;
+                else if (axis == 1) mh[ctr + 1] = 1.0;
+                else mh[ctr + 2] = 1.0;
+                upload(mh.data(), tm);
+                fft.field(tm, T(ms), d_bd, stream, prof);
+                download(d_bd, bf.data());
+                imp.template demag_field<T>(tm, ms, tb, h.sigma_cells, false);
+                download(tb, bm.data());
+                for (int64_t b = -r; b <= r; ++b)
+                    for (int64_t a = -r; a <= r; ++a) {
+                        const size_t id = (size_t)(3 * ((cy + b) * nx + (cx + a)));
+                        const size_t k = (size_t)((b + r) * st + (a + r));
+                        if (axis == 0) {
+                            xx[k] = (bf[id] - bm[id]) * ms_inv;
+                            xy_x[k] = (bf[id + 1] - bm[id + 1]) * ms_inv;
+                        } else if (axis == 1) {
+                            yy[k] = (bf[id + 1] - bm[id + 1]) * ms_inv;
+                            xy_y[k] = (bf[id] - bm[id]) * ms_inv;
+                        } else {
+                            zz[k] = (bf[id + 2] - bm[id + 2]) * ms_inv;
+                        }
+                    }
+            }
+        } catch (...) {
+            (void)hipFree(tm);
+            if (tb) (void)hipFree(tb);
+            throw;
+        }
+        (void)hipFree(tm);
+        (void)hipFree(tb);
+        for (int64_t k = 0; k < nst; ++k) xy[(size_t)k] = 0.5 * (xy_x[(size_t)k] + xy_y[(size_t)k]);
+        // DeltaKernel2D::symmetrize (:664-694), in place in the reference's order
+        auto ki = [&](int64_t a, int64_t b) { return (size_t)((b + r) * st + (a + r)); };
+        for (int64_t b = -r; b <= r; ++b)
+            for (int64_t a = -r; a <= r; ++a) {
+                const size_t k = ki(a, b), kr = ki(-a, -b);
+                xx[k] = 0.5 * (xx[k] + xx[kr]);
+                yy[k] = 0.5 * (yy[k] + yy[kr]);
+                zz[k] = 0.5 * (zz[k] + zz[kr]);
+                xy[k] = 0.5 * (xy[k] + xy[kr]);
+            }
+        if (std::fabs(g.dx - g.dy) <= 1e-12 * std::max({std::fabs(g.dx), std::fabs(g.dy), 1.0})) {
+            for (int64_t b = -r; b <= r; ++b)
+                for (int64_t a = -r; a <= r; ++a) {
+                    const size_t k = ki(a, b), ks = ki(b, a);
+                    const double nxx = 0.5 * (xx[k] + yy[ks]), nyy = 0.5 * (yy[k] + xx[ks]),
+                                 nxy = 0.5 * (xy[k] + xy[ks]);
+                    xx[k] = nxx;
+                    yy[k] = nyy;
+                    xy[k] = nxy;
+                }
+        }
+        // uniform-bias fix on the diagonal stencils (:3562-3571)
+        const size_t center = (size_t)(r * st + r);
+        double sxx = 0.0, syy = 0.0, szz = 0.0;
+        for (double v : xx) sxx += v;
+        for (double v : yy) syy += v;
+        for (double v : zz) szz += v;
+        xx[center] -= sxx;
+        yy[center] -= syy;
+        zz[center] -= szz;
+        dk_host.assign((size_t)(4 * nst), 0.0);
+        std::copy(xx.begin(), xx.end(), dk_host.begin());
+        std::copy(xy.begin(), xy.end(), dk_host.begin() + nst);
+        std::copy(yy.begin(), yy.end(), dk_host.begin() + 2 * nst);
+        std::copy(zz.begin(), zz.end(), dk_host.begin() + 3 * nst);
+        if (d_dk) (void)hipFree(d_dk);
+        d_dk = nullptr;
+        LLGMI_HIP(hipMalloc(&d_dk, sizeof(double) * dk_host.size()));
+        LLGMI_HIP(hipMemcpy(d_dk, dk_host.data(), sizeof(double) * dk_host.size(), hipMemcpyHostToDevice));
     }
 
     LocalArgs<T> local_args(const llgmi_params &p, const llgmi_material &mat, int mask, bool torque,
@@ -1031,6 +1174,32 @@ int llgmi_compute_demag_field_poisson_mg(llgmi_ctx *ctx, const double *m, double
     if (!out) return fail(LLGMI_ERR_INVALID, "NULL field");
     std::memset(out, 0, sizeof(double) * 3 * (size_t)(ctx->grid.nx * ctx->grid.ny));
     return llgmi_add_demag_field_poisson_mg(ctx, m, out, mat);
+}
+
+int llgmi_mg_hybrid_config_default(llgmi_mg_hybrid_config *cfg) {
+    if (!cfg) return fail(LLGMI_ERR_INVALID, "config is NULL");
+    mg_hybrid_config_default(*cfg);
+    return LLGMI_OK;
+}
+
+int llgmi_mg_hybrid_config_from_env(llgmi_mg_hybrid_config *cfg) {
+    if (!cfg) return fail(LLGMI_ERR_INVALID, "config is NULL");
+    mg_hybrid_config_from_env(*cfg);
+    return LLGMI_OK;
+}
+
+int llgmi_ctx_set_mg_hybrid_config(llgmi_ctx *ctx, const llgmi_mg_hybrid_config *cfg) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (cfg && (cfg->radius_xy < 0 || cfg->delta_v_cycles < 1 || !(cfg->sigma_cells >= 0.0) ||
+                cfg->sigma_cells > 32.0 || cfg->bmargin < 0 || cfg->radius_xy > 4096))
+        return fail(LLGMI_ERR_INVALID, "hybrid config out of range (radius 0..4096, delta_v_cycles >= 1, "
+                                       "sigma 0..32, bmargin >= 0)");
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            eng.has_hyb_cfg = cfg != nullptr;
+            if (cfg) eng.hyb_cfg = *cfg;
+        });
+    });
 }
 
 int llgmi_mg_last_stats(llgmi_ctx *ctx, llgmi_mg_stats *out) {

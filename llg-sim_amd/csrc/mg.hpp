@@ -30,6 +30,8 @@ MGStencil mg_stencil_from_kind(int kind, double dx, double dy, double dz, double
 MGStencil mg_galerkin(const MGStencil &fine, int rx, int ry, int rz, bool trilinear);
 void mg_config_default(llgmi_mg_config &c);
 void mg_config_from_env(llgmi_mg_config &c);
+void mg_hybrid_config_default(llgmi_mg_hybrid_config &h);
+void mg_hybrid_config_from_env(llgmi_mg_hybrid_config &h);
 void mg_sanitize(llgmi_mg_config &c);
 void mg_validate(const llgmi_mg_config &c);  // throws Error(1) on out-of-range fields
 
@@ -43,10 +45,12 @@ public:
     bool same_structure(const llgmi_grid &g, const llgmi_mg_config &cfg) const;
     void apply_cfg(const llgmi_mg_config &cfg);
     // B_demag = -mu0 grad(phi) on the film cells with |m|^2 > 1e-30, 0 elsewhere
-    // (overwrites bd); m and bd are SoA in precision T.
+    // (overwrites bd); m and bd are SoA in precision T.  sigma > 0 screens the
+    // rhs with a Gaussian (hybrid); masked = false fills every film cell.
     template <typename T>
-    void demag_field(const T *m, double ms, T *bd);
+    void demag_field(const T *m, double ms, T *bd, double sigma = 0.0, bool masked = true);
     const llgmi_mg_stats &stats() const { return stats_; }
+    const llgmi_mg_config &config() const { return cfg_; }
 
     struct Level {
         int nx = 0, ny = 0, nz = 0;
@@ -72,6 +76,7 @@ private:
     void v_cycle(int l);
     void v_cycle_graph();
     void solve();
+    void screen_rhs(double sigma);
     double finest_rhs_max();
     template <typename T>
     void gradient(const T *m, T *bd);
@@ -87,6 +92,9 @@ private:
     double *d_scalars_ = nullptr;              // [0] max, [1] nan flag, [2..5] dipole moments
     double *h_scalars_ = nullptr;              // pinned mirror
     Tree *tree_ = nullptr;
+    double *d_screen_w_ = nullptr;             // Gaussian screening weights
+    size_t screen_cap_ = 0;
+    std::vector<double> screen_w_;
     llgmi_mg_stats stats_{};
     // V-cycles captured as HIP graphs, keyed by the smoothing settings and
     // the Jacobi buffer parity of every level at entry
@@ -99,5 +107,12 @@ private:
     bool rb_fused_ = true;     // LLGMI_MG_RB_FUSED=0: two half-sweep launches per sweep
     int rbz_ = 1 << 20;        // LLGMI_MG_RBZ: planes per z-chunk of the fused finest sweep
 };
+
+// Hybrid MG correction (DeltaKernel2D::add_correction, demag_poisson_mg.rs:410-583):
+// b += dK * (ms m) over the (2r+1)^2 stencil dk = [xx | xy | yy | zz] (device,
+// FP64); bscale > 0 scales cells near vacuum (scale_buf: nx*ny doubles).
+template <typename T>
+void launch_hybrid_correction(const double *dk, int r, const T *m, T *b, double ms, double bscale, int bmargin,
+                              double *scale_buf, int nx, int ny, hipStream_t s);
 
 }  // namespace llgmi

@@ -151,7 +151,8 @@ EXPORTED = [
     "llgmi_relax_with_report", "llgmi_dev_rk23_relax_step", "llgmi_dev_compute_energy_geom",
     "llgmi_dev_relax_with_report", "llgmi_write_ovf2_text", "llgmi_write_ovf2_binary4",
     "llgmi_dev_write_ovf2", "llgmi_ovf_total_sim_time_line", "llgmi_demag_cache_path",
-    "llgmi_demag_cache_save", "llgmi_demag_cache_load",
+    "llgmi_demag_cache_save", "llgmi_demag_cache_load", "llgmi_mg_hybrid_config_default",
+    "llgmi_mg_hybrid_config_from_env", "llgmi_ctx_set_mg_hybrid_config",
 ]
 
 
@@ -185,6 +186,28 @@ BC_ZERO, BC_DIPOLE, BC_TREECODE = 0, 1, 2
 STENCIL_7, STENCIL_ISO9, STENCIL_ISO27 = 0, 1, 2
 PROLONG_INJECT, PROLONG_TRILINEAR = 0, 1
 COARSE_REDISCRETIZE, COARSE_GALERKIN = 0, 1
+
+
+class MGHybridConfig(C.Structure):
+    """llgmi_mg_hybrid_config: HybridConfig (demag_poisson_mg.rs:274-361) + BSCALE / BMARGIN."""
+    _fields_ = [("enabled", C.c_int32), ("reserved", C.c_int32), ("radius_xy", C.c_int64),
+                ("delta_v_cycles", C.c_int64), ("sigma_cells", C.c_double), ("bscale", C.c_double),
+                ("bmargin", C.c_int64)]
+
+
+def mg_hybrid_config(**kw) -> MGHybridConfig:
+    """HybridConfig::default() (disabled) with fields from kw."""
+    c = MGHybridConfig()
+    _check(lib().llgmi_mg_hybrid_config_default(C.byref(c)))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def mg_hybrid_config_from_env() -> MGHybridConfig:
+    c = MGHybridConfig()
+    _check(lib().llgmi_mg_hybrid_config_from_env(C.byref(c)))
+    return c
 
 
 def mg_config(**kw) -> MGConfig:
@@ -383,6 +406,9 @@ def lib():
     L.llgmi_add_demag_field_poisson_mg.argtypes = [vp, dp, dp, M]
     L.llgmi_compute_demag_field_poisson_mg.argtypes = [vp, dp, dp, M]
     L.llgmi_mg_last_stats.argtypes = [vp, C.POINTER(MGStats)]
+    L.llgmi_mg_hybrid_config_default.argtypes = [C.POINTER(MGHybridConfig)]
+    L.llgmi_mg_hybrid_config_from_env.argtypes = [C.POINTER(MGHybridConfig)]
+    L.llgmi_ctx_set_mg_hybrid_config.argtypes = [vp, C.POINTER(MGHybridConfig)]
     RS, RR = C.POINTER(RelaxSettings), C.POINTER(RelaxReport)
     L.llgmi_relax_settings_default.argtypes = [RS]
     L.llgmi_relax_settings_mumax_like.argtypes = [RS]
@@ -507,6 +533,10 @@ class Context:
     def set_mg_config(self, cfg: Optional[MGConfig]) -> None:
         """Pins the MG configuration; None reads LLG_DEMAG_MG_* at every call."""
         _check(lib().llgmi_ctx_set_mg_config(self._h, C.byref(cfg) if cfg is not None else None))
+
+    def set_mg_hybrid_config(self, cfg: Optional[MGHybridConfig]) -> None:
+        """Pins the hybrid MG settings; None reads LLG_DEMAG_MG_HYBRID_* at every call."""
+        _check(lib().llgmi_ctx_set_mg_hybrid_config(self._h, C.byref(cfg) if cfg is not None else None))
 
     def add_demag_field_poisson_mg(self, m, b: np.ndarray, mat: Material) -> None:
         m = _f64(m).reshape(-1, 3)

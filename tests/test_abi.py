@@ -159,3 +159,31 @@ def test_relax_arguments_rejected_without_device():
     assert L.llgmi_rk23_relax_step(None, None, None, None, 2, 1e-5, 0.8, 1e-18, 1e-11,
                                    None, None, None) == llgmi.ERR_INVALID
     assert L.llgmi_relax_settings_default(None) == llgmi.ERR_INVALID
+
+
+def test_mg_hybrid_config_from_env(monkeypatch):
+    """HybridConfig::from_env (demag_poisson_mg.rs:296-356): off unless ENABLE is
+    1/true/yes/on (trimmed, any case); then RADIUS, DELTA_VCYCLES (>= 1) and
+    SIGMA (clamped to [0, 32]) are read."""
+    import llgmi
+    assert C.sizeof(llgmi.MGHybridConfig) == 48
+    for k in ("ENABLE", "RADIUS", "DELTA_VCYCLES", "SIGMA"):
+        monkeypatch.delenv("LLG_DEMAG_MG_HYBRID_" + k, raising=False)
+    d = llgmi.mg_hybrid_config()
+    assert (d.enabled, d.radius_xy, d.delta_v_cycles, d.sigma_cells, d.bmargin) == (0, 12, 60, 1.5, 2)
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_RADIUS", "5")
+    assert llgmi.mg_hybrid_config_from_env().radius_xy == 12        # ignored while disabled
+    for v, on in ((" On ", 1), ("YES", 1), ("true", 1), ("1", 1), ("0", 0), ("enable", 0)):
+        monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_ENABLE", v)
+        assert llgmi.mg_hybrid_config_from_env().enabled == on, v
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_ENABLE", "1")
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_DELTA_VCYCLES", "0")
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_SIGMA", "40")
+    c = llgmi.mg_hybrid_config_from_env()
+    assert (c.radius_xy, c.delta_v_cycles, c.sigma_cells) == (5, 1, 32.0)
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_SIGMA", "-3")
+    assert llgmi.mg_hybrid_config_from_env().sigma_cells == 0.0
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_SIGMA", "NaN")
+    assert llgmi.mg_hybrid_config_from_env().sigma_cells == 0.0
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_SIGMA", "x")
+    assert llgmi.mg_hybrid_config_from_env().sigma_cells == 1.5

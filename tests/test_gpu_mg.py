@@ -220,3 +220,90 @@ def test_mg_graph_replay_and_fused_sweeps_match_direct_launches(gpu_lib, monkeyp
     for m, w in zip(ms, want):
         np.testing.assert_array_equal(slow.compute_demag_field_poisson_mg(m, mat), w)
     assert slow.mg_last_stats()["cycles"] == cycles
+
+
+def _xorshift_unit_vectors(n, seed):
+    """init_random_unit_vectors of tests/test_demag_methods.rs:144-180."""
+    import math
+    mask = (1 << 64) - 1
+    x = seed
+    out = np.zeros((n, 3))
+    for c in range(n):
+        v = []
+        for _ in range(3):
+            x ^= (x << 13) & mask
+            x ^= x >> 7
+            x ^= (x << 17) & mask
+            v.append(2.0 * (x / float(mask)) - 1.0)
+        nrm = max(math.sqrt(v[0] ** 2 + v[1] ** 2 + v[2] ** 2), 1e-30)
+        out[c] = [v[0] / nrm, v[1] / nrm, v[2] / nrm]
+    return out
+
+
+@pytest.mark.parametrize("case", ["unscreened_r2", "screened_r5", "holes_bscale", "galerkin_trilinear"])
+def test_mg_hybrid_fp64_matches_oracle(gpu_lib, case):
+    """Hybrid MG (demag_poisson_mg.rs:2512-2528, 3159-3597) against the oracle:
+    the delta kernel from impulse responses, the screened solve and the local
+    correction; the reference's 32 x 32 random case (tests/test_demag_methods.rs:197-267)."""
+    from oracle import oracle as O
+    L = gpu_lib
+    print(f"hybrid {case}", flush=True)
+    nx = ny = 32
+    cell = (5e-9, 5e-9, 1e-9)
+    m = _xorshift_unit_vectors(nx * ny, 12345)
+    kw = dict(warm_start=0, pad_xy=2, n_vac_z=16, v_cycles=4)
+    hyb = dict(enabled=1, radius_xy=2, delta_v_cycles=20, sigma_cells=0.0)
+    if case == "screened_r5":
+        hyb.update(radius_xy=5, sigma_cells=1.5)
+    if case == "holes_bscale":
+        mm = m.reshape(ny, nx, 3)
+        mm[8:14, 10:16] = 0.0
+        mm[20:25, 3:7] = 0.0
+        hyb.update(radius_xy=3, sigma_cells=1.0, bscale=0.7, bmargin=2)
+    if case == "galerkin_trilinear":
+        kw.update(stencil=2, prolong=1, coarse_op=1, smoother=0)
+        hyb.update(radius_xy=3, sigma_cells=1.2)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    oc.set_mg_config(O.mg_config(**kw))
+    oc.set_hyb_config(O.hyb_config(**hyb))
+    want = np.zeros_like(m)
+    oc.add_demag(m, want, O.Material(ms=8e5, demag=True, demag_method=1))
+    ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    ctx.set_mg_config(L.mg_config(**kw))
+    ctx.set_mg_hybrid_config(L.mg_hybrid_config(**hyb))
+    got = ctx.compute_demag_field_poisson_mg(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(got, want) <= 1e-12, rel_err(got, want)
+    # a second call reuses the cached delta kernel
+    again = ctx.compute_demag_field_poisson_mg(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(again, want) <= 1e-12
+    if case in ("unscreened_r2", "screened_r5"):
+        bf = ctx.compute_demag_field(m, L.Material(ms=8e5, demag=True))
+        rel = np.sqrt(((got - bf) ** 2).sum() / (bf ** 2).sum())
+        assert rel < 0.06, rel        # recorded "~0.32 -> ~0.04"
+
+
+def test_mg_hybrid_env_and_fp32(gpu_lib, monkeypatch):
+    """With no pinned hybrid configuration the context reads
+    LLG_DEMAG_MG_HYBRID_* at every call; an FP32 context agrees to 1e-5."""
+    L = gpu_lib
+    nx, ny, cell = 24, 20, (4e-9, 4e-9, 2e-9)
+    m = _rand_m(nx * ny, 31)
+    kw = dict(warm_start=0, pad_xy=3, n_vac_z=4, v_cycles=3)
+    pinned = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    pinned.set_mg_config(L.mg_config(**kw))
+    pinned.set_mg_hybrid_config(L.mg_hybrid_config(enabled=1, radius_xy=3, delta_v_cycles=10, sigma_cells=0.8))
+    want = pinned.compute_demag_field_poisson_mg(m, L.Material(ms=8e5, demag=True))
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_ENABLE", "on")
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_RADIUS", "3")
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_DELTA_VCYCLES", "10")
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_SIGMA", "0.8")
+    env = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    env.set_mg_config(L.mg_config(**kw))
+    np.testing.assert_array_equal(env.compute_demag_field_poisson_mg(m, L.Material(ms=8e5, demag=True)), want)
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_ENABLE", "0")
+    pure = env.compute_demag_field_poisson_mg(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(pure, want) > 1e-3
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_ENABLE", "1")
+    c32 = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F32)
+    c32.set_mg_config(L.mg_config(**kw))
+    assert rel_err(c32.compute_demag_field_poisson_mg(m, L.Material(ms=8e5, demag=True)), want) <= 1e-5
