@@ -23,6 +23,8 @@ The JSON line also carries:
                 step) and energy breakdowns/s;
   host_aos      the same evaluation through the host entry point (AoS f64 in and
                 out, PCIe transfers included);
+  batched_films 64 independent films of mixed sizes (the batched workload),
+                one context and stream per film, evaluated side by side;
   mg            the multigrid Poisson demag on a 512 x 512 film with a square
                 lattice of holes (3 nm cells, 7-point operator, red-black SOR,
                 injection, rediscretised coarse levels, treecode boundary,
@@ -190,6 +192,55 @@ def mg_bench(L, cpu: bool, reps: int = 5):
     return out
 
 
+def films_bench(L, n_films: int = 64, rounds: int = 20):
+    """Independent films evaluated side by side: one context (and HIP stream)
+    per film, sizes from {64,128,256,512} x {64,128,256}, dx in [2, 5] nm,
+    dz = 3 nm, random materials, smoothed random m; a round queues one full
+    field evaluation (Full mask, FFT demag) on every film before one
+    synchronisation, so the small films overlap on the GPU."""
+    import torch
+    rng = np.random.default_rng(64)
+    films = []
+    for f in range(n_films):
+        nx = int(rng.choice([64, 128, 256, 512]))
+        ny = int(rng.choice([64, 128, 256]))
+        dxy = float(rng.uniform(2e-9, 5e-9))
+        g = L.Grid2D(nx, ny, dxy, dxy, 3e-9)
+        mat = L.Material(ms=float(rng.uniform(4e5, 1.2e6)), a_ex=float(rng.uniform(5e-12, 2e-11)),
+                         k_u=float(rng.uniform(0.0, 5e5)), easy_axis=(0.0, 0.0, 1.0),
+                         dmi=float(rng.uniform(0.0, 4e-3)), demag=True)
+        ctx = L.Context(g, precision=L.F32)
+        mh = rng.standard_normal((ny, nx, 3))
+        for _ in range(3):   # smooth
+            mh = 0.25 * (np.roll(mh, 1, 0) + np.roll(mh, -1, 0) + np.roll(mh, 1, 1) + np.roll(mh, -1, 1))
+        mh /= np.linalg.norm(mh, axis=2, keepdims=True)
+        m = torch.empty((3, ny, nx), dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(m)
+        ctx.dev_upload(mh.reshape(-1, 3), m)
+        films.append((ctx, m, b, mat, nx * ny))
+    params = L.LLGParams(alpha=0.02, b_ext=(0.01, 0.0, 0.0))
+    for ctx, m, b, mat, _ in films:          # untimed: kernels, autotune
+        ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
+    for ctx, *_ in films:
+        ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        for ctx, m, b, mat, _ in films:
+            ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
+    for ctx, *_ in films:
+        ctx.synchronize()
+    dt = time.perf_counter() - t0
+    cells = sum(f[4] for f in films)
+    out = {"workload": f"{n_films} films, sizes {{64,128,256,512}}x{{64,128,256}}, FP32, one context and stream each",
+           "films": n_films, "total_cells": cells, "rounds": rounds,
+           "film_evals_per_s": round(n_films * rounds / dt, 1),
+           "cell_updates_per_s": round(cells * rounds / dt, 1),
+           "ms_per_round": round(1e3 * dt / rounds, 3)}
+    for ctx, *_ in films:
+        ctx.close()
+    return out
+
+
 def host_aos_rate(L, m_host, n=20):
     """Field evaluations/s through the host entry point (AoS f64 in and out)."""
     ctx = L.Context(L.Grid2D(NX, NY, *CELL), precision=L.F32)
@@ -348,6 +399,10 @@ def main():
         acc_rep = accuracy_report(L, m_host)
         print(f"[bench] accuracy done: {acc_rep}", file=sys.stderr, flush=True)
         host_rate = host_aos_rate(L, m_host)
+    films = None
+    if not args.quick:
+        films = films_bench(L)
+        print(f"[bench] films: {films}", file=sys.stderr, flush=True)
     mg = None
     if not args.no_mg:
         mg = mg_bench(L, cpu=not args.no_cpu)
@@ -386,6 +441,7 @@ def main():
         "rk23_relax": r23,
         "host_aos_evals_per_s": host_rate,
         "mg": mg,
+        "batched_films": films,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
