@@ -24,7 +24,8 @@ The JSON line also carries:
   host_aos      the same evaluation through the host entry point (AoS f64 in and
                 out, PCIe transfers included);
   batched_films 64 independent films of mixed sizes (the batched workload),
-                one context and stream per film, evaluated side by side;
+                one context and stream per film, evaluated side by side, and
+                per-film adaptive RK45 through the batched stepper;
   mg            the multigrid Poisson demag on a 512 x 512 film with a square
                 lattice of holes (3 nm cells, 7-point operator, red-black SOR,
                 injection, rediscretised coarse levels, treecode boundary,
@@ -192,7 +193,7 @@ def mg_bench(L, cpu: bool, reps: int = 5):
     return out
 
 
-def films_bench(L, n_films: int = 64, rounds: int = 20):
+def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
     """Independent films evaluated side by side: one context (and HIP stream)
     per film, sizes from {64,128,256,512} x {64,128,256}, dx in [2, 5] nm,
     dz = 3 nm, random materials, smoothed random m; a round queues one full
@@ -236,6 +237,37 @@ def films_bench(L, n_films: int = 64, rounds: int = 20):
            "film_evals_per_s": round(n_films * rounds / dt, 1),
            "cell_updates_per_s": round(cells * rounds / dt, 1),
            "ms_per_round": round(1e3 * dt / rounds, 3)}
+    # per-film adaptive RK45 (own alpha and dt per film), one batched attempt
+    # per round (llgmi_dev_rk45_step_batch) against the same films stepped one
+    # after another (llgmi_dev_rk45_step)
+    ctxs = [f[0] for f in films]
+    ms = [f[1] for f in films]
+    mats = [f[3] for f in films]
+    pars = [L.LLGParams(alpha=float(rng.uniform(0.01, 0.5)), dt=1e-13, b_ext=(0.01, 0.0, 0.0))
+            for _ in films]
+    rk = dict(mask=L.FieldMask.Full, max_err=1e-5, headroom=0.8, dt_min=1e-18, dt_max=1e-11)
+    for _ in range(2):
+        L.dev_rk45_step_batch(ctxs, ms, pars, mats, **rk)
+    acc = att = 0
+    t0 = time.perf_counter()
+    for _ in range(rk_rounds):
+        for e, a, u in L.dev_rk45_step_batch(ctxs, ms, pars, mats, **rk):
+            acc += a
+            att += 1
+    dtb = time.perf_counter() - t0
+    acc_s = 0
+    t0 = time.perf_counter()
+    for _ in range(rk_rounds):
+        for c, m, p, mat in zip(ctxs, ms, pars, mats):
+            acc_s += c.dev_rk45_step(m, p, mat, **rk)[1]
+    dts = time.perf_counter() - t0
+    out["rk45"] = {"workload": "per-film adaptive Dormand-Prince RK45, max_err 1e-5, alpha in U[0.01, 0.5], FP32 "
+                               "state and fields, FP64 error norm and controller",
+                   "rounds": rk_rounds, "attempts": att, "accepted": acc,
+                   "accepted_steps_per_s": round(acc / dtb, 1),
+                   "cell_steps_per_s": round(acc / att * cells * rk_rounds / dtb, 1),
+                   "ms_per_round": round(1e3 * dtb / rk_rounds, 3),
+                   "one_film_at_a_time_steps_per_s": round(acc_s / dts, 1)}
     for ctx, *_ in films:
         ctx.close()
     return out

@@ -165,6 +165,19 @@ int llgmi_dev_rk45_step(llgmi_ctx *ctx, void *m_dev, llgmi_params *params,
                         const llgmi_material *mat, int field_mask, double max_err,
                         double headroom, double dt_min, double dt_max, double *eps,
                         int *accepted, double *dt_used);
+/* llgmi_dev_rk45_step for `count` independent films at once: film i is
+ * ctxs[i] (distinct contexts, one stream each) with state m_devs[i],
+ * params[i] (dt in/out, per-film adaptive) and mats[i]; eps, accepted and
+ * dt_used are arrays of `count`.  Every film's attempt is launched before any
+ * is waited for, so the films run side by side on the device.  Each film's
+ * result is what llgmi_dev_rk45_step gives for it alone.  The reference has
+ * no batched stepper: its drivers call llg.rs:795-1026 once per film, which
+ * is what this replaces for a loop over films.  On an error no film's state
+ * or dt changes. */
+int llgmi_dev_rk45_step_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi_params *params,
+                              const llgmi_material *mats, int64_t count, int field_mask,
+                              double max_err, double headroom, double dt_min, double dt_max,
+                              double *eps, int *accepted, double *dt_used);
 /* AoS f64 host <-> SoA device copies in the context precision (synchronous). */
 int llgmi_dev_upload(llgmi_ctx *ctx, const double *host_aos, void *dev_soa);
 int llgmi_dev_download(llgmi_ctx *ctx, const void *dev_soa, double *host_aos);

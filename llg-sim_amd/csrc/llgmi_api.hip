@@ -14,6 +14,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -91,6 +92,7 @@ struct Engine : EngineBase {
     double *h_err = nullptr;  // pinned
     bool fsal_valid = false;
     int last_mask = -1;
+    int issued_mask = -1;    // mask of the attempt rk45_issue left for rk45_finish
     double last_bext[3] = {0, 0, 0};
 
     // RK23 relax scratch (RK23Scratch, llg.rs:196-268): k1..k4, b4 and m4 of
@@ -434,6 +436,16 @@ struct Engine : EngineBase {
     void rk45(T *m, llgmi_params &p, const llgmi_material &mat, int mask, double max_err,
               double headroom, double dt_min, double dt_max, double &eps_out, int &acc_out,
               double &dtu_out) {
+        rk45_issue(m, p, mat, mask);
+        LLGMI_HIP(hipStreamSynchronize(stream));
+        rk45_finish(m, p, mask, max_err, headroom, dt_min, dt_max, eps_out, acc_out, dtu_out);
+    }
+
+    // First half of an attempt: the stages and the error norm, ending with an
+    // asynchronous copy of max|err| into pinned memory.  Nothing waits, so the
+    // attempts of many contexts (one stream each) run side by side; p is read,
+    // not changed.
+    void rk45_issue(T *m, const llgmi_params &p, const llgmi_material &mat, int mask) {
         static const double A[7][5] = {
             {0}, {0},
             {1.0 / 5.0},
@@ -449,7 +461,8 @@ struct Engine : EngineBase {
         bool fsal_ok = fsal_valid && last_mask == mask && last_bext[0] == p.b_ext[0] &&
                        last_bext[1] == p.b_ext[1] && last_bext[2] == p.b_ext[2];
         if (fsal_ok) std::swap(d_k[1], d_k[7]);
-        else field(m, d_b, p, mat, mask, nullptr, d_k[1]);
+        fsal_valid = false;   // set again by rk45_finish on accept; an attempt cut short leaves it off
+        if (!fsal_ok) field(m, d_b, p, mat, mask, nullptr, d_k[1]);
         for (int s = 2; s <= 7; ++s) {
             StageArgs<T> sa{};
             if (s < 7) {
@@ -478,9 +491,17 @@ struct Engine : EngineBase {
                              sizeof(T) == sizeof(float), d_partial, d_err, stream);
         }
         LLGMI_HIP(hipMemcpyAsync(h_err, d_err, sizeof(double), hipMemcpyDeviceToHost, stream));
-        LLGMI_HIP(hipStreamSynchronize(stream));
+        issued_mask = mask;
+    }
+
+    // Second half, after the stream has completed rk45_issue's work: the
+    // controller (llg.rs:999-1025), FP64 on the host, and the accepted state.
+    void rk45_finish(T *m, llgmi_params &p, int mask, double max_err, double headroom, double dt_min,
+                     double dt_max, double &eps_out, int &acc_out, double &dtu_out) {
+        if (issued_mask != mask || mask < 0) throw Error(LLGMI_ERR_INVALID, "internal: rk45_finish without rk45_issue");
+        issued_mask = -1;
+        const double dt0 = p.dt;
         const double err_inf = *h_err;
-        // controller (llg.rs:999-1025), FP64 on the host
         const double eps = err_inf * dt0;
         double dt_next = eps > 0.0 ? dt0 * headroom * std::pow(max_err / eps, 0.2) : dt0 * 2.0;
         if (dt_next < dt_min) dt_next = dt_min;
@@ -1102,6 +1123,66 @@ int llgmi_dev_rk45_step(llgmi_ctx *ctx, void *m_dev, llgmi_params *params, const
                      *accepted, *dt_used);
         });
     });
+}
+
+int llgmi_dev_rk45_step_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi_params *params,
+                              const llgmi_material *mats, int64_t count, int field_mask, double max_err,
+                              double headroom, double dt_min, double dt_max, double *eps, int *accepted,
+                              double *dt_used) {
+    if (count < 0) return fail(LLGMI_ERR_INVALID, "count < 0");
+    if (count == 0) return LLGMI_OK;
+    if (!ctxs || !m_devs || !params || !mats || !eps || !accepted || !dt_used)
+        return fail(LLGMI_ERR_INVALID, "NULL argument");
+    std::string e = check_mask(field_mask);
+    if (e.empty() && dt_min > dt_max) e = "dt_min > dt_max";
+    std::set<const llgmi_ctx *> seen;
+    for (int64_t i = 0; i < count && e.empty(); ++i) {
+        const llgmi_ctx *c = ctxs[i];
+        if (!c) { e = "context is NULL"; break; }
+        if (!seen.insert(c).second) { e = "a context appears twice in one batch"; break; }
+        e = check_mat(&mats[i]);
+        if (e.empty()) {
+            (void)hipSetDevice(c->device);
+            e = check_dev(m_devs[i], 3 * (size_t)(c->grid.nx * c->grid.ny) * elem_size(c), "m");
+        }
+        if (!e.empty()) e = "film " + std::to_string(i) + ": " + e;
+    }
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    // every attempt is launched before any is waited for
+    int64_t issued = 0;
+    int rc = LLGMI_OK;
+    for (; issued < count; ++issued) {
+        llgmi_ctx *c = ctxs[issued];
+        rc = guarded(c, [&] {
+            with_engine(c, [&](auto &eng) {
+                using T = std::remove_pointer_t<decltype(eng.d_m)>;
+                eng.rk45_issue((T *)m_devs[issued], params[issued], mats[issued], field_mask);
+            });
+        });
+        if (rc != LLGMI_OK) break;
+    }
+    std::string first_err = rc != LLGMI_OK ? llgmi_last_error() : "";
+    for (int64_t i = 0; i < issued; ++i) {
+        llgmi_ctx *c = ctxs[i];
+        const bool finish = rc == LLGMI_OK;
+        const int r = guarded(c, [&] {
+            with_engine(c, [&](auto &eng) {
+                using T = std::remove_pointer_t<decltype(eng.d_m)>;
+                LLGMI_HIP(hipStreamSynchronize(eng.stream));
+                if (finish)
+                    eng.rk45_finish((T *)m_devs[i], params[i], field_mask, max_err, headroom, dt_min, dt_max,
+                                    eps[i], accepted[i], dt_used[i]);
+                else
+                    eng.issued_mask = -1;   // the batch failed: no film takes this attempt
+            });
+        });
+        if (r != LLGMI_OK && rc == LLGMI_OK) {
+            rc = r;
+            first_err = llgmi_last_error();
+        }
+    }
+    if (rc != LLGMI_OK) return fail(rc, first_err);
+    return LLGMI_OK;
 }
 
 int llgmi_dev_upload(llgmi_ctx *ctx, const double *host_aos, void *dev_soa) {

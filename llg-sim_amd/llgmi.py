@@ -142,6 +142,7 @@ EXPORTED = [
     "llgmi_add_demag_field_pbc", "llgmi_compute_demag_field_pbc", "llgmi_add_demag_field",
     "llgmi_rk45_step", "llgmi_rk45_step_adaptive", "llgmi_rk45_reset",
     "llgmi_dev_build_h_eff", "llgmi_dev_add_demag_field_pbc", "llgmi_dev_rk45_step",
+    "llgmi_dev_rk45_step_batch",
     "llgmi_dev_upload", "llgmi_dev_download", "llgmi_profile_enable", "llgmi_profile_reset",
     "llgmi_profile_read", "llgmi_mg_config_default", "llgmi_mg_config_from_env",
     "llgmi_ctx_set_mg_config", "llgmi_add_demag_field_poisson_mg",
@@ -395,6 +396,8 @@ def lib():
     L.llgmi_dev_add_demag_field_pbc.argtypes = [vp, vp, vp, M, C.c_uint64, C.c_uint64]
     L.llgmi_dev_rk45_step.argtypes = [vp, vp, P, M, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, dp, ip, dp]
+    L.llgmi_dev_rk45_step_batch.argtypes = [C.POINTER(vp), C.POINTER(vp), P, M, C.c_int64, C.c_int,
+                                            C.c_double, C.c_double, C.c_double, C.c_double, dp, ip, dp]
     L.llgmi_dev_upload.argtypes = [vp, dp, vp]
     L.llgmi_dev_download.argtypes = [vp, vp, dp]
     L.llgmi_profile_enable.argtypes = [vp, C.c_int]
@@ -458,6 +461,29 @@ def _ptr(x) -> Optional[int]:
     if isinstance(x, int):
         return x
     return int(x.data_ptr())
+
+
+def dev_rk45_step_batch(ctxs, m_devs, params, mats, mask: int, max_err: float, headroom: float,
+                        dt_min: float, dt_max: float):
+    """One adaptive RK45 attempt for each of len(ctxs) independent films
+    (llgmi_dev_rk45_step_batch): film i is ctxs[i] with device state m_devs[i],
+    params[i] (its dt is updated in place) and mats[i].  Returns the per-film
+    (eps, accepted, dt_used) tuples of step_llg_rk45_recompute_field_masked_adaptive."""
+    k = len(ctxs)
+    if not (len(m_devs) == len(params) == len(mats) == k):
+        raise ValueError("ctxs, m_devs, params and mats must have one entry per film")
+    hs = (C.c_void_p * max(k, 1))(*[c._h for c in ctxs])
+    ms = (C.c_void_p * max(k, 1))(*[_ptr(m) for m in m_devs])
+    ps = (_Params * max(k, 1))(*[p._c() for p in params])
+    cms = (_Material * max(k, 1))(*[m._c() for m in mats])
+    eps = np.zeros(max(k, 1))
+    dtu = np.zeros(max(k, 1))
+    acc = np.zeros(max(k, 1), dtype=np.intc)
+    _check(lib().llgmi_dev_rk45_step_batch(hs, ms, ps, cms, k, mask, max_err, headroom, dt_min, dt_max,
+                                           _dp(eps), acc.ctypes.data_as(C.POINTER(C.c_int)), _dp(dtu)))
+    for i, p in enumerate(params):
+        p.dt = ps[i].dt
+    return [(float(eps[i]), bool(acc[i]), float(dtu[i])) for i in range(k)]
 
 
 class Context:

@@ -154,3 +154,109 @@ def test_rk45_fp32_matches_fp64_over_steps(gpu_lib):
     r64 = _trajectory(lambda m, p: c64.step_llg_rk45_recompute_field_adaptive(m, p, gm, *args), m0.copy(), p64, 5e-11, 1e-11)
     r32 = _trajectory(lambda m, p: c32.step_llg_rk45_recompute_field_adaptive(m, p, gm, *args), m0.copy(), p32, 5e-11, 1e-11)
     assert np.abs(r32 - r64).max() <= 1e-4
+
+
+def _films(seed, k):
+    """k small independent films with their own sizes and materials."""
+    import llgmi as L
+    from initial_states import smoothed_random
+    rng = np.random.default_rng(seed)
+    films = []
+    for i in range(k):
+        nx, ny = int(rng.choice([8, 16, 24, 32])), int(rng.choice([8, 12, 16]))
+        cell = (float(rng.uniform(2e-9, 5e-9)), float(rng.uniform(2e-9, 5e-9)), 3e-9)
+        mat = L.Material(ms=float(rng.uniform(4e5, 1.2e6)), a_ex=float(rng.uniform(5e-12, 2e-11)),
+                         k_u=float(rng.uniform(0, 5e5)), dmi=float(rng.uniform(0, 4e-3)), demag=True)
+        par = L.LLGParams(alpha=float(rng.uniform(0.01, 0.5)), dt=1e-13, b_ext=(0.0, 0.0, 0.01 * i))
+        films.append((L.Grid2D(nx, ny, *cell), mat, par, smoothed_random(nx, ny, 100 + i).reshape(-1, 3)))
+    return films
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_rk45_batch_equals_films_stepped_alone(gpu_lib, prec):
+    """llgmi_dev_rk45_step_batch gives every film exactly (bitwise) what
+    llgmi_dev_rk45_step gives it alone: state, eps, accept and both dts."""
+    from conftest import DeviceBuffer
+    import copy
+    L = gpu_lib
+    P = L.F64 if prec == "f64" else L.F32
+    es = 8 if prec == "f64" else 4
+    films = _films(3, 6)
+    ctx_b = [L.Context(g, precision=P) for g, _, _, _ in films]
+    ctx_s = [L.Context(g, precision=P) for g, _, _, _ in films]
+    buf_b = [DeviceBuffer(3 * g.n_cells() * es) for g, _, _, _ in films]
+    buf_s = [DeviceBuffer(3 * g.n_cells() * es) for g, _, _, _ in films]
+    for (g, _, _, m0), cb, cs, bb, bs in zip(films, ctx_b, ctx_s, buf_b, buf_s):
+        cb.dev_upload(m0, bb)
+        cs.dev_upload(m0, bs)
+    pb = [copy.deepcopy(f[2]) for f in films]
+    ps = [copy.deepcopy(f[2]) for f in films]
+    pb[1].dt = ps[1].dt = 5e-11          # one film starts with a step that is rejected
+    mats = [f[1] for f in films]
+    n_acc = 0
+    for attempt in range(8):
+        res = L.dev_rk45_step_batch(ctx_b, buf_b, pb, mats, 2, 1e-5, 0.8, 1e-18, 1e-11)
+        for i in range(len(films)):
+            alone = ctx_s[i].dev_rk45_step(buf_s[i], ps[i], mats[i], 2, 1e-5, 0.8, 1e-18, 1e-11)
+            assert res[i] == alone, (attempt, i)
+            assert pb[i].dt == ps[i].dt
+            n_acc += res[i][1]
+            if attempt == 0 and i == 1:
+                assert not res[i][1]
+    assert n_acc > 30
+    for cb, cs, bb, bs in zip(ctx_b, ctx_s, buf_b, buf_s):
+        assert np.array_equal(cb.dev_download(bb), cs.dev_download(bs))
+
+
+def test_rk45_batch_films_match_oracle(gpu_lib):
+    """Per-film accuracy of the batch against the FP64 oracle (FP64 contexts,
+    the single-step bars: state 1e-12, eps and dt 1e-9 relative)."""
+    from conftest import DeviceBuffer
+    from oracle import oracle as O
+    L = gpu_lib
+    films = _films(11, 3)
+    ctxs = [L.Context(g, precision=L.F64) for g, _, _, _ in films]
+    bufs = [DeviceBuffer(3 * g.n_cells() * 8) for g, _, _, _ in films]
+    ocs, oms, ops, mos = [], [], [], []
+    for (g, mat, par, m0), c, b in zip(films, ctxs, bufs):
+        c.dev_upload(m0, b)
+        ocs.append(O.Context(O.Grid(g.nx, g.ny, g.dx, g.dy, g.dz)))
+        oms.append(O.Material(ms=mat.ms, a_ex=mat.a_ex, k_u=mat.k_u, dmi=mat.dmi, demag=True))
+        ops.append(O.Params(alpha=par.alpha, dt=par.dt, b_ext=par.b_ext))
+        mos.append(m0.copy())
+    pars = [f[2] for f in films]
+    for step in range(4):
+        res = L.dev_rk45_step_batch(ctxs, bufs, pars, [f[1] for f in films], 2, 1e-6, 0.8, 1e-19, 1e-11)
+        for i in range(len(films)):
+            eo, ao, uo = ocs[i].rk45_step(mos[i], ops[i], oms[i], 2, 1e-6, 0.8, 1e-19, 1e-11)
+            eg, ag, ug = res[i]
+            assert ag == ao and ug == uo, (step, i)
+            assert abs(eg - eo) <= 1e-9 * abs(eo) + 1e-30
+            assert abs(pars[i].dt - ops[i].dt) <= 1e-9 * ops[i].dt
+            pars[i].dt = ops[i].dt
+            assert np.abs(ctxs[i].dev_download(bufs[i]) - mos[i]).max() <= 1e-12, (step, i)
+
+
+def test_rk45_batch_rejects_bad_input_without_stepping(gpu_lib):
+    from conftest import DeviceBuffer
+    L = gpu_lib
+    films = _films(5, 3)
+    ctxs = [L.Context(g, precision=L.F64) for g, _, _, _ in films]
+    bufs = [DeviceBuffer(3 * g.n_cells() * 8) for g, _, _, _ in films]
+    for (g, _, _, m0), c, b in zip(films, ctxs, bufs):
+        c.dev_upload(m0, b)
+    pars = [f[2] for f in films]
+    mats = [f[1] for f in films]
+    with pytest.raises(L.LlgmiError, match="twice"):
+        L.dev_rk45_step_batch([ctxs[0], ctxs[1], ctxs[0]], bufs, pars, mats, 2, 1e-6, 0.8, 1e-19, 1e-11)
+    bad = list(mats)
+    bad[2] = L.Material(ms=8e5, a_ex=1e-11, demag_method=7)
+    with pytest.raises(L.LlgmiError, match="film 2"):
+        L.dev_rk45_step_batch(ctxs, bufs, pars, bad, 2, 1e-6, 0.8, 1e-19, 1e-11)
+    small = DeviceBuffer(8)
+    with pytest.raises(L.LlgmiError, match="film 1"):
+        L.dev_rk45_step_batch(ctxs, [bufs[0], small, bufs[2]], pars, mats, 2, 1e-6, 0.8, 1e-19, 1e-11)
+    assert [p.dt for p in pars] == [1e-13] * 3
+    for (g, _, _, m0), c, b in zip(films, ctxs, bufs):
+        assert np.array_equal(c.dev_download(b), m0)
+    assert L.dev_rk45_step_batch([], [], [], [], 2, 1e-6, 0.8, 1e-19, 1e-11) == []
