@@ -273,6 +273,37 @@ def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
     return out
 
 
+def warm_start_bench():
+    """The learned multigrid initial guess on the 512 x 512 film with holes:
+    V-cycles to tol_rel 1e-6 from the zero guess, the previous state's
+    potential, the network's potential and the previous potential plus the
+    network's change, along a relaxation the network was not trained on
+    (states 1 and 20 relaxation steps apart), with the network's MFMA rates.
+    Uses the committed weights (llg-sim_amd/weights); they are trained by
+    llg-sim_amd/warmstart.py."""
+    import warmstart as WS
+    if not os.path.exists(WS.DEFAULT_WEIGHTS):
+        return {"skipped": "no trained weights"}
+    params = np.load(WS.DEFAULT_WEIGHTS)
+    info = os.path.splitext(WS.DEFAULT_WEIGHTS)[0] + ".json"
+    prob = WS.Problem()
+    out = {"weights": os.path.relpath(WS.DEFAULT_WEIGHTS, REPO)}
+    if os.path.exists(info):
+        with open(info) as f:
+            d = json.load(f)
+        out["training"] = {k: d[k] for k in ("pairs", "train_steps", "batch", "record_s", "train_s",
+                                             "loss_first_100", "loss_last_100") if k in d}
+    names = ("zero", "previous", "learned", "previous_plus_learned_change")
+    for stride, states in ((1, 20), (20, 10)):
+        c = WS.compare(prob, params, states=states, stride=stride)
+        out[f"stride_{stride}"] = {"workload": c["workload"], "inference_ms": c["inference_ms"],
+                                   "max_dm_between_states": c["max_dm_between_states_mean"],
+                                   **{f"{n}_v_cycles": c[n]["v_cycles_mean"] for n in names},
+                                   **{f"{n}_ms": c[n]["ms_mean_incl_inference"] for n in names}}
+    out["mfma"] = WS.measure_throughput(prob, params)
+    return out
+
+
 def host_aos_rate(L, m_host, n=20):
     """Field evaluations/s through the host entry point (AoS f64 in and out)."""
     ctx = L.Context(L.Grid2D(NX, NY, *CELL), precision=L.F32)
@@ -299,6 +330,8 @@ def main():
     ap.add_argument("--no-mg", action="store_true", help="skip the multigrid leg")
     ap.add_argument("--mg-only", action="store_true",
                     help="run only the multigrid leg (GPU solves, no CPU oracle); for profilers")
+    ap.add_argument("--ws-only", action="store_true",
+                    help="run only the learned initial guess's forward / training-step rates; for profilers")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 256 MiB (x10) to scale HBM byte counters")
     args = ap.parse_args()
@@ -329,6 +362,11 @@ def main():
 
     if args.mg_only:
         print(json.dumps({"mg": mg_bench(L, cpu=False)}), flush=True)
+        return
+    if args.ws_only:
+        import warmstart as WS
+        prob = WS.Problem()
+        print(json.dumps({"mfma": WS.measure_throughput(prob, np.load(WS.DEFAULT_WEIGHTS))}), flush=True)
         return
 
     assert torch.cuda.is_available(), "no GPU visible to torch"
@@ -439,6 +477,10 @@ def main():
     if not args.no_mg:
         mg = mg_bench(L, cpu=not args.no_cpu)
         print(f"[bench] multigrid: {mg}", file=sys.stderr, flush=True)
+    ws = None
+    if not args.no_mg and not args.quick:
+        ws = warm_start_bench()
+        print(f"[bench] mg warm start: {ws}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(m_host)
@@ -474,6 +516,7 @@ def main():
         "host_aos_evals_per_s": host_rate,
         "mg": mg,
         "batched_films": films,
+        "mg_warm_start": ws,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

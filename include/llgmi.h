@@ -178,6 +178,10 @@ int llgmi_dev_rk45_step_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi
                               const llgmi_material *mats, int64_t count, int field_mask,
                               double max_err, double headroom, double dt_min, double dt_max,
                               double *eps, int *accepted, double *dt_used);
+/* m = 0 on the cells whose keep byte (device, n bytes) is 0 (synchronous):
+ * resets the holes of a geometry after a step whose stage normalisation
+ * (normalize(0) = +z, vec3.rs:21-28) filled them. */
+int llgmi_dev_mask_cells(llgmi_ctx *ctx, void *m_dev, const uint8_t *keep_dev);
 /* AoS f64 host <-> SoA device copies in the context precision (synchronous). */
 int llgmi_dev_upload(llgmi_ctx *ctx, const double *host_aos, void *dev_soa);
 int llgmi_dev_download(llgmi_ctx *ctx, const void *dev_soa, double *host_aos);
@@ -267,6 +271,57 @@ int llgmi_add_demag_field_poisson_mg(llgmi_ctx *ctx, const double *m, double *b_
 int llgmi_compute_demag_field_poisson_mg(llgmi_ctx *ctx, const double *m, double *out,
                                          const llgmi_material *mat);
 int llgmi_mg_last_stats(llgmi_ctx *ctx, llgmi_mg_stats *out);
+/* The finest multigrid box of the context's last solve: dims[0..5] =
+ * (px, py, pz, offx, offy, offz), film cell (i, j) at box (offx+i, offy+j,
+ * offz).  No reference counterpart: these three serve the learned initial
+ * guess below. */
+int llgmi_mg_box(llgmi_ctx *ctx, int64_t dims[6]);
+/* Copies the finest potential phi[z][y][x] (px*py*pz doubles, device memory)
+ * out of, or into, the solver.  With warm_start on, a solve starts from the
+ * phi written here (the boundary faces are restamped by the solve). */
+int llgmi_mg_phi_read(llgmi_ctx *ctx, double *phi_dev);
+int llgmi_mg_phi_write(llgmi_ctx *ctx, const double *phi_dev);
+
+/* ---- learned initial guess for the multigrid solve (no reference counterpart) ---
+ * A 2-D U-Net (4 levels, base_channels * {1,2,4,8} channels, bf16 weights and
+ * activations on MFMA with FP32 accumulation, FP32 master weights and Adam)
+ * mapping per-pixel inputs x[b][y][x][16] (bf16; (m_x, m_y, m_z, mask) in
+ * channels 0..3) to outputs y[b][y][x][c_pad] (FP32, c_pad = out_channels
+ * rounded up to 16); trained on the device.  The loss is the mean squared
+ * error over channels < out_channels and pixels x < valid_width,
+ * y < valid_height, against bf16 targets t[b][y][x][c_pad]. */
+typedef struct llgmi_ws llgmi_ws;
+typedef struct llgmi_ws_shape {
+    int64_t batch, height, width;       /* height, width multiples of 8 */
+    int32_t in_channels, out_channels;  /* 1..16, 1..256 */
+    int32_t base_channels, reserved;    /* 16 or 32 */
+    int64_t valid_width, valid_height;
+} llgmi_ws_shape;                       /* 48 bytes */
+int llgmi_ws_create(const llgmi_ws_shape *shape, uint64_t seed, int device, llgmi_ws **out);
+int llgmi_ws_destroy(llgmi_ws *ws);
+int llgmi_ws_param_count(llgmi_ws *ws, int64_t *n);
+/* FP32 parameters in layer order (enc 0..3 x 2, dec 2..0 x 2, head), each
+ * layer weights [tap][c_out][c_in] (tap = (dy+1)*3 + dx+1) then bias [c_out].
+ * set_params also resets the Adam state. */
+int llgmi_ws_get_params(llgmi_ws *ws, float *host);
+int llgmi_ws_set_params(llgmi_ws *ws, const float *host);
+int llgmi_ws_get_grads(llgmi_ws *ws, float *host);   /* of the last loss_grad */
+int llgmi_ws_forward(llgmi_ws *ws, const uint16_t *x_dev, float *y_dev);
+int llgmi_ws_loss_grad(llgmi_ws *ws, const uint16_t *x_dev, const uint16_t *t_dev, double *loss);
+int llgmi_ws_adam_step(llgmi_ws *ws, double lr);
+int llgmi_ws_train_step(llgmi_ws *ws, const uint16_t *x_dev, const uint16_t *t_dev, double lr,
+                        double *loss);
+/* Packing between the multigrid context and the network's tensors, for
+ * sample `sample` of a batch: the film's m (SoA, FP32 or FP64, nx x ny) at
+ * offset (offx, offy) of the plane; phi[z][y][x] of a px x py x pz box as
+ * target channels z (divided by scale) and back (multiplied by scale; with
+ * accumulate != 0 added to phi instead of overwriting it). */
+int llgmi_ws_pack_input(llgmi_ws *ws, int64_t sample, const void *m_dev, int m_is_f32,
+                        int64_t nx, int64_t ny, int64_t offx, int64_t offy, uint16_t *x_dev);
+int llgmi_ws_pack_phi(llgmi_ws *ws, int64_t sample, const double *phi_dev, int64_t px,
+                      int64_t py, int64_t pz, double scale, uint16_t *t_dev);
+int llgmi_ws_unpack_phi(llgmi_ws *ws, int64_t sample, const float *y_dev, int64_t px,
+                        int64_t py, int64_t pz, double scale, int accumulate, double *phi_dev);
 
 /* ---- relaxation (llg.rs:38-50, 196-436; energy.rs; relax.rs) -------------------- */
 /* TorqueMetric (relax.rs:32-38) */

@@ -27,6 +27,11 @@ struct Error : std::runtime_error {
 
 #define LLGMI_LAUNCH_CHECK() LLGMI_HIP(hipGetLastError())
 
+// Records msg as the library's last error (llgmi_last_error) and returns code.
+int set_last_error(int code, const std::string &msg);
+// "" when [p, p + bytes) lies in one device allocation, else the reason.
+std::string check_device_range(const void *p, size_t bytes, const char *what);
+
 // ---------------------------------------------------------------------------
 // Complex value of precision T (plain struct: 8 or 16 bytes, naturally aligned)
 // ---------------------------------------------------------------------------

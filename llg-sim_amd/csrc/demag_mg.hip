@@ -2023,6 +2023,14 @@ double MGSolver::finest_rhs_max() {
 }
 
 // solve_with_timing (:1947-2057)
+void MGSolver::phi_copy(double *dev, bool into) {
+    Level &L = lev_[0];
+    const size_t bytes = sizeof(double) * (size_t)L.nx * L.ny * L.nz;
+    if (into) LLGMI_HIP(hipMemcpyAsync(L.phi[L.cur], dev, bytes, hipMemcpyDeviceToDevice, stream_));
+    else LLGMI_HIP(hipMemcpyAsync(dev, L.phi[L.cur], bytes, hipMemcpyDeviceToDevice, stream_));
+    LLGMI_HIP(hipStreamSynchronize(stream_));
+}
+
 void MGSolver::solve() {
     Level &L = lev_[0];
     const size_t n0 = (size_t)L.nx * L.ny * L.nz;

@@ -109,6 +109,22 @@ void launch_soa_accumulate(const T *src, T *dst, int64_t count, hipStream_t s) {
     LLGMI_LAUNCH_CHECK();
 }
 
+// m = 0 on cells whose keep byte is 0 (SoA, n cells)
+template <typename T>
+__global__ void k_mask_cells(T *__restrict__ m, const uint8_t *__restrict__ keep, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !keep[i]) {
+        m[i] = T(0);
+        m[n + i] = T(0);
+        m[2 * n + i] = T(0);
+    }
+}
+template <typename T>
+void launch_mask_cells(T *m, const uint8_t *keep, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_mask_cells<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, keep, n);
+    LLGMI_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------
 // RK45 stage state: v = m; v = v + c_q k_q (q in order); normalize (zero -> +z)
 // ---------------------------------------------------------------------------
@@ -237,6 +253,7 @@ void launch_rk_err(const T *k1, const T *k3, const T *k4, const T *k5, const T *
     template void launch_soa_to_aos<T>(const T *, double *, int64_t, hipStream_t);           \
     template void launch_soa_add_to_aos<T>(const T *, double *, int64_t, hipStream_t);       \
     template void launch_soa_accumulate<T>(const T *, T *, int64_t, hipStream_t);            \
+    template void launch_mask_cells<T>(T *, const uint8_t *, int64_t, hipStream_t);          \
     template void launch_rk_stage<T>(const T *, const StageArgs<T> &, T *, int64_t, hipStream_t); \
     template void launch_rk_err<T>(const T *, const T *, const T *, const T *, const T *,    \
                                    const T *, int64_t, bool, double *, double *, hipStream_t);

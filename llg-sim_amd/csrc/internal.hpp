@@ -157,6 +157,8 @@ void launch_soa_add_to_aos(const T *soa, double *aos, int64_t n, hipStream_t s);
 // dst[i] += src[i], i < count
 template <typename T>
 void launch_soa_accumulate(const T *src, T *dst, int64_t count, hipStream_t s);
+template <typename T>
+void launch_mask_cells(T *m, const uint8_t *keep, int64_t n, hipStream_t s);
 
 // RK45 stage state: out = normalize(m + sum_q c[q] k[q]) applied in order q = 0..nk-1.
 template <typename T>

@@ -29,6 +29,13 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+}  // namespace
+
+// the other translation units report errors through the same last-error slot
+int llgmi::set_last_error(int code, const std::string &msg) { return fail(code, msg); }
+
+namespace {
+
 // LLG_DEMAG_METHOD, parsed once (demag.rs:35-44, params.rs:76-83).
 int env_demag_method() {
     static int v = [] {
@@ -887,6 +894,14 @@ std::string check_dev(const void *p, size_t bytes, const char *what) {
     return "";
 }
 
+}  // namespace
+
+std::string llgmi::check_device_range(const void *p, size_t bytes, const char *what) {
+    return check_dev(p, bytes, what);
+}
+
+namespace {
+
 template <typename F>
 int guarded(llgmi_ctx *ctx, F &&f) {
     if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
@@ -1185,6 +1200,21 @@ int llgmi_dev_rk45_step_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi
     return LLGMI_OK;
 }
 
+int llgmi_dev_mask_cells(llgmi_ctx *ctx, void *m_dev, const uint8_t *keep_dev) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    std::string e = check_dev(m_dev, 3 * n * elem_size(ctx), "m");
+    if (e.empty()) e = check_dev(keep_dev, n, "keep");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            launch_mask_cells<T>((T *)m_dev, keep_dev, eng.n, eng.stream);
+            LLGMI_HIP(hipStreamSynchronize(eng.stream));
+        });
+    });
+}
+
 int llgmi_dev_upload(llgmi_ctx *ctx, const double *host_aos, void *dev_soa) {
     if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
     if (!host_aos) return fail(LLGMI_ERR_INVALID, "host buffer is NULL");
@@ -1289,6 +1319,42 @@ int llgmi_mg_last_stats(llgmi_ctx *ctx, llgmi_mg_stats *out) {
         with_engine(ctx, [&](auto &eng) {
             if (!eng.mg) throw Error(LLGMI_ERR_INVALID, "no multigrid solve has run on this context");
             *out = eng.mg->stats();
+        });
+    });
+}
+
+int llgmi_mg_box(llgmi_ctx *ctx, int64_t dims[6]) {
+    if (!dims) return fail(LLGMI_ERR_INVALID, "dims is NULL");
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            if (!eng.mg) throw Error(LLGMI_ERR_INVALID, "no multigrid solve has run on this context");
+            eng.mg->box(dims);
+        });
+    });
+}
+
+int llgmi_mg_phi_read(llgmi_ctx *ctx, double *phi_dev) {
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            if (!eng.mg) throw Error(LLGMI_ERR_INVALID, "no multigrid solve has run on this context");
+            int64_t d[6];
+            eng.mg->box(d);
+            const std::string e = check_dev(phi_dev, sizeof(double) * (size_t)(d[0] * d[1] * d[2]), "phi");
+            if (!e.empty()) throw Error(LLGMI_ERR_INVALID, e);
+            eng.mg->phi_copy(phi_dev, false);
+        });
+    });
+}
+
+int llgmi_mg_phi_write(llgmi_ctx *ctx, const double *phi_dev) {
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            if (!eng.mg) throw Error(LLGMI_ERR_INVALID, "no multigrid solve has run on this context");
+            int64_t d[6];
+            eng.mg->box(d);
+            const std::string e = check_dev(phi_dev, sizeof(double) * (size_t)(d[0] * d[1] * d[2]), "phi");
+            if (!e.empty()) throw Error(LLGMI_ERR_INVALID, e);
+            eng.mg->phi_copy(const_cast<double *>(phi_dev), true);
         });
     });
 }

@@ -51,6 +51,12 @@ public:
     void demag_field(const T *m, double ms, T *bd, double sigma = 0.0, bool masked = true);
     const llgmi_mg_stats &stats() const { return stats_; }
     const llgmi_mg_config &config() const { return cfg_; }
+    // finest box (px, py, pz, offx, offy, offz)
+    void box(int64_t d[6]) const {
+        d[0] = px_; d[1] = py_; d[2] = pz_; d[3] = offx_; d[4] = offy_; d[5] = offz_;
+    }
+    // device copy of the finest phi out of (into = false) or into the solver
+    void phi_copy(double *dev, bool into);
 
     struct Level {
         int nx = 0, ny = 0, nz = 0;

@@ -154,7 +154,20 @@ EXPORTED = [
     "llgmi_dev_write_ovf2", "llgmi_ovf_total_sim_time_line", "llgmi_demag_cache_path",
     "llgmi_demag_cache_save", "llgmi_demag_cache_load", "llgmi_mg_hybrid_config_default",
     "llgmi_mg_hybrid_config_from_env", "llgmi_ctx_set_mg_hybrid_config",
+    "llgmi_mg_box", "llgmi_mg_phi_read", "llgmi_mg_phi_write", "llgmi_dev_mask_cells",
+    "llgmi_ws_create", "llgmi_ws_destroy", "llgmi_ws_param_count", "llgmi_ws_get_params",
+    "llgmi_ws_set_params", "llgmi_ws_get_grads", "llgmi_ws_forward", "llgmi_ws_loss_grad",
+    "llgmi_ws_adam_step", "llgmi_ws_train_step", "llgmi_ws_pack_input", "llgmi_ws_pack_phi",
+    "llgmi_ws_unpack_phi",
 ]
+
+
+class WSShape(C.Structure):
+    """llgmi_ws_shape (48 bytes)."""
+    _fields_ = [("batch", C.c_int64), ("height", C.c_int64), ("width", C.c_int64),
+                ("in_channels", C.c_int32), ("out_channels", C.c_int32),
+                ("base_channels", C.c_int32), ("reserved", C.c_int32),
+                ("valid_width", C.c_int64), ("valid_height", C.c_int64)]
 
 
 class MGConfig(C.Structure):
@@ -432,6 +445,26 @@ def lib():
     L.llgmi_demag_cache_path.argtypes = [G, C.c_uint64, C.c_uint64, C.c_char_p, C.c_size_t]
     L.llgmi_demag_cache_save.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_char_p]
     L.llgmi_demag_cache_load.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_char_p, ip]
+    L.llgmi_mg_box.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.llgmi_mg_phi_read.argtypes = [vp, vp]
+    L.llgmi_mg_phi_write.argtypes = [vp, vp]
+    fp = C.POINTER(C.c_float)
+    L.llgmi_ws_create.argtypes = [C.POINTER(WSShape), C.c_uint64, C.c_int, C.POINTER(vp)]
+    L.llgmi_ws_destroy.argtypes = [vp]
+    L.llgmi_ws_param_count.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.llgmi_ws_get_params.argtypes = [vp, fp]
+    L.llgmi_ws_set_params.argtypes = [vp, fp]
+    L.llgmi_ws_get_grads.argtypes = [vp, fp]
+    L.llgmi_ws_forward.argtypes = [vp, vp, vp]
+    L.llgmi_ws_loss_grad.argtypes = [vp, vp, vp, dp]
+    L.llgmi_ws_adam_step.argtypes = [vp, C.c_double]
+    L.llgmi_ws_train_step.argtypes = [vp, vp, vp, C.c_double, dp]
+    L.llgmi_ws_pack_input.argtypes = [vp, C.c_int64, vp, C.c_int, C.c_int64, C.c_int64, C.c_int64,
+                                      C.c_int64, vp]
+    L.llgmi_ws_pack_phi.argtypes = [vp, C.c_int64, vp, C.c_int64, C.c_int64, C.c_int64, C.c_double, vp]
+    L.llgmi_ws_unpack_phi.argtypes = [vp, C.c_int64, vp, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int,
+                                      vp]
+    L.llgmi_dev_mask_cells.argtypes = [vp, vp, vp]
     _lib = L
     return L
 
@@ -461,6 +494,164 @@ def _ptr(x) -> Optional[int]:
     if isinstance(x, int):
         return x
     return int(x.data_ptr())
+
+
+class DeviceArray:
+    """Device memory from the HIP runtime the library runs on (hipMalloc
+    through ctypes): buffers for the llgmi_dev_* / llgmi_ws_* entry points
+    without a second runtime in the process.  Copies are synchronous."""
+
+    _hip = None
+
+    @classmethod
+    def _rt(cls):
+        if cls._hip is None:
+            lib()                                  # loads libamdhip64 with the library
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        rc = self._rt().hipMalloc(C.byref(p), int(nbytes))
+        if rc != 0:
+            raise LlgmiError(f"hipMalloc({nbytes}) failed: {rc}")
+        self.ptr, self.nbytes = int(p.value), int(nbytes)
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def at(self, offset: int) -> int:
+        """Device address offset bytes into the buffer."""
+        if not 0 <= offset <= self.nbytes:
+            raise ValueError("offset outside the buffer")
+        return self.ptr + int(offset)
+
+    def _copy(self, dst: int, src: int, n: int, kind: int) -> None:
+        rc = self._rt().hipMemcpy(dst, src, int(n), kind)
+        if rc == 0 and kind != 2:
+            # host-to-device copies from pageable memory (and device-to-device
+            # copies) can return before the data has landed; the library's
+            # streams do not wait for the null stream, so finish them here
+            rc = self._rt().hipDeviceSynchronize()
+        if rc != 0:
+            raise LlgmiError(f"hipMemcpy failed: {rc}")
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(a)
+        if offset + a.nbytes > self.nbytes:
+            raise ValueError("upload beyond the buffer")
+        self._copy(self.ptr + int(offset), a.ctypes.data, a.nbytes, 1)
+
+    def download(self, dtype, count: int, offset: int = 0) -> np.ndarray:
+        out = np.empty(int(count), dtype=dtype)
+        if offset + out.nbytes > self.nbytes:
+            raise ValueError("download beyond the buffer")
+        self._copy(out.ctypes.data, self.ptr + int(offset), out.nbytes, 2)
+        return out
+
+    def copy_from(self, src: "DeviceArray", nbytes: int, src_offset: int = 0, offset: int = 0) -> None:
+        if src_offset + nbytes > src.nbytes or offset + nbytes > self.nbytes:
+            raise ValueError("copy beyond a buffer")
+        self._copy(self.ptr + int(offset), src.ptr + int(src_offset), nbytes, 3)
+
+    def zero(self) -> None:
+        rc = self._rt().hipMemset(self.ptr, 0, self.nbytes)
+        if rc == 0:
+            rc = self._rt().hipDeviceSynchronize()
+        if rc != 0:
+            raise LlgmiError(f"hipMemset failed: {rc}")
+
+    def close(self) -> None:
+        if self.ptr:
+            self._rt().hipFree(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WarmStartNet:
+    """The learned initial guess of the multigrid solve (llgmi_ws_*): a U-Net
+    trained on the device.  Inputs x: bf16 [batch, H, W, 16]; targets: bf16
+    [batch, H, W, c_pad]; outputs: FP32 [batch, H, W, c_pad] (device buffers,
+    e.g. torch tensors of dtype bfloat16 / float32)."""
+
+    def __init__(self, batch: int, height: int, width: int, out_channels: int, base_channels: int = 32,
+                 in_channels: int = 4, valid_width: Optional[int] = None, valid_height: Optional[int] = None,
+                 seed: int = 1, device: int = 0):
+        self.shape = WSShape(batch, height, width, in_channels, out_channels, base_channels, 0,
+                             valid_width or width, valid_height or height)
+        self.c_pad = (out_channels + 15) // 16 * 16
+        self._h = C.c_void_p()
+        _check(lib().llgmi_ws_create(C.byref(self.shape), seed, device, C.byref(self._h)))
+        n = C.c_int64()
+        _check(lib().llgmi_ws_param_count(self._h, C.byref(n)))
+        self.n_params = int(n.value)
+
+    def close(self) -> None:
+        if self._h:
+            lib().llgmi_ws_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        h = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib().llgmi_ws_get_params(self._h, h.ctypes.data_as(C.POINTER(C.c_float))))
+        return h
+
+    def set_params(self, p: np.ndarray) -> None:
+        h = np.ascontiguousarray(p, dtype=np.float32)
+        if h.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {h.size}")
+        _check(lib().llgmi_ws_set_params(self._h, h.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def get_grads(self) -> np.ndarray:
+        h = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib().llgmi_ws_get_grads(self._h, h.ctypes.data_as(C.POINTER(C.c_float))))
+        return h
+
+    def forward(self, x_dev, y_dev) -> None:
+        _check(lib().llgmi_ws_forward(self._h, _ptr(x_dev), _ptr(y_dev)))
+
+    def loss_grad(self, x_dev, t_dev) -> float:
+        loss = C.c_double()
+        _check(lib().llgmi_ws_loss_grad(self._h, _ptr(x_dev), _ptr(t_dev), C.byref(loss)))
+        return loss.value
+
+    def adam_step(self, lr: float) -> None:
+        _check(lib().llgmi_ws_adam_step(self._h, lr))
+
+    def train_step(self, x_dev, t_dev, lr: float) -> float:
+        loss = C.c_double()
+        _check(lib().llgmi_ws_train_step(self._h, _ptr(x_dev), _ptr(t_dev), lr, C.byref(loss)))
+        return loss.value
+
+    def pack_input(self, sample: int, m_dev, m_is_f32: bool, nx: int, ny: int, offx: int, offy: int,
+                   x_dev) -> None:
+        _check(lib().llgmi_ws_pack_input(self._h, sample, _ptr(m_dev), int(m_is_f32), nx, ny, offx, offy,
+                                         _ptr(x_dev)))
+
+    def pack_phi(self, sample: int, phi_dev, px: int, py: int, pz: int, scale: float, t_dev) -> None:
+        _check(lib().llgmi_ws_pack_phi(self._h, sample, _ptr(phi_dev), px, py, pz, scale, _ptr(t_dev)))
+
+    def unpack_phi(self, sample: int, y_dev, px: int, py: int, pz: int, scale: float, phi_dev,
+                   accumulate: bool = False) -> None:
+        _check(lib().llgmi_ws_unpack_phi(self._h, sample, _ptr(y_dev), px, py, pz, scale, int(accumulate),
+                                         _ptr(phi_dev)))
 
 
 def dev_rk45_step_batch(ctxs, m_devs, params, mats, mask: int, max_err: float, headroom: float,
@@ -581,6 +772,18 @@ class Context:
         st = MGStats()
         _check(lib().llgmi_mg_last_stats(self._h, C.byref(st)))
         return st.as_dict()
+
+    def mg_box(self) -> Tuple[int, int, int, int, int, int]:
+        """(px, py, pz, offx, offy, offz) of the finest multigrid box."""
+        d = (C.c_int64 * 6)()
+        _check(lib().llgmi_mg_box(self._h, d))
+        return tuple(int(v) for v in d)
+
+    def mg_phi_read(self, phi_dev) -> None:
+        _check(lib().llgmi_mg_phi_read(self._h, _ptr(phi_dev)))
+
+    def mg_phi_write(self, phi_dev) -> None:
+        _check(lib().llgmi_mg_phi_write(self._h, _ptr(phi_dev)))
 
     def step_llg_rk45_recompute_field_masked_adaptive(self, m: np.ndarray, params: LLGParams,
                                                       mat: Material, mask: int, max_err: float,
@@ -726,6 +929,10 @@ class Context:
                                          C.byref(acc), C.byref(dtu)))
         params.dt = p.dt
         return eps.value, bool(acc.value), dtu.value
+
+    def dev_mask_cells(self, m_dev, keep_dev) -> None:
+        """m = 0 where the keep bytes (device, one per cell) are 0."""
+        _check(lib().llgmi_dev_mask_cells(self._h, _ptr(m_dev), _ptr(keep_dev)))
 
     def dev_upload(self, host_aos, dev_soa) -> None:
         h = _f64(host_aos).reshape(-1)

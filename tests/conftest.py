@@ -60,6 +60,42 @@ class DeviceBuffer:
     def data_ptr(self) -> int:
         return self.ptr
 
+    def upload(self, a, offset: int = 0) -> None:
+        """Synchronous host-to-device copy of a numpy array at byte offset."""
+        import ctypes as C
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        offset = int(offset)
+        assert offset + a.nbytes <= self.nbytes
+        DeviceBuffer._hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        rc = DeviceBuffer._hip.hipMemcpy(self.ptr + offset, a.ctypes.data, a.nbytes, 1)
+        assert rc == 0, rc
+        # a pageable upload can return before the DMA lands; the library's
+        # streams do not wait for the null stream
+        assert DeviceBuffer._hip.hipDeviceSynchronize() == 0
+
+    def download(self, dtype, count: int, offset: int = 0):
+        """Synchronous device-to-host copy of count elements of dtype."""
+        import ctypes as C
+        import numpy as np
+        out = np.empty(int(count), dtype=dtype)
+        offset = int(offset)
+        assert offset + out.nbytes <= self.nbytes
+        DeviceBuffer._hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        rc = DeviceBuffer._hip.hipMemcpy(out.ctypes.data, self.ptr + offset, out.nbytes, 2)
+        assert rc == 0, rc
+        return out
+
+    def copy_from(self, src: "DeviceBuffer", nbytes: int, src_offset: int = 0, offset: int = 0) -> None:
+        """Synchronous device-to-device copy."""
+        import ctypes as C
+        nbytes, src_offset, offset = int(nbytes), int(src_offset), int(offset)
+        assert src_offset + nbytes <= src.nbytes and offset + nbytes <= self.nbytes
+        DeviceBuffer._hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        rc = DeviceBuffer._hip.hipMemcpy(self.ptr + offset, src.ptr + src_offset, nbytes, 3)
+        assert rc == 0, rc
+        assert DeviceBuffer._hip.hipDeviceSynchronize() == 0
+
     def close(self) -> None:
         if self.ptr:
             DeviceBuffer._hip.hipFree(self.ptr)
