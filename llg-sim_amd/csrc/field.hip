@@ -136,7 +136,7 @@ k_rk_stage(const T *__restrict__ m, StageArgs<T> a, T *__restrict__ out, int64_t
     T v0 = m[i], v1 = m[n + i], v2 = m[2 * n + i];
     for (int q = 0; q < a.nk; ++q) {
         const T *k = a.k[q];
-        T c = a.c[q];
+        const T c = a.dt ? T(*a.dt * a.a[q]) : a.c[q];
         v0 = v0 + c * k[i];
         v1 = v1 + c * k[n + i];
         v2 = v2 + c * k[2 * n + i];

@@ -166,6 +166,10 @@ struct StageArgs {
     const T *k[6];
     T c[6];
     int nk;
+    // with dt non-null the coefficients are T(*dt * a[q]) (the same double
+    // product the host forms for c[q]), so a captured launch follows the step size
+    const double *dt;
+    double a[6];
 };
 template <typename T>
 void launch_rk_stage(const T *m, const StageArgs<T> &a, T *out, int64_t n, hipStream_t s);

@@ -100,6 +100,17 @@ struct Engine : EngineBase {
     bool fsal_valid = false;
     int last_mask = -1;
     int issued_mask = -1;    // mask of the attempt rk45_issue left for rk45_finish
+    // HIP-graph replay of RK45 attempts: one graph per (FSAL used, k1/k7
+    // swapped), keyed by what the captured launches read from the host;
+    // `seen` is the configuration run eagerly once before it is captured
+    struct RkGraph {
+        hipGraphExec_t exec = nullptr;
+        std::string key, seen;
+    };
+    RkGraph rk_graph[2][2];
+    bool k_swapped = false;
+    uint64_t fft_gen = 0;                      // bumped whenever an FFT operator is (re)built
+    double *d_dt = nullptr, *h_dt = nullptr;   // the attempt's step size (pinned host -> device)
     double last_bext[3] = {0, 0, 0};
 
     // RK23 relax scratch (RK23Scratch, llg.rs:196-268): k1..k4, b4 and m4 of
@@ -126,6 +137,11 @@ struct Engine : EngineBase {
             if (d_r23k[i]) (void)hipFree(d_r23k[i]);
         if (h_err) (void)hipHostFree(h_err);
         if (h_red) (void)hipHostFree(h_red);
+        if (h_dt) (void)hipHostFree(h_dt);
+        if (d_dt) (void)hipFree(d_dt);
+        for (auto &row : rk_graph)
+            for (auto &G : row)
+                if (G.exec) (void)hipGraphExecDestroy(G.exec);
     }
 
     template <typename U>
@@ -147,6 +163,7 @@ struct Engine : EngineBase {
         return e && std::string(e) == "1";
     }
     void build_fft(DemagFFT<T> &op, int64_t pbc_x, int64_t pbc_y) {
+        ++fft_gen;   // captured RK45 attempts name the operator's buffers
         std::vector<double> base;
         const std::string path = kernel_cache_path(g, pbc_x, pbc_y);
         const bool cached = auto_cache() && read_kernel_cache(path, g, pbc_x, pbc_y, base);
@@ -436,7 +453,9 @@ struct Engine : EngineBase {
         alloc(d_b, 3 * n);
         alloc(d_partial, kErrBlocks);
         alloc(d_err, 1);
+        alloc(d_dt, 1);
         if (!h_err) LLGMI_HIP(hipHostMalloc((void **)&h_err, sizeof(double), hipHostMallocDefault));
+        if (!h_dt) LLGMI_HIP(hipHostMalloc((void **)&h_dt, sizeof(double), hipHostMallocDefault));
     }
 
     // step_llg_rk45_recompute_field_masked_adaptive (llg.rs:795-1026) on device state m.
@@ -453,6 +472,61 @@ struct Engine : EngineBase {
     // attempts of many contexts (one stream each) run side by side; p is read,
     // not changed.
     void rk45_issue(T *m, const llgmi_params &p, const llgmi_material &mat, int mask) {
+        ensure_rk();
+        bool fsal_ok = fsal_valid && last_mask == mask && last_bext[0] == p.b_ext[0] &&
+                       last_bext[1] == p.b_ext[1] && last_bext[2] == p.b_ext[2];
+        if (fsal_ok) {
+            std::swap(d_k[1], d_k[7]);
+            k_swapped = !k_swapped;
+        }
+        fsal_valid = false;   // set again by rk45_finish on accept; an attempt cut short leaves it off
+        const bool demag = mask == LLGMI_MASK_FULL && mat.demag;
+        const bool capturable = (!demag || resolved_demag_method(mat) == LLGMI_DEMAG_FFT) && prof->mask == 0 &&
+                                rk_graphs_enabled();
+        if (!capturable) {
+            rk45_launches(m, p, mat, mask, fsal_ok, nullptr);
+        } else {
+            // one eager attempt per launch configuration (lazy set-up happens
+            // there), then capture on the next and replay after that
+            RkGraph &G = rk_graph[fsal_ok][k_swapped];
+            const std::string key = rk_graph_key(m, p, mat, mask);
+            *h_dt = p.dt;
+            LLGMI_HIP(hipMemcpyAsync(d_dt, h_dt, sizeof(double), hipMemcpyHostToDevice, stream));
+            if (G.exec && G.key == key) {
+                LLGMI_HIP(hipGraphLaunch(G.exec, stream));
+            } else if (G.seen == key) {
+                if (G.exec) (void)hipGraphExecDestroy(G.exec);
+                G.exec = nullptr;
+                G.key.clear();
+                hipGraph_t graph = nullptr;
+                LLGMI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    rk45_launches(m, p, mat, mask, fsal_ok, d_dt);
+                } catch (...) {
+                    (void)hipStreamEndCapture(stream, &graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                    throw;
+                }
+                LLGMI_HIP(hipStreamEndCapture(stream, &graph));
+                const hipError_t e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                LLGMI_HIP(e);
+                G.key = key;
+                LLGMI_HIP(hipGraphLaunch(G.exec, stream));
+            } else {
+                G.seen = key;
+                rk45_launches(m, p, mat, mask, fsal_ok, d_dt);
+            }
+        }
+        issued_mask = mask;
+    }
+
+    // The launches of one attempt: stages 2..7 with their field evaluations
+    // (stage 1 too without FSAL), the FP64 error norm and the copy of
+    // max|err| into pinned memory.  With dt_dev the stage coefficients are
+    // formed on the device from the step size there (for graph replay).
+    void rk45_launches(T *m, const llgmi_params &p, const llgmi_material &mat, int mask, bool fsal_ok,
+                       const double *dt_dev) {
         static const double A[7][5] = {
             {0}, {0},
             {1.0 / 5.0},
@@ -463,25 +537,23 @@ struct Engine : EngineBase {
         };
         static const double Bw[5] = {35.0 / 384.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0};
         static const int Bk[5] = {1, 3, 4, 5, 6};
-        ensure_rk();
         const double dt0 = p.dt;
-        bool fsal_ok = fsal_valid && last_mask == mask && last_bext[0] == p.b_ext[0] &&
-                       last_bext[1] == p.b_ext[1] && last_bext[2] == p.b_ext[2];
-        if (fsal_ok) std::swap(d_k[1], d_k[7]);
-        fsal_valid = false;   // set again by rk45_finish on accept; an attempt cut short leaves it off
         if (!fsal_ok) field(m, d_b, p, mat, mask, nullptr, d_k[1]);
         for (int s = 2; s <= 7; ++s) {
             StageArgs<T> sa{};
+            sa.dt = dt_dev;
             if (s < 7) {
                 sa.nk = s - 1;
                 for (int j = 1; j < s; ++j) {
                     sa.k[j - 1] = d_k[j];
+                    sa.a[j - 1] = A[s][j - 1];
                     sa.c[j - 1] = T(dt0 * A[s][j - 1]);
                 }
             } else {
                 sa.nk = 5;
                 for (int q = 0; q < 5; ++q) {
                     sa.k[q] = d_k[Bk[q]];
+                    sa.a[q] = Bw[q];
                     sa.c[q] = T(dt0 * Bw[q]);
                 }
             }
@@ -498,7 +570,30 @@ struct Engine : EngineBase {
                              sizeof(T) == sizeof(float), d_partial, d_err, stream);
         }
         LLGMI_HIP(hipMemcpyAsync(h_err, d_err, sizeof(double), hipMemcpyDeviceToHost, stream));
-        issued_mask = mask;
+    }
+
+    static bool rk_graphs_enabled() {
+        const char *e = std::getenv("LLGMI_RK45_GRAPH");
+        return !(e && e[0] == '0');
+    }
+    // everything a captured attempt reads from the host besides the step size
+    std::string rk_graph_key(const T *m, const llgmi_params &p, const llgmi_material &mat, int mask) const {
+        std::string k;
+        auto put = [&](const void *v, size_t b) { k.append((const char *)v, b); };
+        put(&m, sizeof m);
+        put(&mask, sizeof mask);
+        put(&p.gamma, sizeof p.gamma);
+        put(&p.alpha, sizeof p.alpha);
+        put(p.b_ext, sizeof p.b_ext);
+        put(&mat, sizeof mat);
+        put(d_k, sizeof d_k);
+        put(&d_stage, sizeof d_stage);
+        put(&d_m7, sizeof d_m7);
+        put(&d_b, sizeof d_b);
+        const int method = resolved_demag_method(mat);
+        put(&method, sizeof method);
+        put(&fft_gen, sizeof fft_gen);
+        return k;
     }
 
     // Second half, after the stream has completed rk45_issue's work: the

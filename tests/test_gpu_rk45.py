@@ -260,3 +260,38 @@ def test_rk45_batch_rejects_bad_input_without_stepping(gpu_lib):
     for (g, _, _, m0), c, b in zip(films, ctxs, bufs):
         assert np.array_equal(c.dev_download(b), m0)
     assert L.dev_rk45_step_batch([], [], [], [], 2, 1e-6, 0.8, 1e-19, 1e-11) == []
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_rk45_graph_replay_equals_eager(gpu_lib, prec, monkeypatch):
+    """Attempts replayed from captured HIP graphs (the default on the FFT
+    path) against the same attempts launched eagerly (LLGMI_RK45_GRAPH=0),
+    step by step: state, eps, accept and dt bitwise equal over 12 attempts,
+    including a rejected one and a B_ext change (new capture), with the two
+    contexts' eager work and synchronisations interleaved between replays."""
+    from conftest import DeviceBuffer
+    L = gpu_lib
+    P, es = (L.F32, 4) if prec == "f32" else (L.F64, 8)
+    nx, ny, cell = 48, 32, (3e-9, 3e-9, 2e-9)
+    m0, _, mat = _setup(nx, ny, 17, dmi=2e-3, k_u=3e4)
+    cg = L.Context(L.Grid2D(nx, ny, *cell), precision=P)
+    ce = L.Context(L.Grid2D(nx, ny, *cell), precision=P)
+    bg, be = DeviceBuffer(3 * nx * ny * es), DeviceBuffer(3 * nx * ny * es)
+    cg.dev_upload(m0, bg)
+    ce.dev_upload(m0, be)
+    pg = L.LLGParams(alpha=0.1, dt=1e-13, b_ext=(0.0, 0.0, 0.02))
+    pe = L.LLGParams(alpha=0.1, dt=1e-13, b_ext=(0.0, 0.0, 0.02))
+    for step in range(12):
+        if step == 5:
+            pg.dt = pe.dt = 4e-11                # rejected
+        if step == 8:
+            pg.b_ext = pe.b_ext = (0.01, 0.0, 0.02)
+        monkeypatch.delenv("LLGMI_RK45_GRAPH", raising=False)
+        rg = cg.dev_rk45_step(bg, pg, mat, 2, 1e-5, 0.8, 1e-18, 1e-11)
+        monkeypatch.setenv("LLGMI_RK45_GRAPH", "0")
+        re_ = ce.dev_rk45_step(be, pe, mat, 2, 1e-5, 0.8, 1e-18, 1e-11)
+        assert rg == re_, step
+        assert pg.dt == pe.dt
+        assert np.array_equal(cg.dev_download(bg), ce.dev_download(be)), step
+        if step == 5:
+            assert not rg[1]
