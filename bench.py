@@ -276,8 +276,10 @@ def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
 def warm_start_bench():
     """The learned multigrid initial guess on the 512 x 512 film with holes:
     V-cycles to tol_rel 1e-6 from the zero guess, the previous state's
-    potential, the network's potential and the previous potential plus the
-    network's change, along a relaxation the network was not trained on
+    potential, the network's potential, the previous potential plus the
+    network's change, and the previous potential plus the difference
+    network's potential of m - m_prev, along a relaxation the networks were
+    not trained on
     (states 1 and 20 relaxation steps apart), with the network's MFMA rates.
     Uses the committed weights (llg-sim_amd/weights); they are trained by
     llg-sim_amd/warmstart.py."""
@@ -293,9 +295,11 @@ def warm_start_bench():
             d = json.load(f)
         out["training"] = {k: d[k] for k in ("pairs", "train_steps", "batch", "record_s", "train_s",
                                              "loss_first_100", "loss_last_100") if k in d}
-    names = ("zero", "previous", "learned", "previous_plus_learned_change")
+    p_delta = np.load(WS.DELTA_WEIGHTS) if os.path.exists(WS.DELTA_WEIGHTS) else None
+    names = ("zero", "previous", "learned", "previous_plus_learned_change") + \
+        (("previous_plus_learned_delta",) if p_delta is not None else ())
     for stride, states in ((1, 20), (20, 10)):
-        c = WS.compare(prob, params, states=states, stride=stride)
+        c = WS.compare(prob, params, states=states, stride=stride, params_delta=p_delta)
         out[f"stride_{stride}"] = {"workload": c["workload"], "inference_ms": c["inference_ms"],
                                    "max_dm_between_states": c["max_dm_between_states_mean"],
                                    **{f"{n}_v_cycles": c[n]["v_cycles_mean"] for n in names},

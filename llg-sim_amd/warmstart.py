@@ -54,6 +54,7 @@ SOLVER = dict(stencil=0, prolong=0, coarse_op=0, smoother=1, bc=2, v_cycles_max=
 TOL_RECORD = 1e-8
 TOL_SOLVE = 1e-6
 DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "mg_warmstart_512_holes.npy")
+DELTA_WEIGHTS = os.path.join(HERE, "weights", "mg_warmstart_512_holes_delta.npy")
 
 
 class Problem:
@@ -67,6 +68,7 @@ class Problem:
         self.mat_dyn = L.Material(ms=MS, a_ex=A_EX, demag=True, demag_method=L.DemagMethod.FftUniform)
         self.m_dyn = L.DeviceArray(3 * nx * ny * 8)
         self.m = L.DeviceArray(3 * nx * ny * 8)
+        self.dm = L.DeviceArray(3 * nx * ny * 8)       # a normalised state difference
         self.b = L.DeviceArray(3 * nx * ny * 8)
         self.keep = L.DeviceArray(nx * ny)
         self.film(0, 96e-9)
@@ -81,11 +83,11 @@ class Problem:
     def set_tol(self, tol: float, warm: bool) -> None:
         self.ctx.set_mg_config(L.mg_config(warm_start=int(warm), tol_rel=tol, v_cycles=1, **SOLVER))
 
-    def solve(self) -> int:
-        """MG demag of the current state (the solver's phi is the initial
-        guess when warm starts are on); returns the V-cycles taken."""
+    def solve(self, m=None) -> int:
+        """MG demag of the current state (or of m; the solver's phi is the
+        initial guess when warm starts are on); returns the V-cycles taken."""
         self.b.zero()
-        self.ctx.dev_add_demag_field_pbc(self.m, self.b, self.mat)
+        self.ctx.dev_add_demag_field_pbc(self.m if m is None else m, self.b, self.mat)
         self.ctx.synchronize()
         return int(self.ctx.mg_last_stats()["cycles"])
 
@@ -173,8 +175,22 @@ class Dataset:
         self.t = L.DeviceArray(n * self.st)
 
 
-def record(prob: Problem, runs: int, steps: int, seed0: int = 0, pitch_range=(64e-9, 128e-9), log=print):
-    """Training pairs along `runs` relaxations of `steps` steps each."""
+def set_delta(prob: Problem, m_now: np.ndarray, m_prev: np.ndarray) -> float:
+    """prob.dm = (m_now - m_prev) / s with s the RMS of m_now - m_prev over
+    the components (changes concentrate in walls: the RMS, not the maximum,
+    keeps typical inputs of order one); returns s."""
+    d = m_now - m_prev
+    s = float(np.sqrt(np.mean(d * d)))
+    prob.ctx.dev_upload(d / s if s > 0 else d, prob.dm)
+    return s
+
+
+def record(prob: Problem, runs: int, steps: int, seed0: int = 0, pitch_range=(64e-9, 128e-9), log=print,
+           delta: bool = False):
+    """Training pairs along `runs` relaxations of `steps` steps each: the
+    state and its potential, or with delta the normalised change between
+    consecutive states and the potential of that change (phi is linear in m
+    for a fixed geometry, so phi(m) - phi(m_prev) = phi(m - m_prev))."""
     data = Dataset(prob, runs * steps)
     packer = L.WarmStartNet(1, prob.H, prob.W, prob.pz, 16, valid_width=prob.px, valid_height=prob.py)
     rng = np.random.default_rng(seed0)
@@ -186,12 +202,19 @@ def record(prob: Problem, runs: int, steps: int, seed0: int = 0, pitch_range=(64
         params = L.LLGParams(alpha=0.5, dt=1e-13)
         m_first = prob.state()
         n_acc = 0
+        m_prev = m_first
         for _ in range(steps):
             n_acc += prob.relax_step(params)
-            prob.set_tol(TOL_RECORD, warm=True)
-            prob.solve()
+            src = prob.m
+            if delta:
+                m_now = prob.state()
+                set_delta(prob, m_now, m_prev)
+                m_prev = m_now
+                src = prob.dm
+            prob.set_tol(TOL_RECORD, warm=not delta)
+            prob.solve(src)
             prob.ctx.mg_phi_read(prob.phi)
-            packer.pack_input(0, prob.m, False, prob.nx, prob.ny, prob.offx, prob.offy, data.x.at(k * data.sx))
+            packer.pack_input(0, src, False, prob.nx, prob.ny, prob.offx, prob.offy, data.x.at(k * data.sx))
             packer.pack_phi(0, prob.phi, prob.px, prob.py, prob.pz, prob.scale, data.t.at(k * data.st))
             k += 1
         dm = float(np.abs(prob.state() - m_first).max())
@@ -236,10 +259,10 @@ class Predictor:
         self.y_bytes = prob.H * prob.W * self.net.c_pad * 4
         self.y = L.DeviceArray(self.y_bytes)
 
-    def run(self) -> None:
-        """Network output for the problem's current state into self.y."""
+    def run(self, m=None) -> None:
+        """Network output for the problem's current state (or m) into self.y."""
         p = self.prob
-        self.net.pack_input(0, p.m, False, p.nx, p.ny, p.offx, p.offy, self.x)
+        self.net.pack_input(0, p.m if m is None else m, False, p.nx, p.ny, p.offx, p.offy, self.x)
         self.net.forward(self.x, self.y)
 
     def potential(self, y, phi, sign: float = 1.0, accumulate: bool = False) -> None:
@@ -248,12 +271,15 @@ class Predictor:
 
 
 def compare(prob: Problem, params: np.ndarray, states: int = 30, seed: int = 424242, pitch: float = 96e-9,
-            skip: int = 5, stride: int = 1):
-    """V-cycles and wall time to tol_rel 1e-6 from four initial guesses,
-    along a relaxation the network was not trained on; consecutive compared
+            skip: int = 5, stride: int = 1, params_delta=None):
+    """V-cycles and wall time to tol_rel 1e-6 from the initial guesses below,
+    along a relaxation the networks were not trained on; consecutive compared
     states are `stride` relaxation steps apart (the previous-step guess is
-    the potential of the state `stride` steps earlier)."""
+    the potential of the state `stride` steps earlier).  With params_delta,
+    also phi_prev + s N_delta((m - m_prev) / s).  The difference m - m_prev is
+    formed on the host in this harness and is not in the timed figures."""
     pred = Predictor(prob, params)
+    pdel = Predictor(prob, params_delta) if params_delta is not None else None
     prob.film(seed, pitch)
     p_llg = L.LLGParams(alpha=0.5, dt=1e-13)
     for _ in range(skip):
@@ -265,7 +291,8 @@ def compare(prob: Problem, params: np.ndarray, states: int = 30, seed: int = 424
     prob.ctx.mg_phi_read(phi_prev)
     pred.run()
     y_prev.copy_from(pred.y, pred.y_bytes)
-    names = ("zero", "previous", "learned", "previous_plus_learned_change")
+    names = ("zero", "previous", "learned", "previous_plus_learned_change") + \
+        (("previous_plus_learned_delta",) if pdel else ())
     cyc = {k: [] for k in names}
     sec = {k: [] for k in names}
     t_inf = []
@@ -276,6 +303,7 @@ def compare(prob: Problem, params: np.ndarray, states: int = 30, seed: int = 424
             prob.relax_step(p_llg)
         m_now = prob.state()
         dms.append(float(np.abs(m_now - m_last).max()))
+        s_delta = set_delta(prob, m_now, m_last) if pdel else 0.0
         m_last = m_now
         prob.set_tol(TOL_SOLVE, warm=False)
         t0 = time.perf_counter()
@@ -293,6 +321,12 @@ def compare(prob: Problem, params: np.ndarray, states: int = 30, seed: int = 424
                 t0 = time.perf_counter()
                 pred.potential(pred.y, guess)
                 extra = t_inf[-1] + time.perf_counter() - t0
+            elif name == "previous_plus_learned_delta":
+                t0 = time.perf_counter()
+                pdel.run(prob.dm)
+                guess.copy_from(phi_prev, prob.phi_bytes)
+                pdel.potential(pdel.y, guess, s_delta, True)
+                extra = time.perf_counter() - t0
             else:
                 t0 = time.perf_counter()
                 guess.copy_from(phi_prev, prob.phi_bytes)
@@ -318,6 +352,10 @@ def compare(prob: Problem, params: np.ndarray, states: int = 30, seed: int = 424
     out["learned_change_vs_previous_cycles"] = (
         round(out["previous_plus_learned_change"]["v_cycles_mean"] / prev, 3) if prev else None)
     out["learned_vs_zero_cycles"] = round(out["learned"]["v_cycles_mean"] / out["zero"]["v_cycles_mean"], 3)
+    if pdel:
+        out["learned_delta_vs_previous_cycles"] = (
+            round(out["previous_plus_learned_delta"]["v_cycles_mean"] / prev, 3) if prev else None)
+        pdel.net.close()
     pred.net.close()
     return out
 
@@ -331,10 +369,14 @@ def main():
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--report", default=None, help="JSON file for the training log and comparison")
+    ap.add_argument("--delta", action="store_true",
+                    help="train on state differences (phi of m - m_prev); compares with the state network "
+                         "at --abs-weights")
+    ap.add_argument("--abs-weights", default=DEFAULT_WEIGHTS)
     a = ap.parse_args()
     prob = Problem()
     t0 = time.perf_counter()
-    data = record(prob, a.runs, a.steps)
+    data = record(prob, a.runs, a.steps, seed0=1 if a.delta else 0, delta=a.delta)
     t_rec = time.perf_counter() - t0
     t0 = time.perf_counter()
     net, losses = train(prob, data, a.train_steps, a.batch, a.lr)
@@ -346,7 +388,11 @@ def main():
     n = data.n
     data.x.close()
     data.t.close()
-    cmp_ = {f"stride_{k}": compare(prob, params, states=20, stride=k) for k in (1, 20)}
+    if a.delta:
+        p_abs = np.load(a.abs_weights)
+        cmp_ = {f"stride_{k}": compare(prob, p_abs, states=20, stride=k, params_delta=params) for k in (1, 20)}
+    else:
+        cmp_ = {f"stride_{k}": compare(prob, params, states=20, stride=k) for k in (1, 20)}
     rep = {"pairs": n, "runs": a.runs, "steps_per_run": a.steps, "record_s": round(t_rec, 1),
            "train_steps": a.train_steps, "batch": a.batch, "train_s": round(t_train, 1),
            "loss_first_100": float(np.mean(losses[:100])), "loss_last_100": float(np.mean(losses[-100:])),
