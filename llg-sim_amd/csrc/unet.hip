@@ -40,6 +40,8 @@ constexpr int kLevels = 4;
 constexpr int kNT = 256;
 // levels with at least this many pixels use 64-pixel waves (4 row tiles)
 constexpr int64_t kWideTileMinPixels = 65536;
+// dynamic LDS of a convolution block (the per-tap weight slice)
+constexpr size_t kConvLdsMax = 64 * 1024;
 
 __device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even
     uint32_t u = __float_as_uint(f);
@@ -60,17 +62,24 @@ inline int nblk(int64_t n) { return (int)((n + kNT - 1) / kNT); }
 // ---------------------------------------------------------------------------
 // 3x3 convolution as an implicit GEMM: rows = pixels, columns = output
 // channels, reduction = 9 taps x input channels.  A block is 4 waves; a wave
-// owns 16*MT pixels (MT 16-row MFMA tiles) x 16*NT channels.  Operand
-// fragments come straight from global memory (8-byte loads of 4 consecutive
-// channels).  Also the data gradient of a convolution: the same product of
-// the output gradient with the tap-flipped, transposed weights.
-// Requires: cin % 16 == 0, cout % (16*NT) == 0, xs, xo, ys, yo multiples of 4.
+// owns 16*MT pixels (MT 16-row MFMA tiles) x 16*NT channels; the 4 waves
+// share the block's channels.  Per tap, the block's weight slice
+// (16*NT rows of cin, rows padded by 8 elements against bank conflicts) is
+// staged in LDS once and read from there by all waves; the pixel operands are
+// 8-byte loads of 4 consecutive channels from global memory.  Also the data
+// gradient of a convolution: the same product of the output gradient with
+// the tap-flipped, transposed weights.
+// Requires: cin % 16 == 0, cout % (16*NT) == 0, xs, xo, ys, yo multiples of 4,
+// dynamic LDS of 16*NT*(cin+8)*2 bytes.
 // ---------------------------------------------------------------------------
 template <int MT, int NT, bool RELU, bool OUT_F32>
 __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ x, int xs, int xo, int cin,
                                                  const uint16_t *__restrict__ w, const float *__restrict__ bias,
                                                  void *__restrict__ y, int ys, int yo, int cout, int B, int H,
                                                  int W) {
+    extern __shared__ __align__(16) unsigned char conv_smem[];
+    uint16_t *wsl = reinterpret_cast<uint16_t *>(conv_smem);
+    const int ws_row = cin + 8;   // LDS row stride of the weight slice (elements)
     const int P = B * H * W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = (lane >> 4) * 4;
@@ -101,14 +110,25 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ x,
             const int64_t q = (int64_t)(p_base + mt * 16 + li) + dy * W + dx;
             xa[mt] = x + (qv[mt] ? q * xs + xo + lk : 0);
         }
-        const uint16_t *wt = w + ((int64_t)tap * cout + co_base + li) * cin + lk;
+        // stage w[tap][co_base .. co_base + 16 NT)[0 .. cin) (contiguous) in LDS, 16 bytes at a time
+        __syncthreads();   // the previous tap's reads are done
+        {
+            const uint4 *src = reinterpret_cast<const uint4 *>(w + ((int64_t)tap * cout + co_base) * cin);
+            const int per_row = cin / 8, total = 16 * NT * per_row;
+            for (int e = threadIdx.x; e < total; e += 256) {
+                const int row = e / per_row, c8 = e - row * per_row;
+                *reinterpret_cast<uint4 *>(wsl + row * ws_row + c8 * 8) = src[e];
+            }
+        }
+        __syncthreads();
+        const uint16_t *wt = wsl + li * ws_row + lk;
         for (int ci0 = 0; ci0 < cin; ci0 += 16) {
             bf16x4 a[MT], b[NT];
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
                 a[mt] = qv[mt] ? *(const bf16x4 *)(xa[mt] + ci0) : bf16x4{0, 0, 0, 0};
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) b[nt] = *(const bf16x4 *)(wt + (int64_t)nt * 16 * cin + ci0);
+            for (int nt = 0; nt < NT; ++nt) b[nt] = *(const bf16x4 *)(wt + nt * 16 * ws_row + ci0);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -593,15 +613,18 @@ struct UNet {
                      int ys, int yo, int co, int l) {
         const int64_t P = pl(l);
         const int gx = (int)((P + 64 * MT - 1) / (64 * MT));
+        auto lds = [&](int nt) { return (size_t)16 * nt * (cin + 8) * sizeof(uint16_t); };
+        if (lds(co % 64 == 0 ? 4 : co % 32 == 0 ? 2 : 1) > kConvLdsMax)
+            throw Error(LLGMI_ERR_INVALID, "internal: convolution weight slice exceeds LDS");
         if (co % 64 == 0)
-            hipLaunchKernelGGL((k_conv3x3<MT, 4, RELU, OUT_F32>), dim3(gx, co / 64), dim3(256), 0, stream, x, xs, xo,
-                               cin, w, bias, y, ys, yo, co, B, hl(l), wl(l));
+            hipLaunchKernelGGL((k_conv3x3<MT, 4, RELU, OUT_F32>), dim3(gx, co / 64), dim3(256), lds(4), stream, x,
+                               xs, xo, cin, w, bias, y, ys, yo, co, B, hl(l), wl(l));
         else if (co % 32 == 0)
-            hipLaunchKernelGGL((k_conv3x3<MT, 2, RELU, OUT_F32>), dim3(gx, co / 32), dim3(256), 0, stream, x, xs, xo,
-                               cin, w, bias, y, ys, yo, co, B, hl(l), wl(l));
+            hipLaunchKernelGGL((k_conv3x3<MT, 2, RELU, OUT_F32>), dim3(gx, co / 32), dim3(256), lds(2), stream, x,
+                               xs, xo, cin, w, bias, y, ys, yo, co, B, hl(l), wl(l));
         else
-            hipLaunchKernelGGL((k_conv3x3<MT, 1, RELU, OUT_F32>), dim3(gx, co / 16), dim3(256), 0, stream, x, xs, xo,
-                               cin, w, bias, y, ys, yo, co, B, hl(l), wl(l));
+            hipLaunchKernelGGL((k_conv3x3<MT, 1, RELU, OUT_F32>), dim3(gx, co / 16), dim3(256), lds(1), stream, x,
+                               xs, xo, cin, w, bias, y, ys, yo, co, B, hl(l), wl(l));
         LLGMI_LAUNCH_CHECK();
     }
     void wgrad(const uint16_t *dzp, int co, const uint16_t *x, int xs, int xo, int ci, const UNetConv &cv, int l) {
