@@ -5,9 +5,9 @@
 // activations are bf16 on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32
 // accumulation); gradients, master weights and Adam moments are FP32.  All of
 // training runs here: forward, backward (data and weight gradients as
-// implicit GEMMs), loss and optimiser.  The multigrid solver stays the judge:
-// the prediction only replaces the initial phi, and the solve iterates to the
-// same residual (the reference has no counterpart; its solver starts from
+// implicit GEMMs), loss and optimiser.  The multigrid solver still decides
+// convergence: the prediction only replaces the initial phi, and the solve
+// iterates to the same residual (the reference has no counterpart; its solver starts from
 // zero or from the previous solve, demag_poisson_mg.rs:1947-2057).
 //
 // Layout: activations channels-last, pixel p = (b*H + y)*W + x for batch b,
@@ -50,14 +50,6 @@ __device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-inline uint16_t f2bf_host(float f) {
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-inline int nblk(int64_t n) { return (int)((n + kNT - 1) / kNT); }
 
 // ---------------------------------------------------------------------------
 // 3x3 convolution as an implicit GEMM: rows = pixels, columns = output
@@ -157,13 +149,22 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ x,
 }
 
 // Weight gradient: dw[tap][co][ci] += sum_p dz[p][co] * x[p + off(tap)][ci].
-// Block = 4 waves over one tap and one (16*MT) x (16*NT) tile; the waves take
-// interleaved 16-pixel steps of the block's pixel chunk, their partial tiles
-// are summed through LDS and added to dw with FP32 atomics.
+// Block = 4 waves over one tap and one (16*MT) x (16*NT) tile and a pixel
+// chunk, taken kSlice pixels at a time: the block loads 4-pixel x 4-channel
+// blocks of dz and of the shifted x (four 8-byte loads each), transposes
+// them in registers into LDS tiles [channel][pixel], and each wave multiplies
+// its quarter of the slice in 16-pixel MFMA steps (8-byte LDS reads of 4
+// consecutive pixels).
+// The waves' partial tiles are summed through LDS and added to dw with FP32
+// atomics.
 template <int MT, int NT>
 __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const uint16_t *__restrict__ dz, int cout,
                                                        const uint16_t *__restrict__ x, int xs, int xo, int cin,
                                                        float *__restrict__ dw, int B, int H, int W, int chunk) {
+    constexpr int kSlice = 64;          // pixels per LDS fill
+    constexpr int kRow = kSlice + 4;    // LDS row stride (elements) of the transposed tiles
+    __shared__ __align__(16) uint16_t sa[16 * MT][kRow];   // dz^T: [co][pixel]
+    __shared__ __align__(16) uint16_t sb[16 * NT][kRow];   // x^T:  [ci][pixel]
     __shared__ f32x4 red[4][MT * NT][64];
     const int P = B * H * W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -174,32 +175,55 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const uint16_t *__restric
     const int ci_base = (blockIdx.y / co_tiles) * 16 * NT;
     const int p_begin = blockIdx.x * chunk;
     const int p_end = min(P, p_begin + chunk);
+    constexpr int nA = kSlice / 4 * 4 * MT, nB = kSlice / 4 * 4 * NT;   // (pixel quad, channel quad) items
     f32x4 acc[MT][NT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int pk = p_begin + wave * 16; pk < p_end; pk += 64) {
-        bf16x4 a[MT], b[NT];
+    for (int p0 = p_begin; p0 < p_end; p0 += kSlice) {
+        __syncthreads();   // the previous slice's reads are done
+        for (int e = threadIdx.x; e < nA + nB; e += 256) {
+            const bool isA = e < nA;
+            const int f = isA ? e : e - nA;
+            const int cq = f % (isA ? 4 * MT : 4 * NT), pq = f / (isA ? 4 * MT : 4 * NT);
+            bf16x4 v[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int p = pk + lk + r;
-            const bool v = p < p_end;
-            const int xx = p % W + dx, yy = (p / W) % H + dy;
-            const bool qv = v && xx >= 0 && xx < W && yy >= 0 && yy < H;
-            const int64_t q = (int64_t)p + dy * W + dx;
+            for (int r = 0; r < 4; ++r) {
+                const int p = p0 + pq * 4 + r;
+                v[r] = bf16x4{0, 0, 0, 0};
+                if (p < p_end) {
+                    if (isA) {
+                        v[r] = *(const bf16x4 *)(dz + (int64_t)p * cout + co_base + cq * 4);
+                    } else {
+                        const int xx = p % W + dx, yy = (p / W) % H + dy;
+                        if (xx >= 0 && xx < W && yy >= 0 && yy < H) {
+                            const int64_t q = (int64_t)p + dy * W + dx;
+                            v[r] = *(const bf16x4 *)(x + q * xs + xo + ci_base + cq * 4);
+                        }
+                    }
+                }
+            }
+            uint16_t(*t)[kRow] = isA ? sa : sb;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                *(bf16x4 *)&t[cq * 4 + c][pq * 4] = bf16x4{v[0][c], v[1][c], v[2][c], v[3][c]};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kSlice / 64; ++ks) {
+            const int kp = wave * (kSlice / 4) + ks * 16 + lk;   // this lane's 4 consecutive pixels
+            bf16x4 a[MT], b[NT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) a[mt] = *(const bf16x4 *)&sa[mt * 16 + li][kp];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) b[nt] = *(const bf16x4 *)&sb[nt * 16 + li][kp];
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
-                a[mt][r] = v ? (short)dz[(int64_t)p * cout + co_base + mt * 16 + li] : (short)0;
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                b[nt][r] = qv ? (short)x[q * xs + xo + ci_base + nt * 16 + li] : (short)0;
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
         }
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
     }
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
@@ -209,13 +233,13 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const uint16_t *__restric
     // thread t sums entry (tile t/64, lane t%64) of the four waves
     for (int e = threadIdx.x; e < MT * NT * 64; e += 256) {
         const int t = e / 64, ln = e % 64;
-        const f32x4 s = red[0][t][ln] + red[1][t][ln] + red[2][t][ln] + red[3][t][ln];
+        const f32x4 sum = red[0][t][ln] + red[1][t][ln] + red[2][t][ln] + red[3][t][ln];
         const int mt = t / NT, nt = t % NT;
         const int ci = ci_base + nt * 16 + (ln & 15);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int co = co_base + mt * 16 + (ln >> 4) * 4 + r;
-            atomicAdd(dw + ((int64_t)tap * cout + co) * cin + ci, s[r]);
+            atomicAdd(dw + ((int64_t)tap * cout + co) * cin + ci, sum[r]);
         }
     }
 }
@@ -454,7 +478,6 @@ struct UNet {
     // gradients (FP32)
     float *g_out = nullptr, *g_a[kLevels] = {}, *g_b[kLevels] = {}, *g_cat[kLevels] = {}, *g_eb[kLevels] = {},
           *g_pool[kLevels] = {};
-    const uint16_t *x_in = nullptr;
 
     int c(int l) const { return base << l; }
     int hl(int l) const { return H >> l; }
@@ -631,11 +654,15 @@ struct UNet {
         if (co % 16 || ci % 16 || xo + ci > xs || co != cv.cout || ci != cv.cin)
             throw Error(LLGMI_ERR_INVALID, "internal: weight-gradient shape");
         const int64_t P = pl(l);
-        const int chunk = 2048;
-        const int gx = (int)((P + chunk - 1) / chunk);
         float *dw = d_g + cv.w_off;
         const int mt = (co / 16) % 2 == 0 ? 2 : 1, nt = (ci / 16) % 2 == 0 ? 2 : 1;
-        const dim3 grid(gx, (co / (16 * mt)) * (ci / (16 * nt)), 9);
+        // pixel chunk per block: about 2048 blocks in all (each block ends in
+        // 16*mt x 16*nt x 4 atomics), at least 2048 pixels, a multiple of 64
+        const int tiles = (co / (16 * mt)) * (ci / (16 * nt));
+        const int64_t nchunks = std::max<int64_t>(1, 2048 / (9 * tiles));
+        const int chunk = (int)std::max<int64_t>(2048, ((P + nchunks - 1) / nchunks + 63) / 64 * 64);
+        const int gx = (int)((P + chunk - 1) / chunk);
+        const dim3 grid(gx, tiles, 9);
         if (mt == 2 && nt == 2)
             hipLaunchKernelGGL((k_conv3x3_wgrad<2, 2>), grid, dim3(256), 0, stream, dzp, co, x, xs, xo, ci, dw, B,
                                hl(l), wl(l), chunk);
@@ -661,7 +688,6 @@ struct UNet {
     const UNetConv &head() const { return conv.back(); }
 
     void forward(const uint16_t *x) {
-        x_in = x;
         for (int l = 0; l < kLevels; ++l) {
             const uint16_t *src = l == 0 ? x : pool[l];
             const int cs = l == 0 ? 16 : c(l - 1);

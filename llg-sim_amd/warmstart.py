@@ -25,7 +25,7 @@ predicts phi / (Ms dz) on all 36 z planes of the box.
 
 The reference has no learned component; its solve starts from zero or from
 the previous solve (demag_poisson_mg.rs:1947-2057).  The multigrid solver
-remains the judge: every guess is iterated to the same residual.
+still decides convergence: every guess is iterated to the same residual.
 
 Run as a script to train and save weights:
   python llg-sim_amd/warmstart.py --runs 20 --steps 100 --out <file.npy>
