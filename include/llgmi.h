@@ -296,7 +296,7 @@ typedef struct llgmi_ws_shape {
     int32_t in_channels, out_channels;  /* 1..16, 1..256 */
     int32_t base_channels, reserved;    /* 16 or 32 */
     int64_t valid_width, valid_height;
-} llgmi_ws_shape;                       /* 48 bytes */
+} llgmi_ws_shape;                       /* 56 bytes */
 int llgmi_ws_create(const llgmi_ws_shape *shape, uint64_t seed, int device, llgmi_ws **out);
 int llgmi_ws_destroy(llgmi_ws *ws);
 int llgmi_ws_param_count(llgmi_ws *ws, int64_t *n);

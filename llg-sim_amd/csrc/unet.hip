@@ -188,20 +188,26 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const uint16_t *__restric
             const int f = isA ? e : e - nA;
             const int cq = f % (isA ? 4 * MT : 4 * NT), pq = f / (isA ? 4 * MT : 4 * NT);
             bf16x4 v[4];
+            const int pb = p0 + pq * 4;
+            int xc = pb % W, yc = (pb / W) % H;   // column and row of pixel pb, advanced below
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int p = p0 + pq * 4 + r;
+                const int p = pb + r;
                 v[r] = bf16x4{0, 0, 0, 0};
                 if (p < p_end) {
                     if (isA) {
                         v[r] = *(const bf16x4 *)(dz + (int64_t)p * cout + co_base + cq * 4);
                     } else {
-                        const int xx = p % W + dx, yy = (p / W) % H + dy;
+                        const int xx = xc + dx, yy = yc + dy;
                         if (xx >= 0 && xx < W && yy >= 0 && yy < H) {
                             const int64_t q = (int64_t)p + dy * W + dx;
                             v[r] = *(const bf16x4 *)(x + q * xs + xo + ci_base + cq * 4);
                         }
                     }
+                }
+                if (++xc == W) {   // next pixel starts a new row (and perhaps a new batch image)
+                    xc = 0;
+                    if (++yc == H) yc = 0;
                 }
             }
             uint16_t(*t)[kRow] = isA ? sa : sb;

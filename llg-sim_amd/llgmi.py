@@ -163,7 +163,7 @@ EXPORTED = [
 
 
 class WSShape(C.Structure):
-    """llgmi_ws_shape (48 bytes)."""
+    """llgmi_ws_shape (56 bytes)."""
     _fields_ = [("batch", C.c_int64), ("height", C.c_int64), ("width", C.c_int64),
                 ("in_channels", C.c_int32), ("out_channels", C.c_int32),
                 ("base_channels", C.c_int32), ("reserved", C.c_int32),

@@ -187,3 +187,39 @@ def test_mg_hybrid_config_from_env(monkeypatch):
     assert llgmi.mg_hybrid_config_from_env().sigma_cells == 0.0
     monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_SIGMA", "x")
     assert llgmi.mg_hybrid_config_from_env().sigma_cells == 1.5
+
+
+def test_warm_start_shape_and_batch_arguments_rejected_without_device():
+    """llgmi_ws_shape is 56 bytes; llgmi_ws_create rejects every bad shape
+    and llgmi_dev_rk45_step_batch its bad arguments before any device call
+    (so these run without a GPU); an empty batch is a no-op."""
+    import llgmi
+    L = llgmi.lib()
+    assert C.sizeof(llgmi.WSShape) == 56
+    h = C.c_void_p()
+    good = dict(batch=1, height=64, width=64, in_channels=4, out_channels=36, base_channels=32,
+                valid_width=60, valid_height=60)
+    bad = [dict(height=60), dict(width=4), dict(batch=0), dict(in_channels=0), dict(in_channels=17),
+           dict(out_channels=0), dict(out_channels=257), dict(base_channels=24), dict(base_channels=64),
+           dict(valid_width=0), dict(batch=1 << 20, height=1024, width=1024)]
+    for b in bad:
+        kw = {**good, **b}
+        s = llgmi.WSShape(kw["batch"], kw["height"], kw["width"], kw["in_channels"], kw["out_channels"],
+                          kw["base_channels"], 0, kw["valid_width"], kw["valid_height"])
+        assert L.llgmi_ws_create(C.byref(s), 1, 0, C.byref(h)) == 1, b
+        assert not h.value
+    assert L.llgmi_ws_create(None, 1, 0, C.byref(h)) == 1
+    n = C.c_int64()
+    assert L.llgmi_ws_param_count(None, C.byref(n)) == 1
+    assert b"NULL" in L.llgmi_last_error()
+    assert L.llgmi_ws_destroy(None) == 0
+    # batched stepper: empty batch, negative count, NULL arrays
+    z = C.c_double()
+    assert L.llgmi_dev_rk45_step_batch(None, None, None, None, 0, 2, 1e-5, 0.8, 1e-18, 1e-11, None, None,
+                                       None) == 0
+    assert L.llgmi_dev_rk45_step_batch(None, None, None, None, -1, 2, 1e-5, 0.8, 1e-18, 1e-11, None, None,
+                                       None) == 1
+    assert L.llgmi_dev_rk45_step_batch(None, None, None, None, 3, 2, 1e-5, 0.8, 1e-18, 1e-11,
+                                       C.byref(z), None, None) == 1
+    assert L.llgmi_mg_box(None, None) == 1
+    assert L.llgmi_dev_mask_cells(None, None, None) == 1
