@@ -295,3 +295,53 @@ def test_rk45_graph_replay_equals_eager(gpu_lib, prec, monkeypatch):
         assert np.array_equal(cg.dev_download(bg), ce.dev_download(be)), step
         if step == 5:
             assert not rg[1]
+
+
+def test_c1_one_nanosecond_trajectory_matches_oracle(gpu_lib):
+    """The 128 x 64 film of 4 x 4 x 3 nm cells (Ms 8e5, A 1.3e-11, alpha 0.02,
+    B = (-24.6, 4.3, 0) mT, FFT demag padded to 256 x 128), seeded
+    box-smoothed random m0, adaptive RK45 with max_err 1e-6 for 1 ns, outputs
+    every 10 ps: the FP64 path's spatially averaged m stays within 1e-4 of the
+    CPU oracle's (measured 5.4e-10) with the same number of attempts (4252).
+    The FP32 path follows the FP64 one within 1e-4 for the first 150 ps; from
+    random m0 the film then forms domains, which amplify FP32 rounding into
+    different textures (measured 2e-4 at 300 ps, 5e-2 by 1 ns)."""
+    from conftest import DeviceBuffer
+    from oracle import oracle as O
+    L = gpu_lib
+    nx, ny, cell = 128, 64, (4e-9, 4e-9, 3e-9)
+    m0, om, mat = _setup(nx, ny, 4)
+    bext = (-24.6e-3, 4.3e-3, 0.0)
+
+    def run(step, m_get, p, n_out):
+        avg, t, n = [], 0.0, 0
+        for k in range(1, n_out + 1):
+            t_target = k * 1e-11
+            while t + 1e-18 < t_target:
+                if p.dt > t_target - t:
+                    p.dt = t_target - t
+                _, acc, dtu = step(p)
+                n += 1
+                if acc:
+                    t += dtu
+            t = t_target
+            avg.append(m_get().mean(axis=0))
+        return np.array(avg), n
+
+    gpu = {}
+    for prec, es, n_out in ((L.F64, 8, 100), (L.F32, 4, 15)):
+        ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=prec)
+        buf = DeviceBuffer(3 * nx * ny * es)
+        ctx.dev_upload(m0, buf)
+        p = L.LLGParams(alpha=0.02, dt=1e-13, b_ext=bext)
+        gpu[prec] = run(lambda q, c=ctx, b=buf: c.dev_rk45_step(b, q, mat, 2, 1e-6, 0.8, 1e-19, 1e-11),
+                        lambda c=ctx, b=buf: c.dev_download(b), p, n_out)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    mo = m0.copy()
+    ro, no = run(lambda q: oc.rk45_step(mo, q, om, 2, 1e-6, 0.8, 1e-19, 1e-11), lambda: mo,
+                 O.Params(alpha=0.02, dt=1e-13, b_ext=bext), 100)
+    d64 = np.abs(gpu[L.F64][0] - ro).max()
+    d32 = np.abs(gpu[L.F32][0] - gpu[L.F64][0][:15]).max()
+    print(f"1 ns FP64 vs oracle {d64:.2e} ({gpu[L.F64][1]} / {no} attempts); 150 ps FP32 vs FP64 {d32:.2e}")
+    assert d64 <= 1e-4 and gpu[L.F64][1] == no
+    assert d32 <= 1e-4
