@@ -490,8 +490,7 @@ struct Engine : EngineBase {
             // there), then capture on the next and replay after that
             RkGraph &G = rk_graph[fsal_ok][k_swapped];
             const std::string key = rk_graph_key(m, p, mat, mask);
-            *h_dt = p.dt;
-            LLGMI_HIP(hipMemcpyAsync(d_dt, h_dt, sizeof(double), hipMemcpyHostToDevice, stream));
+            *h_dt = p.dt;   // the graph's first node copies it (pinned memory, read when the graph runs)
             if (G.exec && G.key == key) {
                 LLGMI_HIP(hipGraphLaunch(G.exec, stream));
             } else if (G.seen == key) {
@@ -523,10 +522,11 @@ struct Engine : EngineBase {
 
     // The launches of one attempt: stages 2..7 with their field evaluations
     // (stage 1 too without FSAL), the FP64 error norm and the copy of
-    // max|err| into pinned memory.  With dt_dev the stage coefficients are
-    // formed on the device from the step size there (for graph replay).
+    // max|err| into pinned memory.  With dt_dev the step size is first copied
+    // there from pinned h_dt and the stage coefficients are formed on the
+    // device from it (for graph replay: the copy node reads h_dt when it runs).
     void rk45_launches(T *m, const llgmi_params &p, const llgmi_material &mat, int mask, bool fsal_ok,
-                       const double *dt_dev) {
+                       double *dt_dev) {
         static const double A[7][5] = {
             {0}, {0},
             {1.0 / 5.0},
@@ -538,6 +538,7 @@ struct Engine : EngineBase {
         static const double Bw[5] = {35.0 / 384.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0};
         static const int Bk[5] = {1, 3, 4, 5, 6};
         const double dt0 = p.dt;
+        if (dt_dev) LLGMI_HIP(hipMemcpyAsync(dt_dev, h_dt, sizeof(double), hipMemcpyHostToDevice, stream));
         if (!fsal_ok) field(m, d_b, p, mat, mask, nullptr, d_k[1]);
         for (int s = 2; s <= 7; ++s) {
             StageArgs<T> sa{};
