@@ -312,7 +312,11 @@ __global__ void __launch_bounds__(RBX * RBY) k_mg_rb_sweep(const double *__restr
 // place half-sweep by half-sweep (each half-sweep leaves one more outer ring
 // stale), and writes the central TBT x TBT columns to B; after h <= TBH
 // half-sweeps those are exactly the values the in-place half-sweeps produce.
-constexpr int TBT = 16, TBH = 8, TBW = TBT + 2 * TBH, TB_MAXZ = 9, TB_NT = 1024;
+// TBT = 10 puts the 131 x 131 coarsest level of the 512^2 film in 169 blocks
+// (one per CU, one round on 256 CUs, 97 KB of LDS each); measured against 8,
+// 12 and 16 it gives the shortest launches (fewer cells per block, no second
+// round of blocks).
+constexpr int TBT = 10, TBH = 8, TBW = TBT + 2 * TBH, TB_MAXZ = 9, TB_NT = 1024;
 __device__ __forceinline__ int tb_at(int k, int hj, int hi) { return (k * TBW + hj) * TBW + hi; }
 __global__ void __launch_bounds__(TB_NT) k_mg_rb_tb(const double *__restrict__ A, double *__restrict__ B,
                                                     const double *__restrict__ rhs, int nx, int ny, int nz,
