@@ -9,15 +9,22 @@ One step = one full field evaluation (build_h_eff_masked, FieldMask::Full:
 Zeeman + exchange + anisotropy + DMI + FFT demag) with m already resident in
 HBM, called through the C ABI (llgmi_dev_build_h_eff).
 
-The JSON line also carries:
-  roofline      the dominant kernel (column FFT * tensor * inverse column FFT),
-                timed live with HIP events on the library stream during the
-                timed region; algorithmic bytes per launch as below;
+The timed region holds K evaluations and nothing else.  The JSON line also
+carries:
+  per_eval      median and p10/p90 of per-evaluation durations (a second pass
+                of K evaluations with one HIP event after each);
+  roofline      the dominant kernel (column FFT * tensor * inverse column FFT):
+                its average duration from launches back to back between two
+                HIP events on the library stream, algorithmic bytes per launch
+                as below;
   cpu_baseline  the CPU oracle (C FP64 restatement of the reference path,
-                OpenMP over the host cores) on the same 1024^2 workload, a few
-                evaluations after an untimed warm-up that builds its kernel;
+                OpenMP over the host cores, and one thread) on the same 1024^2
+                workload, 5 evaluations each after an untimed warm-up that
+                builds its kernel;
+  fp64_gpu      the same evaluation in an FP64 context;
   rel_err_vs_fp64  FP32 against the FP64 GPU path on the same m, per term;
-  kernels_us    per-kernel average durations from a separate event-timed pass;
+  kernels_us_event_timed  per-kernel durations from a separate pass with events
+                around every launch (includes dispatch latency);
   rk45          accepted RK45 steps/s (FSAL, 6 field evaluations per step);
   rk23_relax    accepted relax-mode RK23 steps/s (FSAL, 3 field evaluations per
                 step) and energy breakdowns/s;
@@ -112,8 +119,23 @@ def accuracy_report(L, m_host):
     return out
 
 
-def cpu_baseline(m_host, n_eval=2):
-    """CPU oracle (C FP64 restatement of the reference path) on this workload."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(m_host, n_eval=5):
+    """CPU oracle (C FP64 restatement of the reference path: C2C FFTs of the
+    2nx x 2ny padded grid, serial stencils, OpenMP where the reference uses
+    rayon) on this workload, at the host's OpenMP thread count and at one
+    thread; median and p10/p90 of n_eval evaluations after an untimed warm-up
+    that builds the kernel."""
     from oracle import oracle as O
     threads = O.get_threads()
     ctx = O.Context(O.Grid(NX, NY, *CELL))
@@ -122,13 +144,59 @@ def cpu_baseline(m_host, n_eval=2):
     t0 = time.perf_counter()
     ctx.build_h_eff(m_host, p, mat, O.FULL)  # warm-up: builds the kernel, untimed
     t_build = time.perf_counter() - t0
+
+    def timed(n):
+        ts = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            ctx.build_h_eff(m_host, p, mat, O.FULL)
+            ts.append(time.perf_counter() - t0)
+        return np.array(ts)
+
+    t_all = timed(n_eval)
+    O.set_threads(1)
+    try:
+        t_one = timed(n_eval)
+    finally:
+        O.set_threads(threads)
+    med = float(np.median(t_all))
+    return {"value": round(1.0 / med, 3), "unit": "field evals/s", "cores": threads, "kind": "port",
+            "sample": f"{n_eval} full field evaluations (1024x1024 FP64, C2C FFT demag 2048^2) at {threads} "
+                      f"threads and {n_eval} at 1 thread, after one untimed warm-up that built the kernel "
+                      f"({t_build:.1f} s); value = 1 / median",
+            "cpu_model": _cpu_model(),
+            "ms_median": round(1e3 * med, 1), "ms_p10": round(1e3 * float(np.percentile(t_all, 10)), 1),
+            "ms_p90": round(1e3 * float(np.percentile(t_all, 90)), 1),
+            "one_thread": {"value": round(1.0 / float(np.median(t_one)), 4),
+                           "ms_median": round(1e3 * float(np.median(t_one)), 1),
+                           "ms_p10": round(1e3 * float(np.percentile(t_one, 10)), 1),
+                           "ms_p90": round(1e3 * float(np.percentile(t_one, 90)), 1)}}
+
+
+def fp64_rate(L, m_host, n_eval):
+    """The same field evaluation in an FP64 context (device buffers, state
+    resident): evaluations/s, the like-for-like figure for the FP64 CPU path."""
+    import torch
+    ctx = L.Context(L.Grid2D(NX, NY, *CELL), precision=L.F64)
+    m = torch.empty((3, NY, NX), dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(m)
+    ctx.dev_upload(m_host, m)
+    params = L.LLGParams(alpha=ALPHA, b_ext=BEXT)
+    mat = L.Material(**MAT)
+    for _ in range(5):
+        ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
+    ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(n_eval):
-        ctx.build_h_eff(m_host, p, mat, O.FULL)
+        ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
+    ctx.synchronize()
     dt = time.perf_counter() - t0
-    return {"value": n_eval / dt, "unit": "field evals/s", "cores": threads, "kind": "port",
-            "sample": f"{n_eval} full field evaluations (1024x1024 FP64, FFT demag 2048^2) after "
-                      f"one untimed warm-up that built the kernel ({t_build:.1f} s)"}
+    passes = ctx.profile_fft_passes(m, MAT["ms"], 50)
+    ctx.close()
+    return {"value": round(n_eval / dt, 1), "unit": "field evals/s", "evals": n_eval,
+            "fft_passes_back_to_back_us": {k: round(v, 2) for k, v in passes.items()},
+            "col_pass_frac_hbm": round(col_pass_bytes(NX, NY, 8) / (passes["fft_col_mul"] * 1e-6) / 1e9
+                                       / HBM_PEAK_GBS, 4)}
 
 
 MG_N = 512
@@ -200,25 +268,16 @@ def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
     field evaluation (Full mask, FFT demag) on every film before one
     synchronisation, so the small films overlap on the GPU."""
     import torch
-    rng = np.random.default_rng(64)
+    from initial_states import batched_films
+    specs = batched_films(64, n_films)
     films = []
-    for f in range(n_films):
-        nx = int(rng.choice([64, 128, 256, 512]))
-        ny = int(rng.choice([64, 128, 256]))
-        dxy = float(rng.uniform(2e-9, 5e-9))
-        g = L.Grid2D(nx, ny, dxy, dxy, 3e-9)
-        mat = L.Material(ms=float(rng.uniform(4e5, 1.2e6)), a_ex=float(rng.uniform(5e-12, 2e-11)),
-                         k_u=float(rng.uniform(0.0, 5e5)), easy_axis=(0.0, 0.0, 1.0),
-                         dmi=float(rng.uniform(0.0, 4e-3)), demag=True)
-        ctx = L.Context(g, precision=L.F32)
-        mh = rng.standard_normal((ny, nx, 3))
-        for _ in range(3):   # smooth
-            mh = 0.25 * (np.roll(mh, 1, 0) + np.roll(mh, -1, 0) + np.roll(mh, 1, 1) + np.roll(mh, -1, 1))
-        mh /= np.linalg.norm(mh, axis=2, keepdims=True)
+    for f in specs:
+        nx, ny = f["nx"], f["ny"]
+        ctx = L.Context(L.Grid2D(nx, ny, *f["cell"]), precision=L.F32)
         m = torch.empty((3, ny, nx), dtype=torch.float32, device="cuda:0")
         b = torch.empty_like(m)
-        ctx.dev_upload(mh.reshape(-1, 3), m)
-        films.append((ctx, m, b, mat, nx * ny))
+        ctx.dev_upload(f["m0"], m)
+        films.append((ctx, m, b, L.Material(**f["material"]), nx * ny))
     params = L.LLGParams(alpha=0.02, b_ext=(0.01, 0.0, 0.0))
     for ctx, m, b, mat, _ in films:          # untimed: kernels, autotune
         ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
@@ -243,8 +302,7 @@ def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
     ctxs = [f[0] for f in films]
     ms = [f[1] for f in films]
     mats = [f[3] for f in films]
-    pars = [L.LLGParams(alpha=float(rng.uniform(0.01, 0.5)), dt=1e-13, b_ext=(0.01, 0.0, 0.0))
-            for _ in films]
+    pars = [L.LLGParams(alpha=f["alpha"], dt=1e-13, b_ext=(0.01, 0.0, 0.0)) for f in specs]
     rk = dict(mask=L.FieldMask.Full, max_err=1e-5, headroom=0.8, dt_min=1e-18, dt_max=1e-11)
     for _ in range(6):       # untimed: the eager first attempts and the captures of both FSAL parities
         L.dev_rk45_step_batch(ctxs, ms, pars, mats, **rk)
@@ -268,6 +326,17 @@ def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
                    "cell_steps_per_s": round(acc / att * cells * rk_rounds / dtb, 1),
                    "ms_per_round": round(1e3 * dtb / rk_rounds, 3),
                    "one_film_at_a_time_steps_per_s": round(acc_s / dts, 1)}
+    # per-film accuracy: the FP32 field of each film's current state against an
+    # FP64 context on the same state (max |dB| / max |B| over the film's cells)
+    errs = []
+    for (ctx, m, b, mat, _), f in zip(films, specs):
+        mh = ctx.dev_download(m)
+        p = L.LLGParams(alpha=f["alpha"], b_ext=(0.01, 0.0, 0.0))
+        c64 = L.Context(L.Grid2D(f["nx"], f["ny"], *f["cell"]), precision=L.F64)
+        errs.append(rel_err(ctx.build_h_eff(mh, p, mat), c64.build_h_eff(mh, p, mat)))
+        c64.close()
+    out["field_rel_err_vs_fp64"] = {"max": float(max(errs)), "median": float(np.median(errs)),
+                                    "films_over_1e-5": int(sum(e > 1e-5 for e in errs))}
     for ctx, *_ in films:
         ctx.close()
     return out
@@ -391,9 +460,7 @@ def main():
         ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
     ctx.synchronize()
 
-    # timed region: the dominant kernel is event-timed on the library stream
-    ctx.profile_reset()
-    ctx.profile_enable([L.K_FFT_COL])
+    # timed region: K evaluations, nothing else on the stream
     torch.cuda.synchronize()
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -402,15 +469,33 @@ def main():
     ctx.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ctx.profile_enable([])
-    col_ms, col_n = ctx.profile_read(L.K_FFT_COL)
     elapsed = t1 - t0
     value = args.steps / elapsed
-    col_us = 1e3 * col_ms / max(col_n, 1)
+
+    # per-evaluation durations: one event after each evaluation on the
+    # library stream (a second pass of K evaluations)
+    lib_stream = torch.cuda.ExternalStream(ctx.stream(), device=dev)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    evs[0].record(lib_stream)
+    for i in range(args.steps):
+        ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
+        evs[i + 1].record(lib_stream)
+    ctx.synchronize()
+    per_eval_us = np.array([1e3 * evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)])
+    dist = {"median_us": round(float(np.median(per_eval_us)), 2),
+            "p10_us": round(float(np.percentile(per_eval_us, 10)), 2),
+            "p90_us": round(float(np.percentile(per_eval_us, 90)), 2), "evals": args.steps}
+
+    # the roofline kernel: each FFT pass launched back to back between two
+    # HIP events on the library stream (no event between launches)
+    reps = max(100, args.steps)
+    passes_us = ctx.profile_fft_passes(m, MAT["ms"], reps)
+    col_us = passes_us["fft_col_mul"]
     col_bytes = col_pass_bytes(NX, NY, 4)
     achieved = col_bytes / (col_us * 1e-6) / 1e9
 
-    # per-kernel breakdown (separate pass, all classes timed)
+    # per-kernel breakdown inside the pipeline (separate pass, every launch
+    # between its own pair of events, which adds the dispatch latency)
     ctx.profile_reset()
     ctx.profile_enable(None)
     for _ in range(50):
@@ -422,6 +507,11 @@ def main():
         t, c = ctx.profile_read(cls)
         if c:
             kernels[name] = round(1e3 * t / c, 2)
+
+    fp64 = None
+    if not args.quick:
+        fp64 = fp64_rate(L, m_host, max(20, args.steps // 5))
+        print(f"[bench] fp64: {fp64}", file=sys.stderr, flush=True)
 
     # RK45 accepted steps/s (FSAL; state resident)
     rk = {}
@@ -509,11 +599,15 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": COL_TRAFFIC_BYTES,
                      "kernel": "k_col_demag (column FFT x tensor x inverse column FFT)",
-                     "bytes_per_launch": col_bytes, "avg_us": round(col_us, 2)},
+                     "bytes_per_launch": col_bytes, "avg_us": round(col_us, 2),
+                     "timing": f"{reps} launches back to back between two HIP events on the library stream"},
+        "per_eval": dist,
+        "fft_passes_back_to_back_us": {k: round(v, 2) for k, v in passes_us.items()},
+        "fp64_gpu": fp64,
         "cpu_baseline": cpu,
         "pipeline_bytes_per_eval": field_eval_bytes(NX, NY, 4),
         "pipeline_frac_hbm": round(field_eval_bytes(NX, NY, 4) * value / 1e9 / HBM_PEAK_GBS, 4),
-        "kernels_us": kernels,
+        "kernels_us_event_timed": kernels,
         "rel_err_vs_fp64": acc_rep,
         "rk45": rk,
         "rk23_relax": r23,

@@ -478,6 +478,15 @@ int llgmi_profile_enable(llgmi_ctx *ctx, int class_mask);
 int llgmi_profile_reset(llgmi_ctx *ctx);
 /* Sum of event-timed durations (ms) and number of launches of a class. */
 int llgmi_profile_read(llgmi_ctx *ctx, int kernel_class, double *total_ms, int64_t *count);
+/* Average duration (us) of each FFT demag pass of the context's open-boundary
+ * operator, launched `reps` times back to back between two events (no event
+ * between launches): avg_us[0] row R2C of ms*m_dev, avg_us[1] column pass (in
+ * place on the spectrum, whose values it scales down each time), avg_us[2]
+ * row C2R into the context's demag buffer.  Leaves the spectrum changed; the
+ * next field evaluation writes it again.  Synchronous.  Measurement aid with
+ * no counterpart in the reference (which times whole calls,
+ * bin/demag_fft_vs_mg.rs:229-240). */
+int llgmi_profile_fft_passes(llgmi_ctx *ctx, const void *m_dev, double ms, int reps, double *avg_us);
 
 #ifdef __cplusplus
 }

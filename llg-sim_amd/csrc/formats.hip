@@ -18,6 +18,7 @@
 
 #include <sys/stat.h>
 #include <sys/types.h>
+#include <unistd.h>
 
 #include <cerrno>
 #include <cmath>
@@ -369,7 +370,10 @@ void write_kernel_cache(const std::string &path, const llgmi_grid &g, int64_t pb
     realspace_from_base(base, g.nx, g.ny, pbc_x, pbc_y, px, py, k);
     for (int c = 0; c < 4; ++c) fft2(k[c], (size_t)px, (size_t)py, -1);
     ensure_parent_dir(path);
-    File f(path, "wb");
+    // written beside the target and renamed over it, so a reader never sees a
+    // partly written file
+    const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+    File f(tmp, "wb");
     f.bytes(kMagic, 8);
     const uint32_t u[7] = {2u, (uint32_t)g.nx, (uint32_t)g.ny, (uint32_t)pbc_x, (uint32_t)pbc_y, (uint32_t)px,
                            (uint32_t)py};
@@ -378,6 +382,10 @@ void write_kernel_cache(const std::string &path, const llgmi_grid &g, int64_t pb
     f.bytes(d, sizeof d);
     for (int c = 0; c < 4; ++c) f.bytes(k[c].data(), sizeof(C64) * k[c].size());
     f.close();
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) {
+        std::remove(tmp.c_str());
+        throw Error(LLGMI_ERR_INVALID, "cannot rename " + tmp + " to " + path);
+    }
 }
 
 // try_load_kernel_kspace (demag_fft_uniform.rs:322-429): false when the file is

@@ -139,9 +139,7 @@ struct Engine : EngineBase {
         if (h_red) (void)hipHostFree(h_red);
         if (h_dt) (void)hipHostFree(h_dt);
         if (d_dt) (void)hipFree(d_dt);
-        for (auto &row : rk_graph)
-            for (auto &G : row)
-                if (G.exec) (void)hipGraphExecDestroy(G.exec);
+        drop_rk_graphs();
     }
 
     template <typename U>
@@ -162,12 +160,37 @@ struct Engine : EngineBase {
         const char *e = std::getenv("LLGMI_DEMAG_CACHE");
         return e && std::string(e) == "1";
     }
+    // Every (re)build of an operator frees the buffers that captured RK45
+    // attempts name: the generation in the graph key changes first (so a
+    // failed build cannot leave a stale key behind) and the captured graphs
+    // are dropped.
+    void rebuild_fft(DemagFFT<T> &op, int64_t pbc_x, int64_t pbc_y, const std::vector<double> *base) {
+        ++fft_gen;
+        drop_rk_graphs();
+        op.build(g.nx, g.ny, g.dx, g.dy, g.dz, stream, pbc_x, pbc_y, base);
+    }
+    void drop_rk_graphs() {
+        for (auto &row : rk_graph)
+            for (auto &G : row) {
+                if (G.exec) (void)hipGraphExecDestroy(G.exec);
+                G = RkGraph{};
+            }
+    }
     void build_fft(DemagFFT<T> &op, int64_t pbc_x, int64_t pbc_y) {
-        ++fft_gen;   // captured RK45 attempts name the operator's buffers
         std::vector<double> base;
         const std::string path = kernel_cache_path(g, pbc_x, pbc_y);
-        const bool cached = auto_cache() && read_kernel_cache(path, g, pbc_x, pbc_y, base);
-        op.build(g.nx, g.ny, g.dx, g.dy, g.dz, stream, pbc_x, pbc_y, cached ? &base : nullptr);
+        bool cached = false;
+        if (auto_cache()) {
+            // as try_load_kernel_kspace(..).unwrap_or(false): an unreadable or
+            // short file is a miss, rebuilt and written again
+            try {
+                cached = read_kernel_cache(path, g, pbc_x, pbc_y, base);
+            } catch (const std::exception &e) {
+                std::fprintf(stderr, "[llgmi] WARN: demag kernel cache not read (%s)\n", e.what());
+                cached = false;
+            }
+        }
+        rebuild_fft(op, pbc_x, pbc_y, cached ? &base : nullptr);
         if (auto_cache() && !cached) {
             try {
                 write_kernel_cache(path, g, pbc_x, pbc_y, op.base_host);
@@ -1757,9 +1780,50 @@ int llgmi_demag_cache_load(llgmi_ctx *ctx, uint64_t pbc_x, uint64_t pbc_y, const
             std::vector<double> base;
             if (!read_kernel_cache(p, ctx->grid, (int64_t)pbc_x, (int64_t)pbc_y, base)) return;
             auto &op = (pbc_x == 0 && pbc_y == 0) ? eng.fft : eng.fft_pbc;
-            op.build(ctx->grid.nx, ctx->grid.ny, ctx->grid.dx, ctx->grid.dy, ctx->grid.dz, eng.stream,
-                     (int64_t)pbc_x, (int64_t)pbc_y, &base);
+            eng.rebuild_fft(op, (int64_t)pbc_x, (int64_t)pbc_y, &base);
             *loaded = 1;
+        });
+    });
+}
+
+int llgmi_profile_fft_passes(llgmi_ctx *ctx, const void *m_dev, double ms, int reps, double *avg_us) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!avg_us) return fail(LLGMI_ERR_INVALID, "avg_us is NULL");
+    if (reps < 1 || reps > 100000) return fail(LLGMI_ERR_INVALID, "reps must be in [1, 100000]");
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    std::string e = check_dev(m_dev, 3 * n * elem_size(ctx), "m");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.ensure_fft();
+            eng.alloc(eng.d_bd, 3 * eng.n);
+            auto &op = eng.fft;
+            hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+            auto destroy = [&] {
+                for (auto &x : ev)
+                    if (x) (void)hipEventDestroy(x);
+            };
+            try {
+                for (auto &x : ev) LLGMI_HIP(hipEventCreate(&x));
+                LLGMI_HIP(hipEventRecord(ev[0], eng.stream));
+                for (int i = 0; i < reps; ++i) op.launch_row_fwd((const T *)m_dev, T(ms), eng.stream, op.rq_row);
+                LLGMI_HIP(hipEventRecord(ev[1], eng.stream));
+                for (int i = 0; i < reps; ++i) op.launch_col(eng.stream, op.rq_col);
+                LLGMI_HIP(hipEventRecord(ev[2], eng.stream));
+                for (int i = 0; i < reps; ++i) op.launch_row_inv(eng.d_bd, eng.stream, op.rq_row_inv);
+                LLGMI_HIP(hipEventRecord(ev[3], eng.stream));
+                LLGMI_HIP(hipEventSynchronize(ev[3]));
+                for (int k = 0; k < 3; ++k) {
+                    float t = 0.f;
+                    LLGMI_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+                    avg_us[k] = 1e3 * (double)t / reps;
+                }
+            } catch (...) {
+                destroy();
+                throw;
+            }
+            destroy();
         });
     });
 }

@@ -75,3 +75,30 @@ def apply_mask_zero(m: np.ndarray, mask: np.ndarray) -> np.ndarray:
     out = m.copy()
     out[~mask.reshape(out.shape[:2])] = 0.0
     return out
+
+
+def batched_films(seed: int = 64, n_films: int = 64) -> list:
+    """The batched-films workload: `n_films` independent films with sizes
+    drawn from {64,128,256,512} x {64,128,256}, dx = dy in U[2, 5] nm, dz = 3 nm,
+    Ms in U[4e5, 1.2e6] A/m, A in U[5e-12, 2e-11] J/m, Ku in U[0, 5e5] J/m^3
+    (easy axis z), D in U[0, 4e-3] J/m^2, alpha in U[0.01, 0.5], and m0 a
+    Gaussian random field smoothed by three 4-neighbour averages (periodic)
+    and normalised.  Each film is a dict with nx, ny, cell, material (keyword
+    arguments of a material), alpha and m0 (float64, shape (ny*nx, 3))."""
+    rng = np.random.default_rng(seed)
+    films = []
+    for _ in range(n_films):
+        nx = int(rng.choice([64, 128, 256, 512]))
+        ny = int(rng.choice([64, 128, 256]))
+        dxy = float(rng.uniform(2e-9, 5e-9))
+        mat = dict(ms=float(rng.uniform(4e5, 1.2e6)), a_ex=float(rng.uniform(5e-12, 2e-11)),
+                   k_u=float(rng.uniform(0.0, 5e5)), easy_axis=(0.0, 0.0, 1.0),
+                   dmi=float(rng.uniform(0.0, 4e-3)), demag=True)
+        alpha = float(rng.uniform(0.01, 0.5))
+        mh = rng.standard_normal((ny, nx, 3))
+        for _ in range(3):
+            mh = 0.25 * (np.roll(mh, 1, 0) + np.roll(mh, -1, 0) + np.roll(mh, 1, 1) + np.roll(mh, -1, 1))
+        mh /= np.linalg.norm(mh, axis=2, keepdims=True)
+        films.append(dict(nx=nx, ny=ny, cell=(dxy, dxy, 3e-9), material=mat, alpha=alpha,
+                          m0=mh.reshape(-1, 3)))
+    return films

@@ -144,7 +144,7 @@ EXPORTED = [
     "llgmi_dev_build_h_eff", "llgmi_dev_add_demag_field_pbc", "llgmi_dev_rk45_step",
     "llgmi_dev_rk45_step_batch",
     "llgmi_dev_upload", "llgmi_dev_download", "llgmi_profile_enable", "llgmi_profile_reset",
-    "llgmi_profile_read", "llgmi_mg_config_default", "llgmi_mg_config_from_env",
+    "llgmi_profile_read", "llgmi_profile_fft_passes", "llgmi_mg_config_default", "llgmi_mg_config_from_env",
     "llgmi_ctx_set_mg_config", "llgmi_add_demag_field_poisson_mg",
     "llgmi_compute_demag_field_poisson_mg", "llgmi_mg_last_stats",
     "llgmi_relax_settings_default", "llgmi_relax_settings_mumax_like", "llgmi_rk23_relax_step",
@@ -416,6 +416,7 @@ def lib():
     L.llgmi_profile_enable.argtypes = [vp, C.c_int]
     L.llgmi_profile_reset.argtypes = [vp]
     L.llgmi_profile_read.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_int64)]
+    L.llgmi_profile_fft_passes.argtypes = [vp, vp, C.c_double, C.c_int, dp]
     L.llgmi_mg_config_default.argtypes = [C.POINTER(MGConfig)]
     L.llgmi_mg_config_from_env.argtypes = [C.POINTER(MGConfig)]
     L.llgmi_ctx_set_mg_config.argtypes = [vp, C.POINTER(MGConfig)]
@@ -961,3 +962,10 @@ class Context:
         t, c = C.c_double(), C.c_int64()
         _check(lib().llgmi_profile_read(self._h, cls, C.byref(t), C.byref(c)))
         return t.value, c.value
+
+    def profile_fft_passes(self, m_dev, ms: float, reps: int):
+        """Average microseconds of the row R2C, column and row C2R passes, each
+        launched reps times back to back (llgmi_profile_fft_passes)."""
+        out = np.zeros(3)
+        _check(lib().llgmi_profile_fft_passes(self._h, _ptr(m_dev), float(ms), int(reps), _dp(out)))
+        return {"fft_row_fwd": float(out[0]), "fft_col_mul": float(out[1]), "fft_row_inv": float(out[2])}

@@ -324,3 +324,58 @@ def test_kernel_cache_automatic_with_env(gpu_lib, tmp_path, monkeypatch):
     assert path.exists()
     b = L.Context(g, precision=L.F64).compute_demag_field(m, L.Material(ms=8e5, demag=True))
     assert rel_err(b, a) <= 1e-13
+
+
+def test_kernel_cache_automatic_rebuilds_a_truncated_file(gpu_lib, tmp_path, monkeypatch):
+    """LLGMI_DEMAG_CACHE=1 with a short cache file at the automatic path: the
+    file is a miss, as try_load_kernel_kspace(..).unwrap_or(false) makes it
+    (demag_fft_uniform.rs:499-512); the operator is built and the file written
+    again, whole."""
+    L = gpu_lib
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("LLGMI_DEMAG_CACHE", "1")
+    g = L.Grid2D(20, 12, 4e-9, 4e-9, 3e-9)
+    m = _rand_m(240, 78)
+    a = L.Context(g, precision=L.F64).compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    path = tmp_path / L.demag_cache_path(g)
+    whole = path.read_bytes()
+    path.write_bytes(whole[: len(whole) // 3])          # truncated mid-kernel
+    b = L.Context(g, precision=L.F64).compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(b, a) <= 1e-13
+    assert path.read_bytes() == whole
+    path.write_bytes(whole[:20])                        # truncated mid-header
+    c = L.Context(g, precision=L.F64).compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(c, a) <= 1e-13
+    assert path.read_bytes() == whole
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_cache_load_between_graph_replayed_steps(gpu_lib, tmp_path, prec, monkeypatch):
+    """Loading a kernel cache rebuilds the operator that captured RK45 attempts
+    read: steps replayed from graphs, then demag_cache_load, then more steps,
+    stay bitwise equal to the same sequence launched eagerly."""
+    from conftest import DeviceBuffer
+    L = gpu_lib
+    P, es = (L.F32, 4) if prec == "f32" else (L.F64, 8)
+    nx, ny, cell = 24, 16, (3e-9, 3e-9, 2e-9)
+    m0 = _rand_m(nx * ny, 91)
+    mat = L.Material(ms=8e5, a_ex=1.3e-11, k_u=2e4, demag=True)
+    src = str(tmp_path / "k.bin")
+    L.Context(L.Grid2D(nx, ny, *cell), precision=P).demag_cache_save(0, 0, path=src)
+    cg = L.Context(L.Grid2D(nx, ny, *cell), precision=P)
+    ce = L.Context(L.Grid2D(nx, ny, *cell), precision=P)
+    bg, be = DeviceBuffer(3 * nx * ny * es), DeviceBuffer(3 * nx * ny * es)
+    cg.dev_upload(m0, bg)
+    ce.dev_upload(m0, be)
+    pg = L.LLGParams(alpha=0.1, dt=1e-13, b_ext=(0.0, 0.0, 0.02))
+    pe = L.LLGParams(alpha=0.1, dt=1e-13, b_ext=(0.0, 0.0, 0.02))
+    for step in range(10):
+        if step == 5:
+            assert cg.demag_cache_load(0, 0, path=src)
+            assert ce.demag_cache_load(0, 0, path=src)
+        monkeypatch.delenv("LLGMI_RK45_GRAPH", raising=False)
+        rg = cg.dev_rk45_step(bg, pg, mat, 2, 1e-5, 0.8, 1e-18, 1e-11)
+        monkeypatch.setenv("LLGMI_RK45_GRAPH", "0")
+        re_ = ce.dev_rk45_step(be, pe, mat, 2, 1e-5, 0.8, 1e-18, 1e-11)
+        assert rg == re_, step
+        assert np.array_equal(cg.dev_download(bg), ce.dev_download(be)), step
