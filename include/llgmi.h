@@ -157,6 +157,21 @@ int llgmi_rk45_reset(llgmi_ctx *ctx);
 int llgmi_dev_build_h_eff(llgmi_ctx *ctx, const void *m_dev, void *b_dev,
                           const llgmi_params *params, const llgmi_material *mat,
                           int field_mask, const uint8_t *geom_mask_dev);
+/* llgmi_dev_build_h_eff that allocates nothing (no hipMalloc, no library
+ * allocation), for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation.  No
+ * counterpart in the reference.  Its scratch (the FFT spectrum, plus B_demag
+ * when the inverse row pass runs unfused) is one 48 MiB device array per GPU,
+ * held until the process ends: grids whose scratch exceeds it (FP32 up to
+ * 1024 x 1024 fits, FP64 up to 1024 x 512) get LLGMI_ERR_UNSUPPORTED.  The
+ * context's FFT operator must already be built and tuned by one
+ * llgmi_dev_build_h_eff call with the Full mask (LLGMI_ERR_INVALID
+ * otherwise); the multigrid demag is not offered (LLGMI_ERR_UNSUPPORTED).
+ * All calls on one GPU share the array: calls that overlap on a GPU (other
+ * streams, other threads) race. */
+int llgmi_dev_build_h_eff_noalloc(llgmi_ctx *ctx, const void *m_dev, void *b_dev,
+                                  const llgmi_params *params, const llgmi_material *mat,
+                                  int field_mask, const uint8_t *geom_mask_dev);
 /* Same as llgmi_add_demag_field_pbc on device buffers (accumulate). */
 int llgmi_dev_add_demag_field_pbc(llgmi_ctx *ctx, const void *m_dev, void *b_dev,
                                   const llgmi_material *mat, uint64_t pbc_x, uint64_t pbc_y);

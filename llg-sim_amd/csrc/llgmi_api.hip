@@ -428,6 +428,42 @@ struct Engine : EngineBase {
         launch_local_field<T>(a, m, demag ? d_bd : nullptr, geom, b, kout, stream);
     }
 
+    // field() with every buffer beyond m and b taken from `scr` (scr_bytes
+    // long): the FFT spectrum and, when the inverse row pass runs unfused,
+    // B_demag.  Nothing is allocated or built: the FFT operator must exist and
+    // be tuned, and the multigrid demag is not offered.  Not profiled.
+    void field_noalloc(const T *m, T *b, const llgmi_params &p, const llgmi_material &mat, int mask,
+                       const uint8_t *geom, unsigned char *scr, size_t scr_bytes) {
+        const bool demag = mask == LLGMI_MASK_FULL && mat.demag;
+        LocalArgs<T> a = local_args(p, mat, mask, false, false);
+        if (!demag) {
+            launch_local_field<T>(a, m, nullptr, geom, b, nullptr, stream);
+            return;
+        }
+        if (resolved_demag_method(mat) != LLGMI_DEMAG_FFT)
+            throw Error(LLGMI_ERR_UNSUPPORTED, "the allocation-free field evaluation offers the FFT demag only");
+        if (!fft.built || fft.rq_fused == 0)
+            throw Error(LLGMI_ERR_INVALID, "the context's FFT demag operator is not set up: call "
+                                           "llgmi_dev_build_h_eff (Full mask, FFT demag) once first");
+        const size_t spec_bytes = sizeof(cpx<T>) * 3 * (size_t)(fft.px / 2) * (size_t)fft.ny;
+        const size_t bd_bytes = fft.rq_fused < 0 ? sizeof(T) * 3 * (size_t)n : 0;
+        const size_t need = ((spec_bytes + 255) & ~size_t(255)) + bd_bytes;
+        if (need > scr_bytes)
+            throw Error(LLGMI_ERR_UNSUPPORTED, "the allocation-free field evaluation needs " + std::to_string(need) +
+                                                   " bytes of scratch for this grid; the maximum is " +
+                                                   std::to_string(scr_bytes));
+        cpx<T> *own = fft.spec;
+        fft.spec = reinterpret_cast<cpx<T> *>(scr);
+        T *bd = bd_bytes ? reinterpret_cast<T *>(scr + ((spec_bytes + 255) & ~size_t(255))) : nullptr;
+        try {
+            fft.field_local(m, T(mat.ms), a, geom, b, nullptr, bd, stream, nullptr);
+        } catch (...) {
+            fft.spec = own;
+            throw;
+        }
+        fft.spec = own;
+    }
+
     // The staging buffers must exist before a destination is named: callers
     // pass d_m etc. by value, so ensure_io() runs before the call (a null dst
     // is rejected here rather than written through on the device).
@@ -953,6 +989,36 @@ void Engine<T>::relax(T *m, llgmi_params &p, const llgmi_material &mat, int mask
     finish(LLGMI_STOP_TIGHTEN_FLOOR_REACHED, last_stage);
 }
 
+// Device scratch of llgmi_dev_build_h_eff_noalloc: one array per GPU, held
+// until the process ends.  Host code never names the array itself (its host
+// symbol is not a device address): each device's address comes from
+// hipGetSymbolAddress, taken once per device when the first context on it is
+// created.  Every call on a GPU shares that GPU's array; calls that overlap on
+// one GPU race.
+constexpr size_t kNoAllocScratchBytes = size_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// with `device` current
+void ensure_scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error(LLGMI_ERR_UNSUPPORTED, "device index beyond 64");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *ptr = nullptr;
+    LLGMI_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(LLGMI_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[device] = ptr;
+}
+
+void *scratch_address(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
+
 struct llgmi_ctx {
     llgmi_grid grid{};
     int precision = LLGMI_F64;
@@ -1085,6 +1151,7 @@ int llgmi_ctx_create(const llgmi_grid *grid, int precision, int device, llgmi_ct
     c->device = device;
     try {
         LLGMI_HIP(hipSetDevice(device));
+        ensure_scratch_address(device);
         LLGMI_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         if (precision == LLGMI_F32) c->ef.reset(new Engine<float>(*grid, c->stream, &c->prof));
         else c->ed.reset(new Engine<double>(*grid, c->stream, &c->prof));
@@ -1217,6 +1284,28 @@ int llgmi_dev_build_h_eff(llgmi_ctx *ctx, const void *m_dev, void *b_dev, const 
         with_engine(ctx, [&](auto &eng) {
             using T = std::remove_pointer_t<decltype(eng.d_m)>;
             eng.field((const T *)m_dev, (T *)b_dev, *params, *mat, field_mask, geom_mask_dev, nullptr);
+        });
+    });
+}
+
+int llgmi_dev_build_h_eff_noalloc(llgmi_ctx *ctx, const void *m_dev, void *b_dev, const llgmi_params *params,
+                                  const llgmi_material *mat, int field_mask, const uint8_t *geom_mask_dev) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (!params) return fail(LLGMI_ERR_INVALID, "params is NULL");
+    std::string e = check_mat(mat);
+    if (e.empty()) e = check_mask(field_mask);
+    const size_t n = (size_t)(ctx->grid.nx * ctx->grid.ny);
+    if (e.empty()) e = check_dev(m_dev, 3 * n * elem_size(ctx), "m");
+    if (e.empty()) e = check_dev(b_dev, 3 * n * elem_size(ctx), "b");
+    if (e.empty() && geom_mask_dev) e = check_dev(geom_mask_dev, n, "geom_mask");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    void *scr = scratch_address(ctx->device);
+    if (!scr) return fail(LLGMI_ERR_INVALID, "no scratch address for the context's device");
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.field_noalloc((const T *)m_dev, (T *)b_dev, *params, *mat, field_mask, geom_mask_dev,
+                              (unsigned char *)scr, kNoAllocScratchBytes);
         });
     });
 }

@@ -141,7 +141,7 @@ EXPORTED = [
     "llgmi_build_h_eff_masked", "llgmi_build_h_eff_masked_geom", "llgmi_build_h_eff",
     "llgmi_add_demag_field_pbc", "llgmi_compute_demag_field_pbc", "llgmi_add_demag_field",
     "llgmi_rk45_step", "llgmi_rk45_step_adaptive", "llgmi_rk45_reset",
-    "llgmi_dev_build_h_eff", "llgmi_dev_add_demag_field_pbc", "llgmi_dev_rk45_step",
+    "llgmi_dev_build_h_eff", "llgmi_dev_build_h_eff_noalloc", "llgmi_dev_add_demag_field_pbc", "llgmi_dev_rk45_step",
     "llgmi_dev_rk45_step_batch",
     "llgmi_dev_upload", "llgmi_dev_download", "llgmi_profile_enable", "llgmi_profile_reset",
     "llgmi_profile_read", "llgmi_profile_fft_passes", "llgmi_mg_config_default", "llgmi_mg_config_from_env",
@@ -406,6 +406,7 @@ def lib():
                                            C.c_double, dp, ip, dp]
     L.llgmi_rk45_reset.argtypes = [vp]
     L.llgmi_dev_build_h_eff.argtypes = [vp, vp, vp, P, M, C.c_int, vp]
+    L.llgmi_dev_build_h_eff_noalloc.argtypes = [vp, vp, vp, P, M, C.c_int, vp]
     L.llgmi_dev_add_demag_field_pbc.argtypes = [vp, vp, vp, M, C.c_uint64, C.c_uint64]
     L.llgmi_dev_rk45_step.argtypes = [vp, vp, P, M, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, dp, ip, dp]
@@ -915,6 +916,15 @@ class Context:
         p, cm = params._c(), mat._c()
         _check(lib().llgmi_dev_build_h_eff(self._h, _ptr(m_dev), _ptr(b_dev), C.byref(p),
                                            C.byref(cm), mask, _ptr(geom_dev)))
+
+    def dev_build_h_eff_noalloc(self, m_dev, b_dev, params: LLGParams, mat: Material, mask: int,
+                                geom_dev=None) -> None:
+        """dev_build_h_eff with no allocation (llgmi_dev_build_h_eff_noalloc):
+        the scratch is the library's per-GPU static array; the context's FFT
+        operator must already be set up by one dev_build_h_eff call."""
+        p, cm = params._c(), mat._c()
+        _check(lib().llgmi_dev_build_h_eff_noalloc(self._h, _ptr(m_dev), _ptr(b_dev), C.byref(p),
+                                                   C.byref(cm), mask, _ptr(geom_dev)))
 
     def dev_add_demag_field_pbc(self, m_dev, b_dev, mat: Material, pbc_x=0, pbc_y=0) -> None:
         cm = mat._c()
