@@ -121,8 +121,11 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
     cpx<T> *seq = buf + p * SLEN;
     reg_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
     __syncthreads();
+    {
+        cpx<T> *wb = seq + lds_pad(t);
 #pragma unroll
-    for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
+        for (int r = 0; r < R0; ++r) wb[lds_pad(r * TPS)] = v[r];
+    }
     __syncthreads();
     constexpr int HXS = L / 2;  // packed columns
     constexpr int NR = 2 * PAIRS;
@@ -182,15 +185,18 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
     cpx<T> *seq = buf + sidx * SLEN;
     reg_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
     __syncthreads();
+    {
+        cpx<T> *wb = seq + lds_pad(t);
 #pragma unroll
-    for (int r = 0; r < R0; ++r) seq[lds_pad(t + r * TPS)] = v[r];
+        for (int r = 0; r < R0; ++r) wb[lds_pad(r * TPS)] = v[r];
+    }
     __syncthreads();
     // kernel planes [Kxx, Kxy, Kyy, Kzz][hxs + 1][HY]: the (x, y) blocks read
     // three planes, the z blocks only Kzz
     const int64_t plane = (int64_t)(hxs + 1) * HY;
     if (kx != 0) {
         const int64_t base = (int64_t)(active ? kx : 0) * HY;
-        const cpx<T> *other = buf + (sidx ^ 1) * SLEN;
+        const cpx<T> *other = buf + (sidx ^ 1) * SLEN + lds_pad(t);
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int ky = t + r * TPS;
@@ -203,7 +209,7 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
                 const T kd = kern[(comp == 0 ? 0 : 2 * plane) + base + kq];
                 const T k01 = kern[plane + base + kq];
                 const T kxy = ky < HY ? k01 : -k01;
-                const cpx<T> oth = other[lds_pad(ky)];
+                const cpx<T> oth = other[lds_pad(r * TPS)];
                 if (comp == 0)
                     v[r] = {scale * (kd * own.x + kxy * oth.x), scale * (kd * own.y + kxy * oth.y)};
                 else
@@ -278,8 +284,11 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hxs, int nx, int ny, const cpx<T> *_
     const int p = threadIdx.x / TPS, t = threadIdx.x - p * TPS;
     cpx<T> *seq = buf + p * SLEN;
     cpx<T> v[R0];
+    {
+        const cpx<T> *rb = seq + lds_pad(t);
 #pragma unroll
-    for (int r = 0; r < R0; ++r) v[r] = seq[lds_pad(t + r * TPS)];
+        for (int r = 0; r < R0; ++r) v[r] = rb[lds_pad(r * TPS)];
+    }
     reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
     const int ja = j0 + 2 * p;
     T *o = out + comp * n;
@@ -347,8 +356,11 @@ k_row_c2r_local(const cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__r
     const int sq = threadIdx.x / TPS, t = threadIdx.x - sq * TPS;
     cpx<T> *seq = buf + sq * SLEN;
     cpx<T> v[R0];
+    {
+        const cpx<T> *rb = seq + lds_pad(t);
 #pragma unroll
-    for (int r = 0; r < R0; ++r) v[r] = seq[lds_pad(t + r * TPS)];
+        for (int r = 0; r < R0; ++r) v[r] = rb[lds_pad(r * TPS)];
+    }
     reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
     __syncthreads();   // every sequence is out of LDS before the rows go in
     T *bdl = reinterpret_cast<T *>(lds_raw);   // [3][2 PAIRS][NYQ]

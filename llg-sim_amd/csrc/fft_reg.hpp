@@ -15,7 +15,12 @@
 // reordering pass.  L <= 16 is one in-register DFT with no LDS at all.
 //
 // LDS layout of a sequence: element i at lds_pad(i) = i + i/16, which spreads
-// the stride-R0 writes of the first stage over the banks.
+// the stride-R0 writes of the first stage over the banks.  Every access below
+// is a per-thread base plus a compile-time offset: for the power-of-two
+// strides used here lds_pad(a + b) = lds_pad(a) + lds_pad(b) whenever b is a
+// multiple of 16, or a's position within its aligned 16-element group plus b
+// stays inside that group (checked exhaustively for every plan), so the
+// compiler emits one address per stage and immediate ds_read/ds_write offsets.
 #pragma once
 
 #include "common.hpp"
@@ -142,13 +147,14 @@ __device__ __forceinline__ void reg_fft_rest(cpx<T> *v, cpx<T> *seq, int t,
         constexpr int TWS = LOG2L - LOG2NS - LOG2R;
         constexpr bool LAST = LOG2NS + LOG2R == LOG2L;
         __syncthreads();  // previous stage's LDS writes are visible
+        const cpx<T> *rb = seq + lds_pad(t);
 #pragma unroll
         for (int q = 0; q < BPT; ++q) {
             const int j = t + q * TPS;
             const int k = j & (NS - 1);
             cpx<T> *u = v + q * R;
 #pragma unroll
-            for (int r = 0; r < R; ++r) u[r] = seq[lds_pad(j + r * NB)];
+            for (int r = 0; r < R; ++r) u[r] = rb[lds_pad(q * TPS) + lds_pad(r * NB)];
             apply_twiddles<T, INV, R>(u, tw, k << TWS);
             dft_reg<T, INV, R>(u);
         }
@@ -159,8 +165,9 @@ __device__ __forceinline__ void reg_fft_rest(cpx<T> *v, cpx<T> *seq, int t,
                 const int j = t + q * TPS;
                 const int k = j & (NS - 1);
                 const int idxD = ((j >> LOG2NS) << (LOG2NS + LOG2R)) + k;
+                cpx<T> *wb = seq + lds_pad(idxD);
 #pragma unroll
-                for (int r = 0; r < R; ++r) seq[lds_pad(idxD + r * NS)] = v[q * R + r];
+                for (int r = 0; r < R; ++r) wb[lds_pad(r * NS)] = v[q * R + r];
             }
             reg_fft_rest<T, INV, LOG2L, RQ, LOG2NS + LOG2R>(v, seq, t, tw);
         }
@@ -181,8 +188,9 @@ __device__ __forceinline__ void reg_fft(cpx<T> *v, cpx<T> *seq, int t,
     dft_reg<T, INV, P::R0>(v);
     if constexpr (LOG2L > P::LOG2R0) {
         __syncthreads();  // the caller may have been reading seq
+        cpx<T> *wb = seq + lds_pad(t * P::R0);
 #pragma unroll
-        for (int r = 0; r < P::R0; ++r) seq[lds_pad(t * P::R0 + r)] = v[r];
+        for (int r = 0; r < P::R0; ++r) wb[lds_pad(r)] = v[r];
         reg_fft_rest<T, INV, LOG2L, RQ, P::LOG2R0>(v, seq, t, tw);
     }
 }
