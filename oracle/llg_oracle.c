@@ -394,8 +394,17 @@ static inline double delta_cell(int64_t d) {
     if (a > 0.0) a -= 1.0;
     return a;
 }
+/* Planted-mistake switch for the tests that show what pins the quadrature:
+ * 0 = the reference's rule; 1 = source faces not doubled (no nv *= 2,
+ * nw *= 2); 2 = counts truncated instead of rounded (no + 0.5 in mumax_n).
+ * Only tests set it; every product comparison runs with 0. */
+static int g_quad_variant = 0;
+void orc_set_quad_variant(int v) { g_quad_variant = v; }
+
 /* mumax_n (demag_fft_uniform.rs:1158-1161) */
-static inline int64_t mumax_n(double x) { return (int64_t)floor(fmax(x, 1.0) + 0.5); }
+static inline int64_t mumax_n(double x) {
+    return (int64_t)floor(fmax(x, 1.0) + (g_quad_variant == 2 ? 0.0 : 0.5));
+}
 
 /* mumax_like_h_from_unit_m (demag_fft_uniform.rs:1139-1242): same point
  * placement, loop nesting (v, w, x, y, z) and summation order. */
@@ -414,8 +423,10 @@ void orc_h_from_unit_m(int source_axis, const double r_center[3], const double c
     int64_t nz = mumax_n(cell[2] / max_size);
     int64_t nv = mumax_n(cell[v] / max_size);
     int64_t nw = mumax_n(cell[w] / max_size);
-    nv *= 2;
-    nw *= 2;
+    if (g_quad_variant != 1) {
+        nv *= 2;
+        nw *= 2;
+    }
     double scale = 1.0 / (double)(nv * nw * nx * ny * nz);
     double face_area = cell[v] * cell[w];
     double charge = face_area * scale;

@@ -70,6 +70,8 @@ void orc_fft2(double *data_cplx, int64_t nx, int64_t ny, int inverse);
 /* ---- demag kernel (MuMax-like face-charge quadrature) ------------------- */
 /* (mu0 Nxx, mu0 Nxy, mu0 Nyy, mu0 Nzz) for integer displacement (sx, sy).
    demag_fft_uniform.rs:1076-1097, 1139-1253 */
+/* 0 = the reference's quadrature; 1, 2 = planted mistakes (tests only) */
+void orc_set_quad_variant(int v);
 void orc_kernel_components(double dx, double dy, double dz, int64_t sx, int64_t sy,
                            double accuracy, double out4[4]);
 /* Row h = H from a unit M along source_axis (demag_fft_uniform.rs:1139-1242). */

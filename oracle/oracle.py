@@ -59,6 +59,7 @@ def lib():
         L.orc_llg_rhs.argtypes = [C.c_int64, dp, dp, C.c_double, C.c_double, dp]
         L.orc_fft1d.argtypes = [dp, C.c_int64, C.c_int]
         L.orc_fft2.argtypes = [dp, C.c_int64, C.c_int64, C.c_int]
+        L.orc_set_quad_variant.argtypes = [C.c_int]
         L.orc_kernel_components.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int64,
                                             C.c_int64, C.c_double, dp]
         L.orc_build_kernel_realspace.argtypes = [C.POINTER(_Grid), C.c_int64, C.c_int64, dp, dp, dp, dp]
@@ -280,6 +281,13 @@ def fft2(x: np.ndarray, inverse: bool = False) -> np.ndarray:
     ny, nx = a.shape
     lib().orc_fft2(a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), nx, ny, int(inverse))
     return a
+
+
+def set_quad_variant(v: int) -> None:
+    """0 = the reference's quadrature rule; 1 = source faces not doubled;
+    2 = sample counts truncated instead of rounded (planted mistakes that the
+    tests must detect)."""
+    lib().orc_set_quad_variant(int(v))
 
 
 def kernel_components(dx, dy, dz, sx, sy, accuracy=10.0) -> np.ndarray:

@@ -309,3 +309,146 @@ def test_ctx_pbc_dispatch_falls_back_to_fft():
         b = np.zeros_like(m)
         ctx.add_demag_pbc(m, b, O.Material(ms=8e5, demag=True, demag_method=method), 2, 1)
         np.testing.assert_array_equal(b, want)
+
+
+# ---------------------------------------------------------------------------
+# The quadrature's sampling rule, pinned against Newell's analytic tensor
+# (A. J. Newell, W. Williams, D. J. Dunlop, J. Geophys. Res. 98, 9551 (1993)):
+# the cell-averaged field of a uniformly magnetised prism from the closed-form
+# f (diagonal) and g (off-diagonal) functions and their 27-point second
+# differences, evaluated in 40-digit arithmetic (mpmath) where the
+# differences cancel.  The accuracy-10 face-charge sum
+# (mumax_like_h_from_unit_m, demag_fft_uniform.rs:1139-1242) differs from it by
+# 1e-4..1.2e-3 of the largest component, a deterministic error that depends on
+# the sample counts: a source face not doubled (nv *= 2, nw *= 2 dropped)
+# raises it to 5e-2..9e-2 at the nearest cells, and counts truncated instead of
+# rounded (mumax_n without + 0.5) change it wherever cell / max_size has a
+# fractional part >= 0.5.  Each case below carries the error measured for the
+# reference's rule; the test allows 25 % on it.
+# ---------------------------------------------------------------------------
+
+def _newell_f(x, y, z):
+    import mpmath as mp
+    x, y, z = abs(x), abs(y), abs(z)
+    x2, y2, z2 = x * x, y * y, z * z
+    r = mp.sqrt(x2 + y2 + z2)
+    t = mp.mpf(0)
+    if y != 0 and z2 != x2 and x2 + z2 != 0:
+        t += (y / 2) * (z2 - x2) * mp.asinh(y / mp.sqrt(x2 + z2))
+    if z != 0 and y2 != x2 and x2 + y2 != 0:
+        t += (z / 2) * (y2 - x2) * mp.asinh(z / mp.sqrt(x2 + y2))
+    if x != 0 and y != 0 and z != 0:
+        t -= x * y * z * mp.atan(y * z / (x * r))
+    return t + (2 * x2 - y2 - z2) * r / 6
+
+
+def _newell_g(x, y, z):
+    import mpmath as mp
+    sgn = (1 if x >= 0 else -1) * (1 if y >= 0 else -1)
+    x, y, z = abs(x), abs(y), abs(z)
+    x2, y2, z2 = x * x, y * y, z * z
+    r = mp.sqrt(x2 + y2 + z2)
+    t = mp.mpf(0)
+    if x != 0 and y != 0:
+        if z != 0:
+            t += x * y * z * mp.asinh(z / mp.sqrt(x2 + y2))
+            t -= (z ** 3 / 6) * mp.atan(x * y / (z * r))
+            t -= (z * y2 / 2) * mp.atan(x * z / (y * r))
+            t -= (z * x2 / 2) * mp.atan(y * z / (x * r))
+        t += (y / 6) * (3 * z2 - y2) * mp.asinh(x / mp.sqrt(y2 + z2))
+        t += (x / 6) * (3 * z2 - x2) * mp.asinh(y / mp.sqrt(x2 + z2))
+    return sgn * (t - x * y * r / 3)
+
+
+def _newell_n(fun, X, Y, dx, dy, dz):
+    """N = -(1 / 4 pi V) D2x D2y D2z fun at (X, Y, 0), D2 the second difference."""
+    import mpmath as mp
+    s = mp.mpf(0)
+    for i in (-1, 0, 1):
+        for j in (-1, 0, 1):
+            for k in (-1, 0, 1):
+                c = (-2 if i == 0 else 1) * (-2 if j == 0 else 1) * (-2 if k == 0 else 1)
+                s += c * fun(X + i * dx, Y + j * dy, k * dz)
+    return -s / (4 * mp.pi * dx * dy * dz)
+
+
+def newell_tensor(sx, sy, dx, dy, dz):
+    """(Nxx, Nxy, Nyy, Nzz) between two dx x dy x dz cells sx, sy cells apart."""
+    import mpmath as mp
+    with mp.workdps(40):
+        dx, dy, dz = mp.mpf(dx), mp.mpf(dy), mp.mpf(dz)
+        X, Y = sx * dx, sy * dy
+        nxx = _newell_n(_newell_f, X, Y, dx, dy, dz)
+        nyy = _newell_n(lambda a, b, c: _newell_f(b, a, c), X, Y, dx, dy, dz)
+        nzz = _newell_n(lambda a, b, c: _newell_f(c, b, a), X, Y, dx, dy, dz)
+        nxy = _newell_n(_newell_g, X, Y, dx, dy, dz)
+        return np.array([float(nxx), float(nxy), float(nyy), float(nzz)])
+
+
+def test_newell_tensor_is_sound():
+    """Cube self term 1/3, trace 1 on the self term for any prism, Aharoni's
+    prism factors, the point-dipole far field, Nxy = 0 on the axes."""
+    assert np.allclose(newell_tensor(0, 0, 1, 1, 1), [1 / 3, 0, 1 / 3, 1 / 3], atol=1e-15)
+    for cell in [(4, 4, 3), (5, 3, 2), (2, 2, 1)]:
+        n = newell_tensor(0, 0, *cell)
+        assert abs(n[0] + n[2] + n[3] - 1) < 1e-14
+        dx, dy, dz = cell
+        assert abs(n[0] - aharoni_n(dy / 2, dz / 2, dx / 2)) < 1e-13
+        assert abs(n[3] - aharoni_n(dx / 2, dy / 2, dz / 2)) < 1e-13
+        assert n[1] == 0.0 and newell_tensor(3, 0, *cell)[1] == 0.0
+    dx, dy, dz = 4.0, 3.0, 2.0
+    sx, sy = 60, 45
+    r = np.array([sx * dx, sy * dy, 0.0])
+    rn = np.linalg.norm(r)
+    rh = r / rn
+    nd = -(dx * dy * dz) / (4 * np.pi * rn ** 3) * (3 * np.outer(rh, rh) - np.eye(3))
+    n = newell_tensor(sx, sy, dx, dy, dz)
+    assert np.allclose(n, [nd[0, 0], nd[0, 1], nd[1, 1], nd[2, 2]], rtol=2e-3, atol=0)
+
+
+# (cell in nm, displacement) -> max |N_quad - N_newell| / max |N_newell| measured for the reference's rule
+QUAD_VS_NEWELL = {
+    ((4, 4, 3), (0, 0)): 7.91e-4, ((4, 4, 3), (1, 0)): 8.42e-4, ((4, 4, 3), (1, 1)): 8.56e-4,
+    ((4, 4, 3), (2, 0)): 2.15e-4, ((4, 4, 3), (2, 1)): 2.20e-4, ((4, 4, 3), (1, 2)): 2.20e-4,
+    ((4, 4, 3), (3, 2)): 3.42e-4,
+    ((2, 2, 1), (0, 0)): 8.39e-4, ((2, 2, 1), (1, 0)): 6.59e-4, ((2, 2, 1), (2, 1)): 1.69e-4,
+    ((5, 3, 2), (0, 0)): 8.30e-4, ((5, 3, 2), (0, 1)): 8.09e-4, ((5, 3, 2), (1, 2)): 1.82e-4,
+    ((5, 3, 2), (3, 2)): 2.83e-4,
+    ((3, 3, 3), (0, 0)): 5.86e-4, ((3, 3, 3), (1, 1)): 1.16e-3,
+    ((4, 3, 2), (0, 0)): 8.18e-4, ((4, 3, 2), (2, 0)): 2.75e-4, ((4, 3, 2), (1, 2)): 2.16e-4,
+    ((4, 3, 2), (3, 2)): 9.27e-4,
+}
+
+
+def _quad_err(cell_nm, disp):
+    cell = tuple(c * 1e-9 for c in cell_nm)
+    nq = -np.asarray(O.kernel_components(*cell, *disp)) / MU0
+    nw = newell_tensor(*disp, *cell_nm)
+    return float(np.abs(nq - nw).max() / np.abs(nw).max())
+
+
+@pytest.mark.parametrize("case", sorted(QUAD_VS_NEWELL))
+def test_quadrature_error_against_newell(case):
+    """The oracle's accuracy-10 kernel differs from the analytic tensor by the
+    recorded amount (within 25 %), and by at most 1.2e-3 anywhere."""
+    err = _quad_err(*case)
+    ref = QUAD_VS_NEWELL[case]
+    assert 0.8 * ref <= err <= 1.25 * ref, (case, err)
+    assert err <= 1.2e-3
+
+
+def test_planted_quadrature_mistakes_fail_the_newell_bands():
+    """With each planted mistake the oracle leaves the bands above: an
+    undoubled source face on every near-field case, truncated sample counts
+    on every case where a count has a fractional part >= 0.5."""
+    try:
+        O.set_quad_variant(1)
+        for case, ref in QUAD_VS_NEWELL.items():
+            assert _quad_err(*case) > 1.25 * ref, case
+        O.set_quad_variant(2)
+        moved = [c for c, ref in QUAD_VS_NEWELL.items() if not 0.8 * ref <= _quad_err(*c) <= 1.25 * ref]
+        assert set(moved) == {((4, 4, 3), (2, 0)), ((4, 4, 3), (1, 2)), ((5, 3, 2), (1, 2)),
+                              ((5, 3, 2), (3, 2)), ((4, 3, 2), (2, 0)), ((4, 3, 2), (1, 2)),
+                              ((4, 3, 2), (3, 2))}, moved
+    finally:
+        O.set_quad_variant(0)
