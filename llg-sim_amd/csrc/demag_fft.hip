@@ -28,6 +28,7 @@
 #include "common.hpp"
 #include "fft_reg.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -153,37 +154,56 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
 
 // ---------------------------------------------------------------------------
 // Pass 2: column FFT, tensor multiply, inverse column FFT.
-// Every block transforms two sequences, with TPS threads each:
-//   blocks [0, hxs):            (x, y) of column kx = block   (coupled by Kxy)
-//   blocks [hxs, hxs + hxs/2):  z of columns 2b' and 2b'+1    (Kzz only)
-// Two sequences per block (instead of three) keep the block at 2*TPS threads
-// and ~35 KB of LDS, so two blocks fit per CU.  The forward outputs stay in
-// registers; x and y exchange through LDS for the off-diagonal Kxy terms.
+// A work item transforms two sequences, with TPS threads each:
+//   items [0, hxs):            (x, y) of column kx = item   (coupled by Kxy)
+//   items [hxs, hxs + hxs/2):  z of columns 2w' and 2w'+1   (Kzz only)
+// Two sequences per block keep the block at 2*TPS threads and ~35 KB of LDS.
+// The forward outputs stay in registers; x and y exchange through LDS for the
+// off-diagonal Kxy terms.  PIPE = false: one item per block.  PIPE = true:
+// persistent blocks (as many as are resident) step through the items, and
+// the column data of a block's next item is loaded while the current one is
+// transformed, so that loads, arithmetic and stores of different items overlap.
 // ---------------------------------------------------------------------------
+// One work item: v holds the thread's column values (zero where padded) on
+// entry; the item's result is stored to col (null: nothing to store).
 template <typename T, int LOG2L, bool HALF, int RQ>
-__global__ void __launch_bounds__((ColCfg<T, LOG2L, RQ>::NT))
-k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
-            const T *__restrict__ kern, T scale) {
+__device__ __forceinline__ void col_item(cpx<T> *v, cpx<T> *col, int comp, int kx, int hxs, int ny, cpx<T> *buf,
+                                         int sidx, int t, const cpx<T> *__restrict__ tw,
+                                         const T *__restrict__ kern, T scale) {
     using C = ColCfg<T, LOG2L, RQ>;
     constexpr int L = C::L, SLEN = C::SLEN, TPS = C::TPS;
     constexpr int R0 = C::P::R0;
     constexpr int HY = L / 2 + 1;
-    extern __shared__ unsigned char lds_raw[];
-    cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
-    const int sidx = threadIdx.x / TPS, t = threadIdx.x - sidx * TPS;
-    const bool xy = (int)blockIdx.x < hxs;
-    const int comp = xy ? sidx : 2;
-    const int kx = xy ? (int)blockIdx.x : 2 * ((int)blockIdx.x - hxs) + sidx;
-    const bool active = kx < hxs;  // the last z block may have one column only
-    cpx<T> *col = S + ((int64_t)comp * hxs + (active ? kx : 0)) * ny;
-    cpx<T> v[R0];
-#pragma unroll
-    for (int r = 0; r < R0; ++r) {
-        const int j = t + r * TPS;
-        v[r] = (active && (HALF ? r < R0 / 2 : j < ny)) ? col[j] : cpx<T>{T(0), T(0)};
-    }
     cpx<T> *seq = buf + sidx * SLEN;
+    // kernel planes [Kxx, Kxy, Kyy, Kzz][hxs + 1][HY]: the (x, y) items read
+    // two planes per sequence (diagonal + Kxy), the z items only Kzz; the
+    // values are read before the forward transform so that their latency
+    // overlaps it
+    const int64_t plane = (int64_t)(hxs + 1) * HY;
+    const T *pa, *pb;
+    if (kx != 0) {
+        pa = kern + (comp == 0 ? 0 : (comp == 1 ? 2 : 3)) * plane + (int64_t)kx * HY;
+        pb = kern + plane + (int64_t)kx * HY;
+    } else {
+        pa = kern + (comp == 0 ? 0 : (comp == 1 ? 2 : 3)) * plane;
+        pb = pa + (int64_t)hxs * HY;
+    }
+    const bool need_b = kx == 0 || comp != 2;
+    T ka[R0], kb[R0];
+    auto load_k = [&] {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int ky = t + r * TPS;
+            const int kq = ky < HY ? ky : L - ky;
+            ka[r] = pa[kq];
+            kb[r] = need_b ? pb[kq] : T(0);
+        }
+    };
+    // FP32: read before the transform; FP64 keeps the registers for the data
+    constexpr bool KPRE = sizeof(T) == 4;
+    if constexpr (KPRE) load_k();
     reg_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
+    if constexpr (!KPRE) load_k();
     __syncthreads();
     {
         cpx<T> *wb = seq + lds_pad(t);
@@ -191,24 +211,18 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
         for (int r = 0; r < R0; ++r) wb[lds_pad(r * TPS)] = v[r];
     }
     __syncthreads();
-    // kernel planes [Kxx, Kxy, Kyy, Kzz][hxs + 1][HY]: the (x, y) blocks read
-    // three planes, the z blocks only Kzz
-    const int64_t plane = (int64_t)(hxs + 1) * HY;
     if (kx != 0) {
-        const int64_t base = (int64_t)(active ? kx : 0) * HY;
         const cpx<T> *other = buf + (sidx ^ 1) * SLEN + lds_pad(t);
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int ky = t + r * TPS;
-            const int kq = ky < HY ? ky : L - ky;
             const cpx<T> own = v[r];
             if (comp == 2) {
-                const T kzz = kern[3 * plane + base + kq];
+                const T kzz = ka[r];
                 v[r] = {scale * (kzz * own.x), scale * (kzz * own.y)};
             } else {
-                const T kd = kern[(comp == 0 ? 0 : 2 * plane) + base + kq];
-                const T k01 = kern[plane + base + kq];
-                const T kxy = ky < HY ? k01 : -k01;
+                const T kd = ka[r];
+                const T kxy = ky < HY ? kb[r] : -kb[r];
                 const cpx<T> oth = other[lds_pad(r * TPS)];
                 if (comp == 0)
                     v[r] = {scale * (kd * own.x + kxy * oth.x), scale * (kd * own.y + kxy * oth.y)};
@@ -220,13 +234,10 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
         // packed column: Z = A + iB with A, B the transforms of the real kx = 0
         // and kx = px/2 columns; Kxy vanishes on both, so per component
         // Y = Ka A + i Kb B = ((Ka+Kb)/2) Z + ((Ka-Kb)/2) conj(Z[-ky])
-        const T *kp = kern + (comp == 0 ? 0 : (comp == 1 ? 2 : 3)) * plane;
-        const T *kb = kp + (int64_t)hxs * HY;
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int ky = t + r * TPS;
-            const int kq = ky < HY ? ky : L - ky;
-            const T a = kp[kq], b = kb[kq];
+            const T a = ka[r], b = kb[r];
             const T c1 = T(0.5) * (a + b), c2 = T(0.5) * (a - b);
             const cpx<T> z = v[r];
             const cpx<T> zp = seq[lds_pad((L - ky) & (L - 1))];
@@ -234,11 +245,81 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
         }
     }
     reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
-    if (active) {
+    if (col) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int j = t + r * TPS;
             if (HALF ? r < R0 / 2 : j < ny) col[j] = v[r];
+        }
+    }
+}
+
+template <typename T, int LOG2L, bool HALF, int RQ, bool PIPE>
+__global__ void __launch_bounds__((ColCfg<T, LOG2L, RQ>::NT))
+k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
+            const T *__restrict__ kern, T scale, int nwork) {
+    using C = ColCfg<T, LOG2L, RQ>;
+    constexpr int TPS = C::TPS;
+    constexpr int R0 = C::P::R0;
+    constexpr int NL = HALF ? R0 / 2 : R0;   // values a thread loads per sequence
+    extern __shared__ unsigned char lds_raw[];
+    cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
+    const int sidx = threadIdx.x / TPS, t = threadIdx.x - sidx * TPS;
+    // column of this thread's sequence in item w (null: none); comp and kx
+    // as the item defines them (kx clamped to a valid column when there is none)
+    auto column = [&](int w, int &comp, int &kx) -> cpx<T> * {
+        const bool xy = w < hxs;
+        comp = xy ? sidx : 2;
+        kx = xy ? w : 2 * (w - hxs) + sidx;
+        const bool ok = w < nwork && kx < hxs;
+        if (!ok) kx = 0;
+        return ok ? S + ((int64_t)comp * hxs + kx) * ny : nullptr;
+    };
+    auto load = [&](const cpx<T> *col, cpx<T> *dst) {
+#pragma unroll
+        for (int r = 0; r < NL; ++r) {
+            const int j = t + r * TPS;
+            dst[r] = (col && (HALF || j < ny)) ? col[j] : cpx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int r = NL; r < R0; ++r) dst[r] = cpx<T>{T(0), T(0)};
+    };
+    if constexpr (!PIPE) {
+        int comp, kx;
+        cpx<T> *col = column((int)blockIdx.x, comp, kx);
+        cpx<T> v[R0];
+        load(col, v);
+        col_item<T, LOG2L, HALF, RQ>(v, col, comp, kx, hxs, ny, buf, sidx, t, tw, kern, scale);
+    } else {
+        // persistent: the next item's column data is in flight while the
+        // current item is transformed
+        int w = blockIdx.x;
+        cpx<T> nxt[NL];
+        {
+            int c0, k0;
+            const cpx<T> *c = column(w, c0, k0);
+#pragma unroll
+            for (int r = 0; r < NL; ++r) {
+                const int j = t + r * TPS;
+                nxt[r] = (c && (HALF || j < ny)) ? c[j] : cpx<T>{T(0), T(0)};
+            }
+        }
+        for (; w < nwork; w += (int)gridDim.x) {
+            int comp, kx;
+            cpx<T> *col = column(w, comp, kx);
+            cpx<T> v[R0];
+#pragma unroll
+            for (int r = 0; r < R0; ++r) v[r] = r < NL ? nxt[r] : cpx<T>{T(0), T(0)};
+            {
+                int cn, kn;
+                const cpx<T> *c = column(w + (int)gridDim.x, cn, kn);
+#pragma unroll
+                for (int r = 0; r < NL; ++r) {
+                    const int j = t + r * TPS;
+                    nxt[r] = (c && (HALF || j < ny)) ? c[j] : cpx<T>{T(0), T(0)};
+                }
+            }
+            col_item<T, LOG2L, HALF, RQ>(v, col, comp, kx, hxs, ny, buf, sidx, t, tw, kern, scale);
         }
     }
 }
@@ -727,6 +808,26 @@ static void allow_lds(K kernel, size_t bytes) {
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+// Blocks of `kernel` (nt threads, lds bytes of dynamic LDS) resident on the
+// current device at once: the grid of a persistent launch.  Cached per
+// (kernel, shape, device).
+static int resident_blocks(const void *kernel, int nt, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void *, int, size_t, int>, int> cache;
+    int dev = 0;
+    LLGMI_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_tuple(kernel, nt, lds, dev);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, ncu = 0;
+    LLGMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, lds));
+    LLGMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int nb = std::max(1, per_cu) * std::max(1, ncu);
+    cache[key] = nb;
+    return nb;
+}
+
 // A column block holds two sequences, 2 * L/R0 threads (at most 1024).
 template <int LG, int RQ>
 constexpr bool col_rq_ok() { return 2 * ((1 << LG) / (1 << RegPlan<LG, RQ>::LOG2R0)) <= 1024; }
@@ -965,11 +1066,15 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         if constexpr (col_rq_ok<LG, 3>()) {
-            allow_lds(k_col_demag<T, LG, true, 3>, ColCfg<T, LG, 3>::LDS);
-            allow_lds(k_col_demag<T, LG, false, 3>, ColCfg<T, LG, 3>::LDS);
+            allow_lds(k_col_demag<T, LG, true, 3, false>, ColCfg<T, LG, 3>::LDS);
+            allow_lds(k_col_demag<T, LG, false, 3, false>, ColCfg<T, LG, 3>::LDS);
+            allow_lds(k_col_demag<T, LG, true, 3, true>, ColCfg<T, LG, 3>::LDS);
+            allow_lds(k_col_demag<T, LG, false, 3, true>, ColCfg<T, LG, 3>::LDS);
         }
-        allow_lds(k_col_demag<T, LG, true, 4>, ColCfg<T, LG, 4>::LDS);
-        allow_lds(k_col_demag<T, LG, false, 4>, ColCfg<T, LG, 4>::LDS);
+        allow_lds(k_col_demag<T, LG, true, 4, false>, ColCfg<T, LG, 4>::LDS);
+        allow_lds(k_col_demag<T, LG, false, 4, false>, ColCfg<T, LG, 4>::LDS);
+        allow_lds(k_col_demag<T, LG, true, 4, true>, ColCfg<T, LG, 4>::LDS);
+        allow_lds(k_col_demag<T, LG, false, 4, true>, ColCfg<T, LG, 4>::LDS);
     });
     built = true;
     autotune(stream, forced);
@@ -1002,20 +1107,29 @@ void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
     dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         const T scale = T(1.0 / ((double)px * (double)py));
+        // rq: log2 of the first/last radix, plus kColPipe for persistent blocks
+        const bool pipe = (rq & kColPipe) != 0;
         auto go = [&](auto rqc) {
             constexpr int RQ = decltype(rqc)::value;
             using C = ColCfg<T, LG, RQ>;
             const int hxs = (int)(px / 2);
-            const dim3 grid((unsigned)(hxs + (hxs + 1) / 2));  // (x,y) blocks + z-pair blocks
-            if (2 * ny == py)
-                hipLaunchKernelGGL((k_col_demag<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, stream,
-                                   spec, hxs, (int)ny, tw_y, kern, scale);
-            else
-                hipLaunchKernelGGL((k_col_demag<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, stream,
-                                   spec, hxs, (int)ny, tw_y, kern, scale);
+            const int nwork = hxs + (hxs + 1) / 2;  // (x, y) items + z-pair items
+            const bool half = 2 * ny == py;
+            auto launch = [&](auto kern_fn, unsigned nb) {
+                hipLaunchKernelGGL(kern_fn, dim3(nb), dim3(C::NT), C::LDS, stream, spec, hxs, (int)ny, tw_y,
+                                   kern, scale, nwork);
+            };
+            if (pipe) {
+                auto kh = half ? k_col_demag<T, LG, true, RQ, true> : k_col_demag<T, LG, false, RQ, true>;
+                const int nb = std::min(nwork, resident_blocks((const void *)kh, C::NT, C::LDS));
+                launch(kh, (unsigned)nb);
+            } else {
+                auto kh = half ? k_col_demag<T, LG, true, RQ, false> : k_col_demag<T, LG, false, RQ, false>;
+                launch(kh, (unsigned)nwork);
+            }
         };
         if constexpr (col_rq_ok<LG, 3>()) {
-            if (rq == 3) go(std::integral_constant<int, 3>{});
+            if ((rq & 7) == 3) go(std::integral_constant<int, 3>{});
             else go(std::integral_constant<int, 4>{});
         } else {
             go(std::integral_constant<int, 4>{});
@@ -1033,12 +1147,11 @@ void DemagFFT<T>::launch_row_inv(T *bd, hipStream_t stream, int rq) {
             constexpr int RQ = decltype(rqc)::value;
             using C = RowCfg<T, LG, RQ>;
             const dim3 grid((unsigned)((ny + 2 * C::PAIRS - 1) / (2 * C::PAIRS)), 3);
-            if (2 * nx == px)
-                hipLaunchKernelGGL((k_row_c2r<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, stream, spec,
-                                   (int)(px / 2), (int)nx, (int)ny, tw_x, bd, n);
-            else
-                hipLaunchKernelGGL((k_row_c2r<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, stream, spec,
-                                   (int)(px / 2), (int)nx, (int)ny, tw_x, bd, n);
+            auto launch = [&](auto kfn) {
+                hipLaunchKernelGGL(kfn, grid, dim3(C::NT), C::LDS, stream, spec, (int)(px / 2), (int)nx, (int)ny,
+                                   tw_x, bd, n);
+            };
+            launch(2 * nx == px ? k_row_c2r<T, LG, true, RQ> : k_row_c2r<T, LG, false, RQ>);
         };
         if (rq == 3) go(std::integral_constant<int, 3>{});
         else go(std::integral_constant<int, 4>{});
@@ -1053,7 +1166,12 @@ void DemagFFT<T>::launch_row_inv(T *bd, hipStream_t stream, int rq) {
 // are launch-bound and keep radix 16 without timing.
 template <typename T>
 void DemagFFT<T>::autotune(hipStream_t stream, int forced) {
+    // LLGMI_FFT_COLPIPE=0|1 forces the one-item-per-block or the persistent
+    // column pass
+    const char *pe = std::getenv("LLGMI_FFT_COLPIPE");
+    const int pipe_forced = pe ? (pe[0] == '1' ? 1 : 0) : -1;
     rq_row = rq_col = rq_row_inv = forced ? forced : 4;
+    if (pipe_forced == 1) rq_col |= kColPipe;
     if (forced || px * py < 256 * 256) return;
     const int64_t n = nx * ny;
     T *m = nullptr, *bd = nullptr;
@@ -1083,11 +1201,17 @@ void DemagFFT<T>::autotune(hipStream_t stream, int forced) {
         // (a pass alone sees other cache contents than inside the pipeline);
         // combinations alternate over three rounds, the minimum per
         // combination decides.
-        float best[8];
+        float best[16];
         for (float &b : best) b = 1e30f;
+        auto allowed = [&](int c) {
+            const int p = (c >> 3) & 1;
+            return pipe_forced < 0 || p == pipe_forced;
+        };
         for (int round = 0; round < 3; ++round)
-            for (int c = 0; c < 8; ++c) {
-                const int rr = 3 + (c & 1), rc = 3 + ((c >> 1) & 1), ri = 3 + ((c >> 2) & 1);
+            for (int c = 0; c < 16; ++c) {
+                if (!allowed(c)) continue;
+                const int rr = 3 + (c & 1), rc = (3 + ((c >> 1) & 1)) | ((c & 8) ? kColPipe : 0),
+                          ri = 3 + ((c >> 2) & 1);
                 const float t = time([&] {
                     launch_row_fwd(m, T(1), stream, rr);
                     launch_col(stream, rc);
@@ -1095,11 +1219,11 @@ void DemagFFT<T>::autotune(hipStream_t stream, int forced) {
                 });
                 if (t < best[c]) best[c] = t;
             }
-        int bc = 0;
-        for (int c = 1; c < 8; ++c)
-            if (best[c] < best[bc]) bc = c;
+        int bc = -1;
+        for (int c = 0; c < 16; ++c)
+            if (allowed(c) && (bc < 0 || best[c] < best[bc])) bc = c;
         rq_row = 3 + (bc & 1);
-        rq_col = 3 + ((bc >> 1) & 1);
+        rq_col = (3 + ((bc >> 1) & 1)) | ((bc & 8) ? kColPipe : 0);
         rq_row_inv = 3 + ((bc >> 2) & 1);
     } catch (...) {
         cleanup();

@@ -131,8 +131,49 @@ __device__ __forceinline__ void apply_twiddles(cpx<T> *u, const cpx<T> *__restri
     }
 }
 
+// The table values a stage's twiddles are formed from (w, w^4, w^8), read
+// before the stage's barrier so that the load latency overlaps the wait.
+template <typename T, int R>
+struct TwBase {
+    cpx<T> w1, w4, w8;
+};
+template <typename T, int R>
+__device__ __forceinline__ TwBase<T, R> twiddle_load(const cpx<T> *__restrict__ tw, int base) {
+    TwBase<T, R> b;
+    b.w1 = tw[base];
+    if constexpr (R >= 8) b.w4 = tw[4 * base];
+    if constexpr (R == 16) b.w8 = tw[8 * base];
+    return b;
+}
+template <typename T, bool INV, int R>
+__device__ __forceinline__ void twiddle_apply(cpx<T> *u, const TwBase<T, R> &b) {
+    if constexpr (R == 2) {
+        u[1] = INV ? cmulc(u[1], b.w1) : cmul(u[1], b.w1);
+    } else {
+        cpx<T> w[R];
+        w[1] = b.w1;
+        w[2] = cmul(w[1], w[1]);
+        w[3] = cmul(w[2], w[1]);
+        if constexpr (R >= 8) {
+            w[4] = b.w4;
+            w[5] = cmul(w[4], w[1]);
+            w[6] = cmul(w[4], w[2]);
+            w[7] = cmul(w[4], w[3]);
+        }
+        if constexpr (R == 16) {
+            w[8] = b.w8;
+#pragma unroll
+            for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+        }
+#pragma unroll
+        for (int r = 1; r < R; ++r) u[r] = INV ? cmulc(u[r], w[r]) : cmul(u[r], w[r]);
+    }
+}
+
 // Middle/last stages.  Called after the first stage wrote its outputs to LDS.
-// If this is the last stage its outputs stay in v.
+// If this is the last stage its outputs stay in v.  The butterfly class k of
+// a thread is the same for all its butterflies of a stage (NS divides TPS),
+// so one set of twiddles serves them all.
 template <typename T, bool INV, int LOG2L, int RQ, int LOG2NS>
 __device__ __forceinline__ void reg_fft_rest(cpx<T> *v, cpx<T> *seq, int t,
                                              const cpx<T> *__restrict__ tw) {
@@ -146,16 +187,16 @@ __device__ __forceinline__ void reg_fft_rest(cpx<T> *v, cpx<T> *seq, int t,
         constexpr int NB = L / R;                    // butterflies per sequence
         constexpr int TWS = LOG2L - LOG2NS - LOG2R;
         constexpr bool LAST = LOG2NS + LOG2R == LOG2L;
+        static_assert(TPS % NS == 0, "butterfly class must not depend on the butterfly");
+        const TwBase<T, R> twb = twiddle_load<T, R>(tw, (t & (NS - 1)) << TWS);
         __syncthreads();  // previous stage's LDS writes are visible
         const cpx<T> *rb = seq + lds_pad(t);
 #pragma unroll
         for (int q = 0; q < BPT; ++q) {
-            const int j = t + q * TPS;
-            const int k = j & (NS - 1);
             cpx<T> *u = v + q * R;
 #pragma unroll
             for (int r = 0; r < R; ++r) u[r] = rb[lds_pad(q * TPS) + lds_pad(r * NB)];
-            apply_twiddles<T, INV, R>(u, tw, k << TWS);
+            twiddle_apply<T, INV, R>(u, twb);
             dft_reg<T, INV, R>(u);
         }
         if constexpr (!LAST) {

@@ -80,6 +80,9 @@ struct ProfScope {
 template <typename T>
 struct LocalArgs;
 
+// rq_col flag: the column pass runs as persistent blocks (k_col_demag PIPE)
+constexpr int kColPipe = 8;
+
 template <typename T>
 struct DemagFFT {
     bool built = false;

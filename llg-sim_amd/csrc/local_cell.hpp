@@ -26,6 +26,29 @@ __device__ __forceinline__ void ghost_y(const T *m, T n_y, T eta, T dy, T *o) {
     o[2] = m[2] + n_y * dy * (eta * m[1]);
 }
 
+// The torque of cell c from its m and total field B: llg_rhs (llg.rs:20-32),
+// -gamma/(1+alpha^2) [m x B + alpha m x (m x B)], or for relaxation
+// llg_rhs_relax (llg.rs:38-50), -gamma alpha/(1+alpha^2) m x (m x B).
+template <typename T>
+__device__ __forceinline__ void torque_cell(const LocalArgs<T> &a, const T *m0, const T *bv, T *__restrict__ kout,
+                                            int64_t c, int64_t n) {
+    T cx = m0[1] * bv[2] - m0[2] * bv[1];
+    T cy = m0[2] * bv[0] - m0[0] * bv[2];
+    T cz = m0[0] * bv[1] - m0[1] * bv[0];
+    T ccx = m0[1] * cz - m0[2] * cy;
+    T ccy = m0[2] * cx - m0[0] * cz;
+    T ccz = m0[0] * cy - m0[1] * cx;
+    if (a.torque == 2) {
+        kout[c] = a.rhs_pref * ccx;
+        kout[n + c] = a.rhs_pref * ccy;
+        kout[2 * n + c] = a.rhs_pref * ccz;
+    } else {
+        kout[c] = a.rhs_pref * (cx + a.alpha * ccx);
+        kout[n + c] = a.rhs_pref * (cy + a.alpha * ccy);
+        kout[2 * n + c] = a.rhs_pref * (cz + a.alpha * ccz);
+    }
+}
+
 // Cell (i, j) of an nx x ny SoA field; bd3 = the cell's demag field (DEMAG).
 template <typename T, bool GEOM, bool DEMAG, bool TORQUE>
 __device__ __forceinline__ void local_cell(const LocalArgs<T> &a, const T *__restrict__ m,
@@ -130,25 +153,7 @@ __device__ __forceinline__ void local_cell(const LocalArgs<T> &a, const T *__res
     b[c] = bv[0];
     b[n + c] = bv[1];
     b[2 * n + c] = bv[2];
-    if (TORQUE) {
-        // llg_rhs: -gamma/(1+alpha^2) [m x B + alpha m x (m x B)]
-        T cx = m0[1] * bv[2] - m0[2] * bv[1];
-        T cy = m0[2] * bv[0] - m0[0] * bv[2];
-        T cz = m0[0] * bv[1] - m0[1] * bv[0];
-        T ccx = m0[1] * cz - m0[2] * cy;
-        T ccy = m0[2] * cx - m0[0] * cz;
-        T ccz = m0[0] * cy - m0[1] * cx;
-        if (a.torque == 2) {
-            // llg_rhs_relax (llg.rs:38-50): -gamma alpha/(1+alpha^2) m x (m x B)
-            kout[c] = a.rhs_pref * ccx;
-            kout[n + c] = a.rhs_pref * ccy;
-            kout[2 * n + c] = a.rhs_pref * ccz;
-        } else {
-            kout[c] = a.rhs_pref * (cx + a.alpha * ccx);
-            kout[n + c] = a.rhs_pref * (cy + a.alpha * ccy);
-            kout[2 * n + c] = a.rhs_pref * (cz + a.alpha * ccz);
-        }
-    }
+    if (TORQUE) torque_cell(a, m0, bv, kout, c, n);
 }
 
 }  // namespace llgmi
