@@ -225,29 +225,43 @@ __global__ void __launch_bounds__(kNT) k_mg_rb(double *__restrict__ phi, const d
 // the colour-0 update only old values, so with separate input and output
 // buffers the sweep is the reference's in-place pair of half-sweeps exactly.
 // A block owns an RBX x RBY tile of interior (i, j) columns and a chunk of
-// RBZ planes and marches up k; the new colour-0 values of planes k-1, k, k+1
-// on the tile plus a one-cell halo sit in LDS (halo cells and the plane below
-// the chunk recomputed, boundary cells keep their value).  rbz = planes per
-// chunk (z-chunks add blocks at the cost of one recomputed plane each).
-constexpr int RBX = 32, RBY = 8, RBW = RBX + 2, RBH = RBY + 2;
+// RBZ planes and marches up k.  Two rings in LDS: the old values A of four
+// planes on the tile plus a two-cell halo (each plane read from HBM once,
+// when the march reaches it), and the new colour-0 values of planes k-1, k,
+// k+1 on the tile plus a one-cell halo (halo cells and the plane below the
+// chunk recomputed, boundary cells keep their value).  rbz = planes per
+// chunk (z-chunks add blocks at the cost of recomputed planes).
+constexpr int RBX = 32, RBY = 8, RBW = RBX + 2, RBH = RBY + 2, RAW = RBX + 4, RAH = RBY + 4;
 __global__ void __launch_bounds__(RBX * RBY) k_mg_rb_sweep(const double *__restrict__ A, double *__restrict__ B,
                                                        const double *__restrict__ rhs, int nx, int ny,
                                                        int nz, double sx, double sy, double sz, double denom,
                                                        double omega, int rbz) {
     __shared__ double red[3][RBH][RBW];
+    __shared__ double av[4][RAH][RAW];   // A on plane p in slot p % 4, origin (i0 - 2, j0 - 2)
     const int tid = threadIdx.x;
     const int i0 = 1 + (int)blockIdx.x * RBX, j0 = 1 + (int)blockIdx.y * RBY;
     const int kb = 1 + (int)blockIdx.z * rbz, ke = min(nz - 2, kb + rbz - 1);
-    const int64_t plane = (int64_t)nx * ny;
+    auto load = [&](int k) {
+        if (k < 0 || k >= nz) return;
+        double(*a)[RAW] = av[k & 3];
+        for (int q = tid; q < RAH * RAW; q += RBX * RBY) {
+            const int hj = q / RAW, hi = q - hj * RAW;
+            const int i = i0 - 2 + hi, j = j0 - 2 + hj;
+            a[hj][hi] = (i >= 0 && i < nx && j >= 0 && j < ny) ? A[idx3(i, j, k, nx, ny)] : 0.0;
+        }
+    };
     // new colour-0 value at an even cell (i, j, k), or the unchanged value of a
-    // boundary cell
+    // boundary cell; A from the LDS ring (planes k-1, k, k+1 loaded)
     auto red_new = [&](int i, int j, int k) -> double {
-        const int64_t id = idx3(i, j, k, nx, ny);
-        if (i <= 0 || i >= nx - 1 || j <= 0 || j >= ny - 1 || k <= 0 || k >= nz - 1) return A[id];
-        const double off = sx * (A[id - 1] + A[id + 1]) + sy * (A[id - nx] + A[id + nx]) +
-                           sz * (A[id - plane] + A[id + plane]);
-        const double pn = (off - rhs[id]) / denom;
-        const double po = A[id];
+        const int hi = i - (i0 - 2), hj = j - (j0 - 2);
+        const double(*ac)[RAW] = av[k & 3];
+        if (i <= 0 || i >= nx - 1 || j <= 0 || j >= ny - 1 || k <= 0 || k >= nz - 1) return ac[hj][hi];
+        const double(*am)[RAW] = av[(k - 1) & 3];
+        const double(*ap)[RAW] = av[(k + 1) & 3];
+        const double off = sx * (ac[hj][hi - 1] + ac[hj][hi + 1]) + sy * (ac[hj - 1][hi] + ac[hj + 1][hi]) +
+                           sz * (am[hj][hi] + ap[hj][hi]);
+        const double pn = (off - rhs[idx3(i, j, k, nx, ny)]) / denom;
+        const double po = ac[hj][hi];
         return po + omega * (pn - po);
     };
     // only even-parity positions are ever read by the colour-1 update: the
@@ -260,6 +274,8 @@ __global__ void __launch_bounds__(RBX * RBY) k_mg_rb_sweep(const double *__restr
             red[slot][hj][hi] = (i < nx && j < ny) ? red_new(i, j, k) : 0.0;
         }
     };
+    for (int k = kb - 2; k <= kb + 1; ++k) load(k);
+    __syncthreads();
     fill((kb - 1) % 3, kb - 1);
     fill(kb % 3, kb);
     const int ti = tid % RBX, tj = tid / RBX;
@@ -281,11 +297,13 @@ __global__ void __launch_bounds__(RBX * RBY) k_mg_rb_sweep(const double *__restr
         if (i == nx - 2 && j == 1) keep(nx - 1, 0);
         if (i == 1 && j == ny - 2) keep(0, ny - 1);
         if (i == nx - 2 && j == ny - 2) keep(nx - 1, ny - 1);
-        if (kb == 1) B[idx3(i, j, 0, nx, ny)] = A[idx3(i, j, 0, nx, ny)];
+        if (kb == 1) B[idx3(i, j, 0, nx, ny)] = av[0][tj + 2][ti + 2];
         if (ke == nz - 2) B[idx3(i, j, nz - 1, nx, ny)] = A[idx3(i, j, nz - 1, nx, ny)];
     }
     for (int k = kb; k <= ke; ++k) {
         const int sm = (k - 1) % 3, sc = k % 3, sp = (k + 1) % 3;
+        load(k + 2);   // slot of plane k - 2, which no step below reads
+        __syncthreads();
         fill(sp, k + 1);
         __syncthreads();
         if (inside) {
@@ -298,11 +316,10 @@ __global__ void __launch_bounds__(RBX * RBY) k_mg_rb_sweep(const double *__restr
                                    sy * (red[sc][hj - 1][hi] + red[sc][hj + 1][hi]) +
                                    sz * (red[sm][hj][hi] + red[sp][hj][hi]);
                 const double pn = (off - rhs[id]) / denom;
-                const double po = A[id];
+                const double po = av[k & 3][tj + 2][ti + 2];
                 B[id] = po + omega * (pn - po);
             }
         }
-        __syncthreads();
     }
 }
 
@@ -885,7 +902,10 @@ __device__ __forceinline__ void cell_pos(const BoxGeom &g, int i, int j, int k, 
 // the wave's next node is n + 1 if any lane opened n and skip[n] otherwise.
 // Node, moment and charge loads are wave-uniform.  Each lane sees exactly its
 // own traversal in its own order, and keeps one partial sum per open depth,
-// so its value is the reference's nested recursive sum bit for bit.
+// so its value is the reference's nested recursive sum bit for bit.  The
+// partial sums live in LDS, one row of kNT lanes per depth (dynamic LDS of
+// (deepest node depth + 1) rows): indexed by the run-time depth they would
+// otherwise go to per-lane scratch memory.
 __global__ void __launch_bounds__(kNT) k_tc_eval(const TNode *__restrict__ nodes, int nnodes,
                                                  const double *__restrict__ mom, const double *__restrict__ pos,
                                                  const double *__restrict__ qp, BoxGeom g, double theta,
@@ -898,7 +918,8 @@ __global__ void __launch_bounds__(kNT) k_tc_eval(const TNode *__restrict__ nodes
               k = valid ? cell / (g.nx * g.ny) : 0;
     double tx, ty, tz;
     cell_pos(g, i, j, k, tx, ty, tz);
-    double part[kMaxDepth + 2];
+    extern __shared__ double part_lds[];
+    double *part = part_lds + threadIdx.x;   // part[d * kNT]: depth d of this lane
     double total = 0.0;
     bool done = !valid;
     int next = 0;       // this lane's next node
@@ -947,21 +968,21 @@ __global__ void __launch_bounds__(kNT) k_tc_eval(const TNode *__restrict__ nodes
                 }
             }
             if (!accept) {
-                part[d] = 0.0;
+                part[d * kNT] = 0.0;
                 next = n + 1;
                 opened = true;
             } else if (d == 0) {
                 total = v;
                 done = true;
             } else {
-                part[d - 1] += v;
+                part[(d - 1) * kNT] += v;
                 for (int l = d - 1; l >= nd.skip_depth; --l) {
                     if (l == 0) {
                         total = part[0];
                         done = true;
                         break;
                     }
-                    part[l - 1] += part[l];
+                    part[(l - 1) * kNT] += part[l * kNT];
                 }
                 next = nd.skip;
             }
@@ -1460,6 +1481,7 @@ struct MGSolver::Tree {
     int leaf = -1, max_depth = -1;
     bool valid = false;
     int nnodes = 0;
+    int deepest = 0;   // largest node depth of the built tree
     // compaction
     int *d_counts = nullptr, *d_offsets = nullptr, *d_meta = nullptr, *h_meta = nullptr;
     int64_t cap_blocks = 0;
@@ -1784,6 +1806,8 @@ void MGSolver::treecode_bc() {
         T.free_tree();
         T.nnodes = (int)b.nodes.size();
         T.nleaves = (int)b.leaves.size();
+        T.deepest = 0;
+        for (const auto &nd : b.nodes) T.deepest = std::max(T.deepest, nd.depth);
         dev_alloc(T.d_nodes, b.nodes.size());
         dev_alloc(T.d_mom, 4 * b.nodes.size());
         dev_alloc(T.d_pos, pos_leaf.size());
@@ -1845,7 +1869,13 @@ void MGSolver::treecode_bc() {
         dev_alloc(T.d_targets, tg.size());
         LLGMI_HIP(hipMemcpy(T.d_targets, tg.data(), sizeof(int) * tg.size(), hipMemcpyHostToDevice));
     }
-    hipLaunchKernelGGL(k_tc_eval, dim3(nblocks(T.ntargets)), dim3(kNT), 0, stream_, T.d_nodes, T.nnodes, T.d_mom,
+    const size_t lds = sizeof(double) * kNT * (size_t)(T.deepest + 1);
+    if (lds > 64 * 1024) {
+        if (lds > 160 * 1024) throw Error(LLGMI_ERR_UNSUPPORTED, "treecode deeper than the LDS holds");
+        LLGMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_tc_eval),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    hipLaunchKernelGGL(k_tc_eval, dim3(nblocks(T.ntargets)), dim3(kNT), lds, stream_, T.d_nodes, T.nnodes, T.d_mom,
                        T.d_pos, T.d_qp, bg, cfg_.tree_theta, T.d_targets, T.ntargets, bc_, L.phi[L.cur]);
     LLGMI_LAUNCH_CHECK();
 }

@@ -101,38 +101,10 @@ struct RegPlan {
     }
 };
 
-// Twiddles of stage (Ns, R) for butterfly k: w^r with w = tw[k L/(Ns R)], r < R.
-// Only w, w^4 and w^8 are gathered from the table; the other powers are
-// products of at most three table values (a few ulp in FP32), which replaces
-// R-1 scattered loads per butterfly by at most three.
-template <typename T, bool INV, int R>
-__device__ __forceinline__ void apply_twiddles(cpx<T> *u, const cpx<T> *__restrict__ tw, int base) {
-    if constexpr (R == 2) {
-        const cpx<T> w1 = tw[base];
-        u[1] = INV ? cmulc(u[1], w1) : cmul(u[1], w1);
-    } else {
-        cpx<T> w[R];
-        w[1] = tw[base];
-        w[2] = cmul(w[1], w[1]);
-        w[3] = cmul(w[2], w[1]);
-        if constexpr (R >= 8) {
-            w[4] = tw[4 * base];
-            w[5] = cmul(w[4], w[1]);
-            w[6] = cmul(w[4], w[2]);
-            w[7] = cmul(w[4], w[3]);
-        }
-        if constexpr (R == 16) {
-            w[8] = tw[8 * base];
-#pragma unroll
-            for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
-        }
-#pragma unroll
-        for (int r = 1; r < R; ++r) u[r] = INV ? cmulc(u[r], w[r]) : cmul(u[r], w[r]);
-    }
-}
-
-// The table values a stage's twiddles are formed from (w, w^4, w^8), read
-// before the stage's barrier so that the load latency overlaps the wait.
+// Twiddles of stage (Ns, R) for butterfly class k: w^r with w = tw[k L/(Ns R)],
+// r < R.  Only w, w^4 and w^8 are gathered from the table, before the stage's
+// barrier so that the load latency overlaps the wait; the other powers are
+// products of at most three table values (a few ulp in FP32).
 template <typename T, int R>
 struct TwBase {
     cpx<T> w1, w4, w8;
@@ -145,29 +117,31 @@ __device__ __forceinline__ TwBase<T, R> twiddle_load(const cpx<T> *__restrict__ 
     if constexpr (R == 16) b.w8 = tw[8 * base];
     return b;
 }
-template <typename T, bool INV, int R>
-__device__ __forceinline__ void twiddle_apply(cpx<T> *u, const TwBase<T, R> &b) {
-    if constexpr (R == 2) {
-        u[1] = INV ? cmulc(u[1], b.w1) : cmul(u[1], b.w1);
-    } else {
-        cpx<T> w[R];
-        w[1] = b.w1;
+// All powers w^r, r = 1..R-1, from the table values (formed once per stage
+// and thread, shared by the thread's butterflies of that stage).
+template <typename T, int R>
+__device__ __forceinline__ void twiddle_powers(const TwBase<T, R> &b, cpx<T> *w) {
+    w[1] = b.w1;
+    if constexpr (R > 2) {
         w[2] = cmul(w[1], w[1]);
         w[3] = cmul(w[2], w[1]);
-        if constexpr (R >= 8) {
-            w[4] = b.w4;
-            w[5] = cmul(w[4], w[1]);
-            w[6] = cmul(w[4], w[2]);
-            w[7] = cmul(w[4], w[3]);
-        }
-        if constexpr (R == 16) {
-            w[8] = b.w8;
-#pragma unroll
-            for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
-        }
-#pragma unroll
-        for (int r = 1; r < R; ++r) u[r] = INV ? cmulc(u[r], w[r]) : cmul(u[r], w[r]);
     }
+    if constexpr (R >= 8) {
+        w[4] = b.w4;
+        w[5] = cmul(w[4], w[1]);
+        w[6] = cmul(w[4], w[2]);
+        w[7] = cmul(w[4], w[3]);
+    }
+    if constexpr (R == 16) {
+        w[8] = b.w8;
+#pragma unroll
+        for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+    }
+}
+template <typename T, bool INV, int R>
+__device__ __forceinline__ void twiddle_apply(cpx<T> *u, const cpx<T> *w) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) u[r] = INV ? cmulc(u[r], w[r]) : cmul(u[r], w[r]);
 }
 
 // Middle/last stages.  Called after the first stage wrote its outputs to LDS.
@@ -191,12 +165,14 @@ __device__ __forceinline__ void reg_fft_rest(cpx<T> *v, cpx<T> *seq, int t,
         const TwBase<T, R> twb = twiddle_load<T, R>(tw, (t & (NS - 1)) << TWS);
         __syncthreads();  // previous stage's LDS writes are visible
         const cpx<T> *rb = seq + lds_pad(t);
+        cpx<T> w[R];
+        twiddle_powers<T, R>(twb, w);
 #pragma unroll
         for (int q = 0; q < BPT; ++q) {
             cpx<T> *u = v + q * R;
 #pragma unroll
             for (int r = 0; r < R; ++r) u[r] = rb[lds_pad(q * TPS) + lds_pad(r * NB)];
-            twiddle_apply<T, INV, R>(u, twb);
+            twiddle_apply<T, INV, R>(u, w);
             dft_reg<T, INV, R>(u);
         }
         if constexpr (!LAST) {
