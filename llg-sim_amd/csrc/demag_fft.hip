@@ -91,19 +91,16 @@ struct ExtractCfg {
 // dst[comp*dst_cstride + k*nrows + j] for k <= L/2, j < nrows.
 // ---------------------------------------------------------------------------
 // HALF: ncols_valid == L/2, so the upper half of every row is a compile-time zero.
+// The block of component comp whose first row is j0.
 template <typename T, typename Tin, int LOG2L, bool HALF, int RQ>
-__global__ void __launch_bounds__((RowCfg<T, LOG2L, RQ>::NT))
-k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int ncols_valid,
-          int nrows, T scale, const cpx<T> *__restrict__ tw, cpx<T> *__restrict__ dst,
-          int64_t dst_cstride) {
+__device__ __forceinline__ void row_r2c_body(const Tin *__restrict__ src, int64_t src_cstride, int row_stride,
+                                             int ncols_valid, int nrows, T scale, const cpx<T> *__restrict__ tw,
+                                             cpx<T> *__restrict__ dst, int64_t dst_cstride, int comp, int j0) {
     using C = RowCfg<T, LOG2L, RQ>;
     constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
     constexpr int R0 = C::P::R0;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
-    const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int comp = lb / gridDim.x;
-    const int j0 = (lb - comp * gridDim.x) * PAIRS * 2;
     const int p = threadIdx.x / TPS, t = threadIdx.x - p * TPS;
     const int ja = j0 + 2 * p, jb = ja + 1;
     const Tin *ra = src + comp * src_cstride + (int64_t)ja * row_stride;
@@ -150,6 +147,19 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
         }
         d[(int64_t)k * nrows + j] = X;
     }
+}
+
+template <typename T, typename Tin, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((RowCfg<T, LOG2L, RQ>::NT))
+k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int ncols_valid,
+          int nrows, T scale, const cpx<T> *__restrict__ tw, cpx<T> *__restrict__ dst,
+          int64_t dst_cstride) {
+    constexpr int PAIRS = RowCfg<T, LOG2L, RQ>::PAIRS;
+    const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int comp = lb / gridDim.x;
+    const int j0 = (lb - comp * gridDim.x) * PAIRS * 2;
+    row_r2c_body<T, Tin, LOG2L, HALF, RQ>(src, src_cstride, row_stride, ncols_valid, nrows, scale, tw, dst,
+                                          dst_cstride, comp, j0);
 }
 
 // ---------------------------------------------------------------------------
@@ -254,6 +264,32 @@ __device__ __forceinline__ void col_item(cpx<T> *v, cpx<T> *col, int comp, int k
     }
 }
 
+// Item w of the column pass (items [0, nwork)), one block.
+template <typename T, int LOG2L, bool HALF, int RQ>
+__device__ __forceinline__ void col_work(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
+                                         const T *__restrict__ kern, T scale, int nwork, int w) {
+    using C = ColCfg<T, LOG2L, RQ>;
+    constexpr int TPS = C::TPS;
+    constexpr int R0 = C::P::R0;
+    constexpr int NL = HALF ? R0 / 2 : R0;
+    extern __shared__ unsigned char lds_raw[];
+    cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
+    const int sidx = threadIdx.x / TPS, t = threadIdx.x - sidx * TPS;
+    const bool xy = w < hxs;
+    const int comp = xy ? sidx : 2;
+    int kx = xy ? w : 2 * (w - hxs) + sidx;
+    const bool ok = w < nwork && kx < hxs;
+    if (!ok) kx = 0;
+    cpx<T> *col = ok ? S + ((int64_t)comp * hxs + kx) * ny : nullptr;
+    cpx<T> v[R0];
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const int j = t + r * TPS;
+        v[r] = (col && r < NL && (HALF || j < ny)) ? col[j] : cpx<T>{T(0), T(0)};
+    }
+    col_item<T, LOG2L, HALF, RQ>(v, col, comp, kx, hxs, ny, buf, sidx, t, tw, kern, scale);
+}
+
 template <typename T, int LOG2L, bool HALF, int RQ, bool PIPE>
 __global__ void __launch_bounds__((ColCfg<T, LOG2L, RQ>::NT))
 k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
@@ -275,21 +311,8 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
         if (!ok) kx = 0;
         return ok ? S + ((int64_t)comp * hxs + kx) * ny : nullptr;
     };
-    auto load = [&](const cpx<T> *col, cpx<T> *dst) {
-#pragma unroll
-        for (int r = 0; r < NL; ++r) {
-            const int j = t + r * TPS;
-            dst[r] = (col && (HALF || j < ny)) ? col[j] : cpx<T>{T(0), T(0)};
-        }
-#pragma unroll
-        for (int r = NL; r < R0; ++r) dst[r] = cpx<T>{T(0), T(0)};
-    };
     if constexpr (!PIPE) {
-        int comp, kx;
-        cpx<T> *col = column((int)blockIdx.x, comp, kx);
-        cpx<T> v[R0];
-        load(col, v);
-        col_item<T, LOG2L, HALF, RQ>(v, col, comp, kx, hxs, ny, buf, sidx, t, tw, kern, scale);
+        col_work<T, LOG2L, HALF, RQ>(S, hxs, ny, tw, kern, scale, nwork, (int)blockIdx.x);
     } else {
         // persistent: the next item's column data is in flight while the
         // current item is transformed
@@ -332,17 +355,14 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
 // HALF: nx == L/2, so only the lower half of each inverse-transformed row is
 // formed and stored.
 template <typename T, int LOG2L, bool HALF, int RQ>
-__global__ void __launch_bounds__((RowCfg<T, LOG2L, RQ>::NT))
-k_row_c2r(const cpx<T> *__restrict__ S, int hxs, int nx, int ny, const cpx<T> *__restrict__ tw,
-          T *__restrict__ out, int64_t n) {
+__device__ __forceinline__ void row_c2r_body(const cpx<T> *__restrict__ S, int hxs, int nx, int ny,
+                                             const cpx<T> *__restrict__ tw, T *__restrict__ out, int64_t n,
+                                             int comp, int j0) {
     using C = RowCfg<T, LOG2L, RQ>;
     constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
     constexpr int R0 = C::P::R0;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
-    const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int comp = lb / gridDim.x;
-    const int j0 = (lb - comp * gridDim.x) * PAIRS * 2;
     const cpx<T> *s = S + (int64_t)comp * hxs * ny;
     constexpr int NYQ = L >> 1;
     for (int idx = threadIdx.x; idx < NYQ * PAIRS; idx += NT) {
@@ -383,6 +403,17 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hxs, int nx, int ny, const cpx<T> *_
     }
 }
 
+template <typename T, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((RowCfg<T, LOG2L, RQ>::NT))
+k_row_c2r(const cpx<T> *__restrict__ S, int hxs, int nx, int ny, const cpx<T> *__restrict__ tw,
+          T *__restrict__ out, int64_t n) {
+    constexpr int PAIRS = RowCfg<T, LOG2L, RQ>::PAIRS;
+    const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int comp = lb / gridDim.x;
+    const int j0 = (lb - comp * gridDim.x) * PAIRS * 2;
+    row_c2r_body<T, LOG2L, HALF, RQ>(S, hxs, nx, ny, tw, out, n, comp, j0);
+}
+
 // ---------------------------------------------------------------------------
 // Pass 3 fused with the local field: a block takes PAIRS pairs of rows for the
 // three components, inverse-transforms them, parks the real B_demag rows in
@@ -405,10 +436,10 @@ template <typename T, int LG, int RQ>
 constexpr bool fused_ok() { return FusedCfg<T, LG, RQ>::OK; }
 
 template <typename T, int LOG2L, bool HALF, int RQ, bool GEOM, bool TORQUE>
-__global__ void __launch_bounds__((FusedCfg<T, LOG2L, RQ>::NT))
-k_row_c2r_local(const cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw, LocalArgs<T> a,
-                const T *__restrict__ m, const uint8_t *__restrict__ geom, T *__restrict__ b,
-                T *__restrict__ kout) {
+__device__ __forceinline__ void row_c2r_local_body(const cpx<T> *__restrict__ S, int hxs, int ny,
+                                                   const cpx<T> *__restrict__ tw, const LocalArgs<T> &a,
+                                                   const T *__restrict__ m, const uint8_t *__restrict__ geom,
+                                                   T *__restrict__ b, T *__restrict__ kout, int j0) {
     using C = FusedCfg<T, LOG2L, RQ>;
     constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
     constexpr int R0 = C::P::R0;
@@ -416,7 +447,6 @@ k_row_c2r_local(const cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__r
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int nx = a.nx;
-    const int j0 = xcd_remap(blockIdx.x, gridDim.x) * PAIRS * 2;
     for (int idx = threadIdx.x; idx < NYQ * 3 * PAIRS; idx += NT) {
         const int p = idx % PAIRS, rem = idx / PAIRS;
         const int comp = rem % 3, k = rem / 3;
@@ -467,6 +497,16 @@ k_row_c2r_local(const cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__r
                           bdl[(2 * 2 * PAIRS + rr) * NYQ + i]};
         local_cell<T, GEOM, true, TORQUE>(a, m, geom, i, j, bd3, b, kout);
     }
+}
+
+template <typename T, int LOG2L, bool HALF, int RQ, bool GEOM, bool TORQUE>
+__global__ void __launch_bounds__((FusedCfg<T, LOG2L, RQ>::NT))
+k_row_c2r_local(const cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw, LocalArgs<T> a,
+                const T *__restrict__ m, const uint8_t *__restrict__ geom, T *__restrict__ b,
+                T *__restrict__ kout) {
+    constexpr int PAIRS = FusedCfg<T, LOG2L, RQ>::PAIRS;
+    const int j0 = xcd_remap(blockIdx.x, gridDim.x) * PAIRS * 2;
+    row_c2r_local_body<T, LOG2L, HALF, RQ, GEOM, TORQUE>(S, hxs, ny, tw, a, m, geom, b, kout, j0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1360,6 +1400,166 @@ void DemagFFT<T>::read_kernel(std::vector<double> &out) const {
     for (size_t i = 0; i < np; ++i)
         for (size_t c = 0; c < 4; ++c) out[4 * i + c] = (double)h[c * np + i];
 }
+
+// ---------------------------------------------------------------------------
+// Batched passes: the block of table entry blockIdx.x runs the single-film
+// body for its film and block.
+// ---------------------------------------------------------------------------
+template <typename T, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((RowCfg<T, LOG2L, RQ>::NT))
+kb_row_r2c(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int which, int stage1) {
+    constexpr int PAIRS = RowCfg<T, LOG2L, RQ>::PAIRS;
+    const BEntry e = tab[blockIdx.x];
+    const BFilm<T> &f = F[e.film];
+    if (stage1 && f.skip1) return;
+    const int nb = (f.ny + 2 * PAIRS - 1) / (2 * PAIRS);
+    const int comp = e.blk / nb, j0 = (e.blk - comp * nb) * 2 * PAIRS;
+    row_r2c_body<T, T, LOG2L, HALF, RQ>(f.in[which], (int64_t)f.nx * f.ny, f.nx, f.nx, f.ny, f.ms, f.tw_x, f.spec,
+                                        (int64_t)(f.px / 2) * f.ny, comp, j0);
+}
+
+template <typename T, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((ColCfg<T, LOG2L, RQ>::NT))
+kb_col(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int stage1) {
+    const BEntry e = tab[blockIdx.x];
+    const BFilm<T> &f = F[e.film];
+    if (stage1 && f.skip1) return;
+    const int hxs = f.px / 2;
+    col_work<T, LOG2L, HALF, RQ>(f.spec, hxs, f.ny, f.tw_y, f.kern, f.scale, hxs + (hxs + 1) / 2, e.blk);
+}
+
+template <typename T, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((RowCfg<T, LOG2L, RQ>::NT))
+kb_row_c2r(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int stage1) {
+    constexpr int PAIRS = RowCfg<T, LOG2L, RQ>::PAIRS;
+    const BEntry e = tab[blockIdx.x];
+    const BFilm<T> &f = F[e.film];
+    if (stage1 && f.skip1) return;
+    const int nb = (f.ny + 2 * PAIRS - 1) / (2 * PAIRS);
+    const int comp = e.blk / nb, j0 = (e.blk - comp * nb) * 2 * PAIRS;
+    row_c2r_body<T, LOG2L, HALF, RQ>(f.spec, f.px / 2, f.nx, f.ny, f.tw_x, f.bd, (int64_t)f.nx * f.ny, comp, j0);
+}
+
+template <typename T, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((FusedCfg<T, LOG2L, RQ>::NT))
+kb_row_c2r_local(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int which, int ks, int stage1) {
+    constexpr int PAIRS = FusedCfg<T, LOG2L, RQ>::PAIRS;
+    const BEntry e = tab[blockIdx.x];
+    const BFilm<T> &f = F[e.film];
+    if (stage1 && f.skip1) return;
+    row_c2r_local_body<T, LOG2L, HALF, RQ, false, true>(f.spec, f.px / 2, f.ny, f.tw_x, f.la, f.in[which], nullptr,
+                                                        f.b, f.k[ks], e.blk * 2 * PAIRS);
+}
+
+template <typename T>
+int batch_row_blocks(int kind, int lg, int rq, int ny) {
+    int pairs = 1;
+    dispatch_log2<1, Range<T>::ROW>(lg, [&](auto lgc) {
+        constexpr int LG = decltype(lgc)::value;
+        if (kind == 3) {
+            if (rq == 3) {
+                if constexpr (fused_ok<T, LG, 3>()) pairs = FusedCfg<T, LG, 3>::PAIRS;
+            } else {
+                if constexpr (fused_ok<T, LG, 4>()) pairs = FusedCfg<T, LG, 4>::PAIRS;
+            }
+        } else {
+            pairs = rq == 3 ? RowCfg<T, LG, 3>::PAIRS : RowCfg<T, LG, 4>::PAIRS;
+        }
+    });
+    return (ny + 2 * pairs - 1) / (2 * pairs);
+}
+
+template <typename T>
+void batch_allow_lds(int lg_x, int lg_y) {
+    dispatch_log2<1, Range<T>::ROW>(lg_x, [&](auto lgc) {
+        constexpr int LG = decltype(lgc)::value;
+        auto rows = [&](auto rqc) {
+            constexpr int RQ = decltype(rqc)::value;
+            allow_lds(kb_row_r2c<T, LG, true, RQ>, RowCfg<T, LG, RQ>::LDS);
+            allow_lds(kb_row_r2c<T, LG, false, RQ>, RowCfg<T, LG, RQ>::LDS);
+            allow_lds(kb_row_c2r<T, LG, true, RQ>, RowCfg<T, LG, RQ>::LDS);
+            allow_lds(kb_row_c2r<T, LG, false, RQ>, RowCfg<T, LG, RQ>::LDS);
+            if constexpr (fused_ok<T, LG, RQ>()) {
+                allow_lds(kb_row_c2r_local<T, LG, true, RQ>, FusedCfg<T, LG, RQ>::LDS);
+                allow_lds(kb_row_c2r_local<T, LG, false, RQ>, FusedCfg<T, LG, RQ>::LDS);
+            }
+        };
+        rows(std::integral_constant<int, 3>{});
+        rows(std::integral_constant<int, 4>{});
+    });
+    dispatch_log2<1, Range<T>::COL>(lg_y, [&](auto lgc) {
+        constexpr int LG = decltype(lgc)::value;
+        if constexpr (col_rq_ok<LG, 3>()) {
+            allow_lds(kb_col<T, LG, true, 3>, ColCfg<T, LG, 3>::LDS);
+            allow_lds(kb_col<T, LG, false, 3>, ColCfg<T, LG, 3>::LDS);
+        }
+        allow_lds(kb_col<T, LG, true, 4>, ColCfg<T, LG, 4>::LDS);
+        allow_lds(kb_col<T, LG, false, 4>, ColCfg<T, LG, 4>::LDS);
+    });
+}
+
+template <typename T>
+void launch_batch_fft(int kind, int lg, int rq, bool half, const BFilm<T> *F, const BEntry *tab, int nent,
+                      int which, int ks, int stage1, hipStream_t s) {
+    if (nent <= 0) return;
+    const dim3 grid((unsigned)nent);
+    auto go = [&](auto lgc, auto rqc) {
+        constexpr int LG = decltype(lgc)::value, RQ = decltype(rqc)::value;
+        if (kind == 0) {
+            using C = RowCfg<T, LG, RQ>;
+            if (half) hipLaunchKernelGGL((kb_row_r2c<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, which, stage1);
+            else hipLaunchKernelGGL((kb_row_r2c<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, which, stage1);
+        } else if (kind == 2) {
+            using C = RowCfg<T, LG, RQ>;
+            if (half) hipLaunchKernelGGL((kb_row_c2r<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, stage1);
+            else hipLaunchKernelGGL((kb_row_c2r<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, stage1);
+        } else if (kind == 3) {
+            if constexpr (fused_ok<T, LG, RQ>()) {
+                using C = FusedCfg<T, LG, RQ>;
+                if (half)
+                    hipLaunchKernelGGL((kb_row_c2r_local<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab,
+                                       which, ks, stage1);
+                else
+                    hipLaunchKernelGGL((kb_row_c2r_local<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab,
+                                       which, ks, stage1);
+            } else {
+                throw Error(3, "fused inverse row pass: block too large for this radix");
+            }
+        }
+    };
+    if (kind == 1) {
+        dispatch_log2<1, Range<T>::COL>(lg, [&](auto lgc) {
+            constexpr int LG = decltype(lgc)::value;
+            auto col = [&](auto rqc) {
+                constexpr int RQ = decltype(rqc)::value;
+                using C = ColCfg<T, LG, RQ>;
+                if (half) hipLaunchKernelGGL((kb_col<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, stage1);
+                else hipLaunchKernelGGL((kb_col<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, stage1);
+            };
+            if constexpr (col_rq_ok<LG, 3>()) {
+                if ((rq & 7) == 3) col(std::integral_constant<int, 3>{});
+                else col(std::integral_constant<int, 4>{});
+            } else {
+                col(std::integral_constant<int, 4>{});
+            }
+        });
+    } else {
+        dispatch_log2<1, Range<T>::ROW>(lg, [&](auto lgc) {
+            if (rq == 3) go(lgc, std::integral_constant<int, 3>{});
+            else go(lgc, std::integral_constant<int, 4>{});
+        });
+    }
+    LLGMI_LAUNCH_CHECK();
+}
+
+template int batch_row_blocks<float>(int, int, int, int);
+template int batch_row_blocks<double>(int, int, int, int);
+template void batch_allow_lds<float>(int, int);
+template void batch_allow_lds<double>(int, int);
+template void launch_batch_fft<float>(int, int, int, bool, const BFilm<float> *, const BEntry *, int, int, int, int,
+                                      hipStream_t);
+template void launch_batch_fft<double>(int, int, int, bool, const BFilm<double> *, const BEntry *, int, int, int,
+                                       int, hipStream_t);
 
 template struct DemagFFT<float>;
 template struct DemagFFT<double>;

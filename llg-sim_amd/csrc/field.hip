@@ -128,18 +128,16 @@ void launch_mask_cells(T *m, const uint8_t *keep, int64_t n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // RK45 stage state: v = m; v = v + c_q k_q (q in order); normalize (zero -> +z)
 // ---------------------------------------------------------------------------
+// cell i: v = m + c_q k_q (q in order), normalised, into out
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_rk_stage(const T *__restrict__ m, StageArgs<T> a, T *__restrict__ out, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void rk_stage_cell(const T *__restrict__ m, const T *const *k, const T *c, int nk,
+                                              T *__restrict__ out, int64_t i, int64_t n) {
     T v0 = m[i], v1 = m[n + i], v2 = m[2 * n + i];
-    for (int q = 0; q < a.nk; ++q) {
-        const T *k = a.k[q];
-        const T c = a.dt ? T(*a.dt * a.a[q]) : a.c[q];
-        v0 = v0 + c * k[i];
-        v1 = v1 + c * k[n + i];
-        v2 = v2 + c * k[2 * n + i];
+    for (int q = 0; q < nk; ++q) {
+        const T *kq = k[q];
+        v0 = v0 + c[q] * kq[i];
+        v1 = v1 + c[q] * kq[n + i];
+        v2 = v2 + c[q] * kq[2 * n + i];
     }
     T n2 = v0 * v0 + v1 * v1 + v2 * v2;
     if (n2 == T(0)) {
@@ -151,6 +149,16 @@ k_rk_stage(const T *__restrict__ m, StageArgs<T> a, T *__restrict__ out, int64_t
     out[i] = v0;
     out[n + i] = v1;
     out[2 * n + i] = v2;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_rk_stage(const T *__restrict__ m, StageArgs<T> a, T *__restrict__ out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T c[6];
+    for (int q = 0; q < a.nk; ++q) c[q] = a.dt ? T(*a.dt * a.a[q]) : a.c[q];
+    rk_stage_cell<T>(m, a.k, c, a.nk, out, i, n);
 }
 
 template <typename T>
@@ -178,6 +186,31 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// max over the three components of cell t of |error estimate|
+template <typename T, bool COMBINED>
+__device__ __forceinline__ double rk_err_cell(const T *__restrict__ k1, const T *__restrict__ k3,
+                                              const T *__restrict__ k4, const T *__restrict__ k5,
+                                              const T *__restrict__ k6, const T *__restrict__ k7, int64_t t,
+                                              int64_t n) {
+    double d = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        int64_t o = q * n + t;
+        double a1 = k1[o], a3 = k3[o], a4 = k4[o], a5 = k5[o], a6 = k6[o], a7 = k7[o];
+        double e;
+        if (COMBINED) {
+            e = (DP::B1 - DP::BH1) * a1 + (DP::B3 - DP::BH3) * a3 + (DP::B4 - DP::BH4) * a4 +
+                (DP::B5 - DP::BH5) * a5 + (DP::B6 - DP::BH6) * a6 - DP::BH7 * a7;
+        } else {
+            double th = DP::B1 * a1 + DP::B3 * a3 + DP::B4 * a4 + DP::B5 * a5 + DP::B6 * a6;
+            double tl = DP::BH1 * a1 + DP::BH3 * a3 + DP::BH4 * a4 + DP::BH5 * a5 + DP::BH6 * a6 + DP::BH7 * a7;
+            e = th - tl;
+        }
+        d = q == 0 ? fabs(e) : fmax(d, fabs(e));
+    }
+    return d;
+}
+
 template <typename T, bool COMBINED>
 __global__ void __launch_bounds__(256)
 k_rk_err(const T *__restrict__ k1, const T *__restrict__ k3, const T *__restrict__ k4,
@@ -185,27 +218,10 @@ k_rk_err(const T *__restrict__ k1, const T *__restrict__ k3, const T *__restrict
          double *__restrict__ partial) {
     __shared__ double red[4];
     double local = 0.0;
-    const int64_t n3 = 3 * n;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        double d = 0.0;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            int64_t o = q * n + t;
-            double a1 = k1[o], a3 = k3[o], a4 = k4[o], a5 = k5[o], a6 = k6[o], a7 = k7[o];
-            double e;
-            if (COMBINED) {
-                e = (DP::B1 - DP::BH1) * a1 + (DP::B3 - DP::BH3) * a3 + (DP::B4 - DP::BH4) * a4 +
-                    (DP::B5 - DP::BH5) * a5 + (DP::B6 - DP::BH6) * a6 - DP::BH7 * a7;
-            } else {
-                double th = DP::B1 * a1 + DP::B3 * a3 + DP::B4 * a4 + DP::B5 * a5 + DP::B6 * a6;
-                double tl = DP::BH1 * a1 + DP::BH3 * a3 + DP::BH4 * a4 + DP::BH5 * a5 + DP::BH6 * a6 + DP::BH7 * a7;
-                e = th - tl;
-            }
-            d = q == 0 ? fabs(e) : fmax(d, fabs(e));
-        }
+        const double d = rk_err_cell<T, COMBINED>(k1, k3, k4, k5, k6, k7, t, n);
         if (d > local) local = d;
     }
-    (void)n3;
     local = wave_max(local);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = local;
@@ -246,7 +262,151 @@ void launch_rk_err(const T *k1, const T *k3, const T *k4, const T *k5, const T *
     LLGMI_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------------------
+// Batched element-wise kernels: entry blockIdx.x = (film, block of kBatchCells
+// cells of that film).
+// ---------------------------------------------------------------------------
+template <typename T, bool DEMAG>
+__global__ void __launch_bounds__(kBatchCells)
+kb_local(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int which, int ks, int stage1) {
+    const BEntry e = tab[blockIdx.x];
+    const BFilm<T> &f = F[e.film];
+    if (stage1 && f.skip1) return;
+    const int64_t n = (int64_t)f.nx * f.ny;
+    const int64_t c = (int64_t)e.blk * kBatchCells + threadIdx.x;
+    if (c >= n) return;
+    const int j = (int)(c / f.nx), i = (int)(c - (int64_t)j * f.nx);
+    T bd3[3] = {T(0), T(0), T(0)};
+    if (DEMAG) {
+        bd3[0] = f.bd[c];
+        bd3[1] = f.bd[n + c];
+        bd3[2] = f.bd[2 * n + c];
+    }
+    local_cell<T, false, DEMAG, true>(f.la, f.in[which], nullptr, i, j, bd3, f.b, f.k[ks]);
+}
+
+// Dormand-Prince stage coefficients a_sj (llg.rs:813-846) and the 5th-order
+// weights of m7, with the k indices they multiply
+struct DPStage {
+    int nk, kidx[5];
+    double a[5];
+};
+__constant__ DPStage kDP[8] = {
+    {0, {0}, {0}}, {0, {0}, {0}},
+    {1, {1}, {1.0 / 5.0}},
+    {2, {1, 2}, {3.0 / 40.0, 9.0 / 40.0}},
+    {3, {1, 2, 3}, {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0}},
+    {4, {1, 2, 3, 4}, {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0}},
+    {5, {1, 2, 3, 4, 5}, {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0}},
+    {5, {1, 3, 4, 5, 6}, {35.0 / 384.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}},
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBatchCells)
+kb_stage(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int stage) {
+    const BEntry e = tab[blockIdx.x];
+    const BFilm<T> &f = F[e.film];
+    const int64_t n = (int64_t)f.nx * f.ny;
+    const int64_t c = (int64_t)e.blk * kBatchCells + threadIdx.x;
+    if (c >= n) return;
+    const DPStage &st = kDP[stage];
+    const T *k[6];
+    T co[6];
+    for (int q = 0; q < st.nk; ++q) {
+        k[q] = f.k[st.kidx[q]];
+        co[q] = T(f.dt * st.a[q]);
+    }
+    rk_stage_cell<T>(f.m, k, co, st.nk, stage == 7 ? f.m7 : f.stage, c, n);
+}
+
+template <typename T, bool COMBINED>
+__global__ void __launch_bounds__(kBatchCells)
+kb_err(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, double *__restrict__ partial) {
+    __shared__ double red[kBatchCells / 64];
+    const BEntry e = tab[blockIdx.x];
+    const BFilm<T> &f = F[e.film];
+    const int64_t n = (int64_t)f.nx * f.ny;
+    const int64_t c = (int64_t)e.blk * kBatchCells + threadIdx.x;
+    double local = c < n ? rk_err_cell<T, COMBINED>(f.k[1], f.k[3], f.k[4], f.k[5], f.k[6], f.k[7], c, n) : 0.0;
+    local = wave_max(local);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = red[0];
+        for (int w = 1; w < kBatchCells / 64; ++w) v = fmax(v, red[w]);
+        partial[blockIdx.x] = v;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+kb_err_reduce(const BFilm<T> *__restrict__ F, const double *__restrict__ partial, double *__restrict__ err) {
+    __shared__ double red[4];
+    const BFilm<T> &f = F[blockIdx.x];
+    double v = 0.0;
+    for (int q = threadIdx.x; q < f.ne; q += blockDim.x) v = fmax(v, partial[f.e0 + q]);
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) err[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBatchCells)
+kb_accept(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab) {
+    const BEntry e = tab[blockIdx.x];
+    const BFilm<T> &f = F[e.film];
+    if (!f.accept) return;
+    const int64_t n = (int64_t)f.nx * f.ny;
+    const int64_t c = (int64_t)e.blk * kBatchCells + threadIdx.x;
+    if (c >= n) return;
+    f.m[c] = f.m7[c];
+    f.m[n + c] = f.m7[n + c];
+    f.m[2 * n + c] = f.m7[2 * n + c];
+}
+
+template <typename T>
+void launch_batch_local(const BFilm<T> *F, const BEntry *tab, int nent, int demag, int which, int ks, int stage1,
+                        hipStream_t s) {
+    if (nent <= 0) return;
+    if (demag)
+        hipLaunchKernelGGL((kb_local<T, true>), dim3((unsigned)nent), dim3(kBatchCells), 0, s, F, tab, which, ks, stage1);
+    else
+        hipLaunchKernelGGL((kb_local<T, false>), dim3((unsigned)nent), dim3(kBatchCells), 0, s, F, tab, which, ks, stage1);
+    LLGMI_LAUNCH_CHECK();
+}
+template <typename T>
+void launch_batch_stage(const BFilm<T> *F, const BEntry *tab, int nent, int stage, hipStream_t s) {
+    if (nent <= 0) return;
+    hipLaunchKernelGGL((kb_stage<T>), dim3((unsigned)nent), dim3(kBatchCells), 0, s, F, tab, stage);
+    LLGMI_LAUNCH_CHECK();
+}
+template <typename T>
+void launch_batch_err(const BFilm<T> *F, const BEntry *tab, int nent, int nfilms, double *partial, double *err,
+                      hipStream_t s) {
+    if (nent <= 0) return;
+    if (sizeof(T) == sizeof(float))
+        hipLaunchKernelGGL((kb_err<T, true>), dim3((unsigned)nent), dim3(kBatchCells), 0, s, F, tab, partial);
+    else
+        hipLaunchKernelGGL((kb_err<T, false>), dim3((unsigned)nent), dim3(kBatchCells), 0, s, F, tab, partial);
+    LLGMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL((kb_err_reduce<T>), dim3((unsigned)nfilms), dim3(256), 0, s, F, partial, err);
+    LLGMI_LAUNCH_CHECK();
+}
+template <typename T>
+void launch_batch_accept(const BFilm<T> *F, const BEntry *tab, int nent, hipStream_t s) {
+    if (nent <= 0) return;
+    hipLaunchKernelGGL((kb_accept<T>), dim3((unsigned)nent), dim3(kBatchCells), 0, s, F, tab);
+    LLGMI_LAUNCH_CHECK();
+}
+
 #define LLGMI_INST(T)                                                                        \
+    template void launch_batch_local<T>(const BFilm<T> *, const BEntry *, int, int, int, int, int, \
+                                        hipStream_t);                                        \
+    template void launch_batch_stage<T>(const BFilm<T> *, const BEntry *, int, int, hipStream_t); \
+    template void launch_batch_err<T>(const BFilm<T> *, const BEntry *, int, int, double *,  \
+                                      double *, hipStream_t);                                \
+    template void launch_batch_accept<T>(const BFilm<T> *, const BEntry *, int, hipStream_t); \
     template void launch_local_field<T>(const LocalArgs<T> &, const T *, const T *,          \
                                         const uint8_t *, T *, T *, hipStream_t);             \
     template void launch_aos_to_soa<T>(const double *, T *, int64_t, hipStream_t);           \

@@ -151,6 +151,67 @@ template <typename T>
 void launch_local_field(const LocalArgs<T> &a, const T *m, const T *bd, const uint8_t *geom,
                         T *b, T *k, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// Batched films: one record per film in device memory; a launch covers all
+// films of one kernel class through a table of (film, block) entries.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct BFilm {
+    // FFT demag operator of the film (fft = 0: no demag term)
+    cpx<T> *spec;
+    const cpx<T> *tw_x, *tw_y;
+    const T *kern;
+    int nx, ny, px, py;
+    T ms, scale;
+    int fft;
+    int skip1;           // nonzero: the stage-1 field evaluation is skipped (FSAL)
+    int accept;          // the controller's decision (for the accepted-state copy)
+    int e0, ne;          // this film's entries in the element table
+    double dt;           // step size of the attempt
+    LocalArgs<T> la;     // local terms and torque (llg_rhs)
+    const T *in[3];      // field-evaluation inputs: m, stage state, m7
+    T *m, *stage, *m7, *b, *bd;
+    T *k[8];             // k1..k7 (k[0] unused)
+};
+// an entry of a launch table: film index and the block's index within the film
+struct BEntry {
+    int film, blk;
+};
+
+// Field evaluation of every film whose entries the tables hold (FFT passes by
+// kernel class, see DemagFFT), element-wise kernels over all films:
+// which = input (0 m, 1 stage state, 2 m7), ks = torque output k[ks];
+// stage1 = skip films whose skip1 is set.
+template <typename T>
+void launch_batch_local(const BFilm<T> *F, const BEntry *tab, int nent, int demag, int which, int ks, int stage1,
+                        hipStream_t s);
+template <typename T>
+void launch_batch_stage(const BFilm<T> *F, const BEntry *tab, int nent, int stage, hipStream_t s);
+// error norm of every film: partial[e] per element-table entry, err[f] per film
+template <typename T>
+void launch_batch_err(const BFilm<T> *F, const BEntry *tab, int nent, int nfilms, double *partial, double *err,
+                      hipStream_t s);
+// m = m7 for the films whose accept is set
+template <typename T>
+void launch_batch_accept(const BFilm<T> *F, const BEntry *tab, int nent, hipStream_t s);
+// cells per element-table entry
+constexpr int kBatchCells = 256;
+
+// Batched FFT passes over the films of one kernel class (the same lengths and
+// radix choices): kind 0 row R2C of in[which] (entries comp * row blocks +
+// row block), 1 column pass (entries = items), 2 row C2R into bd (as 0),
+// 3 row C2R fused with the local field into b and k[ks] (entries = row
+// blocks).  lg: log2 length of the pass, rq: its first/last radix, half: the
+// film's nx (rows) or ny (columns) is half the length.
+template <typename T>
+void launch_batch_fft(int kind, int lg, int rq, bool half, const BFilm<T> *F, const BEntry *tab, int nent,
+                      int which, int ks, int stage1, hipStream_t s);
+// row blocks of a film's row pass (kind 0 / 2) or fused inverse (kind 3)
+template <typename T>
+int batch_row_blocks(int kind, int lg, int rq, int ny);
+template <typename T>
+void batch_allow_lds(int lg_x, int lg_y);
+
 template <typename T>
 void launch_aos_to_soa(const double *aos, T *soa, int64_t n, hipStream_t s);
 template <typename T>

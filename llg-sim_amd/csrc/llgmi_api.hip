@@ -12,10 +12,12 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <set>
 #include <string>
+#include <tuple>
 #include <vector>
 
 using namespace llgmi;
@@ -660,17 +662,44 @@ struct Engine : EngineBase {
     // controller (llg.rs:999-1025), FP64 on the host, and the accepted state.
     void rk45_finish(T *m, llgmi_params &p, int mask, double max_err, double headroom, double dt_min,
                      double dt_max, double &eps_out, int &acc_out, double &dtu_out) {
+        rk45_decide(m, p, mask, max_err, headroom, dt_min, dt_max, *h_err, true, eps_out, acc_out, dtu_out);
+    }
+
+    // The attempt's FSAL decision and scratch state without launching anything
+    // (the batched stepper launches the stages of many films itself): the
+    // state changes rk45_issue makes.  Returns whether k1 is reused.
+    bool rk45_prepare(const llgmi_params &p, const llgmi_material &mat, int mask) {
+        ensure_rk();
+        const bool fsal_ok = fsal_valid && last_mask == mask && last_bext[0] == p.b_ext[0] &&
+                             last_bext[1] == p.b_ext[1] && last_bext[2] == p.b_ext[2];
+        if (fsal_ok) {
+            std::swap(d_k[1], d_k[7]);
+            k_swapped = !k_swapped;
+        }
+        fsal_valid = false;
+        if (mask == LLGMI_MASK_FULL && mat.demag && resolved_demag_method(mat) == LLGMI_DEMAG_FFT) {
+            ensure_fft();
+            alloc(d_bd, 3 * n);
+        }
+        issued_mask = mask;
+        return fsal_ok;
+    }
+
+    // The controller (llg.rs:999-1025) from the attempt's max |err|; with copy
+    // the accepted state m7 is copied into m on the context stream (the
+    // batched stepper copies the accepted films itself).
+    void rk45_decide(T *m, llgmi_params &p, int mask, double max_err, double headroom, double dt_min,
+                     double dt_max, double err_inf, bool copy, double &eps_out, int &acc_out, double &dtu_out) {
         if (issued_mask != mask || mask < 0) throw Error(LLGMI_ERR_INVALID, "internal: rk45_finish without rk45_issue");
         issued_mask = -1;
         const double dt0 = p.dt;
-        const double err_inf = *h_err;
         const double eps = err_inf * dt0;
         double dt_next = eps > 0.0 ? dt0 * headroom * std::pow(max_err / eps, 0.2) : dt0 * 2.0;
         if (dt_next < dt_min) dt_next = dt_min;
         if (dt_next > dt_max) dt_next = dt_max;
         const bool accept = (eps <= max_err) || (dt0 <= dt_min * 1.0000000001);
         if (accept) {
-            LLGMI_HIP(hipMemcpyAsync(m, d_m7, sizeof(T) * 3 * n, hipMemcpyDeviceToDevice, stream));
+            if (copy) LLGMI_HIP(hipMemcpyAsync(m, d_m7, sizeof(T) * 3 * n, hipMemcpyDeviceToDevice, stream));
             fsal_valid = true;
             last_mask = mask;
             for (int q = 0; q < 3; ++q) last_bext[q] = p.b_ext[q];
@@ -1111,6 +1140,286 @@ void with_engine(llgmi_ctx *ctx, F &&f) {
 
 size_t elem_size(const llgmi_ctx *ctx) { return ctx->precision == LLGMI_F32 ? 4 : 8; }
 
+// ---------------------------------------------------------------------------
+// Batched RK45 attempts of many films (llgmi_dev_rk45_step_batch): every
+// kernel launch covers all films of one kernel class (films whose FFT passes
+// have the same lengths and radix choices), through tables of (film, block)
+// entries; the stage and error kernels cover all films at once.  One attempt
+// of every film is one replay of a captured graph, then one host
+// synchronisation for all controllers.  Each film runs exactly the kernels
+// (bodies, radix choices, fused or unfused inverse pass) its own context runs
+// for llgmi_dev_rk45_step, so its results are the single-film results.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct BatchPlan {
+    struct Launch {
+        int kind;   // 0..3: launch_batch_fft kinds; 4: local field (demag = with B_demag)
+        int lg, rq;
+        bool half;
+        int demag;
+        int off, n;   // entries in the table
+    };
+    int dev = -1;
+    int nfilms = 0;
+    hipStream_t stream = nullptr;
+    BFilm<T> *d_films = nullptr, *h_films = nullptr;   // h_films pinned
+    BEntry *d_tab = nullptr;
+    std::vector<Launch> field;   // one field evaluation, in launch order
+    int elem_off = 0, elem_n = 0;
+    double *d_partial = nullptr, *d_err = nullptr, *h_err = nullptr;   // h_err pinned
+    hipGraphExec_t exec = nullptr;
+
+    ~BatchPlan() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (d_films) (void)hipFree(d_films);
+        if (d_tab) (void)hipFree(d_tab);
+        if (d_partial) (void)hipFree(d_partial);
+        if (d_err) (void)hipFree(d_err);
+        if (h_films) (void)hipHostFree(h_films);
+        if (h_err) (void)hipHostFree(h_err);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void launch_field(int which, int ks, int stage1) {
+        for (const Launch &l : field) {
+            if (l.kind < 4)
+                launch_batch_fft<T>(l.kind, l.lg, l.rq, l.half, d_films, d_tab + l.off, l.n, which, ks, stage1, stream);
+            else
+                launch_batch_local<T>(d_films, d_tab + l.off, l.n, l.demag, which, ks, stage1, stream);
+        }
+    }
+    // the attempt: H2D of the film records, stage 1 (films without FSAL),
+    // stages 2..7 with their field evaluations, the error norms, D2H of them
+    void launch_attempt() {
+        LLGMI_HIP(hipMemcpyAsync(d_films, h_films, sizeof(BFilm<T>) * nfilms, hipMemcpyHostToDevice, stream));
+        launch_field(0, 1, 1);
+        for (int st = 2; st <= 7; ++st) {
+            launch_batch_stage<T>(d_films, d_tab + elem_off, elem_n, st, stream);
+            launch_field(st == 7 ? 2 : 1, st, 0);
+        }
+        launch_batch_err<T>(d_films, d_tab + elem_off, elem_n, nfilms, d_partial, d_err, stream);
+        LLGMI_HIP(hipMemcpyAsync(h_err, d_err, sizeof(double) * nfilms, hipMemcpyDeviceToHost, stream));
+    }
+};
+
+// the structure a plan depends on: per film its grid, whether the FFT demag
+// term is on, and the context's kernel choices
+template <typename T>
+std::string batch_key(const std::vector<Engine<T> *> &engs, const std::vector<int> &fft_on) {
+    std::string k;
+    auto put = [&](const void *v, size_t b) { k.append((const char *)v, b); };
+    for (size_t f = 0; f < engs.size(); ++f) {
+        const Engine<T> &e = *engs[f];
+        put(&e.g.nx, sizeof e.g.nx);
+        put(&e.g.ny, sizeof e.g.ny);
+        put(&fft_on[f], sizeof(int));
+        if (fft_on[f]) {
+            const int ch[5] = {e.fft.lx, e.fft.ly, e.fft.rq_row, e.fft.rq_col & 7, e.fft.rq_fused > 0 ? e.fft.rq_fused : -e.fft.rq_row_inv};
+            put(ch, sizeof ch);
+        }
+    }
+    return k;
+}
+
+template <typename T>
+std::unique_ptr<BatchPlan<T>> build_batch_plan(const std::vector<Engine<T> *> &engs, const std::vector<int> &fft_on,
+                                               int dev) {
+    auto P = std::make_unique<BatchPlan<T>>();
+    P->dev = dev;
+    P->nfilms = (int)engs.size();
+    std::vector<BEntry> tab;
+    // launch classes of each kind, keyed by (lg, rq, half) / demag flag
+    std::map<std::tuple<int, int, int, int>, std::vector<BEntry>> cls;
+    std::vector<BEntry> elem;
+    std::vector<int> e0((size_t)P->nfilms), ne((size_t)P->nfilms);
+    for (int f = 0; f < P->nfilms; ++f) {
+        const Engine<T> &e = *engs[(size_t)f];
+        const int nx = (int)e.g.nx, ny = (int)e.g.ny;
+        const int ncell_blocks = (int)((e.n + kBatchCells - 1) / kBatchCells);
+        e0[(size_t)f] = (int)elem.size();
+        ne[(size_t)f] = ncell_blocks;
+        for (int b = 0; b < ncell_blocks; ++b) elem.push_back({f, b});
+        if (!fft_on[(size_t)f]) {
+            auto &v = cls[{4, 0, 0, 0}];
+            for (int b = 0; b < ncell_blocks; ++b) v.push_back({f, b});
+            continue;
+        }
+        const DemagFFT<T> &op = e.fft;
+        const bool hx = 2 * op.nx == op.px, hy = 2 * op.ny == op.py;
+        {
+            auto &v = cls[{0, op.lx, op.rq_row, hx}];
+            const int nb = batch_row_blocks<T>(0, op.lx, op.rq_row, ny);
+            for (int b = 0; b < 3 * nb; ++b) v.push_back({f, b});
+        }
+        {
+            auto &v = cls[{1, op.ly, op.rq_col & 7, hy}];
+            const int hxs = (int)(op.px / 2), nwork = hxs + (hxs + 1) / 2;
+            for (int w = 0; w < nwork; ++w) v.push_back({f, w});
+        }
+        if (op.rq_fused > 0) {
+            auto &v = cls[{3, op.lx, op.rq_fused, hx}];
+            const int nb = batch_row_blocks<T>(3, op.lx, op.rq_fused, ny);
+            for (int b = 0; b < nb; ++b) v.push_back({f, b});
+        } else {
+            auto &v = cls[{2, op.lx, op.rq_row_inv, hx}];
+            const int nb = batch_row_blocks<T>(2, op.lx, op.rq_row_inv, ny);
+            for (int b = 0; b < 3 * nb; ++b) v.push_back({f, b});
+            auto &w = cls[{5, 0, 0, 0}];   // local field with B_demag, after the inverse passes
+            for (int b = 0; b < ncell_blocks; ++b) w.push_back({f, b});
+        }
+        (void)nx;
+    }
+    // launch order: all row R2C, all column, all inverse (fused and unfused),
+    // then the local fields (the map orders keys by kind)
+    for (auto &kv : cls) {
+        const int kind = std::get<0>(kv.first);
+        typename BatchPlan<T>::Launch l{};
+        l.kind = kind >= 4 ? 4 : kind;
+        l.lg = std::get<1>(kv.first);
+        l.rq = std::get<2>(kv.first);
+        l.half = std::get<3>(kv.first) != 0;
+        l.demag = kind == 5 ? 1 : 0;
+        l.off = (int)tab.size();
+        l.n = (int)kv.second.size();
+        tab.insert(tab.end(), kv.second.begin(), kv.second.end());
+        P->field.push_back(l);
+    }
+    P->elem_off = (int)tab.size();
+    P->elem_n = (int)elem.size();
+    tab.insert(tab.end(), elem.begin(), elem.end());
+    for (auto &l : P->field)
+        if (l.kind < 4) batch_allow_lds<T>(l.kind == 1 ? 1 : l.lg, l.kind == 1 ? l.lg : 1);
+    LLGMI_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+    LLGMI_HIP(hipMalloc(&P->d_films, sizeof(BFilm<T>) * P->nfilms));
+    LLGMI_HIP(hipHostMalloc((void **)&P->h_films, sizeof(BFilm<T>) * P->nfilms, hipHostMallocDefault));
+    LLGMI_HIP(hipMalloc(&P->d_tab, sizeof(BEntry) * std::max<size_t>(tab.size(), 1)));
+    LLGMI_HIP(hipMemcpy(P->d_tab, tab.data(), sizeof(BEntry) * tab.size(), hipMemcpyHostToDevice));
+    LLGMI_HIP(hipMalloc(&P->d_partial, sizeof(double) * std::max(P->elem_n, 1)));
+    LLGMI_HIP(hipMalloc(&P->d_err, sizeof(double) * P->nfilms));
+    LLGMI_HIP(hipHostMalloc((void **)&P->h_err, sizeof(double) * P->nfilms, hipHostMallocDefault));
+    std::memset(P->h_films, 0, sizeof(BFilm<T>) * P->nfilms);
+    for (int f = 0; f < P->nfilms; ++f) {
+        P->h_films[f].e0 = e0[(size_t)f];
+        P->h_films[f].ne = ne[(size_t)f];
+    }
+    return P;
+}
+
+// Process-wide plans, keyed by device, precision and structure (contexts
+// that come and go with the same structure reuse a plan: every per-film
+// pointer is written into the film records at each attempt).
+std::mutex g_batch_mu;
+std::map<std::string, std::unique_ptr<BatchPlan<float>>> g_plans_f;
+std::map<std::string, std::unique_ptr<BatchPlan<double>>> g_plans_d;
+template <typename T>
+std::map<std::string, std::unique_ptr<BatchPlan<T>>> &batch_plans();
+template <>
+std::map<std::string, std::unique_ptr<BatchPlan<float>>> &batch_plans<float>() { return g_plans_f; }
+template <>
+std::map<std::string, std::unique_ptr<BatchPlan<double>>> &batch_plans<double>() { return g_plans_d; }
+
+bool batch_graphs_enabled() {
+    const char *e = std::getenv("LLGMI_BATCH_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+// One batched attempt of every film (the films' engines on one device, all
+// with their operators built and tuned).  eps / accepted / dt_used per film.
+template <typename T>
+void batch_attempt(const std::vector<Engine<T> *> &engs, const std::vector<T *> &ms, llgmi_params *params,
+                   const llgmi_material *mats, int mask, double max_err, double headroom, double dt_min,
+                   double dt_max, double *eps, int *accepted, double *dt_used, int dev) {
+    const int nf = (int)engs.size();
+    std::vector<int> fft_on((size_t)nf);
+    for (int f = 0; f < nf; ++f)
+        fft_on[(size_t)f] = mask == LLGMI_MASK_FULL && mats[f].demag && resolved_demag_method(mats[f]) == LLGMI_DEMAG_FFT;
+    // every film's earlier work on its own stream is done before the batch stream touches its buffers
+    for (auto *e : engs) LLGMI_HIP(hipStreamSynchronize(e->stream));
+    std::lock_guard<std::mutex> lock(g_batch_mu);
+    std::string key = batch_key<T>(engs, fft_on);
+    key.append((const char *)&dev, sizeof dev);
+    auto &plans = batch_plans<T>();
+    auto it = plans.find(key);
+    if (it == plans.end()) {
+        if (plans.size() >= 8) plans.clear();
+        it = plans.emplace(key, build_batch_plan<T>(engs, fft_on, dev)).first;
+    }
+    BatchPlan<T> &P = *it->second;
+    for (int f = 0; f < nf; ++f) {
+        Engine<T> &e = *engs[(size_t)f];
+        const bool fsal_ok = e.rk45_prepare(params[f], mats[f], mask);
+        BFilm<T> &r = P.h_films[f];
+        const int e0 = r.e0, ne = r.ne;
+        r = BFilm<T>{};
+        r.e0 = e0;
+        r.ne = ne;
+        r.nx = (int)e.g.nx;
+        r.ny = (int)e.g.ny;
+        r.fft = fft_on[(size_t)f];
+        if (r.fft) {
+            r.spec = e.fft.spec;
+            r.tw_x = e.fft.tw_x;
+            r.tw_y = e.fft.tw_y;
+            r.kern = e.fft.kern;
+            r.px = (int)e.fft.px;
+            r.py = (int)e.fft.py;
+            r.scale = T(1.0 / ((double)e.fft.px * (double)e.fft.py));
+        }
+        r.ms = T(mats[f].ms);
+        r.skip1 = fsal_ok ? 1 : 0;
+        r.dt = params[f].dt;
+        r.la = e.local_args(params[f], mats[f], mask, true, false);
+        r.m = ms[(size_t)f];
+        r.in[0] = ms[(size_t)f];
+        r.in[1] = e.d_stage;
+        r.in[2] = e.d_m7;
+        r.stage = e.d_stage;
+        r.m7 = e.d_m7;
+        r.b = e.d_b;
+        r.bd = e.d_bd;
+        for (int q = 0; q < 8; ++q) r.k[q] = e.d_k[q];
+    }
+    try {
+        if (batch_graphs_enabled()) {
+            if (!P.exec) {
+                hipGraph_t graph = nullptr;
+                LLGMI_HIP(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    P.launch_attempt();
+                } catch (...) {
+                    (void)hipStreamEndCapture(P.stream, &graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                    throw;
+                }
+                LLGMI_HIP(hipStreamEndCapture(P.stream, &graph));
+                const hipError_t ge = hipGraphInstantiate(&P.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                LLGMI_HIP(ge);
+            }
+            LLGMI_HIP(hipGraphLaunch(P.exec, P.stream));
+        } else {
+            P.launch_attempt();
+        }
+        LLGMI_HIP(hipStreamSynchronize(P.stream));
+    } catch (...) {
+        for (auto *e : engs) e->issued_mask = -1;   // no film takes this attempt
+        throw;
+    }
+    bool any = false;
+    for (int f = 0; f < nf; ++f) {
+        engs[(size_t)f]->rk45_decide(ms[(size_t)f], params[f], mask, max_err, headroom, dt_min, dt_max,
+                                     P.h_err[f], false, eps[f], accepted[f], dt_used[f]);
+        P.h_films[f].accept = accepted[f];
+        any = any || accepted[f];
+    }
+    if (any) {
+        LLGMI_HIP(hipMemcpyAsync(P.d_films, P.h_films, sizeof(BFilm<T>) * nf, hipMemcpyHostToDevice, P.stream));
+        launch_batch_accept<T>(P.d_films, P.d_tab + P.elem_off, P.elem_n, P.stream);
+        LLGMI_HIP(hipStreamSynchronize(P.stream));
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1371,6 +1680,50 @@ int llgmi_dev_rk45_step_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi
         if (!e.empty()) e = "film " + std::to_string(i) + ": " + e;
     }
     if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    // Batched kernels once every film's operator is built and tuned (the
+    // first call for a set of films goes through the per-film path, which
+    // does that), for films of one precision on one device, the FFT or no
+    // demag term, profiling off; LLGMI_BATCH_KERNELS=0 keeps the per-film path.
+    {
+        const char *bk = std::getenv("LLGMI_BATCH_KERNELS");
+        bool ok = count >= 2 && !(bk && bk[0] == '0');
+        for (int64_t i = 0; i < count && ok; ++i) {
+            const llgmi_ctx *c = ctxs[i];
+            ok = c->precision == ctxs[0]->precision && c->device == ctxs[0]->device && c->prof.mask == 0;
+            if (!ok) break;
+            const bool demag = field_mask == LLGMI_MASK_FULL && mats[i].demag;
+            if (demag && resolved_demag_method(mats[i]) != LLGMI_DEMAG_FFT) { ok = false; break; }
+            auto ready = [&](auto &eng) {
+                if (!demag) return true;
+                return eng.fft.built && eng.fft.rq_fused != 0;
+            };
+            ok = c->precision == LLGMI_F32 ? ready(*c->ef) : ready(*c->ed);
+        }
+        if (ok) {
+            llgmi_ctx *c0 = ctxs[0];
+            return guarded(c0, [&] {
+                if (c0->precision == LLGMI_F32) {
+                    std::vector<Engine<float> *> engs;
+                    std::vector<float *> ms;
+                    for (int64_t i = 0; i < count; ++i) {
+                        engs.push_back(ctxs[i]->ef.get());
+                        ms.push_back((float *)m_devs[i]);
+                    }
+                    batch_attempt<float>(engs, ms, params, mats, field_mask, max_err, headroom, dt_min, dt_max, eps,
+                                         accepted, dt_used, c0->device);
+                } else {
+                    std::vector<Engine<double> *> engs;
+                    std::vector<double *> ms;
+                    for (int64_t i = 0; i < count; ++i) {
+                        engs.push_back(ctxs[i]->ed.get());
+                        ms.push_back((double *)m_devs[i]);
+                    }
+                    batch_attempt<double>(engs, ms, params, mats, field_mask, max_err, headroom, dt_min, dt_max,
+                                          eps, accepted, dt_used, c0->device);
+                }
+            });
+        }
+    }
     // every attempt is launched before any is waited for
     int64_t issued = 0;
     int rc = LLGMI_OK;

@@ -173,3 +173,38 @@ def test_c5_learned_guess_saves_30_percent_of_v_cycles(gpu_lib):
     previous-step guess, states one relaxation step apart."""
     c = _c5_compare(states=10, stride=1)
     assert c["previous_plus_learned_delta"]["v_cycles_mean"] <= 0.7 * c["previous"]["v_cycles_mean"]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_c3_batched_kernels_equal_per_film_path(gpu_lib, prec, monkeypatch):
+    """The batched kernels (one launch per kernel class over all films,
+    LLGMI_BATCH_KERNELS unset) against the per-film path
+    (LLGMI_BATCH_KERNELS=0) on 16 films of the 64-film workload, with the
+    kernel choices pinned (radix 16, unfused and fused inverse passes by film
+    size, one column item per block) so both sets of contexts run the same
+    kernels: every attempt's eps, accept and dt, and the final states, bitwise
+    equal over 6 attempts (the first goes through the per-film path and tunes
+    the contexts, one film starts with a rejected step)."""
+    L = gpu_lib
+    from initial_states import batched_films
+    films = batched_films(64, 64)[:16]
+    P = L.F32 if prec == "f32" else L.F64
+    monkeypatch.setenv("LLGMI_FFT_RADIX", "16")
+    monkeypatch.setenv("LLGMI_FFT_COLPIPE", "0")
+    print(f"C3 batched vs per-film {prec}", flush=True)
+    out = {}
+    for mode in ("batched", "per_film"):
+        if mode == "per_film":
+            monkeypatch.setenv("LLGMI_BATCH_KERNELS", "0")
+        else:
+            monkeypatch.delenv("LLGMI_BATCH_KERNELS", raising=False)
+        ctxs, bufs, mats = _c3_contexts(L, films, P)
+        pars = [L.LLGParams(alpha=f["alpha"], dt=1e-13, b_ext=(0.01, 0.0, 0.0)) for f in films]
+        pars[3].dt = 5e-11
+        res = [L.dev_rk45_step_batch(ctxs, bufs, pars, mats, 2, 1e-5, 0.8, 1e-18, 1e-11) for _ in range(6)]
+        out[mode] = (res, [p.dt for p in pars], [c.dev_download(b) for c, b in zip(ctxs, bufs)])
+    assert out["batched"][0] == out["per_film"][0]
+    assert out["batched"][1] == out["per_film"][1]
+    for a, b in zip(out["batched"][2], out["per_film"][2]):
+        assert np.array_equal(a, b)
+    assert not out["batched"][0][0][3][1]          # the large first step is rejected
