@@ -76,6 +76,17 @@ struct ColCfg {
     static constexpr size_t LDS = sizeof(cpx<T>) * 2 * SLEN;
 };
 
+// One-item-per-block column launches group G items (G * 2 sequences) per
+// block when a sequence needs few threads, so that a block fills at least two
+// waves (short columns would otherwise leave most lanes of a wave idle).
+template <typename T, int LOG2L, int RQ>
+struct ColGrp {
+    using C = ColCfg<T, LOG2L, RQ>;
+    static constexpr int G = 2 * C::TPS >= 128 ? 1 : 128 / (2 * C::TPS);
+    static constexpr int NT = G * 2 * C::TPS;
+    static constexpr size_t LDS = sizeof(cpx<T>) * 2 * G * C::SLEN;
+};
+
 template <int LOG2L>
 struct ExtractCfg {
     using P = RegPlan<LOG2L>;
@@ -264,17 +275,21 @@ __device__ __forceinline__ void col_item(cpx<T> *v, cpx<T> *col, int comp, int k
     }
 }
 
-// Item w of the column pass (items [0, nwork)), one block.
+// Items wg * G .. wg * G + G - 1 of the column pass (items [0, nwork)), one
+// block (ColGrp); each item has its own two sequences of LDS.
 template <typename T, int LOG2L, bool HALF, int RQ>
 __device__ __forceinline__ void col_work(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
-                                         const T *__restrict__ kern, T scale, int nwork, int w) {
+                                         const T *__restrict__ kern, T scale, int nwork, int wg) {
     using C = ColCfg<T, LOG2L, RQ>;
-    constexpr int TPS = C::TPS;
+    constexpr int TPS = C::TPS, SLEN = C::SLEN;
     constexpr int R0 = C::P::R0;
     constexpr int NL = HALF ? R0 / 2 : R0;
+    constexpr int G = ColGrp<T, LOG2L, RQ>::G;
     extern __shared__ unsigned char lds_raw[];
-    cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
-    const int sidx = threadIdx.x / TPS, t = threadIdx.x - sidx * TPS;
+    const int g = threadIdx.x / (2 * TPS), tl = threadIdx.x - g * 2 * TPS;
+    cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw) + g * 2 * SLEN;
+    const int sidx = tl / TPS, t = tl - sidx * TPS;
+    const int w = wg * G + g;
     const bool xy = w < hxs;
     const int comp = xy ? sidx : 2;
     int kx = xy ? w : 2 * (w - hxs) + sidx;
@@ -291,7 +306,7 @@ __device__ __forceinline__ void col_work(cpx<T> *__restrict__ S, int hxs, int ny
 }
 
 template <typename T, int LOG2L, bool HALF, int RQ, bool PIPE>
-__global__ void __launch_bounds__((ColCfg<T, LOG2L, RQ>::NT))
+__global__ void __launch_bounds__((PIPE ? ColCfg<T, LOG2L, RQ>::NT : ColGrp<T, LOG2L, RQ>::NT))
 k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
             const T *__restrict__ kern, T scale, int nwork) {
     using C = ColCfg<T, LOG2L, RQ>;
@@ -1106,13 +1121,13 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         if constexpr (col_rq_ok<LG, 3>()) {
-            allow_lds(k_col_demag<T, LG, true, 3, false>, ColCfg<T, LG, 3>::LDS);
-            allow_lds(k_col_demag<T, LG, false, 3, false>, ColCfg<T, LG, 3>::LDS);
+            allow_lds(k_col_demag<T, LG, true, 3, false>, ColGrp<T, LG, 3>::LDS);
+            allow_lds(k_col_demag<T, LG, false, 3, false>, ColGrp<T, LG, 3>::LDS);
             allow_lds(k_col_demag<T, LG, true, 3, true>, ColCfg<T, LG, 3>::LDS);
             allow_lds(k_col_demag<T, LG, false, 3, true>, ColCfg<T, LG, 3>::LDS);
         }
-        allow_lds(k_col_demag<T, LG, true, 4, false>, ColCfg<T, LG, 4>::LDS);
-        allow_lds(k_col_demag<T, LG, false, 4, false>, ColCfg<T, LG, 4>::LDS);
+        allow_lds(k_col_demag<T, LG, true, 4, false>, ColGrp<T, LG, 4>::LDS);
+        allow_lds(k_col_demag<T, LG, false, 4, false>, ColGrp<T, LG, 4>::LDS);
         allow_lds(k_col_demag<T, LG, true, 4, true>, ColCfg<T, LG, 4>::LDS);
         allow_lds(k_col_demag<T, LG, false, 4, true>, ColCfg<T, LG, 4>::LDS);
     });
@@ -1155,17 +1170,16 @@ void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
             const int hxs = (int)(px / 2);
             const int nwork = hxs + (hxs + 1) / 2;  // (x, y) items + z-pair items
             const bool half = 2 * ny == py;
-            auto launch = [&](auto kern_fn, unsigned nb) {
-                hipLaunchKernelGGL(kern_fn, dim3(nb), dim3(C::NT), C::LDS, stream, spec, hxs, (int)ny, tw_y,
-                                   kern, scale, nwork);
-            };
             if (pipe) {
                 auto kh = half ? k_col_demag<T, LG, true, RQ, true> : k_col_demag<T, LG, false, RQ, true>;
                 const int nb = std::min(nwork, resident_blocks((const void *)kh, C::NT, C::LDS));
-                launch(kh, (unsigned)nb);
+                hipLaunchKernelGGL(kh, dim3((unsigned)nb), dim3(C::NT), C::LDS, stream, spec, hxs, (int)ny, tw_y,
+                                   kern, scale, nwork);
             } else {
+                using Gc = ColGrp<T, LG, RQ>;
                 auto kh = half ? k_col_demag<T, LG, true, RQ, false> : k_col_demag<T, LG, false, RQ, false>;
-                launch(kh, (unsigned)nwork);
+                hipLaunchKernelGGL(kh, dim3((unsigned)((nwork + Gc::G - 1) / Gc::G)), dim3(Gc::NT), Gc::LDS, stream,
+                                   spec, hxs, (int)ny, tw_y, kern, scale, nwork);
             }
         };
         if constexpr (col_rq_ok<LG, 3>()) {
@@ -1419,7 +1433,7 @@ kb_row_r2c(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int w
 }
 
 template <typename T, int LOG2L, bool HALF, int RQ>
-__global__ void __launch_bounds__((ColCfg<T, LOG2L, RQ>::NT))
+__global__ void __launch_bounds__((ColGrp<T, LOG2L, RQ>::NT))
 kb_col(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int stage1) {
     const BEntry e = tab[blockIdx.x];
     const BFilm<T> &f = F[e.film];
@@ -1453,6 +1467,19 @@ kb_row_c2r_local(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab,
 
 template <typename T>
 int batch_row_blocks(int kind, int lg, int rq, int ny) {
+    if (kind == 1) {
+        // column pass: blocks of G items; ny here is the item count
+        int g = 1;
+        dispatch_log2<1, Range<T>::COL>(lg, [&](auto lgc) {
+            constexpr int LG = decltype(lgc)::value;
+            if constexpr (col_rq_ok<LG, 3>()) {
+                g = (rq & 7) == 3 ? ColGrp<T, LG, 3>::G : ColGrp<T, LG, 4>::G;
+            } else {
+                g = ColGrp<T, LG, 4>::G;
+            }
+        });
+        return (ny + g - 1) / g;
+    }
     int pairs = 1;
     dispatch_log2<1, Range<T>::ROW>(lg, [&](auto lgc) {
         constexpr int LG = decltype(lgc)::value;
@@ -1490,11 +1517,11 @@ void batch_allow_lds(int lg_x, int lg_y) {
     dispatch_log2<1, Range<T>::COL>(lg_y, [&](auto lgc) {
         constexpr int LG = decltype(lgc)::value;
         if constexpr (col_rq_ok<LG, 3>()) {
-            allow_lds(kb_col<T, LG, true, 3>, ColCfg<T, LG, 3>::LDS);
-            allow_lds(kb_col<T, LG, false, 3>, ColCfg<T, LG, 3>::LDS);
+            allow_lds(kb_col<T, LG, true, 3>, ColGrp<T, LG, 3>::LDS);
+            allow_lds(kb_col<T, LG, false, 3>, ColGrp<T, LG, 3>::LDS);
         }
-        allow_lds(kb_col<T, LG, true, 4>, ColCfg<T, LG, 4>::LDS);
-        allow_lds(kb_col<T, LG, false, 4>, ColCfg<T, LG, 4>::LDS);
+        allow_lds(kb_col<T, LG, true, 4>, ColGrp<T, LG, 4>::LDS);
+        allow_lds(kb_col<T, LG, false, 4>, ColGrp<T, LG, 4>::LDS);
     });
 }
 
@@ -1532,7 +1559,7 @@ void launch_batch_fft(int kind, int lg, int rq, bool half, const BFilm<T> *F, co
             constexpr int LG = decltype(lgc)::value;
             auto col = [&](auto rqc) {
                 constexpr int RQ = decltype(rqc)::value;
-                using C = ColCfg<T, LG, RQ>;
+                using C = ColGrp<T, LG, RQ>;
                 if (half) hipLaunchKernelGGL((kb_col<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, stage1);
                 else hipLaunchKernelGGL((kb_col<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, stage1);
             };

@@ -206,7 +206,8 @@ constexpr int kBatchCells = 256;
 template <typename T>
 void launch_batch_fft(int kind, int lg, int rq, bool half, const BFilm<T> *F, const BEntry *tab, int nent,
                       int which, int ks, int stage1, hipStream_t s);
-// row blocks of a film's row pass (kind 0 / 2) or fused inverse (kind 3)
+// row blocks of a film's row pass (kind 0 / 2) or fused inverse (kind 3);
+// for the column pass (kind 1, ny = item count) the blocks of items
 template <typename T>
 int batch_row_blocks(int kind, int lg, int rq, int ny);
 template <typename T>

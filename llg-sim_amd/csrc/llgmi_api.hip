@@ -1255,7 +1255,8 @@ std::unique_ptr<BatchPlan<T>> build_batch_plan(const std::vector<Engine<T> *> &e
         {
             auto &v = cls[{1, op.ly, op.rq_col & 7, hy}];
             const int hxs = (int)(op.px / 2), nwork = hxs + (hxs + 1) / 2;
-            for (int w = 0; w < nwork; ++w) v.push_back({f, w});
+            const int nb = batch_row_blocks<T>(1, op.ly, op.rq_col & 7, nwork);
+            for (int w = 0; w < nb; ++w) v.push_back({f, w});
         }
         if (op.rq_fused > 0) {
             auto &v = cls[{3, op.lx, op.rq_fused, hx}];
