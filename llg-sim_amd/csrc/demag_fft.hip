@@ -887,13 +887,17 @@ static int resident_blocks(const void *kernel, int nt, size_t lds) {
 template <int LG, int RQ>
 constexpr bool col_rq_ok() { return 2 * ((1 << LG) / (1 << RegPlan<LG, RQ>::LOG2R0)) <= 1024; }
 
-// Built ranges: rows up to 2^13 (FP32) / 2^12 (FP64), columns up to 2^12 / 2^11
-// (the column pass holds 3 padded sequences in LDS and 3*TPS <= 1024 threads).
+// Built ranges (padded lengths, so nx, ny <= 4096 in FP32 and nx <= 4096,
+// ny <= 2048 in FP64): rows up to 2^13 in both precisions; columns up to 2^13
+// (FP32) / 2^12 (FP64).  A column block holds the two padded sequences of one
+// item in LDS (2 * 8712 * 16 B = 272 KiB for an FP64 column of 2^13, beyond a
+// CU's 160 KiB) and 2 * TPS <= 1024 threads.  The FP32 column of 2^13 spills
+// a few hundred bytes per lane; it is correct but not tuned.
 template <typename T> struct Range;
-template <> struct Range<float> { static constexpr int ROW = 13, COL = 12; };
-template <> struct Range<double> { static constexpr int ROW = 12, COL = 11; };
+template <> struct Range<float> { static constexpr int ROW = 13, COL = 13; };
+template <> struct Range<double> { static constexpr int ROW = 13, COL = 12; };
 static constexpr int kRowF64Build = 13;  // FP64 row pass of the kernel build (FP32 contexts)
-static constexpr int kExtractMax = 12;
+static constexpr int kExtractMax = 13;
 
 template <typename T>
 DemagFFT<T>::~DemagFFT() {

@@ -133,6 +133,71 @@ def test_demag_forced_radix_fp64_matches_oracle(gpu_lib, monkeypatch, radix, sha
     assert rel_err(b, b_ref) <= 1e-12
 
 
+
+# Long FFT lengths: padded 4096 and 8192 along each axis (rows are the x
+# transforms, columns the y transforms).  FP64 builds columns up to 4096
+# (ny <= 2048), FP32 up to 8192; rows go to 8192 in both.
+@pytest.mark.parametrize("radix", ["auto", "8", "16"])
+@pytest.mark.parametrize("shape", [(8, 1536), (1536, 8), (2100, 8), (5, 1100)])
+def test_demag_long_lengths_fp64_matches_oracle(gpu_lib, monkeypatch, radix, shape):
+    from oracle import oracle as O
+    if radix != "auto":
+        monkeypatch.setenv("LLGMI_FFT_RADIX", radix)
+    L = gpu_lib
+    nx, ny = shape
+    cell = (3e-9, 2e-9, 1e-9)
+    m = _rand_m(nx * ny, 11 * nx + ny)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    b_ref = np.zeros_like(m)
+    oc.add_demag(m, b_ref, O.Material(ms=8e5, demag=True))
+    ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    b = ctx.compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(b, b_ref) <= 1e-12
+
+
+def test_demag_fp64_1536_square_matches_oracle(gpu_lib):
+    """1536 x 1536 pads to 4096 x 4096 (both passes at length 4096)."""
+    from oracle import oracle as O
+    L = gpu_lib
+    nx = ny = 1536
+    cell = (2e-9, 2e-9, 1e-9)
+    m = _rand_m(nx * ny, 1536)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    b_ref = np.zeros_like(m)
+    oc.add_demag(m, b_ref, O.Material(ms=8e5, demag=True))
+    ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    b = ctx.compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(b, b_ref) <= 1e-12
+
+
+@pytest.mark.parametrize("shape", [(8, 2100), (4096, 4096)])
+def test_demag_fp32_length_8192_matches_oracle(gpu_lib, shape):
+    """FP32 columns (and rows) of padded length 8192, against the FP64 oracle
+    at the FP32 bar."""
+    from oracle import oracle as O
+    L = gpu_lib
+    nx, ny = shape
+    cell = (3e-9, 2e-9, 1e-9)
+    m = _rand_m(nx * ny, nx + 3 * ny)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    b_ref = np.zeros_like(m)
+    oc.add_demag(m, b_ref, O.Material(ms=8e5, demag=True))
+    ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F32)
+    b = ctx.compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(b, b_ref) <= 1e-5
+
+
+@pytest.mark.parametrize("shape", [(8, 2049), (4097, 8)])
+def test_demag_beyond_built_lengths_is_unsupported(gpu_lib, shape):
+    """An FP64 column of 8192 (ny > 2048) or a row of 16384 (nx > 4096) is
+    refused with LLGMI_ERR_UNSUPPORTED, before any launch."""
+    L = gpu_lib
+    nx, ny = shape
+    ctx = L.Context(L.Grid2D(nx, ny, 3e-9, 2e-9, 1e-9), precision=L.F64)
+    with pytest.raises(L.LlgmiError) as e:
+        ctx.compute_demag_field(_rand_m(nx * ny, 5), L.Material(ms=8e5, demag=True))
+    assert e.value.code == L.ERR_UNSUPPORTED
+
 @pytest.mark.parametrize("prec", [1, 0])
 def test_demag_radix_variants_agree_1024(gpu_lib, monkeypatch, prec):
     """At the 1024 x 1024 workload the radix-8 and radix-16 passes agree to
