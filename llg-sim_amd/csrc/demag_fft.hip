@@ -363,6 +363,101 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Split column pass, for columns whose item (two padded sequences) does not
+// fit in a CU's LDS (FP64 columns of 2^13): one sequence per block and an HBM
+// round trip between the transforms.  k_colsplit_fwd transforms one column
+// (zero past ny) and stores its full-length spectrum in F[comp][kx][L];
+// k_colsplit_mul_inv forms the product with the kernel from F (its own column
+// and, for the x and y components, the other one at the same ky), transforms
+// it back and stores the first ny values into S.  The products are those of
+// col_item.
+// ---------------------------------------------------------------------------
+template <typename T, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((RegPlan<LOG2L, RQ>::TPS))
+k_colsplit_fwd(const cpx<T> *__restrict__ S, cpx<T> *__restrict__ F, int ny, const cpx<T> *__restrict__ tw) {
+    using P = RegPlan<LOG2L, RQ>;
+    constexpr int L = P::L, TPS = P::TPS, R0 = P::R0;
+    constexpr int NL = HALF ? R0 / 2 : R0;
+    extern __shared__ unsigned char lds_raw[];
+    cpx<T> *seq = reinterpret_cast<cpx<T> *>(lds_raw);
+    const int t = threadIdx.x;
+    const int64_t c = blockIdx.x;  // comp * hxs + kx
+    const cpx<T> *col = S + c * ny;
+    cpx<T> v[R0];
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const int j = t + r * TPS;
+        v[r] = (r < NL && (HALF || j < ny)) ? col[j] : cpx<T>{T(0), T(0)};
+    }
+    reg_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
+    cpx<T> *f = F + c * L;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) f[t + r * TPS] = v[r];
+}
+
+template <typename T, int LOG2L, bool HALF, int RQ>
+__global__ void __launch_bounds__((RegPlan<LOG2L, RQ>::TPS))
+k_colsplit_mul_inv(cpx<T> *__restrict__ S, const cpx<T> *__restrict__ F, int hxs, int ny,
+                   const cpx<T> *__restrict__ tw, const T *__restrict__ kern, T scale) {
+    using P = RegPlan<LOG2L, RQ>;
+    constexpr int L = P::L, TPS = P::TPS, R0 = P::R0;
+    constexpr int HY = L / 2 + 1;
+    extern __shared__ unsigned char lds_raw[];
+    cpx<T> *seq = reinterpret_cast<cpx<T> *>(lds_raw);
+    const int t = threadIdx.x;
+    const int comp = (int)blockIdx.x / hxs, kx = (int)blockIdx.x - comp * hxs;
+    const cpx<T> *own = F + ((int64_t)comp * hxs + kx) * L;
+    // kernel planes [Kxx, Kxy, Kyy, Kzz][hxs + 1][HY], as in col_item
+    const int64_t plane = (int64_t)(hxs + 1) * HY;
+    const int dplane = comp == 0 ? 0 : (comp == 1 ? 2 : 3);
+    cpx<T> v[R0];
+    if (kx != 0) {
+        const T *pa = kern + dplane * plane + (int64_t)kx * HY;
+        const T *pb = kern + plane + (int64_t)kx * HY;
+        const cpx<T> *oth = comp < 2 ? F + ((int64_t)(1 - comp) * hxs + kx) * L : own;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int ky = t + r * TPS;
+            const int kq = ky < HY ? ky : L - ky;
+            const cpx<T> o = own[ky];
+            if (comp == 2) {
+                const T kzz = pa[kq];
+                v[r] = {scale * (kzz * o.x), scale * (kzz * o.y)};
+            } else {
+                const T kd = pa[kq];
+                const T kxy = ky < HY ? pb[kq] : -pb[kq];
+                const cpx<T> q = oth[ky];
+                if (comp == 0)
+                    v[r] = {scale * (kd * o.x + kxy * q.x), scale * (kd * o.y + kxy * q.y)};
+                else
+                    v[r] = {scale * (kxy * q.x + kd * o.x), scale * (kxy * q.y + kd * o.y)};
+            }
+        }
+    } else {
+        // packed DC / Nyquist column, as in col_item
+        const T *pa = kern + dplane * plane;
+        const T *pb = pa + (int64_t)hxs * HY;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int ky = t + r * TPS;
+            const int kq = ky < HY ? ky : L - ky;
+            const T a = pa[kq], b = pb[kq];
+            const T c1 = T(0.5) * (a + b), c2 = T(0.5) * (a - b);
+            const cpx<T> z = own[ky];
+            const cpx<T> zp = own[(L - ky) & (L - 1)];
+            v[r] = {scale * (c1 * z.x + c2 * zp.x), scale * (c1 * z.y - c2 * zp.y)};
+        }
+    }
+    reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
+    cpx<T> *col = S + ((int64_t)comp * hxs + kx) * ny;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const int j = t + r * TPS;
+        if (HALF ? r < R0 / 2 : j < ny) col[j] = v[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Pass 3: row C2R.  Block = PAIRS pairs of rows of one component.  The k = 0
 // and k = L/2 bins use their real parts only, which is what taking Re() of the
 // reference's full complex inverse transform amounts to.
@@ -896,6 +991,11 @@ constexpr bool col_rq_ok() { return 2 * ((1 << LG) / (1 << RegPlan<LG, RQ>::LOG2
 template <typename T> struct Range;
 template <> struct Range<float> { static constexpr int ROW = 13, COL = 13; };
 template <> struct Range<double> { static constexpr int ROW = 13, COL = 12; };
+// Columns beyond Range<T>::COL, up to this length, take the split column pass
+// (FP64 columns of 2^13; an FP32 column of 2^13 still fits the paired block).
+static constexpr int kColSplitMax = 13;
+template <typename T, int LG>
+constexpr size_t colsplit_lds() { return sizeof(cpx<T>) * lds_seq_len(1 << LG); }
 static constexpr int kRowF64Build = 13;  // FP64 row pass of the kernel build (FP32 contexts)
 static constexpr int kExtractMax = 13;
 
@@ -906,12 +1006,13 @@ DemagFFT<T>::~DemagFFT() {
 
 template <typename T>
 void DemagFFT<T>::release() {
-    void *ps[] = {tw_x, tw_y, kern, spec};
+    void *ps[] = {tw_x, tw_y, kern, spec, colfull};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     tw_x = tw_y = nullptr;
     kern = nullptr;
     spec = nullptr;
+    colfull = nullptr;
     built = false;
 }
 
@@ -976,7 +1077,7 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     hy = py / 2 + 1;
     lx = ilog2(px);
     ly = ilog2(py);
-    if (lx > Range<T>::ROW || ly > Range<T>::COL || ly > kExtractMax)
+    if (lx > Range<T>::ROW || ly > kColSplitMax || ly > kExtractMax)
         throw Error(3, "FFT demag: padded size " + std::to_string(px) + "x" + std::to_string(py) +
                            " is beyond the built FFT lengths of this precision");
     const double mu0 = 4.0 * M_PI * 1e-7;
@@ -1092,6 +1193,7 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     tw_x = make_twiddles<T>((int)px);
     tw_y = make_twiddles<T>((int)py);
     LLGMI_HIP(hipMalloc(&spec, sizeof(cpx<T>) * 3 * (px / 2) * ny));
+    if (ly > Range<T>::COL) LLGMI_HIP(hipMalloc(&colfull, sizeof(cpx<T>) * 3 * (px / 2) * py));
     dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         allow_lds(k_row_r2c<T, T, LG, true, 3>, RowCfg<T, LG, 3>::LDS);
@@ -1122,7 +1224,19 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         allow(std::integral_constant<int, 3>{});
         allow(std::integral_constant<int, 4>{});
     });
-    dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
+    if (ly > Range<T>::COL) {
+        dispatch_log2<Range<T>::COL + 1, kColSplitMax>(ly, [&](auto lg) {
+            constexpr int LG = decltype(lg)::value;
+            allow_lds(k_colsplit_fwd<T, LG, true, 3>, colsplit_lds<T, LG>());
+            allow_lds(k_colsplit_fwd<T, LG, false, 3>, colsplit_lds<T, LG>());
+            allow_lds(k_colsplit_fwd<T, LG, true, 4>, colsplit_lds<T, LG>());
+            allow_lds(k_colsplit_fwd<T, LG, false, 4>, colsplit_lds<T, LG>());
+            allow_lds(k_colsplit_mul_inv<T, LG, true, 3>, colsplit_lds<T, LG>());
+            allow_lds(k_colsplit_mul_inv<T, LG, false, 3>, colsplit_lds<T, LG>());
+            allow_lds(k_colsplit_mul_inv<T, LG, true, 4>, colsplit_lds<T, LG>());
+            allow_lds(k_colsplit_mul_inv<T, LG, false, 4>, colsplit_lds<T, LG>());
+        });
+    } else dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         if constexpr (col_rq_ok<LG, 3>()) {
             allow_lds(k_col_demag<T, LG, true, 3, false>, ColGrp<T, LG, 3>::LDS);
@@ -1163,6 +1277,36 @@ void DemagFFT<T>::launch_row_fwd(const T *m, T ms, hipStream_t stream, int rq) {
 
 template <typename T>
 void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
+    if (ly > Range<T>::COL) {
+        // split column pass (the persistent flag does not apply)
+        dispatch_log2<Range<T>::COL + 1, kColSplitMax>(ly, [&](auto lg) {
+            constexpr int LG = decltype(lg)::value;
+            const T scale = T(1.0 / ((double)px * (double)py));
+            const int hxs = (int)(px / 2);
+            const bool half = 2 * ny == py;
+            const dim3 grid((unsigned)(3 * hxs));
+            constexpr size_t lds = colsplit_lds<T, LG>();
+            auto go = [&](auto rqc) {
+                constexpr int RQ = decltype(rqc)::value;
+                constexpr int NT = RegPlan<LG, RQ>::TPS;
+                if (half) {
+                    hipLaunchKernelGGL((k_colsplit_fwd<T, LG, true, RQ>), grid, dim3(NT), lds, stream, spec, colfull,
+                                       (int)ny, tw_y);
+                    hipLaunchKernelGGL((k_colsplit_mul_inv<T, LG, true, RQ>), grid, dim3(NT), lds, stream, spec,
+                                       colfull, hxs, (int)ny, tw_y, kern, scale);
+                } else {
+                    hipLaunchKernelGGL((k_colsplit_fwd<T, LG, false, RQ>), grid, dim3(NT), lds, stream, spec, colfull,
+                                       (int)ny, tw_y);
+                    hipLaunchKernelGGL((k_colsplit_mul_inv<T, LG, false, RQ>), grid, dim3(NT), lds, stream, spec,
+                                       colfull, hxs, (int)ny, tw_y, kern, scale);
+                }
+            };
+            if ((rq & 7) == 3) go(std::integral_constant<int, 3>{});
+            else go(std::integral_constant<int, 4>{});
+            LLGMI_LAUNCH_CHECK();
+        });
+        return;
+    }
     dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         const T scale = T(1.0 / ((double)px * (double)py));

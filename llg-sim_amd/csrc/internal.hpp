@@ -93,6 +93,9 @@ struct DemagFFT {
     cpx<T> *tw_x = nullptr, *tw_y = nullptr;
     T *kern = nullptr;  // 4 planes [Kxx, Kxy, Kyy, Kzz], each [hx][hy]
     cpx<T> *spec = nullptr;    // [3][hx][ny]
+    // full-length forward column spectra [3][px/2][py] of the split column pass
+    // (columns whose paired block would not fit in LDS), else null
+    cpx<T> *colfull = nullptr;
     int rq_row = 4, rq_col = 4, rq_row_inv = 4;  // log2 first/last radix, picked by autotune()
     int rq_fused = 0;   // inverse row pass fused with the local field: log2 radix, -1 = unfused, 0 = untuned
     // real-space base quadrant of the kernel, [(sy*nx + sx)*4 + {xx, xy, yy, zz}]

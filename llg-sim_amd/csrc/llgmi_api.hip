@@ -1696,7 +1696,8 @@ int llgmi_dev_rk45_step_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi
             if (demag && resolved_demag_method(mats[i]) != LLGMI_DEMAG_FFT) { ok = false; break; }
             auto ready = [&](auto &eng) {
                 if (!demag) return true;
-                return eng.fft.built && eng.fft.rq_fused != 0;
+                // split column passes (FP64 columns of 2^13) are not in the batched kernels
+                return eng.fft.built && eng.fft.rq_fused != 0 && eng.fft.colfull == nullptr;
             };
             ok = c->precision == LLGMI_F32 ? ready(*c->ef) : ready(*c->ed);
         }

@@ -135,10 +135,11 @@ def test_demag_forced_radix_fp64_matches_oracle(gpu_lib, monkeypatch, radix, sha
 
 
 # Long FFT lengths: padded 4096 and 8192 along each axis (rows are the x
-# transforms, columns the y transforms).  FP64 builds columns up to 4096
-# (ny <= 2048), FP32 up to 8192; rows go to 8192 in both.
+# transforms, columns the y transforms), up to 8192 in both precisions.  FP64
+# columns of 8192 (ny > 2048) take the split column pass (one sequence per
+# block, forward spectra through HBM).
 @pytest.mark.parametrize("radix", ["auto", "8", "16"])
-@pytest.mark.parametrize("shape", [(8, 1536), (1536, 8), (2100, 8), (5, 1100)])
+@pytest.mark.parametrize("shape", [(8, 1536), (1536, 8), (2100, 8), (5, 1100), (8, 2100), (3, 4096)])
 def test_demag_long_lengths_fp64_matches_oracle(gpu_lib, monkeypatch, radix, shape):
     from oracle import oracle as O
     if radix != "auto":
@@ -187,13 +188,30 @@ def test_demag_fp32_length_8192_matches_oracle(gpu_lib, shape):
     assert rel_err(b, b_ref) <= 1e-5
 
 
-@pytest.mark.parametrize("shape", [(8, 2049), (4097, 8)])
-def test_demag_beyond_built_lengths_is_unsupported(gpu_lib, shape):
-    """An FP64 column of 8192 (ny > 2048) or a row of 16384 (nx > 4096) is
-    refused with LLGMI_ERR_UNSUPPORTED, before any launch."""
+def test_demag_fp64_4096_square_matches_oracle(gpu_lib):
+    """4096 x 4096 in FP64: rows of 8192 and split columns of 8192."""
+    from oracle import oracle as O
+    L = gpu_lib
+    nx = ny = 4096
+    cell = (3e-9, 2e-9, 1e-9)
+    m = _rand_m(nx * ny, 4096)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    b_ref = np.zeros_like(m)
+    oc.add_demag(m, b_ref, O.Material(ms=8e5, demag=True))
+    del oc
+    ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    b = ctx.compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(b, b_ref) <= 1e-12
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("shape", [(8, 4097), (4097, 8)])
+def test_demag_beyond_built_lengths_is_unsupported(gpu_lib, prec, shape):
+    """A padded length of 16384 (nx or ny > 4096) is refused with
+    LLGMI_ERR_UNSUPPORTED, before any launch."""
     L = gpu_lib
     nx, ny = shape
-    ctx = L.Context(L.Grid2D(nx, ny, 3e-9, 2e-9, 1e-9), precision=L.F64)
+    ctx = L.Context(L.Grid2D(nx, ny, 3e-9, 2e-9, 1e-9), precision=prec)
     with pytest.raises(L.LlgmiError) as e:
         ctx.compute_demag_field(_rand_m(nx * ny, 5), L.Material(ms=8e5, demag=True))
     assert e.value.code == L.ERR_UNSUPPORTED
