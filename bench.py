@@ -373,6 +373,9 @@ def warm_start_bench():
                                    "max_dm_between_states": c["max_dm_between_states_mean"],
                                    **{f"{n}_v_cycles": c[n]["v_cycles_mean"] for n in names},
                                    **{f"{n}_ms": c[n]["ms_mean_incl_inference"] for n in names}}
+    if p_delta is not None:
+        out["guess_error_study"] = {f"stride_{k}": WS.guess_error_study(prob, p_delta, stride=k, states=n)
+                                    for k, n in ((1, 10), (20, 5))}
     out["mfma"] = WS.measure_throughput(prob, params)
     return out
 

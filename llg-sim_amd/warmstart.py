@@ -360,6 +360,106 @@ def compare(prob: Problem, params: np.ndarray, states: int = 30, seed: int = 424
     return out
 
 
+
+def _smooth121(c: np.ndarray) -> np.ndarray:
+    """[1 2 1] / 4 along the last two axes (y, x), edges clamped."""
+    for ax in (1, 2):
+        lo = np.concatenate([np.take(c, [0], axis=ax), np.take(c, range(c.shape[ax] - 1), axis=ax)], axis=ax)
+        hi = np.concatenate([np.take(c, range(1, c.shape[ax]), axis=ax), np.take(c, [-1], axis=ax)], axis=ax)
+        c = 0.25 * lo + 0.5 * c + 0.25 * hi
+    return c
+
+
+def guess_error_study(prob: Problem, params_delta, states: int = 10, seed: int = 424242, pitch: float = 96e-9,
+                      skip: int = 5, stride: int = 1, eps=(0.0, 0.02, 0.05, 0.1, 0.2, 0.3, 0.5, 0.7, 1.0),
+                      tol_exact: float = 1e-10, smooth=(2,)):
+    """How accurate an initial guess must be to save V-cycles, against what
+    the trained change network reaches, along the relaxation of compare().
+
+    For each state, phi* is its potential converged to tol_exact (from the
+    previous potential) and phi_prev the previous state's potential at
+    tol_rel 1e-6, as in compare().  The guesses
+        g_eps = phi_prev + (1 - eps) (phi* - phi_prev)
+    have error eps times the previous-step guess's error (eps = 1 is the
+    previous-step guess, eps = 0 the exact potential); the learned guess is
+    phi_prev + s N((m - m_prev) / s) and its error ratio
+    ||g - phi*|| / ||phi_prev - phi*|| (L2 over the box) is measured.  Every
+    guess is iterated to tol_rel 1e-6 (at least one V-cycle)."""
+    pdel = Predictor(prob, params_delta)
+    prob.film(seed, pitch)
+    p_llg = L.LLGParams(alpha=0.5, dt=1e-13)
+    for _ in range(skip):
+        prob.relax_step(p_llg)
+    nphi = prob.phi_bytes // 8
+    phi_prev, guess = L.DeviceArray(prob.phi_bytes), L.DeviceArray(prob.phi_bytes)
+    prob.set_tol(TOL_SOLVE, warm=False)
+    prob.solve()
+    prob.ctx.mg_phi_read(phi_prev)
+    m_last = prob.state()
+    cyc = {e: [] for e in eps}
+    cyc_net, ratio_net, ratio_inf, worst = [], [], [], []
+    cyc_s = {k: [] for k in smooth}
+    ratio_s = {k: [] for k in smooth}
+    for _ in range(states):
+        for _ in range(stride):
+            prob.relax_step(p_llg)
+        m_now = prob.state()
+        s_delta = set_delta(prob, m_now, m_last)
+        m_last = m_now
+        h_prev = phi_prev.download(np.float64, nphi)
+        prob.ctx.mg_phi_write(phi_prev)
+        prob.set_tol(tol_exact, warm=True)
+        prob.solve()
+        prob.ctx.mg_phi_read(guess)
+        h_star = guess.download(np.float64, nphi)
+        e_prev = float(np.linalg.norm(h_prev - h_star))
+        prob.set_tol(TOL_SOLVE, warm=True)
+        for e in eps:
+            guess.upload(h_prev + (1.0 - e) * (h_star - h_prev))
+            prob.ctx.mg_phi_write(guess)
+            cyc[e].append(prob.solve())
+        pdel.run(prob.dm)
+        guess.copy_from(phi_prev, prob.phi_bytes)
+        pdel.potential(pdel.y, guess, s_delta, True)
+        g = guess.download(np.float64, nphi)
+        ratio_net.append(float(np.linalg.norm(g - h_star)) / e_prev if e_prev > 0 else 0.0)
+        einf_prev = float(np.abs(h_prev - h_star).max())
+        err_net = np.abs(g - h_star)
+        ratio_inf.append(float(err_net.max()) / einf_prev if einf_prev > 0 else 0.0)
+        kz, jy, ix = np.unravel_index(int(err_net.argmax()), (prob.pz, prob.py, prob.px))
+        worst.append((int(kz - prob.offz), int(jy - prob.offy), int(ix - prob.offx)))
+        prob.ctx.mg_phi_write(guess)
+        cyc_net.append(prob.solve())
+        # the learned correction low-pass filtered ([1 2 1]/4 along x and y, k times)
+        corr = (g - h_prev).reshape(prob.pz, prob.py, prob.px)
+        for k in smooth:
+            c = corr
+            for _ in range(k):
+                c = _smooth121(c)
+            gs = h_prev + c.reshape(-1)
+            ratio_s[k].append(float(np.linalg.norm(gs - h_star)) / e_prev if e_prev > 0 else 0.0)
+            guess.upload(gs)
+            prob.ctx.mg_phi_write(guess)
+            cyc_s[k].append(prob.solve())
+        # the previous-step guess of the next state: this state at tol_rel 1e-6
+        prob.ctx.mg_phi_write(phi_prev)
+        prob.solve()
+        prob.ctx.mg_phi_read(phi_prev)
+    pdel.net.close()
+    prev = float(np.mean(cyc[1.0]))
+    return {"workload": f"{prob.nx}x{prob.ny} film, holes pitch {pitch * 1e9:.0f} nm, tol_rel {TOL_SOLVE:g}, "
+                        f"{states} states {stride} step(s) apart, exact potential at tol_rel {tol_exact:g}",
+            "v_cycles_by_error_ratio": {f"{e:g}": round(float(np.mean(cyc[e])), 2) for e in eps},
+            "learned_change": {"error_ratio_mean": round(float(np.mean(ratio_net)), 4),
+                               "error_ratio_min": round(float(np.min(ratio_net)), 4),
+                               "max_norm_error_ratio_mean": round(float(np.mean(ratio_inf)), 4),
+                               "worst_cell_vs_film_origin_kji": worst[:4],
+                               "v_cycles": round(float(np.mean(cyc_net)), 2)},
+            "learned_change_smoothed": {f"{k}": {"error_ratio_mean": round(float(np.mean(ratio_s[k])), 4),
+                                                 "v_cycles": round(float(np.mean(cyc_s[k])), 2)} for k in smooth},
+            "previous_step_v_cycles": round(prev, 2),
+            "v_cycles_for_30pct_fewer": round(0.7 * prev, 2)}
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--runs", type=int, default=20)
