@@ -545,12 +545,29 @@ struct FusedCfg {
 template <typename T, int LG, int RQ>
 constexpr bool fused_ok() { return FusedCfg<T, LG, RQ>::OK; }
 
-template <typename T, int LOG2L, bool HALF, int RQ, bool GEOM, bool TORQUE>
+// The batched fused pass groups as many row pairs per block as keep it at
+// 3 * PAIRS * TPS = 768 threads (within the LDS and thread limits), so that
+// films of different lengths share one launch.  Rows are transformed and
+// their cells evaluated the same way for any PAIRS.
+template <typename T, int LOG2L, int RQ>
+struct BatchFusedCfg {
+    using P = RegPlan<LOG2L, RQ>;
+    static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
+    static constexpr size_t kLdsCU = 160 * 1024;
+    static constexpr int fit(int p) {
+        return (p > 1 && (3 * p * TPS > 1024 || sizeof(cpx<T>) * 3 * p * SLEN > kLdsCU)) ? fit(p / 2) : p;
+    }
+    static constexpr int PAIRS = fit(TPS >= 256 ? 1 : 256 / TPS);
+    static constexpr int NT = 3 * PAIRS * TPS;
+    static constexpr size_t LDS = sizeof(cpx<T>) * 3 * PAIRS * SLEN;
+    static constexpr bool OK = NT <= 1024 && LDS <= kLdsCU;
+};
+
+template <typename T, int LOG2L, bool HALF, int RQ, bool GEOM, bool TORQUE, typename C = FusedCfg<T, LOG2L, RQ>>
 __device__ __forceinline__ void row_c2r_local_body(const cpx<T> *__restrict__ S, int hxs, int ny,
                                                    const cpx<T> *__restrict__ tw, const LocalArgs<T> &a,
                                                    const T *__restrict__ m, const uint8_t *__restrict__ geom,
                                                    T *__restrict__ b, T *__restrict__ kout, int j0) {
-    using C = FusedCfg<T, LOG2L, RQ>;
     constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
     constexpr int R0 = C::P::R0;
     constexpr int NYQ = L >> 1;
@@ -1565,52 +1582,136 @@ void DemagFFT<T>::read_kernel(std::vector<double> &out) const {
 
 // ---------------------------------------------------------------------------
 // Batched passes: the block of table entry blockIdx.x runs the single-film
-// body for its film and block.
+// body for its film and block.  One launch covers the films of every length
+// in [LO, HI] whose blocks have the same shape (threads per block); a block
+// runs the body of its own film's length (the kernel takes the largest LDS
+// of the range).
 // ---------------------------------------------------------------------------
-template <typename T, int LOG2L, bool HALF, int RQ>
-__global__ void __launch_bounds__((RowCfg<T, LOG2L, RQ>::NT))
+template <typename T, int KIND, int LG, int RQ>
+struct BatchCfg;
+template <typename T, int LG, int RQ>
+struct BatchCfg<T, 0, LG, RQ> {
+    static constexpr bool OK = LG >= 1 && LG <= Range<T>::ROW;
+    static constexpr int NT = RowCfg<T, LG, RQ>::NT;
+    static constexpr size_t LDS = RowCfg<T, LG, RQ>::LDS;
+};
+template <typename T, int LG, int RQ>
+struct BatchCfg<T, 2, LG, RQ> : BatchCfg<T, 0, LG, RQ> {};
+template <typename T, int LG, int RQ>
+struct BatchCfg<T, 1, LG, RQ> {
+    static constexpr bool OK = LG >= 1 && LG <= Range<T>::COL && (RQ == 4 || col_rq_ok<LG, 3>());
+    static constexpr int NT = ColGrp<T, LG, RQ>::NT;
+    static constexpr size_t LDS = ColGrp<T, LG, RQ>::LDS;
+};
+template <typename T, int LG, int RQ>
+struct BatchCfg<T, 3, LG, RQ> {
+    static constexpr bool OK = LG >= 1 && LG <= Range<T>::ROW && BatchFusedCfg<T, LG, RQ>::OK;
+    static constexpr int NT = BatchFusedCfg<T, LG, RQ>::NT;
+    static constexpr size_t LDS = BatchFusedCfg<T, LG, RQ>::LDS;
+};
+template <typename T, int KIND>
+constexpr int batch_max_lg() { return KIND == 1 ? Range<T>::COL : Range<T>::ROW; }
+
+// the range of lengths around LG whose blocks have LG's shape
+template <typename T, int KIND, int RQ, int LG>
+constexpr int batch_run_lo() {
+    if constexpr (LG > 1) {
+        if constexpr (BatchCfg<T, KIND, LG - 1, RQ>::OK &&
+                      BatchCfg<T, KIND, LG - 1, RQ>::NT == BatchCfg<T, KIND, LG, RQ>::NT)
+            return batch_run_lo<T, KIND, RQ, LG - 1>();
+        else
+            return LG;
+    } else {
+        return LG;
+    }
+}
+template <typename T, int KIND, int RQ, int LG>
+constexpr int batch_run_hi() {
+    if constexpr (LG < batch_max_lg<T, KIND>()) {
+        if constexpr (BatchCfg<T, KIND, LG + 1, RQ>::OK &&
+                      BatchCfg<T, KIND, LG + 1, RQ>::NT == BatchCfg<T, KIND, LG, RQ>::NT)
+            return batch_run_hi<T, KIND, RQ, LG + 1>();
+        else
+            return LG;
+    } else {
+        return LG;
+    }
+}
+template <typename T, int KIND, int RQ, int LO, int HI>
+constexpr size_t batch_run_lds() {
+    if constexpr (LO <= HI) {
+        constexpr size_t a = BatchCfg<T, KIND, LO, RQ>::LDS, b = batch_run_lds<T, KIND, RQ, LO + 1, HI>();
+        return a > b ? a : b;
+    } else {
+        return 0;
+    }
+}
+
+// the body for length lg of [LO, HI] (lg is uniform over the block)
+template <int LO, int HI, typename Fn>
+__device__ __forceinline__ void dev_dispatch_log2(int lg, Fn &&fn) {
+    if constexpr (LO <= HI) {
+        if (lg == LO) fn(std::integral_constant<int, LO>{});
+        else dev_dispatch_log2<LO + 1, HI>(lg, static_cast<Fn &&>(fn));
+    }
+}
+
+template <typename T, int LO, int HI, bool HALF, int RQ>
+__global__ void __launch_bounds__((RowCfg<T, LO, RQ>::NT))
 kb_row_r2c(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int which, int stage1) {
-    constexpr int PAIRS = RowCfg<T, LOG2L, RQ>::PAIRS;
     const BEntry e = tab[blockIdx.x];
     const BFilm<T> &f = F[e.film];
     if (stage1 && f.skip1) return;
-    const int nb = (f.ny + 2 * PAIRS - 1) / (2 * PAIRS);
-    const int comp = e.blk / nb, j0 = (e.blk - comp * nb) * 2 * PAIRS;
-    row_r2c_body<T, T, LOG2L, HALF, RQ>(f.in[which], (int64_t)f.nx * f.ny, f.nx, f.nx, f.ny, f.ms, f.tw_x, f.spec,
-                                        (int64_t)(f.px / 2) * f.ny, comp, j0);
+    dev_dispatch_log2<LO, HI>(__builtin_ctz((unsigned)f.px), [&](auto lgc) {
+        constexpr int LG = decltype(lgc)::value;
+        constexpr int PAIRS = RowCfg<T, LG, RQ>::PAIRS;
+        const int nb = (f.ny + 2 * PAIRS - 1) / (2 * PAIRS);
+        const int comp = e.blk / nb, j0 = (e.blk - comp * nb) * 2 * PAIRS;
+        row_r2c_body<T, T, LG, HALF, RQ>(f.in[which], (int64_t)f.nx * f.ny, f.nx, f.nx, f.ny, f.ms, f.tw_x, f.spec,
+                                         (int64_t)(f.px / 2) * f.ny, comp, j0);
+    });
 }
 
-template <typename T, int LOG2L, bool HALF, int RQ>
-__global__ void __launch_bounds__((ColGrp<T, LOG2L, RQ>::NT))
+template <typename T, int LO, int HI, bool HALF, int RQ>
+__global__ void __launch_bounds__((ColGrp<T, LO, RQ>::NT))
 kb_col(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int stage1) {
     const BEntry e = tab[blockIdx.x];
     const BFilm<T> &f = F[e.film];
     if (stage1 && f.skip1) return;
     const int hxs = f.px / 2;
-    col_work<T, LOG2L, HALF, RQ>(f.spec, hxs, f.ny, f.tw_y, f.kern, f.scale, hxs + (hxs + 1) / 2, e.blk);
+    dev_dispatch_log2<LO, HI>(__builtin_ctz((unsigned)f.py), [&](auto lgc) {
+        constexpr int LG = decltype(lgc)::value;
+        col_work<T, LG, HALF, RQ>(f.spec, hxs, f.ny, f.tw_y, f.kern, f.scale, hxs + (hxs + 1) / 2, e.blk);
+    });
 }
 
-template <typename T, int LOG2L, bool HALF, int RQ>
-__global__ void __launch_bounds__((RowCfg<T, LOG2L, RQ>::NT))
+template <typename T, int LO, int HI, bool HALF, int RQ>
+__global__ void __launch_bounds__((RowCfg<T, LO, RQ>::NT))
 kb_row_c2r(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int stage1) {
-    constexpr int PAIRS = RowCfg<T, LOG2L, RQ>::PAIRS;
     const BEntry e = tab[blockIdx.x];
     const BFilm<T> &f = F[e.film];
     if (stage1 && f.skip1) return;
-    const int nb = (f.ny + 2 * PAIRS - 1) / (2 * PAIRS);
-    const int comp = e.blk / nb, j0 = (e.blk - comp * nb) * 2 * PAIRS;
-    row_c2r_body<T, LOG2L, HALF, RQ>(f.spec, f.px / 2, f.nx, f.ny, f.tw_x, f.bd, (int64_t)f.nx * f.ny, comp, j0);
+    dev_dispatch_log2<LO, HI>(__builtin_ctz((unsigned)f.px), [&](auto lgc) {
+        constexpr int LG = decltype(lgc)::value;
+        constexpr int PAIRS = RowCfg<T, LG, RQ>::PAIRS;
+        const int nb = (f.ny + 2 * PAIRS - 1) / (2 * PAIRS);
+        const int comp = e.blk / nb, j0 = (e.blk - comp * nb) * 2 * PAIRS;
+        row_c2r_body<T, LG, HALF, RQ>(f.spec, f.px / 2, f.nx, f.ny, f.tw_x, f.bd, (int64_t)f.nx * f.ny, comp, j0);
+    });
 }
 
-template <typename T, int LOG2L, bool HALF, int RQ>
-__global__ void __launch_bounds__((FusedCfg<T, LOG2L, RQ>::NT))
+template <typename T, int LO, int HI, bool HALF, int RQ>
+__global__ void __launch_bounds__((BatchFusedCfg<T, LO, RQ>::NT))
 kb_row_c2r_local(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int which, int ks, int stage1) {
-    constexpr int PAIRS = FusedCfg<T, LOG2L, RQ>::PAIRS;
     const BEntry e = tab[blockIdx.x];
     const BFilm<T> &f = F[e.film];
     if (stage1 && f.skip1) return;
-    row_c2r_local_body<T, LOG2L, HALF, RQ, false, true>(f.spec, f.px / 2, f.ny, f.tw_x, f.la, f.in[which], nullptr,
-                                                        f.b, f.k[ks], e.blk * 2 * PAIRS);
+    dev_dispatch_log2<LO, HI>(__builtin_ctz((unsigned)f.px), [&](auto lgc) {
+        constexpr int LG = decltype(lgc)::value;
+        using C = BatchFusedCfg<T, LG, RQ>;
+        row_c2r_local_body<T, LG, HALF, RQ, false, true, C>(f.spec, f.px / 2, f.ny, f.tw_x, f.la, f.in[which],
+                                                            nullptr, f.b, f.k[ks], e.blk * 2 * C::PAIRS);
+    });
 }
 
 template <typename T>
@@ -1632,11 +1733,7 @@ int batch_row_blocks(int kind, int lg, int rq, int ny) {
     dispatch_log2<1, Range<T>::ROW>(lg, [&](auto lgc) {
         constexpr int LG = decltype(lgc)::value;
         if (kind == 3) {
-            if (rq == 3) {
-                if constexpr (fused_ok<T, LG, 3>()) pairs = FusedCfg<T, LG, 3>::PAIRS;
-            } else {
-                if constexpr (fused_ok<T, LG, 4>()) pairs = FusedCfg<T, LG, 4>::PAIRS;
-            }
+            pairs = rq == 3 ? BatchFusedCfg<T, LG, 3>::PAIRS : BatchFusedCfg<T, LG, 4>::PAIRS;
         } else {
             pairs = rq == 3 ? RowCfg<T, LG, 3>::PAIRS : RowCfg<T, LG, 4>::PAIRS;
         }
@@ -1644,32 +1741,62 @@ int batch_row_blocks(int kind, int lg, int rq, int ny) {
     return (ny + 2 * pairs - 1) / (2 * pairs);
 }
 
+// Calls fn(KIND, LO, HI, RQ as integral constants) for the merged class of
+// (kind, lg, rq).
+template <typename T, typename Fn>
+static void batch_class_dispatch(int kind, int lg, int rq, Fn &&fn) {
+    auto with_kind = [&](auto kc) {
+        constexpr int KIND = decltype(kc)::value;
+        dispatch_log2<1, batch_max_lg<T, KIND>()>(lg, [&](auto lgc) {
+            constexpr int LG = decltype(lgc)::value;
+            auto with_rq = [&](auto rqc) {
+                constexpr int RQ = decltype(rqc)::value;
+                if constexpr (BatchCfg<T, KIND, LG, RQ>::OK) {
+                    constexpr int LO = batch_run_lo<T, KIND, RQ, LG>(), HI = batch_run_hi<T, KIND, RQ, LG>();
+                    fn(kc, std::integral_constant<int, LO>{}, std::integral_constant<int, HI>{}, rqc);
+                } else {
+                    throw Error(3, "batched FFT pass: no kernel for this length and radix");
+                }
+            };
+            if ((rq & 7) == 3) with_rq(std::integral_constant<int, 3>{});
+            else with_rq(std::integral_constant<int, 4>{});
+        });
+    };
+    switch (kind) {
+        case 0: with_kind(std::integral_constant<int, 0>{}); break;
+        case 1: with_kind(std::integral_constant<int, 1>{}); break;
+        case 2: with_kind(std::integral_constant<int, 2>{}); break;
+        case 3: with_kind(std::integral_constant<int, 3>{}); break;
+        default: throw Error(1, "batched FFT pass: unknown kind");
+    }
+}
+
 template <typename T>
-void batch_allow_lds(int lg_x, int lg_y) {
-    dispatch_log2<1, Range<T>::ROW>(lg_x, [&](auto lgc) {
-        constexpr int LG = decltype(lgc)::value;
-        auto rows = [&](auto rqc) {
-            constexpr int RQ = decltype(rqc)::value;
-            allow_lds(kb_row_r2c<T, LG, true, RQ>, RowCfg<T, LG, RQ>::LDS);
-            allow_lds(kb_row_r2c<T, LG, false, RQ>, RowCfg<T, LG, RQ>::LDS);
-            allow_lds(kb_row_c2r<T, LG, true, RQ>, RowCfg<T, LG, RQ>::LDS);
-            allow_lds(kb_row_c2r<T, LG, false, RQ>, RowCfg<T, LG, RQ>::LDS);
-            if constexpr (fused_ok<T, LG, RQ>()) {
-                allow_lds(kb_row_c2r_local<T, LG, true, RQ>, FusedCfg<T, LG, RQ>::LDS);
-                allow_lds(kb_row_c2r_local<T, LG, false, RQ>, FusedCfg<T, LG, RQ>::LDS);
-            }
-        };
-        rows(std::integral_constant<int, 3>{});
-        rows(std::integral_constant<int, 4>{});
-    });
-    dispatch_log2<1, Range<T>::COL>(lg_y, [&](auto lgc) {
-        constexpr int LG = decltype(lgc)::value;
-        if constexpr (col_rq_ok<LG, 3>()) {
-            allow_lds(kb_col<T, LG, true, 3>, ColGrp<T, LG, 3>::LDS);
-            allow_lds(kb_col<T, LG, false, 3>, ColGrp<T, LG, 3>::LDS);
+int batch_class_lg(int kind, int lg, int rq) {
+    int lo = lg;
+    batch_class_dispatch<T>(kind, lg, rq, [&](auto, auto loc, auto, auto) { lo = decltype(loc)::value; });
+    return lo;
+}
+
+template <typename T>
+void batch_allow_lds(int kind, int lg, int rq) {
+    batch_class_dispatch<T>(kind, lg, rq, [&](auto kc, auto loc, auto hic, auto rqc) {
+        constexpr int KIND = decltype(kc)::value, LO = decltype(loc)::value, HI = decltype(hic)::value,
+                      RQ = decltype(rqc)::value;
+        constexpr size_t lds = batch_run_lds<T, KIND, RQ, LO, HI>();
+        if constexpr (KIND == 0) {
+            allow_lds(kb_row_r2c<T, LO, HI, true, RQ>, lds);
+            allow_lds(kb_row_r2c<T, LO, HI, false, RQ>, lds);
+        } else if constexpr (KIND == 1) {
+            allow_lds(kb_col<T, LO, HI, true, RQ>, lds);
+            allow_lds(kb_col<T, LO, HI, false, RQ>, lds);
+        } else if constexpr (KIND == 2) {
+            allow_lds(kb_row_c2r<T, LO, HI, true, RQ>, lds);
+            allow_lds(kb_row_c2r<T, LO, HI, false, RQ>, lds);
+        } else {
+            allow_lds(kb_row_c2r_local<T, LO, HI, true, RQ>, lds);
+            allow_lds(kb_row_c2r_local<T, LO, HI, false, RQ>, lds);
         }
-        allow_lds(kb_col<T, LG, true, 4>, ColGrp<T, LG, 4>::LDS);
-        allow_lds(kb_col<T, LG, false, 4>, ColGrp<T, LG, 4>::LDS);
     });
 }
 
@@ -1678,59 +1805,38 @@ void launch_batch_fft(int kind, int lg, int rq, bool half, const BFilm<T> *F, co
                       int which, int ks, int stage1, hipStream_t s) {
     if (nent <= 0) return;
     const dim3 grid((unsigned)nent);
-    auto go = [&](auto lgc, auto rqc) {
-        constexpr int LG = decltype(lgc)::value, RQ = decltype(rqc)::value;
-        if (kind == 0) {
-            using C = RowCfg<T, LG, RQ>;
-            if (half) hipLaunchKernelGGL((kb_row_r2c<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, which, stage1);
-            else hipLaunchKernelGGL((kb_row_r2c<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, which, stage1);
-        } else if (kind == 2) {
-            using C = RowCfg<T, LG, RQ>;
-            if (half) hipLaunchKernelGGL((kb_row_c2r<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, stage1);
-            else hipLaunchKernelGGL((kb_row_c2r<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, stage1);
-        } else if (kind == 3) {
-            if constexpr (fused_ok<T, LG, RQ>()) {
-                using C = FusedCfg<T, LG, RQ>;
-                if (half)
-                    hipLaunchKernelGGL((kb_row_c2r_local<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab,
-                                       which, ks, stage1);
-                else
-                    hipLaunchKernelGGL((kb_row_c2r_local<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab,
-                                       which, ks, stage1);
-            } else {
-                throw Error(3, "fused inverse row pass: block too large for this radix");
-            }
+    batch_class_dispatch<T>(kind, lg, rq, [&](auto kc, auto loc, auto hic, auto rqc) {
+        constexpr int KIND = decltype(kc)::value, LO = decltype(loc)::value, HI = decltype(hic)::value,
+                      RQ = decltype(rqc)::value;
+        constexpr size_t lds = batch_run_lds<T, KIND, RQ, LO, HI>();
+        const dim3 nt((unsigned)BatchCfg<T, KIND, LO, RQ>::NT);
+        if constexpr (KIND == 0) {
+            if (half) hipLaunchKernelGGL((kb_row_r2c<T, LO, HI, true, RQ>), grid, nt, lds, s, F, tab, which, stage1);
+            else hipLaunchKernelGGL((kb_row_r2c<T, LO, HI, false, RQ>), grid, nt, lds, s, F, tab, which, stage1);
+        } else if constexpr (KIND == 1) {
+            if (half) hipLaunchKernelGGL((kb_col<T, LO, HI, true, RQ>), grid, nt, lds, s, F, tab, stage1);
+            else hipLaunchKernelGGL((kb_col<T, LO, HI, false, RQ>), grid, nt, lds, s, F, tab, stage1);
+        } else if constexpr (KIND == 2) {
+            if (half) hipLaunchKernelGGL((kb_row_c2r<T, LO, HI, true, RQ>), grid, nt, lds, s, F, tab, stage1);
+            else hipLaunchKernelGGL((kb_row_c2r<T, LO, HI, false, RQ>), grid, nt, lds, s, F, tab, stage1);
+        } else {
+            if (half)
+                hipLaunchKernelGGL((kb_row_c2r_local<T, LO, HI, true, RQ>), grid, nt, lds, s, F, tab, which, ks,
+                                   stage1);
+            else
+                hipLaunchKernelGGL((kb_row_c2r_local<T, LO, HI, false, RQ>), grid, nt, lds, s, F, tab, which, ks,
+                                   stage1);
         }
-    };
-    if (kind == 1) {
-        dispatch_log2<1, Range<T>::COL>(lg, [&](auto lgc) {
-            constexpr int LG = decltype(lgc)::value;
-            auto col = [&](auto rqc) {
-                constexpr int RQ = decltype(rqc)::value;
-                using C = ColGrp<T, LG, RQ>;
-                if (half) hipLaunchKernelGGL((kb_col<T, LG, true, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, stage1);
-                else hipLaunchKernelGGL((kb_col<T, LG, false, RQ>), grid, dim3(C::NT), C::LDS, s, F, tab, stage1);
-            };
-            if constexpr (col_rq_ok<LG, 3>()) {
-                if ((rq & 7) == 3) col(std::integral_constant<int, 3>{});
-                else col(std::integral_constant<int, 4>{});
-            } else {
-                col(std::integral_constant<int, 4>{});
-            }
-        });
-    } else {
-        dispatch_log2<1, Range<T>::ROW>(lg, [&](auto lgc) {
-            if (rq == 3) go(lgc, std::integral_constant<int, 3>{});
-            else go(lgc, std::integral_constant<int, 4>{});
-        });
-    }
+    });
     LLGMI_LAUNCH_CHECK();
 }
 
 template int batch_row_blocks<float>(int, int, int, int);
 template int batch_row_blocks<double>(int, int, int, int);
-template void batch_allow_lds<float>(int, int);
-template void batch_allow_lds<double>(int, int);
+template void batch_allow_lds<float>(int, int, int);
+template int batch_class_lg<float>(int, int, int);
+template void batch_allow_lds<double>(int, int, int);
+template int batch_class_lg<double>(int, int, int);
 template void launch_batch_fft<float>(int, int, int, bool, const BFilm<float> *, const BEntry *, int, int, int, int,
                                       hipStream_t);
 template void launch_batch_fft<double>(int, int, int, bool, const BFilm<double> *, const BEntry *, int, int, int,

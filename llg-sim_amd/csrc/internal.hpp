@@ -213,8 +213,13 @@ void launch_batch_fft(int kind, int lg, int rq, bool half, const BFilm<T> *F, co
 // for the column pass (kind 1, ny = item count) the blocks of items
 template <typename T>
 int batch_row_blocks(int kind, int lg, int rq, int ny);
+// The representative length of the launch class of (kind, lg, rq): one
+// launch covers the films of every length whose blocks have the same shape.
 template <typename T>
-void batch_allow_lds(int lg_x, int lg_y);
+int batch_class_lg(int kind, int lg, int rq);
+// Opts the class's kernels into the LDS they need.
+template <typename T>
+void batch_allow_lds(int kind, int lg, int rq);
 
 template <typename T>
 void launch_aos_to_soa(const double *aos, T *soa, int64_t n, hipStream_t s);

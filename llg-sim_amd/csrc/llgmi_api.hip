@@ -1229,7 +1229,9 @@ std::unique_ptr<BatchPlan<T>> build_batch_plan(const std::vector<Engine<T> *> &e
     P->dev = dev;
     P->nfilms = (int)engs.size();
     std::vector<BEntry> tab;
-    // launch classes of each kind, keyed by (lg, rq, half) / demag flag
+    // launch classes of each kind, keyed by (class length, rq, half) / demag
+    // flag; a class covers the films of every length whose blocks have the
+    // same shape
     std::map<std::tuple<int, int, int, int>, std::vector<BEntry>> cls;
     std::vector<BEntry> elem;
     std::vector<int> e0((size_t)P->nfilms), ne((size_t)P->nfilms);
@@ -1248,22 +1250,22 @@ std::unique_ptr<BatchPlan<T>> build_batch_plan(const std::vector<Engine<T> *> &e
         const DemagFFT<T> &op = e.fft;
         const bool hx = 2 * op.nx == op.px, hy = 2 * op.ny == op.py;
         {
-            auto &v = cls[{0, op.lx, op.rq_row, hx}];
+            auto &v = cls[{0, batch_class_lg<T>(0, op.lx, op.rq_row), op.rq_row, hx}];
             const int nb = batch_row_blocks<T>(0, op.lx, op.rq_row, ny);
             for (int b = 0; b < 3 * nb; ++b) v.push_back({f, b});
         }
         {
-            auto &v = cls[{1, op.ly, op.rq_col & 7, hy}];
+            auto &v = cls[{1, batch_class_lg<T>(1, op.ly, op.rq_col & 7), op.rq_col & 7, hy}];
             const int hxs = (int)(op.px / 2), nwork = hxs + (hxs + 1) / 2;
             const int nb = batch_row_blocks<T>(1, op.ly, op.rq_col & 7, nwork);
             for (int w = 0; w < nb; ++w) v.push_back({f, w});
         }
         if (op.rq_fused > 0) {
-            auto &v = cls[{3, op.lx, op.rq_fused, hx}];
+            auto &v = cls[{3, batch_class_lg<T>(3, op.lx, op.rq_fused), op.rq_fused, hx}];
             const int nb = batch_row_blocks<T>(3, op.lx, op.rq_fused, ny);
             for (int b = 0; b < nb; ++b) v.push_back({f, b});
         } else {
-            auto &v = cls[{2, op.lx, op.rq_row_inv, hx}];
+            auto &v = cls[{2, batch_class_lg<T>(2, op.lx, op.rq_row_inv), op.rq_row_inv, hx}];
             const int nb = batch_row_blocks<T>(2, op.lx, op.rq_row_inv, ny);
             for (int b = 0; b < 3 * nb; ++b) v.push_back({f, b});
             auto &w = cls[{5, 0, 0, 0}];   // local field with B_demag, after the inverse passes
@@ -1272,7 +1274,16 @@ std::unique_ptr<BatchPlan<T>> build_batch_plan(const std::vector<Engine<T> *> &e
         (void)nx;
     }
     // launch order: all row R2C, all column, all inverse (fused and unfused),
-    // then the local fields (the map orders keys by kind)
+    // then the local fields (the map orders keys by kind); within a class the
+    // blocks of the longest transforms go first
+    for (auto &kv : cls) {
+        const int kind = std::get<0>(kv.first);
+        if (kind < 4)
+            std::stable_sort(kv.second.begin(), kv.second.end(), [&](const BEntry &a, const BEntry &b) {
+                const DemagFFT<T> &fa = engs[(size_t)a.film]->fft, &fb = engs[(size_t)b.film]->fft;
+                return kind == 1 ? fa.ly > fb.ly : fa.lx > fb.lx;
+            });
+    }
     for (auto &kv : cls) {
         const int kind = std::get<0>(kv.first);
         typename BatchPlan<T>::Launch l{};
@@ -1290,7 +1301,7 @@ std::unique_ptr<BatchPlan<T>> build_batch_plan(const std::vector<Engine<T> *> &e
     P->elem_n = (int)elem.size();
     tab.insert(tab.end(), elem.begin(), elem.end());
     for (auto &l : P->field)
-        if (l.kind < 4) batch_allow_lds<T>(l.kind == 1 ? 1 : l.lg, l.kind == 1 ? l.lg : 1);
+        if (l.kind < 4) batch_allow_lds<T>(l.kind, l.lg, l.rq);
     LLGMI_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     LLGMI_HIP(hipMalloc(&P->d_films, sizeof(BFilm<T>) * P->nfilms));
     LLGMI_HIP(hipHostMalloc((void **)&P->h_films, sizeof(BFilm<T>) * P->nfilms, hipHostMallocDefault));
