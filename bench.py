@@ -262,11 +262,13 @@ def mg_bench(L, cpu: bool, reps: int = 5):
 
 
 def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
-    """Independent films evaluated side by side: one context (and HIP stream)
-    per film, sizes from {64,128,256,512} x {64,128,256}, dx in [2, 5] nm,
-    dz = 3 nm, random materials, smoothed random m; a round queues one full
-    field evaluation (Full mask, FFT demag) on every film before one
-    synchronisation, so the small films overlap on the GPU."""
+    """Independent films: one context per film, sizes from
+    {64,128,256,512} x {64,128,256}, dx in [2, 5] nm, dz = 3 nm, random
+    materials, smoothed random m.  Field evaluations: a round queues one full
+    field evaluation (Full mask, FFT demag) on every film's own stream before
+    one synchronisation.  RK45: one llgmi_dev_rk45_step_batch attempt of all
+    films per round (one launch per kernel class over all films, replayed from
+    one HIP graph), against the same films stepped one after another."""
     import torch
     from initial_states import batched_films
     specs = batched_films(64, n_films)
@@ -291,7 +293,8 @@ def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
         ctx.synchronize()
     dt = time.perf_counter() - t0
     cells = sum(f[4] for f in films)
-    out = {"workload": f"{n_films} films, sizes {{64,128,256,512}}x{{64,128,256}}, FP32, one context and stream each",
+    out = {"workload": f"{n_films} films, sizes {{64,128,256,512}}x{{64,128,256}}, FP32, field evaluations "
+                       "queued film by film on one stream per film",
            "films": n_films, "total_cells": cells, "rounds": rounds,
            "film_evals_per_s": round(n_films * rounds / dt, 1),
            "cell_updates_per_s": round(cells * rounds / dt, 1),
@@ -320,7 +323,8 @@ def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
             acc_s += c.dev_rk45_step(m, p, mat, **rk)[1]
     dts = time.perf_counter() - t0
     out["rk45"] = {"workload": "per-film adaptive Dormand-Prince RK45, max_err 1e-5, alpha in U[0.01, 0.5], FP32 "
-                               "state and fields, FP64 error norm and controller",
+                               "state and fields, FP64 error norm and controller; batched kernels over all "
+                               "films, one graph replay and one host synchronisation per round",
                    "rounds": rk_rounds, "attempts": att, "accepted": acc,
                    "accepted_steps_per_s": round(acc / dtb, 1),
                    "cell_steps_per_s": round(acc / att * cells * rk_rounds / dtb, 1),
@@ -380,6 +384,27 @@ def warm_start_bench():
     return out
 
 
+def rk45_rate(L, ctx, m, mat, steps):
+    """Accepted RK45 steps/s at the headline workload (FSAL, state resident,
+    max_err 1e-5), after six untimed attempts (eager first attempts, then the
+    graph captures of both FSAL parities)."""
+    mr = m.clone()
+    rp = L.LLGParams(alpha=ALPHA, dt=1e-14, b_ext=BEXT)
+    for _ in range(6):
+        ctx.dev_rk45_step(mr, rp, mat, L.FieldMask.Full, 1e-5, 0.8, 1e-18, 1e-12)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    acc = att = 0
+    while acc < steps and att < 4 * steps:
+        _, a, _ = ctx.dev_rk45_step(mr, rp, mat, L.FieldMask.Full, 1e-5, 0.8, 1e-18, 1e-12)
+        acc += int(a)
+        att += 1
+    ctx.synchronize()
+    dtr = time.perf_counter() - t0
+    return {"accepted_steps_per_s": round(acc / dtr, 1), "attempts": att, "accepted": acc,
+            "us_per_attempt": round(1e6 * dtr / att, 1), "max_err": 1e-5, "last_dt": rp.dt}
+
+
 def host_aos_rate(L, m_host, n=20):
     """Field evaluations/s through the host entry point (AoS f64 in and out)."""
     ctx = L.Context(L.Grid2D(NX, NY, *CELL), precision=L.F32)
@@ -408,6 +433,10 @@ def main():
                     help="run only the multigrid leg (GPU solves, no CPU oracle); for profilers")
     ap.add_argument("--ws-only", action="store_true",
                     help="run only the learned initial guess's forward / training-step rates; for profilers")
+    ap.add_argument("--rk45-only", action="store_true",
+                    help="run only the RK45 leg at the headline workload (--rk45-steps steps); for profilers")
+    ap.add_argument("--films-only", action="store_true",
+                    help="run only the batched-films leg (64 films); for profilers")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 256 MiB (x10) to scale HBM byte counters")
     args = ap.parse_args()
@@ -439,6 +468,9 @@ def main():
     if args.mg_only:
         print(json.dumps({"mg": mg_bench(L, cpu=False)}), flush=True)
         return
+    if args.films_only:
+        print(json.dumps({"batched_films": films_bench(L)}), flush=True)
+        return
     if args.ws_only:
         import warmstart as WS
         prob = WS.Problem()
@@ -458,6 +490,9 @@ def main():
     ctx.dev_upload(m_host, m)
     params = L.LLGParams(alpha=ALPHA, b_ext=BEXT)
     mat = L.Material(**MAT)
+    if args.rk45_only:
+        print(json.dumps({"rk45": rk45_rate(L, ctx, m, mat, args.rk45_steps)}), flush=True)
+        return
 
     for _ in range(args.warmup):
         ctx.dev_build_h_eff(m, b, params, mat, L.FieldMask.Full)
@@ -517,24 +552,7 @@ def main():
         print(f"[bench] fp64: {fp64}", file=sys.stderr, flush=True)
 
     # RK45 accepted steps/s (FSAL; state resident)
-    rk = {}
-    if args.rk45_steps > 0:
-        mr = m.clone()
-        rp = L.LLGParams(alpha=ALPHA, dt=1e-14, b_ext=BEXT)
-        for _ in range(6):   # untimed: eager first attempts, then the captures of both FSAL parities
-            ctx.dev_rk45_step(mr, rp, mat, L.FieldMask.Full, 1e-5, 0.8, 1e-18, 1e-12)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        acc = 0
-        att = 0
-        while acc < args.rk45_steps and att < 4 * args.rk45_steps:
-            _, a, _ = ctx.dev_rk45_step(mr, rp, mat, L.FieldMask.Full, 1e-5, 0.8, 1e-18, 1e-12)
-            acc += int(a)
-            att += 1
-        ctx.synchronize()
-        dtr = time.perf_counter() - t0
-        rk = {"accepted_steps_per_s": round(acc / dtr, 1), "attempts": att, "accepted": acc,
-              "max_err": 1e-5, "last_dt": rp.dt}
+    rk = rk45_rate(L, ctx, m, mat, args.rk45_steps) if args.rk45_steps > 0 else {}
 
     # relax-mode RK23 accepted steps/s (FSAL, 3 field evaluations per step)
     # and energy breakdowns/s (one demag field + one reduction each)
