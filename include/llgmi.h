@@ -181,10 +181,13 @@ int llgmi_dev_rk45_step(llgmi_ctx *ctx, void *m_dev, llgmi_params *params,
                         double headroom, double dt_min, double dt_max, double *eps,
                         int *accepted, double *dt_used);
 /* llgmi_dev_rk45_step for `count` independent films at once: film i is
- * ctxs[i] (distinct contexts, one stream each) with state m_devs[i],
- * params[i] (dt in/out, per-film adaptive) and mats[i]; eps, accepted and
- * dt_used are arrays of `count`.  Every film's attempt is launched before any
- * is waited for, so the films run side by side on the device.  Each film's
+ * ctxs[i] (distinct contexts on one device, same precision) with state
+ * m_devs[i], params[i] (dt in/out, per-film adaptive) and mats[i]; eps,
+ * accepted and dt_used are arrays of `count`.  With FFT demag (or no demag)
+ * on every film the attempt runs as batched kernels, one launch per kernel
+ * class over all films replayed from one HIP graph, with one host
+ * synchronisation for all films' controllers; otherwise every film's attempt
+ * is launched on its own stream before any is waited for.  Each film's
  * result is what llgmi_dev_rk45_step gives for it alone.  The reference has
  * no batched stepper: its drivers call llg.rs:795-1026 once per film, which
  * is what this replaces for a loop over films.  On an error no film's state
