@@ -190,7 +190,7 @@ k_row_r2c(const Tin *__restrict__ src, int64_t src_cstride, int row_stride, int 
 template <typename T, int LOG2L, bool HALF, int RQ>
 __device__ __forceinline__ void col_item(cpx<T> *v, cpx<T> *col, int comp, int kx, int hxs, int ny, cpx<T> *buf,
                                          int sidx, int t, const cpx<T> *__restrict__ tw,
-                                         const T *__restrict__ kern, T scale) {
+                                         const T *__restrict__ kern) {
     using C = ColCfg<T, LOG2L, RQ>;
     constexpr int L = C::L, SLEN = C::SLEN, TPS = C::TPS;
     constexpr int R0 = C::P::R0;
@@ -240,15 +240,15 @@ __device__ __forceinline__ void col_item(cpx<T> *v, cpx<T> *col, int comp, int k
             const cpx<T> own = v[r];
             if (comp == 2) {
                 const T kzz = ka[r];
-                v[r] = {scale * (kzz * own.x), scale * (kzz * own.y)};
+                v[r] = {kzz * own.x, kzz * own.y};
             } else {
                 const T kd = ka[r];
                 const T kxy = ky < HY ? kb[r] : -kb[r];
                 const cpx<T> oth = other[lds_pad(r * TPS)];
                 if (comp == 0)
-                    v[r] = {scale * (kd * own.x + kxy * oth.x), scale * (kd * own.y + kxy * oth.y)};
+                    v[r] = {kd * own.x + kxy * oth.x, kd * own.y + kxy * oth.y};
                 else
-                    v[r] = {scale * (kxy * oth.x + kd * own.x), scale * (kxy * oth.y + kd * own.y)};
+                    v[r] = {kxy * oth.x + kd * own.x, kxy * oth.y + kd * own.y};
             }
         }
     } else {
@@ -262,7 +262,7 @@ __device__ __forceinline__ void col_item(cpx<T> *v, cpx<T> *col, int comp, int k
             const T c1 = T(0.5) * (a + b), c2 = T(0.5) * (a - b);
             const cpx<T> z = v[r];
             const cpx<T> zp = seq[lds_pad((L - ky) & (L - 1))];
-            v[r] = {scale * (c1 * z.x + c2 * zp.x), scale * (c1 * z.y - c2 * zp.y)};
+            v[r] = {c1 * z.x + c2 * zp.x, c1 * z.y - c2 * zp.y};
         }
     }
     reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
@@ -279,7 +279,7 @@ __device__ __forceinline__ void col_item(cpx<T> *v, cpx<T> *col, int comp, int k
 // block (ColGrp); each item has its own two sequences of LDS.
 template <typename T, int LOG2L, bool HALF, int RQ>
 __device__ __forceinline__ void col_work(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
-                                         const T *__restrict__ kern, T scale, int nwork, int wg) {
+                                         const T *__restrict__ kern, int nwork, int wg) {
     using C = ColCfg<T, LOG2L, RQ>;
     constexpr int TPS = C::TPS, SLEN = C::SLEN;
     constexpr int R0 = C::P::R0;
@@ -302,13 +302,13 @@ __device__ __forceinline__ void col_work(cpx<T> *__restrict__ S, int hxs, int ny
         const int j = t + r * TPS;
         v[r] = (col && r < NL && (HALF || j < ny)) ? col[j] : cpx<T>{T(0), T(0)};
     }
-    col_item<T, LOG2L, HALF, RQ>(v, col, comp, kx, hxs, ny, buf, sidx, t, tw, kern, scale);
+    col_item<T, LOG2L, HALF, RQ>(v, col, comp, kx, hxs, ny, buf, sidx, t, tw, kern);
 }
 
 template <typename T, int LOG2L, bool HALF, int RQ, bool PIPE>
 __global__ void __launch_bounds__((PIPE ? ColCfg<T, LOG2L, RQ>::NT : ColGrp<T, LOG2L, RQ>::NT))
 k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
-            const T *__restrict__ kern, T scale, int nwork) {
+            const T *__restrict__ kern, int nwork) {
     using C = ColCfg<T, LOG2L, RQ>;
     constexpr int TPS = C::TPS;
     constexpr int R0 = C::P::R0;
@@ -327,7 +327,7 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
         return ok ? S + ((int64_t)comp * hxs + kx) * ny : nullptr;
     };
     if constexpr (!PIPE) {
-        col_work<T, LOG2L, HALF, RQ>(S, hxs, ny, tw, kern, scale, nwork, (int)blockIdx.x);
+        col_work<T, LOG2L, HALF, RQ>(S, hxs, ny, tw, kern, nwork, (int)blockIdx.x);
     } else {
         // persistent: the next item's column data is in flight while the
         // current item is transformed
@@ -357,7 +357,7 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
                     nxt[r] = (c && (HALF || j < ny)) ? c[j] : cpx<T>{T(0), T(0)};
                 }
             }
-            col_item<T, LOG2L, HALF, RQ>(v, col, comp, kx, hxs, ny, buf, sidx, t, tw, kern, scale);
+            col_item<T, LOG2L, HALF, RQ>(v, col, comp, kx, hxs, ny, buf, sidx, t, tw, kern);
         }
     }
 }
@@ -398,7 +398,7 @@ k_colsplit_fwd(const cpx<T> *__restrict__ S, cpx<T> *__restrict__ F, int ny, con
 template <typename T, int LOG2L, bool HALF, int RQ>
 __global__ void __launch_bounds__((RegPlan<LOG2L, RQ>::TPS))
 k_colsplit_mul_inv(cpx<T> *__restrict__ S, const cpx<T> *__restrict__ F, int hxs, int ny,
-                   const cpx<T> *__restrict__ tw, const T *__restrict__ kern, T scale) {
+                   const cpx<T> *__restrict__ tw, const T *__restrict__ kern) {
     using P = RegPlan<LOG2L, RQ>;
     constexpr int L = P::L, TPS = P::TPS, R0 = P::R0;
     constexpr int HY = L / 2 + 1;
@@ -422,15 +422,15 @@ k_colsplit_mul_inv(cpx<T> *__restrict__ S, const cpx<T> *__restrict__ F, int hxs
             const cpx<T> o = own[ky];
             if (comp == 2) {
                 const T kzz = pa[kq];
-                v[r] = {scale * (kzz * o.x), scale * (kzz * o.y)};
+                v[r] = {kzz * o.x, kzz * o.y};
             } else {
                 const T kd = pa[kq];
                 const T kxy = ky < HY ? pb[kq] : -pb[kq];
                 const cpx<T> q = oth[ky];
                 if (comp == 0)
-                    v[r] = {scale * (kd * o.x + kxy * q.x), scale * (kd * o.y + kxy * q.y)};
+                    v[r] = {kd * o.x + kxy * q.x, kd * o.y + kxy * q.y};
                 else
-                    v[r] = {scale * (kxy * q.x + kd * o.x), scale * (kxy * q.y + kd * o.y)};
+                    v[r] = {kxy * q.x + kd * o.x, kxy * q.y + kd * o.y};
             }
         }
     } else {
@@ -445,7 +445,7 @@ k_colsplit_mul_inv(cpx<T> *__restrict__ S, const cpx<T> *__restrict__ F, int hxs
             const T c1 = T(0.5) * (a + b), c2 = T(0.5) * (a - b);
             const cpx<T> z = own[ky];
             const cpx<T> zp = own[(L - ky) & (L - 1)];
-            v[r] = {scale * (c1 * z.x + c2 * zp.x), scale * (c1 * z.y - c2 * zp.y)};
+            v[r] = {c1 * z.x + c2 * zp.x, c1 * z.y - c2 * zp.y};
         }
     }
     reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
@@ -638,12 +638,14 @@ k_row_c2r_local(const cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__r
 
 // ---------------------------------------------------------------------------
 // One-time k-space kernel: forward column FFT (FP64) of the row spectra of the
-// real-space kernels and extraction of the real quadrant.  Block = (kx, comp).
+// real-space kernels and extraction of the real quadrant, times the inverse
+// transform's 1 / (px py) (a power of two, so the products of the column pass
+// are exactly what a separate scaling would give).  Block = (kx, comp).
 // ---------------------------------------------------------------------------
 template <typename Tout, int LOG2L>
 __global__ void __launch_bounds__((ExtractCfg<LOG2L>::NT))
 k_col_extract(const cpx<double> *__restrict__ Sk, int hxs, const cpx<double> *__restrict__ tw,
-              Tout *__restrict__ kern) {
+              Tout *__restrict__ kern, double scale) {
     using C = ExtractCfg<LOG2L>;
     constexpr int L = C::L, TPS = C::TPS;
     constexpr int R0 = RegPlan<LOG2L>::R0;
@@ -662,10 +664,10 @@ k_col_extract(const cpx<double> *__restrict__ Sk, int hxs, const cpx<double> *__
         const int ky = t + r * TPS;
         if (ky < HY) {
             const int64_t plane = (int64_t)(hxs + 1) * HY;
-            kern[comp * plane + (int64_t)kx * HY + ky] = Tout(v[r].x);
+            kern[comp * plane + (int64_t)kx * HY + ky] = Tout(v[r].x * scale);
             // packed column 0: the imaginary part is the transform of the kx = px/2
             // column (both transforms are real for the even/odd-odd kernels)
-            if (kx == 0) kern[comp * plane + (int64_t)hxs * HY + ky] = Tout(v[r].y);
+            if (kx == 0) kern[comp * plane + (int64_t)hxs * HY + ky] = Tout(v[r].y * scale);
         }
     }
 }
@@ -1181,7 +1183,7 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
             using C = ExtractCfg<LG>;
             allow_lds(k_col_extract<T, LG>, C::LDS);
             hipLaunchKernelGGL((k_col_extract<T, LG>), dim3((unsigned)hxs, 4), dim3(C::NT), C::LDS,
-                               stream, sk, (int)hxs, twy64, kern);
+                               stream, sk, (int)hxs, twy64, kern, 1.0 / ((double)px * (double)py));
             LLGMI_LAUNCH_CHECK();
         });
         LLGMI_HIP(hipStreamSynchronize(stream));
@@ -1298,7 +1300,6 @@ void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
         // split column pass (the persistent flag does not apply)
         dispatch_log2<Range<T>::COL + 1, kColSplitMax>(ly, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
-            const T scale = T(1.0 / ((double)px * (double)py));
             const int hxs = (int)(px / 2);
             const bool half = 2 * ny == py;
             const dim3 grid((unsigned)(3 * hxs));
@@ -1310,12 +1311,12 @@ void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
                     hipLaunchKernelGGL((k_colsplit_fwd<T, LG, true, RQ>), grid, dim3(NT), lds, stream, spec, colfull,
                                        (int)ny, tw_y);
                     hipLaunchKernelGGL((k_colsplit_mul_inv<T, LG, true, RQ>), grid, dim3(NT), lds, stream, spec,
-                                       colfull, hxs, (int)ny, tw_y, kern, scale);
+                                       colfull, hxs, (int)ny, tw_y, kern);
                 } else {
                     hipLaunchKernelGGL((k_colsplit_fwd<T, LG, false, RQ>), grid, dim3(NT), lds, stream, spec, colfull,
                                        (int)ny, tw_y);
                     hipLaunchKernelGGL((k_colsplit_mul_inv<T, LG, false, RQ>), grid, dim3(NT), lds, stream, spec,
-                                       colfull, hxs, (int)ny, tw_y, kern, scale);
+                                       colfull, hxs, (int)ny, tw_y, kern);
                 }
             };
             if ((rq & 7) == 3) go(std::integral_constant<int, 3>{});
@@ -1326,7 +1327,6 @@ void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
     }
     dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
-        const T scale = T(1.0 / ((double)px * (double)py));
         // rq: log2 of the first/last radix, plus kColPipe for persistent blocks
         const bool pipe = (rq & kColPipe) != 0;
         auto go = [&](auto rqc) {
@@ -1339,12 +1339,12 @@ void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
                 auto kh = half ? k_col_demag<T, LG, true, RQ, true> : k_col_demag<T, LG, false, RQ, true>;
                 const int nb = std::min(nwork, resident_blocks((const void *)kh, C::NT, C::LDS));
                 hipLaunchKernelGGL(kh, dim3((unsigned)nb), dim3(C::NT), C::LDS, stream, spec, hxs, (int)ny, tw_y,
-                                   kern, scale, nwork);
+                                   kern, nwork);
             } else {
                 using Gc = ColGrp<T, LG, RQ>;
                 auto kh = half ? k_col_demag<T, LG, true, RQ, false> : k_col_demag<T, LG, false, RQ, false>;
                 hipLaunchKernelGGL(kh, dim3((unsigned)((nwork + Gc::G - 1) / Gc::G)), dim3(Gc::NT), Gc::LDS, stream,
-                                   spec, hxs, (int)ny, tw_y, kern, scale, nwork);
+                                   spec, hxs, (int)ny, tw_y, kern, nwork);
             }
         };
         if constexpr (col_rq_ok<LG, 3>()) {
@@ -1570,16 +1570,6 @@ void DemagFFT<T>::field_local(const T *m, T ms, const LocalArgs<T> &a, const uin
     else launch_row_inv_local(a, m, geom, b, kout, stream, rq_fused);
 }
 
-template <typename T>
-void DemagFFT<T>::read_kernel(std::vector<double> &out) const {
-    const size_t np = (size_t)(hx * hy);
-    std::vector<T> h(4 * np);
-    LLGMI_HIP(hipMemcpy(h.data(), kern, sizeof(T) * h.size(), hipMemcpyDeviceToHost));
-    out.resize(4 * np);
-    for (size_t i = 0; i < np; ++i)
-        for (size_t c = 0; c < 4; ++c) out[4 * i + c] = (double)h[c * np + i];
-}
-
 // ---------------------------------------------------------------------------
 // Batched passes: the block of table entry blockIdx.x runs the single-film
 // body for its film and block.  One launch covers the films of every length
@@ -1681,7 +1671,7 @@ kb_col(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab, int stage
     const int hxs = f.px / 2;
     dev_dispatch_log2<LO, HI>(__builtin_ctz((unsigned)f.py), [&](auto lgc) {
         constexpr int LG = decltype(lgc)::value;
-        col_work<T, LG, HALF, RQ>(f.spec, hxs, f.ny, f.tw_y, f.kern, f.scale, hxs + (hxs + 1) / 2, e.blk);
+        col_work<T, LG, HALF, RQ>(f.spec, hxs, f.ny, f.tw_y, f.kern, hxs + (hxs + 1) / 2, e.blk);
     });
 }
 

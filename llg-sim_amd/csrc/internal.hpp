@@ -91,7 +91,7 @@ struct DemagFFT {
     int lx = 0, ly = 0;
     double dx = 0, dy = 0, dz = 0;
     cpx<T> *tw_x = nullptr, *tw_y = nullptr;
-    T *kern = nullptr;  // 4 planes [Kxx, Kxy, Kyy, Kzz], each [hx][hy]
+    T *kern = nullptr;  // 4 planes [Kxx, Kxy, Kyy, Kzz], each [hx][hy], times 1 / (px py)
     cpx<T> *spec = nullptr;    // [3][hx][ny]
     // full-length forward column spectra [3][px/2][py] of the split column pass
     // (columns whose paired block would not fit in LDS), else null
@@ -124,7 +124,6 @@ struct DemagFFT {
                      T *bd_scratch, hipStream_t stream, Profiler *prof);
     void launch_row_inv_local(const LocalArgs<T> &a, const T *m, const uint8_t *geom, T *b, T *kout,
                               hipStream_t stream, int rq);
-    void read_kernel(std::vector<double> &out) const;
 };
 
 // ---------------------------------------------------------------------------
@@ -165,7 +164,7 @@ struct BFilm {
     const cpx<T> *tw_x, *tw_y;
     const T *kern;
     int nx, ny, px, py;
-    T ms, scale;
+    T ms;
     int fft;
     int skip1;           // nonzero: the stage-1 field evaluation is skipped (FSAL)
     int accept;          // the controller's decision (for the accepted-state copy)

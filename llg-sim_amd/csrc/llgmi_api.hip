@@ -1376,7 +1376,6 @@ void batch_attempt(const std::vector<Engine<T> *> &engs, const std::vector<T *> 
             r.kern = e.fft.kern;
             r.px = (int)e.fft.px;
             r.py = (int)e.fft.py;
-            r.scale = T(1.0 / ((double)e.fft.px * (double)e.fft.py));
         }
         r.ms = T(mats[f].ms);
         r.skip1 = fsal_ok ? 1 : 0;
