@@ -322,6 +322,12 @@ def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
         for c, m, p, mat in zip(ctxs, ms, pars, mats):
             acc_s += c.dev_rk45_step(m, p, mat, **rk)[1]
     dts = time.perf_counter() - t0
+    # the controller on the device: rk_rounds attempts of every film per call
+    L.dev_rk45_run_batch(ctxs, ms, pars, mats, rk_rounds, **rk)          # untimed: graph capture
+    t0 = time.perf_counter()
+    res_d = L.dev_rk45_run_batch(ctxs, ms, pars, mats, rk_rounds, **rk)
+    dtd = time.perf_counter() - t0
+    acc_d = sum(int(a) for row in res_d for _, a, _ in row)
     out["rk45"] = {"workload": "per-film adaptive Dormand-Prince RK45, max_err 1e-5, alpha in U[0.01, 0.5], FP32 "
                                "state and fields, FP64 error norm and controller; batched kernels over all "
                                "films, one graph replay and one host synchronisation per round",
@@ -329,7 +335,11 @@ def films_bench(L, n_films: int = 64, rounds: int = 20, rk_rounds: int = 10):
                    "accepted_steps_per_s": round(acc / dtb, 1),
                    "cell_steps_per_s": round(acc / att * cells * rk_rounds / dtb, 1),
                    "ms_per_round": round(1e3 * dtb / rk_rounds, 3),
-                   "one_film_at_a_time_steps_per_s": round(acc_s / dts, 1)}
+                   "one_film_at_a_time_steps_per_s": round(acc_s / dts, 1),
+                   "device_controller": {"accepted_steps_per_s": round(acc_d / dtd, 1),
+                                         "attempts": len(films) * rk_rounds, "accepted": acc_d,
+                                         "ms_per_round": round(1e3 * dtd / rk_rounds, 3),
+                                         "attempts_per_call": rk_rounds}}
     # per-film accuracy: the FP32 field of each film's current state against an
     # FP64 context on the same state (max |dB| / max |B| over the film's cells)
     errs = []
@@ -401,8 +411,23 @@ def rk45_rate(L, ctx, m, mat, steps):
         att += 1
     ctx.synchronize()
     dtr = time.perf_counter() - t0
-    return {"accepted_steps_per_s": round(acc / dtr, 1), "attempts": att, "accepted": acc,
-            "us_per_attempt": round(1e6 * dtr / att, 1), "max_err": 1e-5, "last_dt": rp.dt}
+    out = {"accepted_steps_per_s": round(acc / dtr, 1), "attempts": att, "accepted": acc,
+           "us_per_attempt": round(1e6 * dtr / att, 1), "max_err": 1e-5, "last_dt": rp.dt}
+    # the same stepping with the controller on the device: `steps` attempts
+    # per llgmi_dev_rk45_run call (one graph replay, one host synchronisation)
+    rr = L.LLGParams(alpha=ALPHA, dt=rp.dt, b_ext=BEXT)
+    ctx.dev_rk45_run(mr, rr, mat, L.FieldMask.Full, 1e-5, 0.8, 1e-18, 1e-12, steps)   # untimed: capture
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    res = []
+    for _ in range(3):
+        res += ctx.dev_rk45_run(mr, rr, mat, L.FieldMask.Full, 1e-5, 0.8, 1e-18, 1e-12, steps)
+    dtd = time.perf_counter() - t0
+    acc_d = sum(int(a) for _, a, _ in res)
+    out["device_controller"] = {"accepted_steps_per_s": round(acc_d / dtd, 1), "attempts": len(res),
+                                "accepted": acc_d, "us_per_attempt": round(1e6 * dtd / len(res), 1),
+                                "attempts_per_call": steps}
+    return out
 
 
 def host_aos_rate(L, m_host, n=20):

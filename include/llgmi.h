@@ -196,6 +196,30 @@ int llgmi_dev_rk45_step_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi
                               const llgmi_material *mats, int64_t count, int field_mask,
                               double max_err, double headroom, double dt_min, double dt_max,
                               double *eps, int *accepted, double *dt_used);
+/* `attempts` consecutive llgmi_dev_rk45_step attempts of each of `count`
+ * films with the controller (llg.rs:999-1025, FP64) on the device: every
+ * attempt's eps, decision and step size are formed there, the next attempt
+ * starts from them (FSAL on accept), and the host waits once for the whole
+ * run.  eps, accepted and dt_used are attempt-major arrays of
+ * attempts * count ([a * count + i]); params[i].dt returns the step size for
+ * the film's next attempt.  attempts <= 256.  Films as for
+ * llgmi_dev_rk45_step_batch, with the FFT demag or none
+ * (LLGMI_ERR_UNSUPPORTED otherwise, and for FP64 films with ny > 2048).
+ * Results are those of the same attempts taken one call at a time up to the
+ * last-ulp rounding of the controller's power function (device against host
+ * libm): a step size an ulp apart moves the following error norms by up to
+ * ~1e-10 relative (the norm cancels to ~1e-4 of the slopes).  No reference
+ * counterpart: it
+ * replaces a driver's loop over llg.rs:795-1026 for callers that do not need
+ * to look at each attempt before the next. */
+int llgmi_dev_rk45_run_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi_params *params,
+                             const llgmi_material *mats, int64_t count, int64_t attempts, int field_mask,
+                             double max_err, double headroom, double dt_min, double dt_max, double *eps,
+                             int *accepted, double *dt_used);
+/* llgmi_dev_rk45_run_batch for one film. */
+int llgmi_dev_rk45_run(llgmi_ctx *ctx, void *m_dev, llgmi_params *params, const llgmi_material *mat,
+                       int64_t attempts, int field_mask, double max_err, double headroom, double dt_min,
+                       double dt_max, double *eps, int *accepted, double *dt_used);
 /* m = 0 on the cells whose keep byte (device, n bytes) is 0 (synchronous):
  * resets the holes of a geometry after a step whose stage normalisation
  * (normalize(0) = +z, vec3.rs:21-28) filled them. */

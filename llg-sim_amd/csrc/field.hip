@@ -365,6 +365,99 @@ kb_accept(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab) {
     f.m[2 * n + c] = f.m7[2 * n + c];
 }
 
+// One film's controller on the device (the same decision as kb_control):
+// reads max |err| and the step size, writes attempt a's outputs, the next
+// step size and the accept flag.  One thread.
+__global__ void __launch_bounds__(64)
+k_rk_control(const double *__restrict__ err, double *__restrict__ dt, int *__restrict__ flag, double max_err,
+             double headroom, double dt_min, double dt_max, int a, double *__restrict__ o_eps,
+             int *__restrict__ o_acc, double *__restrict__ o_dtu) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double dt0 = *dt;
+    const double eps = *err * dt0;
+    double dt_next = eps > 0.0 ? dt0 * headroom * pow(max_err / eps, 0.2) : dt0 * 2.0;
+    if (dt_next < dt_min) dt_next = dt_min;
+    if (dt_next > dt_max) dt_next = dt_max;
+    const int accept = (eps <= max_err) || (dt0 <= dt_min * 1.0000000001);
+    o_eps[a] = eps;
+    o_acc[a] = accept;
+    o_dtu[a] = dt0;
+    *dt = dt_next;
+    *flag = accept;
+}
+
+// After an accepted attempt: m = m7 and k1 = k7 (the next attempt's first
+// slope, FSAL); nothing after a rejected one (m and k1 = f(m) stay).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_rk_accept(const int *__restrict__ flag, T *__restrict__ m, const T *__restrict__ m7, T *__restrict__ k1,
+            const T *__restrict__ k7, int64_t n3) {
+    if (!*flag) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        m[i] = m7[i];
+        k1[i] = k7[i];
+    }
+}
+
+void launch_rk_control(const double *err, double *dt, int *flag, double max_err, double headroom, double dt_min,
+                       double dt_max, int a, double *o_eps, int *o_acc, double *o_dtu, hipStream_t s) {
+    hipLaunchKernelGGL(k_rk_control, dim3(1), dim3(64), 0, s, err, dt, flag, max_err, headroom, dt_min, dt_max, a,
+                       o_eps, o_acc, o_dtu);
+    LLGMI_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_rk_accept(const int *flag, T *m, const T *m7, T *k1, const T *k7, int64_t n, hipStream_t s) {
+    const int64_t n3 = 3 * n;
+    const int64_t want = (n3 + 255) / 256;
+    const unsigned nb = (unsigned)(want < 4096 ? want : 4096);
+    hipLaunchKernelGGL((k_rk_accept<T>), dim3(nb), dim3(256), 0, s, flag, m, m7, k1, k7, n3);
+    LLGMI_LAUNCH_CHECK();
+}
+
+// The controller of step_llg_rk45_recompute_field_masked_adaptive
+// (llg.rs:999-1025) on the device, one thread per film, FP64: from the
+// attempt's max |err| it forms eps and the next dt, records attempt a's eps,
+// decision and step size, and prepares the film's next attempt (dt, FSAL:
+// on accept k7 becomes k1 and stage 1 is skipped; the accepted state is
+// copied by kb_accept, which runs next).
+template <typename T>
+__global__ void __launch_bounds__(64)
+kb_control(BFilm<T> *__restrict__ F, int nfilms, const double *__restrict__ err, double max_err, double headroom,
+           double dt_min, double dt_max, int a, double *__restrict__ o_eps, int *__restrict__ o_acc,
+           double *__restrict__ o_dtu) {
+    const int f = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (f >= nfilms) return;
+    BFilm<T> &r = F[f];
+    const double dt0 = r.dt;
+    const double eps = err[f] * dt0;
+    double dt_next = eps > 0.0 ? dt0 * headroom * pow(max_err / eps, 0.2) : dt0 * 2.0;
+    if (dt_next < dt_min) dt_next = dt_min;
+    if (dt_next > dt_max) dt_next = dt_max;
+    const int accept = (eps <= max_err) || (dt0 <= dt_min * 1.0000000001);
+    const int64_t o = (int64_t)a * nfilms + f;
+    o_eps[o] = eps;
+    o_acc[o] = accept;
+    o_dtu[o] = dt0;
+    r.dt = dt_next;
+    r.accept = accept;
+    r.skip1 = accept;
+    if (accept) {
+        T *t = r.k[1];
+        r.k[1] = r.k[7];
+        r.k[7] = t;
+    }
+}
+
+template <typename T>
+void launch_batch_control(BFilm<T> *F, int nfilms, const double *err, double max_err, double headroom, double dt_min,
+                          double dt_max, int a, double *o_eps, int *o_acc, double *o_dtu, hipStream_t s) {
+    if (nfilms <= 0) return;
+    hipLaunchKernelGGL((kb_control<T>), dim3((unsigned)((nfilms + 63) / 64)), dim3(64), 0, s, F, nfilms, err, max_err,
+                       headroom, dt_min, dt_max, a, o_eps, o_acc, o_dtu);
+    LLGMI_LAUNCH_CHECK();
+}
+
 template <typename T>
 void launch_batch_local(const BFilm<T> *F, const BEntry *tab, int nent, int demag, int which, int ks, int stage1,
                         hipStream_t s) {
@@ -407,6 +500,9 @@ void launch_batch_accept(const BFilm<T> *F, const BEntry *tab, int nent, hipStre
     template void launch_batch_err<T>(const BFilm<T> *, const BEntry *, int, int, double *,  \
                                       double *, hipStream_t);                                \
     template void launch_batch_accept<T>(const BFilm<T> *, const BEntry *, int, hipStream_t); \
+    template void launch_rk_accept<T>(const int *, T *, const T *, T *, const T *, int64_t, hipStream_t); \
+    template void launch_batch_control<T>(BFilm<T> *, int, const double *, double, double, double, double, int, \
+                                          double *, int *, double *, hipStream_t);             \
     template void launch_local_field<T>(const LocalArgs<T> &, const T *, const T *,          \
                                         const uint8_t *, T *, T *, hipStream_t);             \
     template void launch_aos_to_soa<T>(const double *, T *, int64_t, hipStream_t);           \

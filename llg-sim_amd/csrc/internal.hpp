@@ -196,6 +196,19 @@ void launch_batch_err(const BFilm<T> *F, const BEntry *tab, int nent, int nfilms
 // m = m7 for the films whose accept is set
 template <typename T>
 void launch_batch_accept(const BFilm<T> *F, const BEntry *tab, int nent, hipStream_t s);
+// The RK45 controller of one film on the device for attempt a: outputs at
+// [a], the next step size into *dt and the decision into *flag.
+void launch_rk_control(const double *err, double *dt, int *flag, double max_err, double headroom, double dt_min,
+                       double dt_max, int a, double *o_eps, int *o_acc, double *o_dtu, hipStream_t s);
+// m = m7 and k1 = k7 when *flag is set (3 n values each).
+template <typename T>
+void launch_rk_accept(const int *flag, T *m, const T *m7, T *k1, const T *k7, int64_t n, hipStream_t s);
+// The RK45 controller per film on the device for attempt a (outputs at
+// [a * nfilms + film]); updates dt, accept, FSAL skip and the k1/k7 pointers
+// of each film record.
+template <typename T>
+void launch_batch_control(BFilm<T> *F, int nfilms, const double *err, double max_err, double headroom, double dt_min,
+                          double dt_max, int a, double *o_eps, int *o_acc, double *o_dtu, hipStream_t s);
 // cells per element-table entry
 constexpr int kBatchCells = 256;
 

@@ -110,6 +110,12 @@ struct Engine : EngineBase {
         std::string key, seen;
     };
     RkGraph rk_graph[2][2];
+    // device-controlled runs (rk45_run): the last captured run and its
+    // per-attempt outputs (device and pinned host memory)
+    RkGraph run_graph;
+    double *d_run_eps = nullptr, *d_run_dtu = nullptr, *h_run_eps = nullptr, *h_run_dtu = nullptr;
+    int *d_run_acc = nullptr, *h_run_acc = nullptr, *d_run_flag = nullptr;
+    int64_t run_cap = 0;
     bool k_swapped = false;
     uint64_t fft_gen = 0;                      // bumped whenever an FFT operator is (re)built
     double *d_dt = nullptr, *h_dt = nullptr;   // the attempt's step size (pinned host -> device)
@@ -142,6 +148,19 @@ struct Engine : EngineBase {
         if (h_dt) (void)hipHostFree(h_dt);
         if (d_dt) (void)hipFree(d_dt);
         drop_rk_graphs();
+        free_run_outputs();
+        if (d_run_flag) (void)hipFree(d_run_flag);
+    }
+    void free_run_outputs() {
+        void *dv[] = {d_run_eps, d_run_dtu, d_run_acc};
+        for (void *q : dv)
+            if (q) (void)hipFree(q);
+        void *hv[] = {h_run_eps, h_run_dtu, h_run_acc};
+        for (void *q : hv)
+            if (q) (void)hipHostFree(q);
+        d_run_eps = d_run_dtu = h_run_eps = h_run_dtu = nullptr;
+        d_run_acc = h_run_acc = nullptr;
+        run_cap = 0;
     }
 
     template <typename U>
@@ -177,6 +196,8 @@ struct Engine : EngineBase {
                 if (G.exec) (void)hipGraphExecDestroy(G.exec);
                 G = RkGraph{};
             }
+        if (run_graph.exec) (void)hipGraphExecDestroy(run_graph.exec);
+        run_graph = RkGraph{};
     }
     void build_fft(DemagFFT<T> &op, int64_t pbc_x, int64_t pbc_y) {
         std::vector<double> base;
@@ -588,6 +609,20 @@ struct Engine : EngineBase {
     // device from it (for graph replay: the copy node reads h_dt when it runs).
     void rk45_launches(T *m, const llgmi_params &p, const llgmi_material &mat, int mask, bool fsal_ok,
                        double *dt_dev) {
+        if (dt_dev) LLGMI_HIP(hipMemcpyAsync(dt_dev, h_dt, sizeof(double), hipMemcpyHostToDevice, stream));
+        if (!fsal_ok) field(m, d_b, p, mat, mask, nullptr, d_k[1]);
+        rk45_stages(m, p, mat, mask, dt_dev);
+        {
+            ProfScope ps(prof, LLGMI_K_RK45_ERR, stream);
+            launch_rk_err<T>(d_k[1], d_k[3], d_k[4], d_k[5], d_k[6], d_k[7], n,
+                             sizeof(T) == sizeof(float), d_partial, d_err, stream);
+        }
+        LLGMI_HIP(hipMemcpyAsync(h_err, d_err, sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+
+    // Stages 2..7 with their field evaluations (k1 in place); with dt_dev the
+    // stage kernels take the step size from device memory.
+    void rk45_stages(T *m, const llgmi_params &p, const llgmi_material &mat, int mask, double *dt_dev) {
         static const double A[7][5] = {
             {0}, {0},
             {1.0 / 5.0},
@@ -599,8 +634,6 @@ struct Engine : EngineBase {
         static const double Bw[5] = {35.0 / 384.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0};
         static const int Bk[5] = {1, 3, 4, 5, 6};
         const double dt0 = p.dt;
-        if (dt_dev) LLGMI_HIP(hipMemcpyAsync(dt_dev, h_dt, sizeof(double), hipMemcpyHostToDevice, stream));
-        if (!fsal_ok) field(m, d_b, p, mat, mask, nullptr, d_k[1]);
         for (int s = 2; s <= 7; ++s) {
             StageArgs<T> sa{};
             sa.dt = dt_dev;
@@ -626,12 +659,6 @@ struct Engine : EngineBase {
             }
             field(out, d_b, p, mat, mask, nullptr, d_k[s]);
         }
-        {
-            ProfScope ps(prof, LLGMI_K_RK45_ERR, stream);
-            launch_rk_err<T>(d_k[1], d_k[3], d_k[4], d_k[5], d_k[6], d_k[7], n,
-                             sizeof(T) == sizeof(float), d_partial, d_err, stream);
-        }
-        LLGMI_HIP(hipMemcpyAsync(h_err, d_err, sizeof(double), hipMemcpyDeviceToHost, stream));
     }
 
     static bool rk_graphs_enabled() {
@@ -711,6 +738,104 @@ struct Engine : EngineBase {
         eps_out = eps;
         acc_out = accept ? 1 : 0;
         dtu_out = dt0;
+    }
+
+    // `attempts` consecutive attempts with the controller (llg.rs:999-1025,
+    // FP64) on the device, replayed from one captured graph: per attempt the
+    // stages, the error norm, k_rk_control (eps, decision, next step size)
+    // and k_rk_accept (m = m7 and k1 = k7 after an accepted attempt; after a
+    // rejected one m and k1 = f(m) stay, which is what recomputing k1 gives).
+    // The host waits once; p.dt returns the next step size.
+    void rk45_run(T *m, llgmi_params &p, const llgmi_material &mat, int mask, double max_err, double headroom,
+                  double dt_min, double dt_max, int64_t attempts, double *eps, int *acc, double *dtu) {
+        const bool demag = mask == LLGMI_MASK_FULL && mat.demag;
+        if (demag && resolved_demag_method(mat) != LLGMI_DEMAG_FFT)
+            throw Error(LLGMI_ERR_UNSUPPORTED, "device-controlled runs need the FFT demag (or none)");
+        if (prof->mask != 0) throw Error(LLGMI_ERR_UNSUPPORTED, "device-controlled runs need kernel profiling off");
+        ensure_rk();
+        alloc(d_run_flag, 1);
+        if (run_cap < attempts) {
+            drop_rk_graphs();
+            free_run_outputs();
+            LLGMI_HIP(hipMalloc(&d_run_eps, sizeof(double) * attempts));
+            LLGMI_HIP(hipMalloc(&d_run_dtu, sizeof(double) * attempts));
+            LLGMI_HIP(hipMalloc(&d_run_acc, sizeof(int) * attempts));
+            LLGMI_HIP(hipHostMalloc((void **)&h_run_eps, sizeof(double) * attempts, hipHostMallocDefault));
+            LLGMI_HIP(hipHostMalloc((void **)&h_run_dtu, sizeof(double) * attempts, hipHostMallocDefault));
+            LLGMI_HIP(hipHostMalloc((void **)&h_run_acc, sizeof(int) * attempts, hipHostMallocDefault));
+            run_cap = attempts;
+        }
+        if (demag) {
+            // lazy set-up and tuning happen outside the capture
+            ensure_fft();
+            alloc(d_bd, 3 * n);
+            if (fft.rq_fused == 0) {
+                field(m, d_b, p, mat, mask, nullptr, nullptr);
+                LLGMI_HIP(hipStreamSynchronize(stream));
+            }
+        }
+        const bool fsal_ok = fsal_valid && last_mask == mask && last_bext[0] == p.b_ext[0] &&
+                             last_bext[1] == p.b_ext[1] && last_bext[2] == p.b_ext[2];
+        if (fsal_ok) {
+            std::swap(d_k[1], d_k[7]);
+            k_swapped = !k_swapped;
+        }
+        fsal_valid = false;
+        std::string key = rk_graph_key(m, p, mat, mask);
+        auto put = [&](const void *v, size_t b) { key.append((const char *)v, b); };
+        const double ctl[4] = {max_err, headroom, dt_min, dt_max};
+        put(&attempts, sizeof attempts);
+        put(&fsal_ok, sizeof fsal_ok);
+        put(ctl, sizeof ctl);
+        put(&d_run_eps, sizeof d_run_eps);
+        *h_dt = p.dt;   // the graph's first node copies it
+        if (!run_graph.exec || run_graph.key != key) {
+            if (run_graph.exec) (void)hipGraphExecDestroy(run_graph.exec);
+            run_graph = RkGraph{};
+            hipGraph_t graph = nullptr;
+            LLGMI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                LLGMI_HIP(hipMemcpyAsync(d_dt, h_dt, sizeof(double), hipMemcpyHostToDevice, stream));
+                if (!fsal_ok) field(m, d_b, p, mat, mask, nullptr, d_k[1]);
+                for (int64_t a = 0; a < attempts; ++a) {
+                    rk45_stages(m, p, mat, mask, d_dt);
+                    launch_rk_err<T>(d_k[1], d_k[3], d_k[4], d_k[5], d_k[6], d_k[7], n,
+                                     sizeof(T) == sizeof(float), d_partial, d_err, stream);
+                    launch_rk_control(d_err, d_dt, d_run_flag, max_err, headroom, dt_min, dt_max, (int)a,
+                                      d_run_eps, d_run_acc, d_run_dtu, stream);
+                    launch_rk_accept<T>(d_run_flag, m, d_m7, d_k[1], d_k[7], n, stream);
+                }
+                LLGMI_HIP(hipMemcpyAsync(h_run_eps, d_run_eps, sizeof(double) * attempts, hipMemcpyDeviceToHost,
+                                         stream));
+                LLGMI_HIP(hipMemcpyAsync(h_run_dtu, d_run_dtu, sizeof(double) * attempts, hipMemcpyDeviceToHost,
+                                         stream));
+                LLGMI_HIP(hipMemcpyAsync(h_run_acc, d_run_acc, sizeof(int) * attempts, hipMemcpyDeviceToHost,
+                                         stream));
+                LLGMI_HIP(hipMemcpyAsync(h_dt, d_dt, sizeof(double), hipMemcpyDeviceToHost, stream));
+            } catch (...) {
+                (void)hipStreamEndCapture(stream, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            LLGMI_HIP(hipStreamEndCapture(stream, &graph));
+            const hipError_t ge = hipGraphInstantiate(&run_graph.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            LLGMI_HIP(ge);
+            run_graph.key = key;
+        }
+        LLGMI_HIP(hipGraphLaunch(run_graph.exec, stream));
+        LLGMI_HIP(hipStreamSynchronize(stream));
+        std::memcpy(eps, h_run_eps, sizeof(double) * attempts);
+        std::memcpy(dtu, h_run_dtu, sizeof(double) * attempts);
+        std::memcpy(acc, h_run_acc, sizeof(int) * attempts);
+        p.dt = *h_dt;
+        // k1 holds f(m) after the last attempt, accepted or not: the engine
+        // keeps it in the k7 slot, as after an accepted single attempt
+        std::swap(d_k[1], d_k[7]);
+        k_swapped = !k_swapped;
+        fsal_valid = true;
+        last_mask = mask;
+        for (int q = 0; q < 3; ++q) last_bext[q] = p.b_ext[q];
     }
 
     // ---- relaxation ------------------------------------------------------------
@@ -1168,9 +1293,28 @@ struct BatchPlan {
     int elem_off = 0, elem_n = 0;
     double *d_partial = nullptr, *d_err = nullptr, *h_err = nullptr;   // h_err pinned
     hipGraphExec_t exec = nullptr;
+    // runs of several attempts with the controller on the device, one graph
+    // per (attempts, controller settings); per-attempt outputs in device and
+    // pinned host memory
+    struct Run {
+        hipGraphExec_t exec = nullptr;
+        double *d_eps = nullptr, *d_dtu = nullptr, *h_eps = nullptr, *h_dtu = nullptr;
+        int *d_acc = nullptr, *h_acc = nullptr;
+    };
+    std::map<std::tuple<int64_t, double, double, double, double>, Run> runs;
 
     ~BatchPlan() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &kv : runs) {
+            Run &r = kv.second;
+            if (r.exec) (void)hipGraphExecDestroy(r.exec);
+            void *dv[] = {r.d_eps, r.d_dtu, r.d_acc};
+            for (void *p : dv)
+                if (p) (void)hipFree(p);
+            void *hv[] = {r.h_eps, r.h_dtu, r.h_acc};
+            for (void *p : hv)
+                if (p) (void)hipHostFree(p);
+        }
         if (exec) (void)hipGraphExecDestroy(exec);
         if (d_films) (void)hipFree(d_films);
         if (d_tab) (void)hipFree(d_tab);
@@ -1200,6 +1344,29 @@ struct BatchPlan {
         }
         launch_batch_err<T>(d_films, d_tab + elem_off, elem_n, nfilms, d_partial, d_err, stream);
         LLGMI_HIP(hipMemcpyAsync(h_err, d_err, sizeof(double) * nfilms, hipMemcpyDeviceToHost, stream));
+    }
+    // `attempts` consecutive attempts of every film with the controller on
+    // the device: after each attempt's error norm, kb_control decides and
+    // prepares the next attempt and kb_accept copies the accepted states; the
+    // film records and the per-attempt outputs come back at the end.
+    void launch_run(Run &r, int64_t attempts, double max_err, double headroom, double dt_min, double dt_max) {
+        LLGMI_HIP(hipMemcpyAsync(d_films, h_films, sizeof(BFilm<T>) * nfilms, hipMemcpyHostToDevice, stream));
+        for (int64_t a = 0; a < attempts; ++a) {
+            launch_field(0, 1, 1);
+            for (int st = 2; st <= 7; ++st) {
+                launch_batch_stage<T>(d_films, d_tab + elem_off, elem_n, st, stream);
+                launch_field(st == 7 ? 2 : 1, st, 0);
+            }
+            launch_batch_err<T>(d_films, d_tab + elem_off, elem_n, nfilms, d_partial, d_err, stream);
+            launch_batch_control<T>(d_films, nfilms, d_err, max_err, headroom, dt_min, dt_max, (int)a, r.d_eps,
+                                    r.d_acc, r.d_dtu, stream);
+            launch_batch_accept<T>(d_films, d_tab + elem_off, elem_n, stream);
+        }
+        const size_t no = (size_t)(attempts * nfilms);
+        LLGMI_HIP(hipMemcpyAsync(h_films, d_films, sizeof(BFilm<T>) * nfilms, hipMemcpyDeviceToHost, stream));
+        LLGMI_HIP(hipMemcpyAsync(r.h_eps, r.d_eps, sizeof(double) * no, hipMemcpyDeviceToHost, stream));
+        LLGMI_HIP(hipMemcpyAsync(r.h_dtu, r.d_dtu, sizeof(double) * no, hipMemcpyDeviceToHost, stream));
+        LLGMI_HIP(hipMemcpyAsync(r.h_acc, r.d_acc, sizeof(int) * no, hipMemcpyDeviceToHost, stream));
     }
 };
 
@@ -1336,6 +1503,49 @@ bool batch_graphs_enabled() {
     return !(e && e[0] == '0');
 }
 
+// The records of every film for an attempt (after rk45_prepare).
+template <typename T>
+void fill_film_records(BatchPlan<T> &P, const std::vector<Engine<T> *> &engs, const std::vector<T *> &ms,
+                       const llgmi_params *params, const llgmi_material *mats, int mask,
+                       const std::vector<int> &fft_on, const std::vector<int> &fsal) {
+    const int nf = (int)engs.size();
+    for (int f = 0; f < nf; ++f) {
+        Engine<T> &e = *engs[(size_t)f];
+        BFilm<T> &r = P.h_films[f];
+        const int e0 = r.e0, ne = r.ne;
+        r = BFilm<T>{};
+        r.e0 = e0;
+        r.ne = ne;
+        r.nx = (int)e.g.nx;
+        r.ny = (int)e.g.ny;
+        r.fft = fft_on[(size_t)f];
+        if (r.fft) {
+            r.spec = e.fft.spec;
+            r.tw_x = e.fft.tw_x;
+            r.tw_y = e.fft.tw_y;
+            r.kern = e.fft.kern;
+            r.px = (int)e.fft.px;
+            r.py = (int)e.fft.py;
+        }
+        r.ms = T(mats[f].ms);
+        r.skip1 = fsal[(size_t)f];
+        r.dt = params[f].dt;
+        r.la = e.local_args(params[f], mats[f], mask, true, false);
+        r.m = ms[(size_t)f];
+        r.in[0] = ms[(size_t)f];
+        r.in[1] = e.d_stage;
+        r.in[2] = e.d_m7;
+        r.stage = e.d_stage;
+        r.m7 = e.d_m7;
+        r.b = e.d_b;
+        r.bd = e.d_bd;
+        for (int q = 0; q < 8; ++q) r.k[q] = e.d_k[q];
+    }
+}
+
+// Longest device-controlled run (attempts per call of llgmi_dev_rk45_run*).
+constexpr int64_t kRunAttemptsMax = 256;
+
 // One batched attempt of every film (the films' engines on one device, all
 // with their operators built and tuned).  eps / accepted / dt_used per film.
 template <typename T>
@@ -1358,39 +1568,9 @@ void batch_attempt(const std::vector<Engine<T> *> &engs, const std::vector<T *> 
         it = plans.emplace(key, build_batch_plan<T>(engs, fft_on, dev)).first;
     }
     BatchPlan<T> &P = *it->second;
-    for (int f = 0; f < nf; ++f) {
-        Engine<T> &e = *engs[(size_t)f];
-        const bool fsal_ok = e.rk45_prepare(params[f], mats[f], mask);
-        BFilm<T> &r = P.h_films[f];
-        const int e0 = r.e0, ne = r.ne;
-        r = BFilm<T>{};
-        r.e0 = e0;
-        r.ne = ne;
-        r.nx = (int)e.g.nx;
-        r.ny = (int)e.g.ny;
-        r.fft = fft_on[(size_t)f];
-        if (r.fft) {
-            r.spec = e.fft.spec;
-            r.tw_x = e.fft.tw_x;
-            r.tw_y = e.fft.tw_y;
-            r.kern = e.fft.kern;
-            r.px = (int)e.fft.px;
-            r.py = (int)e.fft.py;
-        }
-        r.ms = T(mats[f].ms);
-        r.skip1 = fsal_ok ? 1 : 0;
-        r.dt = params[f].dt;
-        r.la = e.local_args(params[f], mats[f], mask, true, false);
-        r.m = ms[(size_t)f];
-        r.in[0] = ms[(size_t)f];
-        r.in[1] = e.d_stage;
-        r.in[2] = e.d_m7;
-        r.stage = e.d_stage;
-        r.m7 = e.d_m7;
-        r.b = e.d_b;
-        r.bd = e.d_bd;
-        for (int q = 0; q < 8; ++q) r.k[q] = e.d_k[q];
-    }
+    std::vector<int> fsal((size_t)nf);
+    for (int f = 0; f < nf; ++f) fsal[(size_t)f] = engs[(size_t)f]->rk45_prepare(params[f], mats[f], mask) ? 1 : 0;
+    fill_film_records<T>(P, engs, ms, params, mats, mask, fft_on, fsal);
     try {
         if (batch_graphs_enabled()) {
             if (!P.exec) {
@@ -1428,6 +1608,116 @@ void batch_attempt(const std::vector<Engine<T> *> &engs, const std::vector<T *> 
         LLGMI_HIP(hipMemcpyAsync(P.d_films, P.h_films, sizeof(BFilm<T>) * nf, hipMemcpyHostToDevice, P.stream));
         launch_batch_accept<T>(P.d_films, P.d_tab + P.elem_off, P.elem_n, P.stream);
         LLGMI_HIP(hipStreamSynchronize(P.stream));
+    }
+}
+
+// `attempts` consecutive adaptive attempts of every film with the controller
+// on the device (batched kernels, one graph replay, one host synchronisation
+// for all).  Outputs are attempt-major: [a * films + f].  The films' engines
+// take the state the last attempt leaves (step size, FSAL, k1/k7).
+template <typename T>
+void batch_run(const std::vector<Engine<T> *> &engs, const std::vector<T *> &ms, llgmi_params *params,
+               const llgmi_material *mats, int mask, double max_err, double headroom, double dt_min, double dt_max,
+               int64_t attempts, double *eps, int *accepted, double *dt_used, int dev) {
+    const int nf = (int)engs.size();
+    std::vector<int> fft_on((size_t)nf);
+    for (int f = 0; f < nf; ++f)
+        fft_on[(size_t)f] = mask == LLGMI_MASK_FULL && mats[f].demag && resolved_demag_method(mats[f]) == LLGMI_DEMAG_FFT;
+    for (auto *e : engs) LLGMI_HIP(hipStreamSynchronize(e->stream));
+    std::lock_guard<std::mutex> lock(g_batch_mu);
+    std::string key = batch_key<T>(engs, fft_on);
+    key.append((const char *)&dev, sizeof dev);
+    auto &plans = batch_plans<T>();
+    auto it = plans.find(key);
+    if (it == plans.end()) {
+        if (plans.size() >= 8) plans.clear();
+        it = plans.emplace(key, build_batch_plan<T>(engs, fft_on, dev)).first;
+    }
+    BatchPlan<T> &P = *it->second;
+    std::vector<int> fsal((size_t)nf);
+    for (int f = 0; f < nf; ++f) fsal[(size_t)f] = engs[(size_t)f]->rk45_prepare(params[f], mats[f], mask) ? 1 : 0;
+    fill_film_records<T>(P, engs, ms, params, mats, mask, fft_on, fsal);
+    const auto rkey = std::make_tuple(attempts, max_err, headroom, dt_min, dt_max);
+    try {
+        auto rit = P.runs.find(rkey);
+        if (rit == P.runs.end()) {
+            if (P.runs.size() >= 4) {
+                for (auto &kv : P.runs) {
+                    typename BatchPlan<T>::Run &r = kv.second;
+                    if (r.exec) (void)hipGraphExecDestroy(r.exec);
+                    void *dv[] = {r.d_eps, r.d_dtu, r.d_acc};
+                    for (void *p : dv)
+                        if (p) (void)hipFree(p);
+                    void *hv[] = {r.h_eps, r.h_dtu, r.h_acc};
+                    for (void *p : hv)
+                        if (p) (void)hipHostFree(p);
+                }
+                P.runs.clear();
+            }
+            rit = P.runs.emplace(rkey, typename BatchPlan<T>::Run{}).first;
+            typename BatchPlan<T>::Run &r = rit->second;
+            const size_t no = (size_t)(attempts * nf);
+            LLGMI_HIP(hipMalloc(&r.d_eps, sizeof(double) * no));
+            LLGMI_HIP(hipMalloc(&r.d_dtu, sizeof(double) * no));
+            LLGMI_HIP(hipMalloc(&r.d_acc, sizeof(int) * no));
+            LLGMI_HIP(hipHostMalloc((void **)&r.h_eps, sizeof(double) * no, hipHostMallocDefault));
+            LLGMI_HIP(hipHostMalloc((void **)&r.h_dtu, sizeof(double) * no, hipHostMallocDefault));
+            LLGMI_HIP(hipHostMalloc((void **)&r.h_acc, sizeof(int) * no, hipHostMallocDefault));
+        }
+        typename BatchPlan<T>::Run &r = rit->second;
+        if (batch_graphs_enabled()) {
+            if (!r.exec) {
+                hipGraph_t graph = nullptr;
+                LLGMI_HIP(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    P.launch_run(r, attempts, max_err, headroom, dt_min, dt_max);
+                } catch (...) {
+                    (void)hipStreamEndCapture(P.stream, &graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                    throw;
+                }
+                LLGMI_HIP(hipStreamEndCapture(P.stream, &graph));
+                const hipError_t ge = hipGraphInstantiate(&r.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                LLGMI_HIP(ge);
+            }
+            LLGMI_HIP(hipGraphLaunch(r.exec, P.stream));
+        } else {
+            P.launch_run(r, attempts, max_err, headroom, dt_min, dt_max);
+        }
+        LLGMI_HIP(hipStreamSynchronize(P.stream));
+        const size_t no = (size_t)(attempts * nf);
+        std::memcpy(eps, r.h_eps, sizeof(double) * no);
+        std::memcpy(dt_used, r.h_dtu, sizeof(double) * no);
+        std::memcpy(accepted, r.h_acc, sizeof(int) * no);
+    } catch (...) {
+        // a failed run leaves the films' states undefined: no FSAL reuse
+        for (auto *e : engs) {
+            e->issued_mask = -1;
+            e->fsal_valid = false;
+            e->last_mask = -1;
+        }
+        throw;
+    }
+    for (int f = 0; f < nf; ++f) {
+        Engine<T> &e = *engs[(size_t)f];
+        const BFilm<T> &r = P.h_films[f];
+        const bool acc = r.accept != 0;
+        // The device swaps k1 and k7 right after an accepted attempt, the host
+        // path at the start of the next attempt (rk45_prepare): after an
+        // accepted last attempt the engine keeps the pointers from before
+        // that swap, and its next attempt swaps them itself.
+        T *k1 = acc ? r.k[7] : r.k[1];
+        if (k1 != e.d_k[1]) {
+            std::swap(e.d_k[1], e.d_k[7]);
+            e.k_swapped = !e.k_swapped;
+        }
+        e.fsal_valid = acc;
+        e.last_mask = acc ? mask : -1;
+        if (acc)
+            for (int q = 0; q < 3; ++q) e.last_bext[q] = params[f].b_ext[q];
+        e.issued_mask = -1;
+        params[f].dt = r.dt;
     }
 }
 
@@ -1771,6 +2061,99 @@ int llgmi_dev_rk45_step_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi
     }
     if (rc != LLGMI_OK) return fail(rc, first_err);
     return LLGMI_OK;
+}
+
+int llgmi_dev_rk45_run_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi_params *params,
+                             const llgmi_material *mats, int64_t count, int64_t attempts, int field_mask,
+                             double max_err, double headroom, double dt_min, double dt_max, double *eps,
+                             int *accepted, double *dt_used) {
+    if (count < 0) return fail(LLGMI_ERR_INVALID, "count < 0");
+    if (attempts < 0 || attempts > kRunAttemptsMax)
+        return fail(LLGMI_ERR_INVALID, "attempts must be in [0, " + std::to_string(kRunAttemptsMax) + "]");
+    if (count == 0 || attempts == 0) return LLGMI_OK;
+    if (!ctxs || !m_devs || !params || !mats || !eps || !accepted || !dt_used)
+        return fail(LLGMI_ERR_INVALID, "NULL argument");
+    std::string e = check_mask(field_mask);
+    if (e.empty() && dt_min > dt_max) e = "dt_min > dt_max";
+    std::set<const llgmi_ctx *> seen;
+    for (int64_t i = 0; i < count && e.empty(); ++i) {
+        const llgmi_ctx *c = ctxs[i];
+        if (!c) { e = "context is NULL"; break; }
+        if (!seen.insert(c).second) { e = "a context appears twice in one batch"; break; }
+        e = check_mat(&mats[i]);
+        if (e.empty()) {
+            (void)hipSetDevice(c->device);
+            e = check_dev(m_devs[i], 3 * (size_t)(c->grid.nx * c->grid.ny) * elem_size(c), "m");
+        }
+        if (!e.empty()) e = "film " + std::to_string(i) + ": " + e;
+    }
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    for (int64_t i = 0; i < count; ++i) {
+        const llgmi_ctx *c = ctxs[i];
+        std::string why;
+        if (c->precision != ctxs[0]->precision || c->device != ctxs[0]->device)
+            why = "films of one precision on one device";
+        else if (c->prof.mask != 0)
+            why = "kernel profiling off";
+        else if (field_mask == LLGMI_MASK_FULL && mats[i].demag && resolved_demag_method(mats[i]) != LLGMI_DEMAG_FFT)
+            why = "the FFT demag (or none)";
+        if (!why.empty())
+            return fail(LLGMI_ERR_UNSUPPORTED, "film " + std::to_string(i) + ": device-controlled runs need " + why);
+    }
+    if (count == 1)
+        return llgmi_dev_rk45_run(ctxs[0], m_devs[0], params, mats, attempts, field_mask, max_err, headroom, dt_min,
+                                  dt_max, eps, accepted, dt_used);
+    llgmi_ctx *c0 = ctxs[0];
+    return guarded(c0, [&] {
+        auto run = [&](auto tag, auto getter) {
+            using T = decltype(tag);
+            std::vector<Engine<T> *> engs;
+            std::vector<T *> ms;
+            for (int64_t i = 0; i < count; ++i) {
+                Engine<T> &eng = *getter(ctxs[i]);
+                const bool demag = field_mask == LLGMI_MASK_FULL && mats[i].demag;
+                if (demag && !(eng.fft.built && eng.fft.rq_fused != 0)) {
+                    // builds and tunes the film's operator (the batched kernels use its choices)
+                    eng.ensure_rk();
+                    eng.field((const T *)m_devs[i], eng.d_b, params[i], mats[i], field_mask, nullptr, nullptr);
+                    LLGMI_HIP(hipStreamSynchronize(eng.stream));
+                }
+                if (demag && eng.fft.colfull != nullptr)
+                    throw Error(LLGMI_ERR_UNSUPPORTED, "film " + std::to_string(i) +
+                                                           ": split column passes (FP64, ny > 2048) are not batched");
+                engs.push_back(&eng);
+                ms.push_back((T *)m_devs[i]);
+            }
+            batch_run<T>(engs, ms, params, mats, field_mask, max_err, headroom, dt_min, dt_max, attempts, eps,
+                         accepted, dt_used, c0->device);
+        };
+        if (c0->precision == LLGMI_F32)
+            run(float{}, [](llgmi_ctx *c) { return c->ef.get(); });
+        else
+            run(double{}, [](llgmi_ctx *c) { return c->ed.get(); });
+    });
+}
+
+int llgmi_dev_rk45_run(llgmi_ctx *ctx, void *m_dev, llgmi_params *params, const llgmi_material *mat,
+                       int64_t attempts, int field_mask, double max_err, double headroom, double dt_min,
+                       double dt_max, double *eps, int *accepted, double *dt_used) {
+    if (!ctx) return fail(LLGMI_ERR_INVALID, "context is NULL");
+    if (attempts < 0 || attempts > kRunAttemptsMax)
+        return fail(LLGMI_ERR_INVALID, "attempts must be in [0, " + std::to_string(kRunAttemptsMax) + "]");
+    if (attempts == 0) return LLGMI_OK;
+    if (!params || !mat || !eps || !accepted || !dt_used) return fail(LLGMI_ERR_INVALID, "NULL argument");
+    std::string e = check_mask(field_mask);
+    if (e.empty()) e = check_mat(mat);
+    if (e.empty() && dt_min > dt_max) e = "dt_min > dt_max";
+    if (e.empty()) e = check_dev(m_dev, 3 * (size_t)(ctx->grid.nx * ctx->grid.ny) * elem_size(ctx), "m");
+    if (!e.empty()) return fail(LLGMI_ERR_INVALID, e);
+    return guarded(ctx, [&] {
+        with_engine(ctx, [&](auto &eng) {
+            using T = std::remove_pointer_t<decltype(eng.d_m)>;
+            eng.rk45_run((T *)m_dev, *params, *mat, field_mask, max_err, headroom, dt_min, dt_max, attempts, eps,
+                         accepted, dt_used);
+        });
+    });
 }
 
 int llgmi_dev_mask_cells(llgmi_ctx *ctx, void *m_dev, const uint8_t *keep_dev) {

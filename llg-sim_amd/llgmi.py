@@ -142,7 +142,7 @@ EXPORTED = [
     "llgmi_add_demag_field_pbc", "llgmi_compute_demag_field_pbc", "llgmi_add_demag_field",
     "llgmi_rk45_step", "llgmi_rk45_step_adaptive", "llgmi_rk45_reset",
     "llgmi_dev_build_h_eff", "llgmi_dev_build_h_eff_noalloc", "llgmi_dev_add_demag_field_pbc", "llgmi_dev_rk45_step",
-    "llgmi_dev_rk45_step_batch",
+    "llgmi_dev_rk45_step_batch", "llgmi_dev_rk45_run_batch", "llgmi_dev_rk45_run",
     "llgmi_dev_upload", "llgmi_dev_download", "llgmi_profile_enable", "llgmi_profile_reset",
     "llgmi_profile_read", "llgmi_profile_fft_passes", "llgmi_mg_config_default", "llgmi_mg_config_from_env",
     "llgmi_ctx_set_mg_config", "llgmi_add_demag_field_poisson_mg",
@@ -412,6 +412,10 @@ def lib():
                                       C.c_double, dp, ip, dp]
     L.llgmi_dev_rk45_step_batch.argtypes = [C.POINTER(vp), C.POINTER(vp), P, M, C.c_int64, C.c_int,
                                             C.c_double, C.c_double, C.c_double, C.c_double, dp, ip, dp]
+    L.llgmi_dev_rk45_run_batch.argtypes = [C.POINTER(vp), C.POINTER(vp), P, M, C.c_int64, C.c_int64, C.c_int,
+                                           C.c_double, C.c_double, C.c_double, C.c_double, dp, ip, dp]
+    L.llgmi_dev_rk45_run.argtypes = [vp, vp, P, M, C.c_int64, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, dp, ip, dp]
     L.llgmi_dev_upload.argtypes = [vp, dp, vp]
     L.llgmi_dev_download.argtypes = [vp, vp, dp]
     L.llgmi_profile_enable.argtypes = [vp, C.c_int]
@@ -679,6 +683,30 @@ def dev_rk45_step_batch(ctxs, m_devs, params, mats, mask: int, max_err: float, h
     return [(float(eps[i]), bool(acc[i]), float(dtu[i])) for i in range(k)]
 
 
+def dev_rk45_run_batch(ctxs, m_devs, params, mats, attempts: int, mask: int, max_err: float, headroom: float,
+                       dt_min: float, dt_max: float):
+    """`attempts` consecutive adaptive RK45 attempts of each film with the
+    controller on the device (llgmi_dev_rk45_run_batch); params[i].dt is
+    updated in place.  Returns out[a][i] = (eps, accepted, dt_used)."""
+    k = len(ctxs)
+    if not (len(m_devs) == len(params) == len(mats) == k):
+        raise ValueError("ctxs, m_devs, params and mats must have one entry per film")
+    hs = (C.c_void_p * max(k, 1))(*[c._h for c in ctxs])
+    ms = (C.c_void_p * max(k, 1))(*[_ptr(m) for m in m_devs])
+    ps = (_Params * max(k, 1))(*[p._c() for p in params])
+    cms = (_Material * max(k, 1))(*[m._c() for m in mats])
+    no = max(k * attempts, 1)
+    eps = np.zeros(no)
+    dtu = np.zeros(no)
+    acc = np.zeros(no, dtype=np.intc)
+    _check(lib().llgmi_dev_rk45_run_batch(hs, ms, ps, cms, k, attempts, mask, max_err, headroom, dt_min, dt_max,
+                                          _dp(eps), acc.ctypes.data_as(C.POINTER(C.c_int)), _dp(dtu)))
+    for i, p in enumerate(params):
+        p.dt = ps[i].dt
+    return [[(float(eps[a * k + i]), bool(acc[a * k + i]), float(dtu[a * k + i])) for i in range(k)]
+            for a in range(attempts)]
+
+
 class Context:
     """A grid + precision + GPU: the reference's operator caches and RK45Scratch."""
 
@@ -940,6 +968,20 @@ class Context:
                                          C.byref(acc), C.byref(dtu)))
         params.dt = p.dt
         return eps.value, bool(acc.value), dtu.value
+
+    def dev_rk45_run(self, m_dev, params: LLGParams, mat: Material, mask: int, max_err: float, headroom: float,
+                     dt_min: float, dt_max: float, attempts: int):
+        """`attempts` consecutive adaptive RK45 attempts with the controller on
+        the device (llgmi_dev_rk45_run); returns [(eps, accepted, dt_used)]."""
+        p, cm = params._c(), mat._c()
+        no = max(attempts, 1)
+        eps, dtu = np.zeros(no), np.zeros(no)
+        acc = np.zeros(no, dtype=np.intc)
+        _check(lib().llgmi_dev_rk45_run(self._h, _ptr(m_dev), C.byref(p), C.byref(cm), attempts, mask, max_err,
+                                        headroom, dt_min, dt_max, _dp(eps), acc.ctypes.data_as(C.POINTER(C.c_int)),
+                                        _dp(dtu)))
+        params.dt = p.dt
+        return [(float(eps[a]), bool(acc[a]), float(dtu[a])) for a in range(attempts)]
 
     def dev_mask_cells(self, m_dev, keep_dev) -> None:
         """m = 0 where the keep bytes (device, one per cell) are 0."""

@@ -391,3 +391,100 @@ def test_c2_fp32_rk45_against_fp64_over_200_steps(gpu_lib):
     e_b = np.sqrt(((bf32 - bf64) ** 2).sum(1)).max() / np.sqrt((bf64 ** 2).sum(1)).max()
     print(f"C2 after 200 steps ({t64 * 1e12:.3f} ps): <m> {d_avg:.2e}, max cell {d_cell:.2e}, field {e_b:.2e}")
     assert d_avg <= 1e-6 and d_cell <= 1e-4 and e_b <= 1e-5
+
+
+def _run_vs_steps(L, films, P, es, attempts, split, tail=0, single=False):
+    """Films stepped by device-controlled runs (attempt counts `split`) and
+    then `tail` single attempts, against the same films stepped `attempts`
+    times one attempt per call with the host controller."""
+    from conftest import DeviceBuffer
+    import copy
+    ctx_r = [L.Context(g, precision=P) for g, _, _, _ in films]
+    ctx_s = [L.Context(g, precision=P) for g, _, _, _ in films]
+    buf_r = [DeviceBuffer(3 * g.n_cells() * es) for g, _, _, _ in films]
+    buf_s = [DeviceBuffer(3 * g.n_cells() * es) for g, _, _, _ in films]
+    for (g, _, _, m0), cr, cs, br, bs in zip(films, ctx_r, ctx_s, buf_r, buf_s):
+        cr.dev_upload(m0, br)
+        cs.dev_upload(m0, bs)
+    pr = [copy.deepcopy(f[2]) for f in films]
+    ps = [copy.deepcopy(f[2]) for f in films]
+    pr[1].dt = ps[1].dt = 5e-11          # one film starts with a step that is rejected
+    mats = [f[1] for f in films]
+    out_r = []
+    for k in split:
+        if single:   # llgmi_dev_rk45_run film by film (the engine's own kernels)
+            per = [c.dev_rk45_run(b, p, m, 2, 1e-5, 0.8, 1e-18, 1e-11, k) for c, b, p, m in zip(ctx_r, buf_r, pr, mats)]
+            out_r += [[per[i][a] for i in range(len(films))] for a in range(k)]
+        else:
+            out_r += L.dev_rk45_run_batch(ctx_r, buf_r, pr, mats, k, 2, 1e-5, 0.8, 1e-18, 1e-11)
+    for _ in range(tail):
+        out_r.append([c.dev_rk45_step(b, p, m, 2, 1e-5, 0.8, 1e-18, 1e-11)
+                      for c, b, p, m in zip(ctx_r, buf_r, pr, mats)])
+    out_s = [[c.dev_rk45_step(b, p, m, 2, 1e-5, 0.8, 1e-18, 1e-11) for c, b, p, m in zip(ctx_s, buf_s, ps, mats)]
+             for _ in range(attempts)]
+    states = [(cr.dev_download(br), cs.dev_download(bs)) for cr, cs, br, bs in zip(ctx_r, ctx_s, buf_r, buf_s)]
+    return out_r, out_s, pr, ps, states
+
+
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_rk45_device_controlled_run_matches_attempts_one_at_a_time(gpu_lib, prec, single):
+    """llgmi_dev_rk45_run_batch (controller on the device, 8 attempts in one
+    call) against llgmi_dev_rk45_step per film and attempt: the same accept
+    decisions; step sizes, eps and states close.  The device's pow rounds
+    differently from the host libm's in the last ulp, and a step size an ulp
+    apart moves the next eps by up to ~1e-10 relative (eps is dt times a sum
+    that cancels to ~1e-4 of the slopes), so step sizes are compared at 1e-9
+    relative and eps at 1e-6 (FP64; FP32 stage coefficients round to FP32, so
+    1e-3 and 1e-2); the first attempt of one film is rejected."""
+    L = gpu_lib
+    P = L.F64 if prec == "f64" else L.F32
+    es = 8 if prec == "f64" else 4
+    print(f"device-controlled RK45 run vs single attempts {prec} single={single}", flush=True)
+    out_r, out_s, pr, ps, states = _run_vs_steps(L, _films(3, 6), P, es, 8, [8], single=single)
+    tol_dt, tol_eps, tol_m = (1e-9, 1e-6, 1e-12) if prec == "f64" else (1e-3, 1e-2, 1e-5)
+    assert not out_r[0][1][1] and not out_s[0][1][1]     # the large first step is rejected
+    for a, (rr, rs) in enumerate(zip(out_r, out_s)):
+        for i, ((er, ar, ur), (es_, as_, us)) in enumerate(zip(rr, rs)):
+            assert ar == as_, (a, i)
+            assert abs(ur - us) <= tol_dt * us, (a, i, ur, us)
+            assert abs(er - es_) <= tol_eps * abs(es_) + 1e-30, (a, i, er, es_)
+    for i, (a, b) in enumerate(zip(pr, ps)):
+        assert abs(a.dt - b.dt) <= tol_dt * b.dt, i
+    for i, (mr, ms) in enumerate(states):
+        assert np.abs(mr - ms).max() <= tol_m, i
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_rk45_device_controlled_runs_chain_with_single_steps(gpu_lib, single):
+    """Runs of 3 and 2 attempts, then single attempts on the same contexts:
+    the FSAL state and k1/k7 the runs leave behind carry over (FP64, against
+    8 single attempts)."""
+    L = gpu_lib
+    print(f"device-controlled RK45 runs chained with single attempts single={single}", flush=True)
+    out_r, out_s, pr, ps, states = _run_vs_steps(L, _films(7, 4), L.F64, 8, 8, [3, 2], tail=3, single=single)
+    assert len(out_r) == 8
+    for a, (rr, rs) in enumerate(zip(out_r, out_s)):
+        for i, ((er, ar, ur), (es_, as_, us)) in enumerate(zip(rr, rs)):
+            assert ar == as_ and abs(ur - us) <= 1e-9 * us, (a, i)
+            assert abs(er - es_) <= 1e-6 * abs(es_) + 1e-30, (a, i)
+    for i, (mr, ms) in enumerate(states):
+        assert np.abs(mr - ms).max() <= 1e-12, i
+
+
+def test_rk45_device_controlled_run_needs_the_fft_demag(gpu_lib):
+    from conftest import DeviceBuffer
+    L = gpu_lib
+    g = L.Grid2D(16, 16, 3e-9, 3e-9, 3e-9)
+    c = L.Context(g, precision=L.F64)
+    b = DeviceBuffer(3 * g.n_cells() * 8)
+    c.dev_upload(np.tile([0.0, 0.0, 1.0], (g.n_cells(), 1)), b)
+    p = L.LLGParams(alpha=0.1, dt=1e-13)
+    mat = L.Material(ms=8e5, a_ex=1.3e-11, demag=True, demag_method=L.DemagMethod.PoissonMG)
+    with pytest.raises(L.LlgmiError) as e:
+        c.dev_rk45_run(b, p, mat, 2, 1e-5, 0.8, 1e-18, 1e-11, 4)
+    assert e.value.code == L.ERR_UNSUPPORTED
+    assert p.dt == 1e-13
+    with pytest.raises(L.LlgmiError) as e:
+        c.dev_rk45_run(b, p, L.Material(ms=8e5, a_ex=1.3e-11, demag=True), 2, 1e-5, 0.8, 1e-18, 1e-11, 257)
+    assert e.value.code == L.ERR_INVALID
