@@ -60,7 +60,7 @@ __host__ __device__ constexpr int mini(int a, int b) { return a < b ? a : b; }
 #endif
 template <typename T, int LOG2L, int RQ = 0>
 struct RowCfg {
-    using P = RegPlan<LOG2L, RQ>;
+    using P = Plan<LOG2L, RQ>;
     static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
     static constexpr int PAIRS = clampi(mini((int)(LLGMI_ROW_LDS_KB * 1024 / (sizeof(cpx<T>) * SLEN)),
                                              LLGMI_ROW_NT / TPS), 1, 64);
@@ -70,7 +70,7 @@ struct RowCfg {
 
 template <typename T, int LOG2L, int RQ = 0>
 struct ColCfg {
-    using P = RegPlan<LOG2L, RQ>;
+    using P = Plan<LOG2L, RQ>;
     static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
     static constexpr int NT = 2 * TPS;
     static constexpr size_t LDS = sizeof(cpx<T>) * 2 * SLEN;
@@ -89,7 +89,7 @@ struct ColGrp {
 
 template <int LOG2L>
 struct ExtractCfg {
-    using P = RegPlan<LOG2L>;
+    using P = Plan<LOG2L>;
     static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
     static constexpr int NT = TPS;
     static constexpr size_t LDS = sizeof(cpx<double>) * SLEN;
@@ -108,32 +108,33 @@ __device__ __forceinline__ void row_r2c_body(const Tin *__restrict__ src, int64_
                                              int ncols_valid, int nrows, T scale, const cpx<T> *__restrict__ tw,
                                              cpx<T> *__restrict__ dst, int64_t dst_cstride, int comp, int j0) {
     using C = RowCfg<T, LOG2L, RQ>;
+    using PL = typename C::P;
     constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
-    constexpr int R0 = C::P::R0;
+    constexpr int NV = PL::NV;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int p = threadIdx.x / TPS, t = threadIdx.x - p * TPS;
     const int ja = j0 + 2 * p, jb = ja + 1;
     const Tin *ra = src + comp * src_cstride + (int64_t)ja * row_stride;
     const Tin *rb = ra + row_stride;
-    cpx<T> v[R0];
+    cpx<T> v[NV];
 #pragma unroll
-    for (int r = 0; r < R0; ++r) {
-        const int i = t + r * TPS;
+    for (int r = 0; r < NV; ++r) {
+        const int i = PL::tindex(t, r);
         T re = T(0), im = T(0);
-        if (HALF ? r < R0 / 2 : i < ncols_valid) {
+        if (HALF ? PL::lower(r) : i < ncols_valid) {
             if (ja < nrows) re = scale * T(ra[i]);
             if (jb < nrows) im = scale * T(rb[i]);
         }
         v[r] = {re, im};
     }
     cpx<T> *seq = buf + p * SLEN;
-    reg_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
+    plan_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
     __syncthreads();
     {
         cpx<T> *wb = seq + lds_pad(t);
 #pragma unroll
-        for (int r = 0; r < R0; ++r) wb[lds_pad(r * TPS)] = v[r];
+        for (int r = 0; r < NV; ++r) wb[lds_pad(r * TPS)] = v[r];
     }
     __syncthreads();
     constexpr int HXS = L / 2;  // packed columns
@@ -192,8 +193,9 @@ __device__ __forceinline__ void col_item(cpx<T> *v, cpx<T> *col, int comp, int k
                                          int sidx, int t, const cpx<T> *__restrict__ tw,
                                          const T *__restrict__ kern) {
     using C = ColCfg<T, LOG2L, RQ>;
+    using PL = typename C::P;
     constexpr int L = C::L, SLEN = C::SLEN, TPS = C::TPS;
-    constexpr int R0 = C::P::R0;
+    constexpr int NV = PL::NV;
     constexpr int HY = L / 2 + 1;
     cpx<T> *seq = buf + sidx * SLEN;
     // kernel planes [Kxx, Kxy, Kyy, Kzz][hxs + 1][HY]: the (x, y) items read
@@ -210,32 +212,33 @@ __device__ __forceinline__ void col_item(cpx<T> *v, cpx<T> *col, int comp, int k
         pb = pa + (int64_t)hxs * HY;
     }
     const bool need_b = kx == 0 || comp != 2;
-    T ka[R0], kb[R0];
+    T ka[NV], kb[NV];
     auto load_k = [&] {
 #pragma unroll
-        for (int r = 0; r < R0; ++r) {
+        for (int r = 0; r < NV; ++r) {
             const int ky = t + r * TPS;
             const int kq = ky < HY ? ky : L - ky;
             ka[r] = pa[kq];
             kb[r] = need_b ? pb[kq] : T(0);
         }
     };
-    // FP32: read before the transform; FP64 keeps the registers for the data
-    constexpr bool KPRE = sizeof(T) == 4;
+    // FP32 power-of-two lengths: read before the transform; otherwise the
+    // registers are kept for the data
+    constexpr bool KPRE = sizeof(T) == 4 && !PL::F3;
     if constexpr (KPRE) load_k();
-    reg_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
+    plan_fft<T, false, LOG2L, RQ>(v, seq, t, tw);
     if constexpr (!KPRE) load_k();
     __syncthreads();
     {
         cpx<T> *wb = seq + lds_pad(t);
 #pragma unroll
-        for (int r = 0; r < R0; ++r) wb[lds_pad(r * TPS)] = v[r];
+        for (int r = 0; r < NV; ++r) wb[lds_pad(r * TPS)] = v[r];
     }
     __syncthreads();
     if (kx != 0) {
         const cpx<T> *other = buf + (sidx ^ 1) * SLEN + lds_pad(t);
 #pragma unroll
-        for (int r = 0; r < R0; ++r) {
+        for (int r = 0; r < NV; ++r) {
             const int ky = t + r * TPS;
             const cpx<T> own = v[r];
             if (comp == 2) {
@@ -256,21 +259,21 @@ __device__ __forceinline__ void col_item(cpx<T> *v, cpx<T> *col, int comp, int k
         // and kx = px/2 columns; Kxy vanishes on both, so per component
         // Y = Ka A + i Kb B = ((Ka+Kb)/2) Z + ((Ka-Kb)/2) conj(Z[-ky])
 #pragma unroll
-        for (int r = 0; r < R0; ++r) {
+        for (int r = 0; r < NV; ++r) {
             const int ky = t + r * TPS;
             const T a = ka[r], b = kb[r];
             const T c1 = T(0.5) * (a + b), c2 = T(0.5) * (a - b);
             const cpx<T> z = v[r];
-            const cpx<T> zp = seq[lds_pad((L - ky) & (L - 1))];
+            const cpx<T> zp = seq[lds_pad(ky == 0 ? 0 : L - ky)];
             v[r] = {c1 * z.x + c2 * zp.x, c1 * z.y - c2 * zp.y};
         }
     }
-    reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
+    plan_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
     if (col) {
 #pragma unroll
-        for (int r = 0; r < R0; ++r) {
-            const int j = t + r * TPS;
-            if (HALF ? r < R0 / 2 : j < ny) col[j] = v[r];
+        for (int r = 0; r < NV; ++r) {
+            const int j = PL::tindex(t, r);
+            if (HALF ? PL::lower(r) : j < ny) col[j] = v[r];
         }
     }
 }
@@ -281,9 +284,9 @@ template <typename T, int LOG2L, bool HALF, int RQ>
 __device__ __forceinline__ void col_work(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ tw,
                                          const T *__restrict__ kern, int nwork, int wg) {
     using C = ColCfg<T, LOG2L, RQ>;
+    using PL = typename C::P;
     constexpr int TPS = C::TPS, SLEN = C::SLEN;
-    constexpr int R0 = C::P::R0;
-    constexpr int NL = HALF ? R0 / 2 : R0;
+    constexpr int NV = PL::NV;
     constexpr int G = ColGrp<T, LOG2L, RQ>::G;
     extern __shared__ unsigned char lds_raw[];
     const int g = threadIdx.x / (2 * TPS), tl = threadIdx.x - g * 2 * TPS;
@@ -296,11 +299,11 @@ __device__ __forceinline__ void col_work(cpx<T> *__restrict__ S, int hxs, int ny
     const bool ok = w < nwork && kx < hxs;
     if (!ok) kx = 0;
     cpx<T> *col = ok ? S + ((int64_t)comp * hxs + kx) * ny : nullptr;
-    cpx<T> v[R0];
+    cpx<T> v[NV];
 #pragma unroll
-    for (int r = 0; r < R0; ++r) {
-        const int j = t + r * TPS;
-        v[r] = (col && r < NL && (HALF || j < ny)) ? col[j] : cpx<T>{T(0), T(0)};
+    for (int r = 0; r < NV; ++r) {
+        const int j = PL::tindex(t, r);
+        v[r] = (col && (HALF ? PL::lower(r) : j < ny)) ? col[j] : cpx<T>{T(0), T(0)};
     }
     col_item<T, LOG2L, HALF, RQ>(v, col, comp, kx, hxs, ny, buf, sidx, t, tw, kern);
 }
@@ -313,6 +316,7 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
     constexpr int TPS = C::TPS;
     constexpr int R0 = C::P::R0;
     constexpr int NL = HALF ? R0 / 2 : R0;   // values a thread loads per sequence
+    static_assert(!PIPE || !C::P::F3, "the persistent column pass is built for power-of-two lengths");
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const int sidx = threadIdx.x / TPS, t = threadIdx.x - sidx * TPS;
@@ -375,7 +379,8 @@ k_col_demag(cpx<T> *__restrict__ S, int hxs, int ny, const cpx<T> *__restrict__ 
 template <typename T, int LOG2L, bool HALF, int RQ>
 __global__ void __launch_bounds__((RegPlan<LOG2L, RQ>::TPS))
 k_colsplit_fwd(const cpx<T> *__restrict__ S, cpx<T> *__restrict__ F, int ny, const cpx<T> *__restrict__ tw) {
-    using P = RegPlan<LOG2L, RQ>;
+    static_assert(!Plan<LOG2L, RQ>::F3, "the split column pass is built for power-of-two lengths");
+    using P = Plan<LOG2L, RQ>;
     constexpr int L = P::L, TPS = P::TPS, R0 = P::R0;
     constexpr int NL = HALF ? R0 / 2 : R0;
     extern __shared__ unsigned char lds_raw[];
@@ -399,7 +404,8 @@ template <typename T, int LOG2L, bool HALF, int RQ>
 __global__ void __launch_bounds__((RegPlan<LOG2L, RQ>::TPS))
 k_colsplit_mul_inv(cpx<T> *__restrict__ S, const cpx<T> *__restrict__ F, int hxs, int ny,
                    const cpx<T> *__restrict__ tw, const T *__restrict__ kern) {
-    using P = RegPlan<LOG2L, RQ>;
+    static_assert(!Plan<LOG2L, RQ>::F3, "the split column pass is built for power-of-two lengths");
+    using P = Plan<LOG2L, RQ>;
     constexpr int L = P::L, TPS = P::TPS, R0 = P::R0;
     constexpr int HY = L / 2 + 1;
     extern __shared__ unsigned char lds_raw[];
@@ -469,8 +475,9 @@ __device__ __forceinline__ void row_c2r_body(const cpx<T> *__restrict__ S, int h
                                              const cpx<T> *__restrict__ tw, T *__restrict__ out, int64_t n,
                                              int comp, int j0) {
     using C = RowCfg<T, LOG2L, RQ>;
+    using PL = typename C::P;
     constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
-    constexpr int R0 = C::P::R0;
+    constexpr int NV = PL::NV;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
     const cpx<T> *s = S + (int64_t)comp * hxs * ny;
@@ -494,19 +501,19 @@ __device__ __forceinline__ void row_c2r_body(const cpx<T> *__restrict__ S, int h
     __syncthreads();
     const int p = threadIdx.x / TPS, t = threadIdx.x - p * TPS;
     cpx<T> *seq = buf + p * SLEN;
-    cpx<T> v[R0];
+    cpx<T> v[NV];
     {
         const cpx<T> *rb = seq + lds_pad(t);
 #pragma unroll
-        for (int r = 0; r < R0; ++r) v[r] = rb[lds_pad(r * TPS)];
+        for (int r = 0; r < NV; ++r) v[r] = rb[lds_pad(r * TPS)];
     }
-    reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
+    plan_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
     const int ja = j0 + 2 * p;
     T *o = out + comp * n;
 #pragma unroll
-    for (int r = 0; r < R0; ++r) {
-        const int i = t + r * TPS;
-        if (HALF ? r < R0 / 2 : i < nx) {
+    for (int r = 0; r < NV; ++r) {
+        const int i = PL::tindex(t, r);
+        if (HALF ? PL::lower(r) : i < nx) {
             if (ja < ny) o[(int64_t)ja * nx + i] = v[r].x;
             if (ja + 1 < ny) o[(int64_t)(ja + 1) * nx + i] = v[r].y;
         }
@@ -533,7 +540,7 @@ k_row_c2r(const cpx<T> *__restrict__ S, int hxs, int nx, int ny, const cpx<T> *_
 // ---------------------------------------------------------------------------
 template <typename T, int LOG2L, int RQ>
 struct FusedCfg {
-    using P = RegPlan<LOG2L, RQ>;
+    using P = Plan<LOG2L, RQ>;
     static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
     static constexpr size_t kLdsCU = 160 * 1024;
     static constexpr int PAIRS = (3 * 2 * TPS <= 1024 && sizeof(cpx<T>) * 3 * 2 * SLEN <= kLdsCU) ? 2 : 1;
@@ -551,7 +558,7 @@ constexpr bool fused_ok() { return FusedCfg<T, LG, RQ>::OK; }
 // their cells evaluated the same way for any PAIRS.
 template <typename T, int LOG2L, int RQ>
 struct BatchFusedCfg {
-    using P = RegPlan<LOG2L, RQ>;
+    using P = Plan<LOG2L, RQ>;
     static constexpr int L = P::L, TPS = P::TPS, SLEN = lds_seq_len(L);
     static constexpr size_t kLdsCU = 160 * 1024;
     static constexpr int fit(int p) {
@@ -568,8 +575,9 @@ __device__ __forceinline__ void row_c2r_local_body(const cpx<T> *__restrict__ S,
                                                    const cpx<T> *__restrict__ tw, const LocalArgs<T> &a,
                                                    const T *__restrict__ m, const uint8_t *__restrict__ geom,
                                                    T *__restrict__ b, T *__restrict__ kout, int j0) {
+    using PL = typename C::P;
     constexpr int L = C::L, PAIRS = C::PAIRS, NT = C::NT, SLEN = C::SLEN, TPS = C::TPS;
-    constexpr int R0 = C::P::R0;
+    constexpr int NV = PL::NV;
     constexpr int NYQ = L >> 1;
     extern __shared__ unsigned char lds_raw[];
     cpx<T> *buf = reinterpret_cast<cpx<T> *>(lds_raw);
@@ -593,13 +601,13 @@ __device__ __forceinline__ void row_c2r_local_body(const cpx<T> *__restrict__ S,
     __syncthreads();
     const int sq = threadIdx.x / TPS, t = threadIdx.x - sq * TPS;
     cpx<T> *seq = buf + sq * SLEN;
-    cpx<T> v[R0];
+    cpx<T> v[NV];
     {
         const cpx<T> *rb = seq + lds_pad(t);
 #pragma unroll
-        for (int r = 0; r < R0; ++r) v[r] = rb[lds_pad(r * TPS)];
+        for (int r = 0; r < NV; ++r) v[r] = rb[lds_pad(r * TPS)];
     }
-    reg_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
+    plan_fft<T, true, LOG2L, RQ>(v, seq, t, tw);
     __syncthreads();   // every sequence is out of LDS before the rows go in
     T *bdl = reinterpret_cast<T *>(lds_raw);   // [3][2 PAIRS][NYQ]
     {
@@ -607,9 +615,9 @@ __device__ __forceinline__ void row_c2r_local_body(const cpx<T> *__restrict__ S,
         T *ra = bdl + (comp * 2 * PAIRS + 2 * p) * NYQ;
         T *rb = ra + NYQ;
 #pragma unroll
-        for (int r = 0; r < R0; ++r) {
-            const int i = t + r * TPS;
-            if (HALF ? r < R0 / 2 : i < nx) {
+        for (int r = 0; r < NV; ++r) {
+            const int i = PL::tindex(t, r);
+            if (HALF ? PL::lower(r) : i < nx) {
                 ra[i] = v[r].x;
                 rb[i] = v[r].y;
             }
@@ -647,20 +655,21 @@ __global__ void __launch_bounds__((ExtractCfg<LOG2L>::NT))
 k_col_extract(const cpx<double> *__restrict__ Sk, int hxs, const cpx<double> *__restrict__ tw,
               Tout *__restrict__ kern, double scale) {
     using C = ExtractCfg<LOG2L>;
+    using PL = typename C::P;
     constexpr int L = C::L, TPS = C::TPS;
-    constexpr int R0 = RegPlan<LOG2L>::R0;
+    constexpr int NV = PL::NV;
     extern __shared__ unsigned char lds_raw[];
     cpx<double> *buf = reinterpret_cast<cpx<double> *>(lds_raw);
     const int kx = blockIdx.x, comp = blockIdx.y;
     const int t = threadIdx.x;
     const cpx<double> *s = Sk + ((int64_t)comp * hxs + kx) * L;
-    cpx<double> v[R0];
+    cpx<double> v[NV];
 #pragma unroll
-    for (int r = 0; r < R0; ++r) v[r] = s[t + r * TPS];
-    reg_fft<double, false, LOG2L>(v, buf, t, tw);
+    for (int r = 0; r < NV; ++r) v[r] = s[PL::tindex(t, r)];
+    plan_fft<double, false, LOG2L>(v, buf, t, tw);
     constexpr int HY = L / 2 + 1;
 #pragma unroll
-    for (int r = 0; r < R0; ++r) {
+    for (int r = 0; r < NV; ++r) {
         const int ky = t + r * TPS;
         if (ky < HY) {
             const int64_t plane = (int64_t)(hxs + 1) * HY;
@@ -954,15 +963,25 @@ __global__ void k_fold_images(const double *__restrict__ ext, int ex, int ey, in
 // Host side
 // ---------------------------------------------------------------------------
 template <typename T>
-static cpx<T> *make_twiddles(int L) {
-    std::vector<cpx<T>> h(L);
-    for (int t = 0; t < L; ++t) {
-        double a = -2.0 * M_PI * (double)t / (double)L;
-        h[t] = {T(std::cos(a)), T(std::sin(a))};
+static cpx<T> *make_twiddles(int code) {
+    // exp(-2 pi i t / L), t < L, for L = 2^code; for a 3M code the M-point
+    // table followed by the 3M-point table
+    std::vector<cpx<T>> h;
+    auto add = [&](int L) {
+        for (int t = 0; t < L; ++t) {
+            const double a = -2.0 * M_PI * (double)t / (double)L;
+            h.push_back({T(std::cos(a)), T(std::sin(a))});
+        }
+    };
+    if (code_f3(code)) {
+        add(1 << code_log2m(code));
+        add(3 << code_log2m(code));
+    } else {
+        add(1 << code);
     }
     cpx<T> *d = nullptr;
-    LLGMI_HIP(hipMalloc(&d, sizeof(cpx<T>) * L));
-    LLGMI_HIP(hipMemcpy(d, h.data(), sizeof(cpx<T>) * L, hipMemcpyHostToDevice));
+    LLGMI_HIP(hipMalloc(&d, sizeof(cpx<T>) * h.size()));
+    LLGMI_HIP(hipMemcpy(d, h.data(), sizeof(cpx<T>) * h.size(), hipMemcpyHostToDevice));
     return d;
 }
 
@@ -999,7 +1018,7 @@ static int resident_blocks(const void *kernel, int nt, size_t lds) {
 
 // A column block holds two sequences, 2 * L/R0 threads (at most 1024).
 template <int LG, int RQ>
-constexpr bool col_rq_ok() { return 2 * ((1 << LG) / (1 << RegPlan<LG, RQ>::LOG2R0)) <= 1024; }
+constexpr bool col_rq_ok() { return 2 * Plan<LG, RQ>::TPS <= 1024; }
 
 // Built ranges (padded lengths, so nx, ny <= 4096 in FP32 and nx <= 4096,
 // ny <= 2048 in FP64): rows up to 2^13 in both precisions; columns up to 2^13
@@ -1017,6 +1036,27 @@ template <typename T, int LG>
 constexpr size_t colsplit_lds() { return sizeof(cpx<T>) * lds_seq_len(1 << LG); }
 static constexpr int kRowF64Build = 13;  // FP64 row pass of the kernel build (FP32 contexts)
 static constexpr int kExtractMax = 13;
+// Lengths 3 * 2^k built per precision and pass (k = log2 M); the padding takes
+// the shorter of the next power of two and the next 3 * 2^k (a grid of
+// 3 * 2^(k-1) cells pads to 3 * 2^k, as the reference's 2n does).
+template <typename T> struct Range3;
+template <> struct Range3<float> { static constexpr int LO = 8, ROW = 11, COL = 11; };
+template <> struct Range3<double> { static constexpr int LO = 8, ROW = 11, COL = 10; };
+static constexpr int kF3BuildLo = 8, kF3BuildHi = 11;   // kernel build (FP64 rows and column extraction)
+// dispatch over the length codes 1..HI (powers of two) and kF3 + LO3..kF3 + HI3
+template <int HI, int LO3, int HI3, typename F>
+inline void dispatch_code(int c, F &&f) {
+    if (code_f3(c)) dispatch_log2<kF3 + LO3, kF3 + HI3>(c, static_cast<F &&>(f));
+    else dispatch_log2<1, HI>(c, static_cast<F &&>(f));
+}
+template <typename T, typename F>
+inline void dispatch_row(int c, F &&f) {
+    dispatch_code<Range<T>::ROW, Range3<T>::LO, Range3<T>::ROW>(c, static_cast<F &&>(f));
+}
+template <typename T, typename F>
+inline void dispatch_col(int c, F &&f) {
+    dispatch_code<Range<T>::COL, Range3<T>::LO, Range3<T>::COL>(c, static_cast<F &&>(f));
+}
 
 template <typename T>
 DemagFFT<T>::~DemagFFT() {
@@ -1090,15 +1130,30 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     if (pbc_x < 0 || pbc_y < 0 || pbc_x > (1 << 20) || pbc_y > (1 << 20) || ex * ey > (int64_t(1) << 27))
         throw Error(3, "FFT demag: periodic image range " + std::to_string(ex) + " x " + std::to_string(ey) +
                            " cells is beyond the built limit of 2^27 kernel displacements");
-    px = next_pow2(2 * nx);
-    py = next_pow2(2 * ny);
+    // padded lengths >= 2n: the next power of two, or 3 * 2^k when that is
+    // shorter and built (LLGMI_FFT_POW2=1 keeps powers of two)
+    const char *p2e = std::getenv("LLGMI_FFT_POW2");
+    const bool pow2_only = p2e && p2e[0] == '1';
+    auto pick = [&](int64_t need, int hi3) {
+        const int64_t p2 = next_pow2(need);
+        int code = ilog2(p2);
+        if (!pow2_only)
+            for (int k = Range3<T>::LO; k <= hi3; ++k)
+                if ((int64_t(3) << k) >= need) {
+                    if ((int64_t(3) << k) < p2) code = kF3 + k;
+                    break;
+                }
+        return code;
+    };
+    lx = pick(2 * nx, Range3<T>::ROW);
+    ly = pick(2 * ny, Range3<T>::COL);
+    px = code_len(lx);
+    py = code_len(ly);
     hx = px / 2 + 1;
     hy = py / 2 + 1;
-    lx = ilog2(px);
-    ly = ilog2(py);
-    if (lx > Range<T>::ROW || ly > kColSplitMax || ly > kExtractMax)
-        throw Error(3, "FFT demag: padded size " + std::to_string(px) + "x" + std::to_string(py) +
-                           " is beyond the built FFT lengths of this precision");
+    if ((!code_f3(lx) && lx > Range<T>::ROW) || (!code_f3(ly) && (ly > kColSplitMax || ly > kExtractMax)))
+        throw Error(3, "FFT demag: padded size " + std::to_string(next_pow2(2 * nx)) + "x" +
+                           std::to_string(next_pow2(2 * ny)) + " is beyond the built FFT lengths of this precision");
     const double mu0 = 4.0 * M_PI * 1e-7;
     const double acc = 10.0;  // DEMAG_ACCURACY (demag_fft_uniform.rs:432)
 
@@ -1163,11 +1218,11 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
                            stream, base, (int)nx, (int)ny, (int)px, (int)py, kr);
         LLGMI_LAUNCH_CHECK();
         // 3) row R2C (FP64) of the four arrays
-        twx64 = make_twiddles<double>((int)px);
-        twy64 = make_twiddles<double>((int)py);
+        twx64 = make_twiddles<double>(lx);
+        twy64 = make_twiddles<double>(ly);
         const int64_t hxs = px / 2;
         LLGMI_HIP(hipMalloc(&sk, sizeof(cpx<double>) * 4 * hxs * py));
-        dispatch_log2<1, kRowF64Build>(lx, [&](auto lg) {
+        dispatch_code<kRowF64Build, kF3BuildLo, kF3BuildHi>(lx, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
             using C = RowCfg<double, LG, 0>;
             allow_lds(k_row_r2c<double, double, LG, false, 0>, C::LDS);
@@ -1178,7 +1233,7 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         });
         // 4) column FFT (FP64), real quadrant -> kern (T)
         LLGMI_HIP(hipMalloc(&kern, sizeof(T) * 4 * hx * hy));
-        dispatch_log2<1, kExtractMax>(ly, [&](auto lg) {
+        dispatch_code<kExtractMax, kF3BuildLo, kF3BuildHi>(ly, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
             using C = ExtractCfg<LG>;
             allow_lds(k_col_extract<T, LG>, C::LDS);
@@ -1209,11 +1264,11 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
     (void)hipFree(twx64);
     (void)hipFree(twy64);
 
-    tw_x = make_twiddles<T>((int)px);
-    tw_y = make_twiddles<T>((int)py);
+    tw_x = make_twiddles<T>(lx);
+    tw_y = make_twiddles<T>(ly);
     LLGMI_HIP(hipMalloc(&spec, sizeof(cpx<T>) * 3 * (px / 2) * ny));
-    if (ly > Range<T>::COL) LLGMI_HIP(hipMalloc(&colfull, sizeof(cpx<T>) * 3 * (px / 2) * py));
-    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+    if (col_split()) LLGMI_HIP(hipMalloc(&colfull, sizeof(cpx<T>) * 3 * (px / 2) * py));
+    dispatch_row<T>(lx, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         allow_lds(k_row_r2c<T, T, LG, true, 3>, RowCfg<T, LG, 3>::LDS);
         allow_lds(k_row_r2c<T, T, LG, false, 3>, RowCfg<T, LG, 3>::LDS);
@@ -1224,7 +1279,7 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         allow_lds(k_row_c2r<T, LG, true, 4>, RowCfg<T, LG, 4>::LDS);
         allow_lds(k_row_c2r<T, LG, false, 4>, RowCfg<T, LG, 4>::LDS);
     });
-    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+    dispatch_row<T>(lx, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         auto allow = [&](auto rqc) {
             constexpr int RQ = decltype(rqc)::value;
@@ -1243,7 +1298,7 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
         allow(std::integral_constant<int, 3>{});
         allow(std::integral_constant<int, 4>{});
     });
-    if (ly > Range<T>::COL) {
+    if (col_split()) {
         dispatch_log2<Range<T>::COL + 1, kColSplitMax>(ly, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
             allow_lds(k_colsplit_fwd<T, LG, true, 3>, colsplit_lds<T, LG>());
@@ -1255,27 +1310,37 @@ void DemagFFT<T>::build(int64_t nx_, int64_t ny_, double dx_, double dy_, double
             allow_lds(k_colsplit_mul_inv<T, LG, true, 4>, colsplit_lds<T, LG>());
             allow_lds(k_colsplit_mul_inv<T, LG, false, 4>, colsplit_lds<T, LG>());
         });
-    } else dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
+    } else dispatch_col<T>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
+        constexpr bool pipe_ok = !code_f3(LG);   // the persistent form is built for powers of two
         if constexpr (col_rq_ok<LG, 3>()) {
             allow_lds(k_col_demag<T, LG, true, 3, false>, ColGrp<T, LG, 3>::LDS);
             allow_lds(k_col_demag<T, LG, false, 3, false>, ColGrp<T, LG, 3>::LDS);
-            allow_lds(k_col_demag<T, LG, true, 3, true>, ColCfg<T, LG, 3>::LDS);
-            allow_lds(k_col_demag<T, LG, false, 3, true>, ColCfg<T, LG, 3>::LDS);
+            if constexpr (pipe_ok) {
+                allow_lds(k_col_demag<T, LG, true, 3, true>, ColCfg<T, LG, 3>::LDS);
+                allow_lds(k_col_demag<T, LG, false, 3, true>, ColCfg<T, LG, 3>::LDS);
+            }
         }
         allow_lds(k_col_demag<T, LG, true, 4, false>, ColGrp<T, LG, 4>::LDS);
         allow_lds(k_col_demag<T, LG, false, 4, false>, ColGrp<T, LG, 4>::LDS);
-        allow_lds(k_col_demag<T, LG, true, 4, true>, ColCfg<T, LG, 4>::LDS);
-        allow_lds(k_col_demag<T, LG, false, 4, true>, ColCfg<T, LG, 4>::LDS);
+        if constexpr (pipe_ok) {
+            allow_lds(k_col_demag<T, LG, true, 4, true>, ColCfg<T, LG, 4>::LDS);
+            allow_lds(k_col_demag<T, LG, false, 4, true>, ColCfg<T, LG, 4>::LDS);
+        }
     });
     built = true;
     autotune(stream, forced);
 }
 
 template <typename T>
+bool DemagFFT<T>::col_split() const {
+    return !code_f3(ly) && ly > Range<T>::COL;
+}
+
+template <typename T>
 void DemagFFT<T>::launch_row_fwd(const T *m, T ms, hipStream_t stream, int rq) {
     const int64_t n = nx * ny;
-    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+    dispatch_row<T>(lx, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         auto go = [&](auto rqc) {
             constexpr int RQ = decltype(rqc)::value;
@@ -1296,7 +1361,7 @@ void DemagFFT<T>::launch_row_fwd(const T *m, T ms, hipStream_t stream, int rq) {
 
 template <typename T>
 void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
-    if (ly > Range<T>::COL) {
+    if (col_split()) {
         // split column pass (the persistent flag does not apply)
         dispatch_log2<Range<T>::COL + 1, kColSplitMax>(ly, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
@@ -1325,7 +1390,7 @@ void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
         });
         return;
     }
-    dispatch_log2<1, Range<T>::COL>(ly, [&](auto lg) {
+    dispatch_col<T>(ly, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         // rq: log2 of the first/last radix, plus kColPipe for persistent blocks
         const bool pipe = (rq & kColPipe) != 0;
@@ -1335,12 +1400,17 @@ void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
             const int hxs = (int)(px / 2);
             const int nwork = hxs + (hxs + 1) / 2;  // (x, y) items + z-pair items
             const bool half = 2 * ny == py;
-            if (pipe) {
-                auto kh = half ? k_col_demag<T, LG, true, RQ, true> : k_col_demag<T, LG, false, RQ, true>;
-                const int nb = std::min(nwork, resident_blocks((const void *)kh, C::NT, C::LDS));
-                hipLaunchKernelGGL(kh, dim3((unsigned)nb), dim3(C::NT), C::LDS, stream, spec, hxs, (int)ny, tw_y,
-                                   kern, nwork);
-            } else {
+            bool launched = false;
+            if constexpr (!code_f3(LG)) {   // the persistent form is built for powers of two
+                if (pipe) {
+                    auto kh = half ? k_col_demag<T, LG, true, RQ, true> : k_col_demag<T, LG, false, RQ, true>;
+                    const int nb = std::min(nwork, resident_blocks((const void *)kh, C::NT, C::LDS));
+                    hipLaunchKernelGGL(kh, dim3((unsigned)nb), dim3(C::NT), C::LDS, stream, spec, hxs, (int)ny,
+                                       tw_y, kern, nwork);
+                    launched = true;
+                }
+            }
+            if (!launched) {
                 using Gc = ColGrp<T, LG, RQ>;
                 auto kh = half ? k_col_demag<T, LG, true, RQ, false> : k_col_demag<T, LG, false, RQ, false>;
                 hipLaunchKernelGGL(kh, dim3((unsigned)((nwork + Gc::G - 1) / Gc::G)), dim3(Gc::NT), Gc::LDS, stream,
@@ -1360,7 +1430,7 @@ void DemagFFT<T>::launch_col(hipStream_t stream, int rq) {
 template <typename T>
 void DemagFFT<T>::launch_row_inv(T *bd, hipStream_t stream, int rq) {
     const int64_t n = nx * ny;
-    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+    dispatch_row<T>(lx, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         auto go = [&](auto rqc) {
             constexpr int RQ = decltype(rqc)::value;
@@ -1470,7 +1540,7 @@ void DemagFFT<T>::field(const T *m, T ms, T *bd, hipStream_t stream, Profiler *p
 template <typename T>
 void DemagFFT<T>::launch_row_inv_local(const LocalArgs<T> &a, const T *m, const uint8_t *geom, T *b, T *kout,
                                        hipStream_t stream, int rq) {
-    dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+    dispatch_row<T>(lx, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         auto go = [&](auto rqc) {
             constexpr int RQ = decltype(rqc)::value;
@@ -1516,7 +1586,7 @@ void DemagFFT<T>::field_local(const T *m, T ms, const LocalArgs<T> &a, const uin
     }
     auto fits = [&](int rq) {
         bool ok = false;
-        dispatch_log2<1, Range<T>::ROW>(lx, [&](auto lg) {
+        dispatch_row<T>(lx, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
             ok = rq == 3 ? fused_ok<T, LG, 3>() : fused_ok<T, LG, 4>();
         });

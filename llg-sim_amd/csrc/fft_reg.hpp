@@ -212,6 +212,96 @@ __device__ __forceinline__ void reg_fft(cpx<T> *v, cpx<T> *seq, int t,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Transform-length codes: c < kF3 stands for 2^c points, c >= kF3 for
+// 3 * 2^(c - kF3) points (M = 2^(c - kF3)).  A length 3M is three register
+// transforms of length M over the interleaved subsequences x[3m + g] and a
+// radix-3 combination (decimation in time forward, in frequency inverse):
+//   X[k + qM] = sum_g W^(g k) w3^(g q) Y_g[k],  W = exp(-2 pi i / 3M),
+//   w3 = exp(-2 pi i / 3), Y_g = FFT_M(x[3m + g]).
+// A thread then holds NV = 3 R0 values: in the frequency domain v[r'] =
+// X[t + r' TPS] (as for a power of two), in the time domain v[r'] =
+// x[tindex(t, r')].  Twiddle tables of a 3M code hold the M-point table and
+// then the 3M-point table.
+// ---------------------------------------------------------------------------
+constexpr int kF3 = 32;
+__host__ __device__ constexpr bool code_f3(int c) { return c >= kF3; }
+__host__ __device__ constexpr int code_log2m(int c) { return c >= kF3 ? c - kF3 : c; }
+__host__ __device__ constexpr int code_len(int c) { return (c >= kF3 ? 3 : 1) << code_log2m(c); }
+// entries of a code's twiddle table
+__host__ __device__ constexpr int code_tw_len(int c) { return c >= kF3 ? 4 << code_log2m(c) : 1 << c; }
+
+template <int LC, int RQ = 0>
+struct Plan {
+    static constexpr bool F3 = code_f3(LC);
+    static constexpr int LOG2M = code_log2m(LC);
+    using P = RegPlan<LOG2M, RQ>;
+    static constexpr int M = P::L, L = code_len(LC), R0 = P::R0, TPS = P::TPS;
+    static constexpr int NV = F3 ? 3 * R0 : R0;   // values per thread
+    // time-domain index of value rp of thread t (its frequency index is t + rp * TPS)
+    __host__ __device__ static constexpr int tindex(int t, int rp) {
+        return F3 ? 3 * (t + (rp % R0) * TPS) + rp / R0 : t + rp * TPS;
+    }
+    // whether that time-domain index lies below L / 2
+    __host__ __device__ static constexpr bool lower(int rp) { return (rp % R0) < R0 / 2; }
+};
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft3(cpx<T> &a, cpx<T> &b, cpx<T> &c) {
+    const T h = T(0.86602540378443864676);   // sin(2 pi / 3)
+    const cpx<T> s = cadd(b, c), d = csub(b, c);
+    const cpx<T> m = {a.x - T(0.5) * s.x, a.y - T(0.5) * s.y};
+    const cpx<T> ihd = {-h * d.y, h * d.x};   // i h (b - c)
+    a = cadd(a, s);
+    if (INV) {
+        b = cadd(m, ihd);
+        c = csub(m, ihd);
+    } else {
+        b = csub(m, ihd);
+        c = cadd(m, ihd);
+    }
+}
+
+// Transform of length code LC: time domain -> frequency domain (forward) or
+// back (INV), with the register views above.  `seq` holds lds_seq_len(M)
+// slots for the sub-transforms (the caller's own exchanges need
+// lds_seq_len(L)); the barrier rules of reg_fft apply.
+template <typename T, bool INV, int LC, int RQ = 0>
+__device__ __forceinline__ void plan_fft(cpx<T> *v, cpx<T> *seq, int t, const cpx<T> *__restrict__ tw) {
+    using PL = Plan<LC, RQ>;
+    if constexpr (!PL::F3) {
+        reg_fft<T, INV, LC, RQ>(v, seq, t, tw);
+    } else {
+        constexpr int R0 = PL::R0, TPS = PL::TPS, M = PL::M;
+        const cpx<T> *__restrict__ tw3 = tw + M;
+        if constexpr (!INV) {
+#pragma unroll
+            for (int g = 0; g < 3; ++g) reg_fft<T, false, PL::LOG2M, RQ>(v + g * R0, seq, t, tw);
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                const int k = t + r * TPS;
+                cpx<T> a = v[r], b = cmul(v[R0 + r], tw3[k]), c = cmul(v[2 * R0 + r], tw3[2 * k]);
+                dft3<T, false>(a, b, c);
+                v[r] = a;
+                v[R0 + r] = b;
+                v[2 * R0 + r] = c;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                const int k = t + r * TPS;
+                cpx<T> a = v[r], b = v[R0 + r], c = v[2 * R0 + r];
+                dft3<T, true>(a, b, c);
+                v[r] = a;
+                v[R0 + r] = cmulc(b, tw3[k]);
+                v[2 * R0 + r] = cmulc(c, tw3[2 * k]);
+            }
+#pragma unroll
+            for (int g = 0; g < 3; ++g) reg_fft<T, true, PL::LOG2M, RQ>(v + g * R0, seq, t, tw);
+        }
+    }
+}
+
 // Compile-time dispatch over log2 lengths LO..HI.
 template <int LO, int HI, typename F>
 inline void dispatch_log2(int lg, F &&f) {

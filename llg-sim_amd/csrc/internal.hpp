@@ -88,7 +88,7 @@ struct DemagFFT {
     bool built = false;
     int64_t nx = 0, ny = 0, px = 0, py = 0, hx = 0, hy = 0;
     int64_t pbc_x = 0, pbc_y = 0;   // periodic image counts (0 = open axis)
-    int lx = 0, ly = 0;
+    int lx = 0, ly = 0;   // length codes of px and py: log2 for powers of two, 32 + log2(M) for 3M
     double dx = 0, dy = 0, dz = 0;
     cpx<T> *tw_x = nullptr, *tw_y = nullptr;
     T *kern = nullptr;  // 4 planes [Kxx, Kxy, Kyy, Kzz], each [hx][hy], times 1 / (px py)
@@ -115,6 +115,10 @@ struct DemagFFT {
     // Times the radix-8 and radix-16 variants of each pass on this grid and keeps the faster.
     void autotune(hipStream_t stream, int forced);
     void launch_row_fwd(const T *m, T ms, hipStream_t stream, int rq);
+    // columns taken by the split column pass (see demag_fft.hip)
+    bool col_split() const;
+    // a length is 3 * 2^k along an axis (such operators are not in the batched kernels)
+    bool has_f3() const { return lx >= 32 || ly >= 32; }
     void launch_col(hipStream_t stream, int rq);
     void launch_row_inv(T *bd, hipStream_t stream, int rq);
     // b (overwrite) = local field terms + B_demag of M = ms * m, with the inverse

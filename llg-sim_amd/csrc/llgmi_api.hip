@@ -1996,8 +1996,9 @@ int llgmi_dev_rk45_step_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi
             if (demag && resolved_demag_method(mats[i]) != LLGMI_DEMAG_FFT) { ok = false; break; }
             auto ready = [&](auto &eng) {
                 if (!demag) return true;
-                // split column passes (FP64 columns of 2^13) are not in the batched kernels
-                return eng.fft.built && eng.fft.rq_fused != 0 && eng.fft.colfull == nullptr;
+                // split column passes (FP64 columns of 2^13) and 3 * 2^k lengths are not
+                // in the batched kernels
+                return eng.fft.built && eng.fft.rq_fused != 0 && eng.fft.colfull == nullptr && !eng.fft.has_f3();
             };
             ok = c->precision == LLGMI_F32 ? ready(*c->ef) : ready(*c->ed);
         }
@@ -2118,9 +2119,11 @@ int llgmi_dev_rk45_run_batch(llgmi_ctx *const *ctxs, void *const *m_devs, llgmi_
                     eng.field((const T *)m_devs[i], eng.d_b, params[i], mats[i], field_mask, nullptr, nullptr);
                     LLGMI_HIP(hipStreamSynchronize(eng.stream));
                 }
-                if (demag && eng.fft.colfull != nullptr)
-                    throw Error(LLGMI_ERR_UNSUPPORTED, "film " + std::to_string(i) +
-                                                           ": split column passes (FP64, ny > 2048) are not batched");
+                if (demag && (eng.fft.colfull != nullptr || eng.fft.has_f3()))
+                    throw Error(LLGMI_ERR_UNSUPPORTED,
+                                "film " + std::to_string(i) +
+                                    ": split column passes (FP64, ny > 2048) and padded lengths 3 * 2^k are not "
+                                    "batched; step such films with llgmi_dev_rk45_run one at a time");
                 engs.push_back(&eng);
                 ms.push_back((T *)m_devs[i]);
             }

@@ -188,6 +188,81 @@ def test_demag_fp32_length_8192_matches_oracle(gpu_lib, shape):
     assert rel_err(b, b_ref) <= 1e-5
 
 
+# Lengths 3 * 2^k: a grid whose 2n is 3 * 2^k (or closer to it than to the
+# next power of two) pads to 3 * 2^k, as the reference's 2n does; three
+# register transforms of 2^k points and a radix-3 combination.
+@pytest.mark.parametrize("radix", ["auto", "8", "16"])
+@pytest.mark.parametrize("shape", [(1536, 8), (8, 1536), (384, 384), (1400, 5), (5, 1400), (200, 700)])
+def test_demag_radix3_lengths_fp64_match_oracle(gpu_lib, monkeypatch, radix, shape):
+    from oracle import oracle as O
+    if radix != "auto":
+        monkeypatch.setenv("LLGMI_FFT_RADIX", radix)
+    L = gpu_lib
+    nx, ny = shape
+    cell = (3e-9, 2e-9, 1e-9)
+    m = _rand_m(nx * ny, 13 * nx + ny)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    b_ref = np.zeros_like(m)
+    oc.add_demag(m, b_ref, O.Material(ms=8e5, demag=True))
+    ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+    b = ctx.compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(b, b_ref) <= 1e-12
+
+
+def test_demag_fp64_1536_square_pads_to_3072(gpu_lib, monkeypatch):
+    """1536 x 1536 (2n = 3072 = 3 * 2^10 along both axes) against the oracle,
+    and against the same grid padded to powers of two (LLGMI_FFT_POW2=1)."""
+    from oracle import oracle as O
+    L = gpu_lib
+    nx = ny = 1536
+    cell = (2e-9, 2e-9, 1e-9)
+    m = _rand_m(nx * ny, 3072)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    b_ref = np.zeros_like(m)
+    oc.add_demag(m, b_ref, O.Material(ms=8e5, demag=True))
+    mat = L.Material(ms=8e5, demag=True)
+    b3 = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64).compute_demag_field(m, mat)
+    monkeypatch.setenv("LLGMI_FFT_POW2", "1")
+    b2 = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64).compute_demag_field(m, mat)
+    assert rel_err(b3, b_ref) <= 1e-12
+    assert rel_err(b3, b2) <= 1e-12
+
+
+@pytest.mark.parametrize("shape", [(3000, 8), (8, 3000), (768, 768)])
+def test_demag_radix3_lengths_fp32_match_oracle(gpu_lib, shape):
+    """FP32 at padded lengths 6144 (rows and columns) and 1536."""
+    from oracle import oracle as O
+    L = gpu_lib
+    nx, ny = shape
+    cell = (3e-9, 2e-9, 1e-9)
+    m = _rand_m(nx * ny, nx + 7 * ny)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    b_ref = np.zeros_like(m)
+    oc.add_demag(m, b_ref, O.Material(ms=8e5, demag=True))
+    ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F32)
+    b = ctx.compute_demag_field(m, L.Material(ms=8e5, demag=True))
+    assert rel_err(b, b_ref) <= 1e-5
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_full_field_radix3_lengths_fused_and_unfused(gpu_lib, monkeypatch, prec):
+    """The whole field (all terms, FFT demag padded to 768 x 1536) with the
+    fused and the unfused inverse row pass, against the oracle."""
+    from oracle import oracle as O
+    L = gpu_lib
+    nx, ny = 384, 768
+    cell = (2e-9, 2e-9, 1e-9)
+    m = _rand_m(nx * ny, 77)
+    okw = dict(ms=5.8e5, a_ex=1.5e-11, k_u=8e5, dmi=3e-3, demag=True)
+    oc = O.Context(O.Grid(nx, ny, *cell))
+    b_ref = oc.build_h_eff(m, O.Params(alpha=0.3, b_ext=(-24.6e-3, 4.3e-3, 0.0)), O.Material(**okw))
+    for fused in ("0", "1"):
+        monkeypatch.setenv("LLGMI_FFT_FUSED", fused)
+        ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=prec)
+        b = ctx.build_h_eff(m, L.LLGParams(b_ext=(-24.6e-3, 4.3e-3, 0.0)), L.Material(**okw))
+        assert rel_err(b, b_ref) <= (1e-12 if prec == 1 else 1e-5), fused
+
+
 def test_demag_fp64_4096_square_matches_oracle(gpu_lib):
     """4096 x 4096 in FP64: rows of 8192 and split columns of 8192."""
     from oracle import oracle as O
