@@ -195,6 +195,7 @@ def test_demag_fp32_length_8192_matches_oracle(gpu_lib, shape):
 @pytest.mark.parametrize("shape", [(1536, 8), (8, 1536), (384, 384), (1400, 5), (5, 1400), (200, 700)])
 def test_demag_radix3_lengths_fp64_match_oracle(gpu_lib, monkeypatch, radix, shape):
     from oracle import oracle as O
+    monkeypatch.setenv("LLGMI_FFT_PAD", "3")     # 3 * 2^k whenever shorter, also below 3072
     if radix != "auto":
         monkeypatch.setenv("LLGMI_FFT_RADIX", radix)
     L = gpu_lib
@@ -211,7 +212,7 @@ def test_demag_radix3_lengths_fp64_match_oracle(gpu_lib, monkeypatch, radix, sha
 
 def test_demag_fp64_1536_square_pads_to_3072(gpu_lib, monkeypatch):
     """1536 x 1536 (2n = 3072 = 3 * 2^10 along both axes) against the oracle,
-    and against the same grid padded to powers of two (LLGMI_FFT_POW2=1)."""
+    and against the same grid padded to powers of two (LLGMI_FFT_PAD=pow2)."""
     from oracle import oracle as O
     L = gpu_lib
     nx = ny = 1536
@@ -222,16 +223,17 @@ def test_demag_fp64_1536_square_pads_to_3072(gpu_lib, monkeypatch):
     oc.add_demag(m, b_ref, O.Material(ms=8e5, demag=True))
     mat = L.Material(ms=8e5, demag=True)
     b3 = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64).compute_demag_field(m, mat)
-    monkeypatch.setenv("LLGMI_FFT_POW2", "1")
+    monkeypatch.setenv("LLGMI_FFT_PAD", "pow2")
     b2 = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64).compute_demag_field(m, mat)
     assert rel_err(b3, b_ref) <= 1e-12
     assert rel_err(b3, b2) <= 1e-12
 
 
 @pytest.mark.parametrize("shape", [(3000, 8), (8, 3000), (768, 768)])
-def test_demag_radix3_lengths_fp32_match_oracle(gpu_lib, shape):
+def test_demag_radix3_lengths_fp32_match_oracle(gpu_lib, monkeypatch, shape):
     """FP32 at padded lengths 6144 (rows and columns) and 1536."""
     from oracle import oracle as O
+    monkeypatch.setenv("LLGMI_FFT_PAD", "3")
     L = gpu_lib
     nx, ny = shape
     cell = (3e-9, 2e-9, 1e-9)
@@ -256,6 +258,7 @@ def test_full_field_radix3_lengths_fused_and_unfused(gpu_lib, monkeypatch, prec)
     okw = dict(ms=5.8e5, a_ex=1.5e-11, k_u=8e5, dmi=3e-3, demag=True)
     oc = O.Context(O.Grid(nx, ny, *cell))
     b_ref = oc.build_h_eff(m, O.Params(alpha=0.3, b_ext=(-24.6e-3, 4.3e-3, 0.0)), O.Material(**okw))
+    monkeypatch.setenv("LLGMI_FFT_PAD", "3")
     for fused in ("0", "1"):
         monkeypatch.setenv("LLGMI_FFT_FUSED", fused)
         ctx = L.Context(L.Grid2D(nx, ny, *cell), precision=prec)
@@ -537,3 +540,12 @@ def test_cache_load_between_graph_replayed_steps(gpu_lib, tmp_path, prec, monkey
         re_ = ce.dev_rk45_step(be, pe, mat, 2, 1e-5, 0.8, 1e-18, 1e-11)
         assert rg == re_, step
         assert np.array_equal(cg.dev_download(bg), ce.dev_download(be)), step
+
+
+def test_fft_pad_env_rejects_other_values(gpu_lib, monkeypatch):
+    L = gpu_lib
+    monkeypatch.setenv("LLGMI_FFT_PAD", "5")
+    ctx = L.Context(L.Grid2D(16, 16, 3e-9, 3e-9, 3e-9), precision=L.F64)
+    with pytest.raises(L.LlgmiError) as e:
+        ctx.compute_demag_field(_rand_m(256, 1), L.Material(ms=8e5, demag=True))
+    assert e.value.code == L.ERR_INVALID
