@@ -1493,26 +1493,36 @@ void DemagFFT<T>::autotune(hipStream_t stream, int forced) {
             LLGMI_HIP(hipEventElapsedTime(&ms, e0, e1));
             return ms;
         };
-        // The three passes are timed together for all eight radix combinations
-        // (a pass alone sees other cache contents than inside the pipeline);
-        // combinations alternate over three rounds, the minimum per
-        // combination decides.
+        // The three passes are timed together for all radix / column-pass
+        // combinations (a pass alone sees other cache contents than inside the
+        // pipeline); combinations alternate over five rounds, in reverse order
+        // every other round, and the minimum per combination decides.  A
+        // warm-up of about 20 ms first brings the clocks up, so that no
+        // combination profits from coming later.
+        auto pipeline = [&](int rr, int rc, int ri) {
+            launch_row_fwd(m, T(1), stream, rr);
+            launch_col(stream, rc);
+            launch_row_inv(bd, stream, ri);
+        };
+        {
+            const float t1 = time([&] { pipeline(4, 4, 4); }) / 4;
+            const int reps = std::min(2000, std::max(8, (int)(20.f / std::max(t1, 1e-3f))));
+            for (int i = 0; i < reps; ++i) pipeline(4, 4, 4);
+            LLGMI_HIP(hipStreamSynchronize(stream));
+        }
         float best[16];
         for (float &b : best) b = 1e30f;
         auto allowed = [&](int c) {
             const int p = (c >> 3) & 1;
             return pipe_forced < 0 || p == pipe_forced;
         };
-        for (int round = 0; round < 3; ++round)
-            for (int c = 0; c < 16; ++c) {
+        for (int round = 0; round < 5; ++round)
+            for (int i = 0; i < 16; ++i) {
+                const int c = (round & 1) ? 15 - i : i;
                 if (!allowed(c)) continue;
                 const int rr = 3 + (c & 1), rc = (3 + ((c >> 1) & 1)) | ((c & 8) ? kColPipe : 0),
                           ri = 3 + ((c >> 2) & 1);
-                const float t = time([&] {
-                    launch_row_fwd(m, T(1), stream, rr);
-                    launch_col(stream, rc);
-                    launch_row_inv(bd, stream, ri);
-                });
+                const float t = time([&] { pipeline(rr, rc, ri); });
                 if (t < best[c]) best[c] = t;
             }
         int bc = -1;
@@ -1627,12 +1637,20 @@ void DemagFFT<T>::field_local(const T *m, T ms, const LocalArgs<T> &a, const uin
                     return ms_;
                 };
                 float best[3] = {1e30f, 1e30f, 1e30f};   // unfused, fused radix 8, fused radix 16
-                for (int round = 0; round < 3; ++round) {
-                    best[0] = std::min(best[0], time(unfused));
-                    if (fits(3)) best[1] = std::min(best[1], time([&] { launch_row_inv_local(a, m, geom, b, kout, stream, 3); }));
-                    if (fits(4)) best[2] = std::min(best[2], time([&] { launch_row_inv_local(a, m, geom, b, kout, stream, 4); }));
-                }
-                rq_fused = best[0] <= best[1] && best[0] <= best[2] ? -1 : (best[1] <= best[2] ? 3 : 4);
+                auto run = [&](int v) {
+                    if (v == 0) best[0] = std::min(best[0], time(unfused));
+                    else if (fits(v == 1 ? 3 : 4))
+                        best[v] = std::min(best[v], time([&] { launch_row_inv_local(a, m, geom, b, kout, stream, v == 1 ? 3 : 4); }));
+                };
+                for (int round = 0; round < 5; ++round)   // in reverse order every other round
+                    for (int i = 0; i < 3; ++i) run((round & 1) ? 2 - i : i);
+                // the fused kernel must win by 3 %: within that the two are a
+                // tie on the plain field, and the unfused pair is the faster
+                // one once the torque and the next RK stage ride on the local
+                // field (the fused kernel's one block per CU leaves no slack
+                // for them)
+                const float fb = std::min(best[1], best[2]);
+                rq_fused = fb < 0.97f * best[0] ? (best[1] <= best[2] ? 3 : 4) : -1;
             } catch (...) {
                 if (e0) (void)hipEventDestroy(e0);
                 if (e1) (void)hipEventDestroy(e1);

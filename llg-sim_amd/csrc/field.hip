@@ -132,23 +132,11 @@ void launch_mask_cells(T *m, const uint8_t *keep, int64_t n, hipStream_t s) {
 template <typename T>
 __device__ __forceinline__ void rk_stage_cell(const T *__restrict__ m, const T *const *k, const T *c, int nk,
                                               T *__restrict__ out, int64_t i, int64_t n) {
-    T v0 = m[i], v1 = m[n + i], v2 = m[2 * n + i];
-    for (int q = 0; q < nk; ++q) {
-        const T *kq = k[q];
-        v0 = v0 + c[q] * kq[i];
-        v1 = v1 + c[q] * kq[n + i];
-        v2 = v2 + c[q] * kq[2 * n + i];
-    }
-    T n2 = v0 * v0 + v1 * v1 + v2 * v2;
-    if (n2 == T(0)) {
-        v0 = T(0); v1 = T(0); v2 = T(1);
-    } else {
-        T inv = T(1) / sqrt(n2);
-        v0 *= inv; v1 *= inv; v2 *= inv;
-    }
-    out[i] = v0;
-    out[n + i] = v1;
-    out[2 * n + i] = v2;
+    T v[3] = {m[i], m[n + i], m[2 * n + i]};
+    stage_combine<T>(v, k, c, nk, nullptr, i, n);
+    out[i] = v[0];
+    out[n + i] = v[1];
+    out[2 * n + i] = v[2];
 }
 
 template <typename T>
@@ -192,6 +180,7 @@ __device__ __forceinline__ double rk_err_cell(const T *__restrict__ k1, const T 
                                               const T *__restrict__ k4, const T *__restrict__ k5,
                                               const T *__restrict__ k6, const T *__restrict__ k7, int64_t t,
                                               int64_t n) {
+#pragma clang fp contract(off)   // the same rounding in every kernel that inlines it
     double d = 0.0;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -233,8 +222,26 @@ k_rk_err(const T *__restrict__ k1, const T *__restrict__ k3, const T *__restrict
     }
 }
 
+// One film's controller (llg.rs:999-1025, the same decision as kb_control)
+// from the attempt's max |err|: attempt a's outputs, the next step size into
+// *c.dt and the decision into *c.flag.
+__device__ __forceinline__ void rk_control_one(double err, const RkControl &c) {
+    const double dt0 = *c.dt;
+    const double eps = err * dt0;
+    double dt_next = eps > 0.0 ? dt0 * c.headroom * pow(c.max_err / eps, 0.2) : dt0 * 2.0;
+    if (dt_next < c.dt_min) dt_next = c.dt_min;
+    if (dt_next > c.dt_max) dt_next = c.dt_max;
+    const int accept = (eps <= c.max_err) || (dt0 <= c.dt_min * 1.0000000001);
+    c.o_eps[c.a] = eps;
+    c.o_acc[c.a] = accept;
+    c.o_dtu[c.a] = dt0;
+    *c.dt = dt_next;
+    *c.flag = accept;
+}
+
+// max of the partial rows into out[0]; with c.dt set, then the controller
 __global__ void __launch_bounds__(1024) k_max_reduce(const double *__restrict__ partial, int np,
-                                                     double *__restrict__ out) {
+                                                     double *__restrict__ out, RkControl c) {
     __shared__ double red[16];
     double v = 0.0;
     for (int i = threadIdx.x; i < np; i += blockDim.x) v = fmax(v, partial[i]);
@@ -245,12 +252,13 @@ __global__ void __launch_bounds__(1024) k_max_reduce(const double *__restrict__ 
         double r = red[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, red[w]);
         out[0] = r;
+        if (c.dt) rk_control_one(r, c);
     }
 }
 
 template <typename T>
 void launch_rk_err(const T *k1, const T *k3, const T *k4, const T *k5, const T *k6, const T *k7,
-                   int64_t n, bool combined, double *partial, double *out, hipStream_t s) {
+                   int64_t n, bool combined, double *partial, double *out, hipStream_t s, const RkControl *ctl) {
     int64_t want = (n + 255) / 256;
     int nb = (int)(want < kErrBlocks ? (want > 0 ? want : 1) : kErrBlocks);
     if (combined)
@@ -258,7 +266,7 @@ void launch_rk_err(const T *k1, const T *k3, const T *k4, const T *k5, const T *
     else
         hipLaunchKernelGGL((k_rk_err<T, false>), dim3(nb), dim3(256), 0, s, k1, k3, k4, k5, k6, k7, n, partial);
     LLGMI_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(1024), 0, s, partial, nb, out);
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(1024), 0, s, partial, nb, out, ctl ? *ctl : RkControl{});
     LLGMI_LAUNCH_CHECK();
 }
 
@@ -365,27 +373,6 @@ kb_accept(const BFilm<T> *__restrict__ F, const BEntry *__restrict__ tab) {
     f.m[2 * n + c] = f.m7[2 * n + c];
 }
 
-// One film's controller on the device (the same decision as kb_control):
-// reads max |err| and the step size, writes attempt a's outputs, the next
-// step size and the accept flag.  One thread.
-__global__ void __launch_bounds__(64)
-k_rk_control(const double *__restrict__ err, double *__restrict__ dt, int *__restrict__ flag, double max_err,
-             double headroom, double dt_min, double dt_max, int a, double *__restrict__ o_eps,
-             int *__restrict__ o_acc, double *__restrict__ o_dtu) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const double dt0 = *dt;
-    const double eps = *err * dt0;
-    double dt_next = eps > 0.0 ? dt0 * headroom * pow(max_err / eps, 0.2) : dt0 * 2.0;
-    if (dt_next < dt_min) dt_next = dt_min;
-    if (dt_next > dt_max) dt_next = dt_max;
-    const int accept = (eps <= max_err) || (dt0 <= dt_min * 1.0000000001);
-    o_eps[a] = eps;
-    o_acc[a] = accept;
-    o_dtu[a] = dt0;
-    *dt = dt_next;
-    *flag = accept;
-}
-
 // After an accepted attempt: m = m7 and k1 = k7 (the next attempt's first
 // slope, FSAL); nothing after a rejected one (m and k1 = f(m) stay).
 template <typename T>
@@ -399,10 +386,43 @@ k_rk_accept(const int *__restrict__ flag, T *__restrict__ m, const T *__restrict
     }
 }
 
-void launch_rk_control(const double *err, double *dt, int *flag, double max_err, double headroom, double dt_min,
-                       double dt_max, int a, double *o_eps, int *o_acc, double *o_dtu, hipStream_t s) {
-    hipLaunchKernelGGL(k_rk_control, dim3(1), dim3(64), 0, s, err, dt, flag, max_err, headroom, dt_min, dt_max, a,
-                       o_eps, o_acc, o_dtu);
+// Stage 2 of an attempt inside a device-controlled run: after an accepted
+// attempt (*flag set) the base state and first slope are m7 and k7, which are
+// also copied to m and k1 (what k_rk_accept does), else m and k1; then the
+// stage state normalize(base + c k) goes to out.  a: stage 2's arguments
+// (one slope).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_rk_stage2_accept(const int *__restrict__ flag, T *__restrict__ m, const T *__restrict__ m7, T *__restrict__ k1,
+                   const T *__restrict__ k7, StageArgs<T> a, T *__restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool acc = *flag != 0;
+    const T *mb = acc ? m7 : m;
+    const T *kb = acc ? k7 : k1;
+    const T base[3] = {mb[i], mb[n + i], mb[2 * n + i]};
+    const T kk[3] = {kb[i], kb[n + i], kb[2 * n + i]};
+    const T c0 = a.dt ? T(*a.dt * a.a[0]) : a.c[0];
+    T v[3] = {base[0], base[1], base[2]};
+    stage_combine<T>(v, nullptr, &c0, 1, kk, i, n);
+    out[i] = v[0];
+    out[n + i] = v[1];
+    out[2 * n + i] = v[2];
+    if (acc) {
+        m[i] = base[0];
+        m[n + i] = base[1];
+        m[2 * n + i] = base[2];
+        k1[i] = kk[0];
+        k1[n + i] = kk[1];
+        k1[2 * n + i] = kk[2];
+    }
+}
+
+template <typename T>
+void launch_rk_stage2_accept(const int *flag, T *m, const T *m7, T *k1, const T *k7, const StageArgs<T> &a,
+                             T *out, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_rk_stage2_accept<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flag, m, m7, k1,
+                       k7, a, out, n);
     LLGMI_LAUNCH_CHECK();
 }
 
@@ -501,6 +521,8 @@ void launch_batch_accept(const BFilm<T> *F, const BEntry *tab, int nent, hipStre
                                       double *, hipStream_t);                                \
     template void launch_batch_accept<T>(const BFilm<T> *, const BEntry *, int, hipStream_t); \
     template void launch_rk_accept<T>(const int *, T *, const T *, T *, const T *, int64_t, hipStream_t); \
+    template void launch_rk_stage2_accept<T>(const int *, T *, const T *, T *, const T *, const StageArgs<T> &, \
+                                             T *, int64_t, hipStream_t);                                        \
     template void launch_batch_control<T>(BFilm<T> *, int, const double *, double, double, double, double, int, \
                                           double *, int *, double *, hipStream_t);             \
     template void launch_local_field<T>(const LocalArgs<T> &, const T *, const T *,          \
@@ -512,7 +534,8 @@ void launch_batch_accept(const BFilm<T> *F, const BEntry *tab, int nent, hipStre
     template void launch_mask_cells<T>(T *, const uint8_t *, int64_t, hipStream_t);          \
     template void launch_rk_stage<T>(const T *, const StageArgs<T> &, T *, int64_t, hipStream_t); \
     template void launch_rk_err<T>(const T *, const T *, const T *, const T *, const T *,    \
-                                   const T *, int64_t, bool, double *, double *, hipStream_t);
+                                   const T *, int64_t, bool, double *, double *, hipStream_t,     \
+                                   const RkControl *);
 LLGMI_INST(float)
 LLGMI_INST(double)
 #undef LLGMI_INST

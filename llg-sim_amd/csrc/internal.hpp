@@ -151,6 +151,18 @@ struct LocalArgs {
     int torque;            // also write k: 1 = llg_rhs(m, B), 2 = llg_rhs_relax(m, B)
     T rhs_pref;            // -gamma/(1+alpha^2), or -gamma alpha/(1+alpha^2) for relax
     T alpha;
+    // Next RK stage state, formed after the torque (st_out non-null, torque
+    // written): st_out = normalize(st_m + sum_q c_q k_q), q = 0 .. st_nk - 1 in
+    // order, with k_q = st_k[q] read from memory for q < st_nk - 1 and the
+    // cell's own torque for the last q; c_q = T(*st_dt * st_a[q]) when st_dt is
+    // set, else st_c[q].  The same arithmetic as the stand-alone stage kernel.
+    T *st_out;
+    const T *st_m;
+    const T *st_k[5];
+    T st_c[6];
+    const double *st_dt;
+    double st_a[6];
+    int st_nk;
 };
 
 template <typename T>
@@ -200,10 +212,6 @@ void launch_batch_err(const BFilm<T> *F, const BEntry *tab, int nent, int nfilms
 // m = m7 for the films whose accept is set
 template <typename T>
 void launch_batch_accept(const BFilm<T> *F, const BEntry *tab, int nent, hipStream_t s);
-// The RK45 controller of one film on the device for attempt a: outputs at
-// [a], the next step size into *dt and the decision into *flag.
-void launch_rk_control(const double *err, double *dt, int *flag, double max_err, double headroom, double dt_min,
-                       double dt_max, int a, double *o_eps, int *o_acc, double *o_dtu, hipStream_t s);
 // m = m7 and k1 = k7 when *flag is set (3 n values each).
 template <typename T>
 void launch_rk_accept(const int *flag, T *m, const T *m7, T *k1, const T *k7, int64_t n, hipStream_t s);
@@ -262,14 +270,32 @@ struct StageArgs {
 };
 template <typename T>
 void launch_rk_stage(const T *m, const StageArgs<T> &a, T *out, int64_t n, hipStream_t s);
+// Stage 2 inside a device-controlled run: with *flag set (the previous attempt
+// accepted) first m = m7 and k1 = k7, then out = normalize(m + c k1).
+template <typename T>
+void launch_rk_stage2_accept(const int *flag, T *m, const T *m7, T *k1, const T *k7, const StageArgs<T> &a,
+                             T *out, int64_t n, hipStream_t s);
+
+// The RK45 controller of one film on the device for attempt a: outputs at
+// [a], the next step size into *dt and the decision into *flag.
+struct RkControl {
+    double *dt = nullptr;
+    int *flag = nullptr;
+    double max_err = 0, headroom = 0, dt_min = 0, dt_max = 0;
+    int a = 0;
+    double *o_eps = nullptr;
+    int *o_acc = nullptr;
+    double *o_dtu = nullptr;
+};
 
 // Error norm max_{cell,comp} |tau5 - tau4| over k1,k3,k4,k5,k6,k7 (FP64 arithmetic).
-// partial must hold kErrBlocks doubles; result written to out[0].
+// partial must hold kErrBlocks doubles; result written to out[0].  With ctl,
+// the kernel that forms the maximum then runs the controller on it.
 static constexpr int kErrBlocks = 1024;
 template <typename T>
 void launch_rk_err(const T *k1, const T *k3, const T *k4, const T *k5, const T *k6,
                    const T *k7, int64_t n, bool combined, double *partial, double *out,
-                   hipStream_t s);
+                   hipStream_t s, const RkControl *ctl = nullptr);
 
 // ---------------------------------------------------------------------------
 // Relaxation path (relax.hip)

@@ -96,7 +96,7 @@ struct Engine : EngineBase {
 
     // RK45 scratch (RK45Scratch, llg.rs:126-189): k1..k7, stage state, m7.
     T *d_k[8] = {nullptr};
-    T *d_stage = nullptr, *d_m7 = nullptr;
+    T *d_stage = nullptr, *d_stage2 = nullptr, *d_m7 = nullptr;
     double *d_partial = nullptr, *d_err = nullptr;
     double *h_err = nullptr;  // pinned
     bool fsal_valid = false;
@@ -135,7 +135,7 @@ struct Engine : EngineBase {
     Engine(const llgmi_grid &g_, hipStream_t s, Profiler *p) : g(g_), n(g_.nx * g_.ny), stream(s), prof(p) {}
 
     ~Engine() override {
-        void *ps[] = {d_m, d_b, d_bd, d_aos, d_geom, d_stage, d_m7, d_partial, d_err,
+        void *ps[] = {d_m, d_b, d_bd, d_aos, d_geom, d_stage, d_stage2, d_m7, d_partial, d_err,
                       d_r23b4, d_r23m4, d_red, d_redout, d_dk, d_hyb_scale};
         for (void *q : ps)
             if (q) (void)hipFree(q);
@@ -434,11 +434,28 @@ struct Engine : EngineBase {
     }
 
     // build_h_eff_masked[_geom] on device buffers; optional torque output.
-    // relax: kout = llg_rhs_relax instead of llg_rhs.
+    // relax: kout = llg_rhs_relax instead of llg_rhs.  b null: the field is
+    // not stored (only the torque is wanted).  nxt (with kout): the next RK
+    // stage state normalize(nxt_m + sum_q c_q k_q) is formed from the torque
+    // in the same kernel and written to nxt_out, whose last k_q is kout.
     void field(const T *m, T *b, const llgmi_params &p, const llgmi_material &mat, int mask,
-               const uint8_t *geom, T *kout, bool relax = false) {
+               const uint8_t *geom, T *kout, bool relax = false, const StageArgs<T> *nxt = nullptr,
+               const T *nxt_m = nullptr, T *nxt_out = nullptr) {
         const bool demag = mask == LLGMI_MASK_FULL && mat.demag;
         LocalArgs<T> a = local_args(p, mat, mask, kout != nullptr, relax);
+        if (nxt) {
+            if (!kout || relax || nxt->nk < 1 || nxt->nk > 5 || nxt->k[nxt->nk - 1] != kout)
+                throw Error(LLGMI_ERR_INVALID, "fused RK stage: the last slope must be this evaluation's torque");
+            a.st_out = nxt_out;
+            a.st_m = nxt_m;
+            a.st_nk = nxt->nk;
+            a.st_dt = nxt->dt;
+            for (int q = 0; q < nxt->nk; ++q) {
+                if (q < nxt->nk - 1) a.st_k[q] = nxt->k[q];
+                a.st_c[q] = nxt->c[q];
+                a.st_a[q] = nxt->a[q];
+            }
+        }
         if (demag && resolved_demag_method(mat) == LLGMI_DEMAG_FFT) {
             // FFT demag: inverse row pass and local field share one kernel
             ensure_fft();
@@ -531,6 +548,7 @@ struct Engine : EngineBase {
     void ensure_rk() {
         for (int s = 1; s <= 7; ++s) alloc(d_k[s], 3 * n);
         alloc(d_stage, 3 * n);
+        alloc(d_stage2, 3 * n);
         alloc(d_m7, 3 * n);
         alloc(d_b, 3 * n);
         alloc(d_partial, kErrBlocks);
@@ -621,8 +639,16 @@ struct Engine : EngineBase {
     }
 
     // Stages 2..7 with their field evaluations (k1 in place); with dt_dev the
-    // stage kernels take the step size from device memory.
-    void rk45_stages(T *m, const llgmi_params &p, const llgmi_material &mat, int mask, double *dt_dev) {
+    // stage kernels take the step size from device memory.  The state of
+    // stage s + 1 is formed in the epilogue of stage s's field evaluation
+    // from its torque (stage 2's by the stage kernel; LLGMI_RK45_FUSE=0 runs
+    // every stage combination as its own kernel, with identical results).
+    // Stage states alternate between two buffers, so an epilogue never writes
+    // the state its own field evaluation reads.  The field b itself is not
+    // stored.  acc_flag (device-controlled runs): when the previous attempt
+    // was accepted, the stage-2 kernel first takes m = m7 and k1 = k7.
+    void rk45_stages(T *m, const llgmi_params &p, const llgmi_material &mat, int mask, double *dt_dev,
+                     const int *acc_flag = nullptr) {
         static const double A[7][5] = {
             {0}, {0},
             {1.0 / 5.0},
@@ -634,7 +660,7 @@ struct Engine : EngineBase {
         static const double Bw[5] = {35.0 / 384.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0};
         static const int Bk[5] = {1, 3, 4, 5, 6};
         const double dt0 = p.dt;
-        for (int s = 2; s <= 7; ++s) {
+        auto args = [&](int s) {
             StageArgs<T> sa{};
             sa.dt = dt_dev;
             if (s < 7) {
@@ -652,15 +678,37 @@ struct Engine : EngineBase {
                     sa.c[q] = T(dt0 * Bw[q]);
                 }
             }
-            T *out = s == 7 ? d_m7 : d_stage;
-            {
-                ProfScope ps(prof, LLGMI_K_RK45_STAGE, stream);
-                launch_rk_stage<T>(m, sa, out, n, stream);
+            return sa;
+        };
+        auto state = [&](int s) { return s == 7 ? d_m7 : ((s & 1) ? d_stage2 : d_stage); };
+        auto stage_kernel = [&](int s) {
+            const StageArgs<T> sa = args(s);
+            ProfScope ps(prof, LLGMI_K_RK45_STAGE, stream);
+            launch_rk_stage<T>(m, sa, state(s), n, stream);
+        };
+        const bool fuse = rk45_fuse_enabled();
+        if (acc_flag) {
+            const StageArgs<T> sa = args(2);
+            ProfScope ps(prof, LLGMI_K_RK45_STAGE, stream);
+            launch_rk_stage2_accept<T>(acc_flag, m, d_m7, d_k[1], d_k[7], sa, state(2), n, stream);
+        } else {
+            stage_kernel(2);
+        }
+        for (int s = 2; s <= 7; ++s) {
+            if (s < 7 && fuse) {
+                const StageArgs<T> nxt = args(s + 1);
+                field(state(s), nullptr, p, mat, mask, nullptr, d_k[s], false, &nxt, m, state(s + 1));
+            } else {
+                field(state(s), nullptr, p, mat, mask, nullptr, d_k[s]);
+                if (s < 7) stage_kernel(s + 1);
             }
-            field(out, d_b, p, mat, mask, nullptr, d_k[s]);
         }
     }
 
+    static bool rk45_fuse_enabled() {
+        const char *e = std::getenv("LLGMI_RK45_FUSE");
+        return !(e && e[0] == '0');
+    }
     static bool rk_graphs_enabled() {
         const char *e = std::getenv("LLGMI_RK45_GRAPH");
         return !(e && e[0] == '0');
@@ -677,6 +725,9 @@ struct Engine : EngineBase {
         put(&mat, sizeof mat);
         put(d_k, sizeof d_k);
         put(&d_stage, sizeof d_stage);
+        put(&d_stage2, sizeof d_stage2);
+        const bool fuse = rk45_fuse_enabled();
+        put(&fuse, sizeof fuse);
         put(&d_m7, sizeof d_m7);
         put(&d_b, sizeof d_b);
         const int method = resolved_demag_method(mat);
@@ -742,10 +793,12 @@ struct Engine : EngineBase {
 
     // `attempts` consecutive attempts with the controller (llg.rs:999-1025,
     // FP64) on the device, replayed from one captured graph: per attempt the
-    // stages, the error norm, k_rk_control (eps, decision, next step size)
-    // and k_rk_accept (m = m7 and k1 = k7 after an accepted attempt; after a
-    // rejected one m and k1 = f(m) stay, which is what recomputing k1 gives).
-    // The host waits once; p.dt returns the next step size.
+    // stages and the error norm with the controller in its final reduction
+    // (eps, decision, next step size).  After an accepted attempt m = m7 and
+    // k1 = k7, applied by the next attempt's stage-2 kernel (k_rk_accept after
+    // the last); after a rejected one m and k1 = f(m) stay, which is what
+    // recomputing k1 gives.  The host waits once; p.dt returns the next step
+    // size.
     void rk45_run(T *m, llgmi_params &p, const llgmi_material &mat, int mask, double max_err, double headroom,
                   double dt_min, double dt_max, int64_t attempts, double *eps, int *acc, double *dtu) {
         const bool demag = mask == LLGMI_MASK_FULL && mat.demag;
@@ -798,13 +851,23 @@ struct Engine : EngineBase {
                 LLGMI_HIP(hipMemcpyAsync(d_dt, h_dt, sizeof(double), hipMemcpyHostToDevice, stream));
                 if (!fsal_ok) field(m, d_b, p, mat, mask, nullptr, d_k[1]);
                 for (int64_t a = 0; a < attempts; ++a) {
-                    rk45_stages(m, p, mat, mask, d_dt);
+                    rk45_stages(m, p, mat, mask, d_dt, a > 0 ? d_run_flag : nullptr);
+                    RkControl ctl;
+                    ctl.dt = d_dt;
+                    ctl.flag = d_run_flag;
+                    ctl.max_err = max_err;
+                    ctl.headroom = headroom;
+                    ctl.dt_min = dt_min;
+                    ctl.dt_max = dt_max;
+                    ctl.a = (int)a;
+                    ctl.o_eps = d_run_eps;
+                    ctl.o_acc = d_run_acc;
+                    ctl.o_dtu = d_run_dtu;
                     launch_rk_err<T>(d_k[1], d_k[3], d_k[4], d_k[5], d_k[6], d_k[7], n,
-                                     sizeof(T) == sizeof(float), d_partial, d_err, stream);
-                    launch_rk_control(d_err, d_dt, d_run_flag, max_err, headroom, dt_min, dt_max, (int)a,
-                                      d_run_eps, d_run_acc, d_run_dtu, stream);
-                    launch_rk_accept<T>(d_run_flag, m, d_m7, d_k[1], d_k[7], n, stream);
+                                     sizeof(T) == sizeof(float), d_partial, d_err, stream, &ctl);
                 }
+                // the last attempt's accept (each earlier one is applied by the next stage 2)
+                launch_rk_accept<T>(d_run_flag, m, d_m7, d_k[1], d_k[7], n, stream);
                 LLGMI_HIP(hipMemcpyAsync(h_run_eps, d_run_eps, sizeof(double) * attempts, hipMemcpyDeviceToHost,
                                          stream));
                 LLGMI_HIP(hipMemcpyAsync(h_run_dtu, d_run_dtu, sizeof(double) * attempts, hipMemcpyDeviceToHost,

@@ -1,5 +1,6 @@
 // Per-cell local effective field (Zeeman, exchange, uniaxial anisotropy,
-// interfacial DMI, optional demag addend, optional LLG torque), shared by the
+// interfacial DMI, optional demag addend, optional LLG torque and the next
+// RK stage state formed from it), shared by the
 // stand-alone local-field kernel and the kernel that fuses it with the
 // inverse row FFT of the demag field.
 //
@@ -11,41 +12,88 @@
 
 #include "internal.hpp"
 
+// Every function here is compiled without implicit multiply-add contraction:
+// a sum of two products (a cross product, m.u, the Laplacian's two axes) could
+// be fused either way, and the compiler's choice may differ between the
+// kernels that inline this code (stand-alone, fused with the inverse row FFT,
+// batched).  The multiply-adds are written out instead (mad: fused in FP32,
+// where the FMA halves the arithmetic; a rounded product and sum in FP64, the
+// reference's own rounding), so every kernel rounds the same way and the
+// stepper gives bitwise the same result through any of them.
+
 namespace llgmi {
+
+// a * b + c
+__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double mad(double a, double b, double c) {
+#pragma clang fp contract(off)
+    return a * b + c;
+}
 
 template <typename T>
 __device__ __forceinline__ void ghost_x(const T *m, T n_x, T eta, T dx, T *o) {
-    o[0] = m[0] + n_x * dx * (-eta * m[2]);
+#pragma clang fp contract(off)
+    o[0] = mad(n_x * dx, -eta * m[2], m[0]);
     o[1] = m[1];
-    o[2] = m[2] + n_x * dx * (eta * m[0]);
+    o[2] = mad(n_x * dx, eta * m[0], m[2]);
 }
 template <typename T>
 __device__ __forceinline__ void ghost_y(const T *m, T n_y, T eta, T dy, T *o) {
+#pragma clang fp contract(off)
     o[0] = m[0];
-    o[1] = m[1] + n_y * dy * (-eta * m[2]);
-    o[2] = m[2] + n_y * dy * (eta * m[1]);
+    o[1] = mad(n_y * dy, -eta * m[2], m[1]);
+    o[2] = mad(n_y * dy, eta * m[1], m[2]);
 }
 
 // The torque of cell c from its m and total field B: llg_rhs (llg.rs:20-32),
 // -gamma/(1+alpha^2) [m x B + alpha m x (m x B)], or for relaxation
 // llg_rhs_relax (llg.rs:38-50), -gamma alpha/(1+alpha^2) m x (m x B).
 template <typename T>
-__device__ __forceinline__ void torque_cell(const LocalArgs<T> &a, const T *m0, const T *bv, T *__restrict__ kout,
-                                            int64_t c, int64_t n) {
-    T cx = m0[1] * bv[2] - m0[2] * bv[1];
-    T cy = m0[2] * bv[0] - m0[0] * bv[2];
-    T cz = m0[0] * bv[1] - m0[1] * bv[0];
-    T ccx = m0[1] * cz - m0[2] * cy;
-    T ccy = m0[2] * cx - m0[0] * cz;
-    T ccz = m0[0] * cy - m0[1] * cx;
+__device__ __forceinline__ void torque_cell(const LocalArgs<T> &a, const T *m0, const T *bv, T *kv) {
+#pragma clang fp contract(off)
+    T cx = mad(m0[1], bv[2], -(m0[2] * bv[1]));
+    T cy = mad(m0[2], bv[0], -(m0[0] * bv[2]));
+    T cz = mad(m0[0], bv[1], -(m0[1] * bv[0]));
+    T ccx = mad(m0[1], cz, -(m0[2] * cy));
+    T ccy = mad(m0[2], cx, -(m0[0] * cz));
+    T ccz = mad(m0[0], cy, -(m0[1] * cx));
     if (a.torque == 2) {
-        kout[c] = a.rhs_pref * ccx;
-        kout[n + c] = a.rhs_pref * ccy;
-        kout[2 * n + c] = a.rhs_pref * ccz;
+        kv[0] = a.rhs_pref * ccx;
+        kv[1] = a.rhs_pref * ccy;
+        kv[2] = a.rhs_pref * ccz;
     } else {
-        kout[c] = a.rhs_pref * (cx + a.alpha * ccx);
-        kout[n + c] = a.rhs_pref * (cy + a.alpha * ccy);
-        kout[2 * n + c] = a.rhs_pref * (cz + a.alpha * ccz);
+        kv[0] = a.rhs_pref * mad(a.alpha, ccx, cx);
+        kv[1] = a.rhs_pref * mad(a.alpha, ccy, cy);
+        kv[2] = a.rhs_pref * mad(a.alpha, ccz, cz);
+    }
+}
+
+// v = v + c_q k_q for q = 0 .. nk - 1 in order, then normalised (zero -> +z):
+// the stage state of step_llg_rk45_recompute_field_masked_adaptive
+// (llg.rs:870-941).  k_q is kp[q][o + comp * n] for q < nk - 1; the last term
+// takes klast (when non-null) instead of memory.
+template <typename T>
+__device__ __forceinline__ void stage_combine(T *v, const T *const *kp, const T *c, int nk, const T *klast,
+                                              int64_t o, int64_t n) {
+#pragma clang fp contract(off)
+    for (int q = 0; q < nk; ++q) {
+        if (klast && q == nk - 1) {
+            v[0] = mad(c[q], klast[0], v[0]);
+            v[1] = mad(c[q], klast[1], v[1]);
+            v[2] = mad(c[q], klast[2], v[2]);
+        } else {
+            const T *kq = kp[q];
+            v[0] = mad(c[q], kq[o], v[0]);
+            v[1] = mad(c[q], kq[n + o], v[1]);
+            v[2] = mad(c[q], kq[2 * n + o], v[2]);
+        }
+    }
+    T n2 = mad(v[2], v[2], mad(v[1], v[1], v[0] * v[0]));
+    if (n2 == T(0)) {
+        v[0] = T(0); v[1] = T(0); v[2] = T(1);
+    } else {
+        T inv = T(1) / sqrt(n2);
+        v[0] *= inv; v[1] *= inv; v[2] *= inv;
     }
 }
 
@@ -54,6 +102,7 @@ template <typename T, bool GEOM, bool DEMAG, bool TORQUE>
 __device__ __forceinline__ void local_cell(const LocalArgs<T> &a, const T *__restrict__ m,
                                            const uint8_t *__restrict__ geom, int i, int j, const T *bd3,
                                            T *__restrict__ b, T *__restrict__ kout) {
+#pragma clang fp contract(off)
     const int nx = a.nx, ny = a.ny;
     const int64_t n = (int64_t)nx * ny;
     const int64_t c = (int64_t)j * nx + i;
@@ -106,15 +155,17 @@ __device__ __forceinline__ void local_cell(const LocalArgs<T> &a, const T *__res
             else { yp[0] = m0[0]; yp[1] = m0[1]; yp[2] = m0[2]; }
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                T d2x = nx > 1 ? (xp[q] - T(2) * m0[q] + xm[q]) * a.inv_dx2 : T(0);
-                T d2y = ny > 1 ? (yp[q] - T(2) * m0[q] + ym[q]) * a.inv_dy2 : T(0);
-                bv[q] += a.exch_coeff * (d2x + d2y);
+                // (xp - 2 m + xm) / dx^2 + (yp - 2 m + ym) / dy^2 (2 m is exact,
+                // so the multiply-add rounds xp - 2 m once either way)
+                const T lx = nx > 1 ? mad(T(-2), m0[q], xp[q]) + xm[q] : T(0);
+                const T ly = ny > 1 ? mad(T(-2), m0[q], yp[q]) + ym[q] : T(0);
+                bv[q] = mad(a.exch_coeff, mad(lx, a.inv_dx2, ly * a.inv_dy2), bv[q]);
             }
         }
         if (a.anis_on) {
-            T mdotu = m0[0] * a.u[0] + m0[1] * a.u[1] + m0[2] * a.u[2];
+            const T s = a.anis_coeff * mad(m0[2], a.u[2], mad(m0[1], a.u[1], m0[0] * a.u[0]));
 #pragma unroll
-            for (int q = 0; q < 3; ++q) bv[q] += a.anis_coeff * mdotu * a.u[q];
+            for (int q = 0; q < 3; ++q) bv[q] = mad(s, a.u[q], bv[q]);
         }
         if (a.dmi_on) {
             // dmi.rs:123-181: domain edge -> chiral ghost; masked neighbour -> centre.
@@ -133,27 +184,44 @@ __device__ __forceinline__ void local_cell(const LocalArgs<T> &a, const T *__res
             else { yp[0] = m0[0]; yp[1] = m0[1]; yp[2] = m0[2]; }
             T dmz_dx = nx > 1 ? (xp[2] - xm[2]) * a.inv_2dx : T(0);
             T dmz_dy = ny > 1 ? (yp[2] - ym[2]) * a.inv_2dy : T(0);
-            T dmx_dx = nx > 1 ? (xp[0] - xm[0]) * a.inv_2dx : T(0);
             T dmy_dy = ny > 1 ? (yp[1] - ym[1]) * a.inv_2dy : T(0);
-            bv[0] += a.dmi_pref * dmz_dx;
-            bv[1] += a.dmi_pref * dmz_dy;
-            bv[2] += -a.dmi_pref * (dmx_dx + dmy_dy);
+            T div = nx > 1 ? mad(xp[0] - xm[0], a.inv_2dx, dmy_dy) : dmy_dy;   // dmx/dx + dmy/dy
+            bv[0] = mad(a.dmi_pref, dmz_dx, bv[0]);
+            bv[1] = mad(a.dmi_pref, dmz_dy, bv[1]);
+            bv[2] = mad(-a.dmi_pref, div, bv[2]);
         }
     } else if (a.anis_on) {
         // anisotropy is not masked (mod.rs:91)
-        T mdotu = m0[0] * a.u[0] + m0[1] * a.u[1] + m0[2] * a.u[2];
+        const T s = a.anis_coeff * mad(m0[2], a.u[2], mad(m0[1], a.u[1], m0[0] * a.u[0]));
 #pragma unroll
-        for (int q = 0; q < 3; ++q) bv[q] += a.anis_coeff * mdotu * a.u[q];
+        for (int q = 0; q < 3; ++q) bv[q] = mad(s, a.u[q], bv[q]);
     }
     if (DEMAG) {
         bv[0] += bd3[0];
         bv[1] += bd3[1];
         bv[2] += bd3[2];
     }
-    b[c] = bv[0];
-    b[n + c] = bv[1];
-    b[2 * n + c] = bv[2];
-    if (TORQUE) torque_cell(a, m0, bv, kout, c, n);
+    if (b) {   // null: the field itself is not wanted (RK stages keep only the torque)
+        b[c] = bv[0];
+        b[n + c] = bv[1];
+        b[2 * n + c] = bv[2];
+    }
+    if (TORQUE) {
+        T kv[3];
+        torque_cell(a, m0, bv, kv);
+        kout[c] = kv[0];
+        kout[n + c] = kv[1];
+        kout[2 * n + c] = kv[2];
+        if (a.st_out) {
+            T co[6];
+            for (int q = 0; q < a.st_nk; ++q) co[q] = a.st_dt ? T(*a.st_dt * a.st_a[q]) : a.st_c[q];
+            T v[3] = {a.st_m[c], a.st_m[n + c], a.st_m[2 * n + c]};
+            stage_combine<T>(v, a.st_k, co, a.st_nk, kv, c, n);
+            a.st_out[c] = v[0];
+            a.st_out[n + c] = v[1];
+            a.st_out[2 * n + c] = v[2];
+        }
+    }
 }
 
 }  // namespace llgmi

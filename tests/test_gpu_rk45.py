@@ -297,6 +297,56 @@ def test_rk45_graph_replay_equals_eager(gpu_lib, prec, monkeypatch):
             assert not rg[1]
 
 
+@pytest.mark.parametrize("path", ["fft_unfused", "fft_fused", "no_demag", "mg"])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_rk45_stage_epilogue_equals_stage_kernels(gpu_lib, prec, path, monkeypatch):
+    """The next stage state formed in the epilogue of the field evaluation
+    (the default) against a stage kernel of its own per stage
+    (LLGMI_RK45_FUSE=0): state, eps, accept and dt bitwise equal over 10
+    attempts (one rejected, one B_ext change) and over a device-controlled run
+    of 6 attempts; with the FFT demag (inverse row pass fused with the local
+    field or not), without demag, and with the multigrid demag."""
+    from conftest import DeviceBuffer
+    L = gpu_lib
+    P, es = (L.F32, 4) if prec == "f32" else (L.F64, 8)
+    nx, ny, cell = 40, 24, (3e-9, 3e-9, 2e-9)
+    m0, _, mat = _setup(nx, ny, 23, dmi=2e-3, k_u=3e4, demag=path != "no_demag")
+    if path == "mg":
+        mat.demag_method = L.DemagMethod.PoissonMG
+    mask = 1 if path == "no_demag" else 2
+    monkeypatch.setenv("LLGMI_FFT_FUSED", "1" if path == "fft_fused" else "0")
+    ca = L.Context(L.Grid2D(nx, ny, *cell), precision=P)
+    cb = L.Context(L.Grid2D(nx, ny, *cell), precision=P)
+    ba, bb = DeviceBuffer(3 * nx * ny * es), DeviceBuffer(3 * nx * ny * es)
+    ca.dev_upload(m0, ba)
+    cb.dev_upload(m0, bb)
+    pa = L.LLGParams(alpha=0.1, dt=1e-13, b_ext=(0.0, 0.0, 0.02))
+    pb = L.LLGParams(alpha=0.1, dt=1e-13, b_ext=(0.0, 0.0, 0.02))
+    for step in range(10):
+        if step == 4:
+            pa.dt = pb.dt = 4e-11                # rejected
+        if step == 7:
+            pa.b_ext = pb.b_ext = (0.01, 0.0, 0.02)
+        monkeypatch.delenv("LLGMI_RK45_FUSE", raising=False)
+        ra = ca.dev_rk45_step(ba, pa, mat, mask, 1e-5, 0.8, 1e-18, 1e-11)
+        monkeypatch.setenv("LLGMI_RK45_FUSE", "0")
+        rb = cb.dev_rk45_step(bb, pb, mat, mask, 1e-5, 0.8, 1e-18, 1e-11)
+        assert ra == rb, step
+        assert pa.dt == pb.dt
+        assert np.array_equal(ca.dev_download(ba), cb.dev_download(bb)), step
+        if step == 4:
+            assert not ra[1]
+    if path == "mg":
+        return   # the device-controlled run is built for the FFT demag
+    monkeypatch.delenv("LLGMI_RK45_FUSE", raising=False)
+    oa = ca.dev_rk45_run(ba, pa, mat, mask, 1e-5, 0.8, 1e-18, 1e-11, attempts=6)
+    monkeypatch.setenv("LLGMI_RK45_FUSE", "0")
+    ob = cb.dev_rk45_run(bb, pb, mat, mask, 1e-5, 0.8, 1e-18, 1e-11, attempts=6)
+    assert oa == ob
+    assert pa.dt == pb.dt
+    assert np.array_equal(ca.dev_download(ba), cb.dev_download(bb))
+
+
 def test_c1_one_nanosecond_trajectory_matches_oracle(gpu_lib):
     """The 128 x 64 film of 4 x 4 x 3 nm cells (Ms 8e5, A 1.3e-11, alpha 0.02,
     B = (-24.6, 4.3, 0) mT, FFT demag padded to 256 x 128), seeded
