@@ -1106,6 +1106,32 @@ BaseKey base_key(int64_t nx, int64_t ny, double dx, double dy, double dz, int64_
     std::memcpy(&k.dz, &dz, 8);
     return k;
 }
+
+// Tuned kernel choices, shared process-wide by every operator of one padded
+// shape and precision on one device: the first operator times the variants,
+// later ones (another context, the films of a batch) take its choice, so that
+// operators of the same shape always run the same kernels (their results are
+// then bitwise alike) and the timing is paid once.
+struct TuneKey {
+    int prec, dev;
+    int64_t nx, ny, px, py;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(prec, dev, nx, ny, px, py) < std::tie(o.prec, o.dev, o.nx, o.ny, o.px, o.py);
+    }
+};
+struct TuneVal {
+    int rq_row = 0, rq_col = 0, rq_row_inv = 0;   // 0: not tuned yet
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+// the fused-inverse choice, by shape and the unfused inverse pass's radix
+std::map<std::pair<TuneKey, int>, int> g_tune_fused;
+template <typename T>
+TuneKey tune_key(const DemagFFT<T> &op) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+    return TuneKey{(int)sizeof(T), dev, op.nx, op.ny, op.px, op.py};
+}
 }  // namespace
 
 template <typename T>
@@ -1469,6 +1495,18 @@ void DemagFFT<T>::autotune(hipStream_t stream, int forced) {
     rq_row = rq_col = rq_row_inv = forced ? forced : 4;
     if (pipe_forced == 1) rq_col |= kColPipe;
     if (forced || px * py < 256 * 256) return;
+    const bool shared = pipe_forced < 0;   // forced settings neither read nor set the shared choice
+    const TuneKey key = tune_key(*this);
+    if (shared) {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end() && it->second.rq_row) {
+            rq_row = it->second.rq_row;
+            rq_col = it->second.rq_col;
+            rq_row_inv = it->second.rq_row_inv;
+            return;
+        }
+    }
     const int64_t n = nx * ny;
     T *m = nullptr, *bd = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1531,6 +1569,13 @@ void DemagFFT<T>::autotune(hipStream_t stream, int forced) {
         rq_row = 3 + (bc & 1);
         rq_col = (3 + ((bc >> 1) & 1)) | ((bc & 8) ? kColPipe : 0);
         rq_row_inv = 3 + ((bc >> 2) & 1);
+        if (shared) {
+            std::lock_guard<std::mutex> lock(g_tune_mu);
+            TuneVal &v = g_tune[key];
+            v.rq_row = rq_row;
+            v.rq_col = rq_col;
+            v.rq_row_inv = rq_row_inv;
+        }
     } catch (...) {
         cleanup();
         throw;
@@ -1623,6 +1668,14 @@ void DemagFFT<T>::field_local(const T *m, T ms, const LocalArgs<T> &a, const uin
         } else if (px * py < 256 * 256) {
             rq_fused = fits(4) ? 4 : (fits(3) ? 3 : -1);
         } else {
+            const std::pair<TuneKey, int> fkey{tune_key(*this), rq_row_inv};
+            {
+                std::lock_guard<std::mutex> lock(g_tune_mu);
+                auto it = g_tune_fused.find(fkey);
+                if (it != g_tune_fused.end()) rq_fused = it->second;
+            }
+        }
+        if (rq_fused == 0) {
             hipEvent_t e0 = nullptr, e1 = nullptr;
             try {
                 LLGMI_HIP(hipEventCreate(&e0));
@@ -1651,6 +1704,8 @@ void DemagFFT<T>::field_local(const T *m, T ms, const LocalArgs<T> &a, const uin
                 // for them)
                 const float fb = std::min(best[1], best[2]);
                 rq_fused = fb < 0.97f * best[0] ? (best[1] <= best[2] ? 3 : 4) : -1;
+                std::lock_guard<std::mutex> lock(g_tune_mu);
+                g_tune_fused[{tune_key(*this), rq_row_inv}] = rq_fused;
             } catch (...) {
                 if (e0) (void)hipEventDestroy(e0);
                 if (e1) (void)hipEventDestroy(e1);

@@ -1600,7 +1600,7 @@ void fill_film_records(BatchPlan<T> &P, const std::vector<Engine<T> *> &engs, co
         r.in[2] = e.d_m7;
         r.stage = e.d_stage;
         r.m7 = e.d_m7;
-        r.b = e.d_b;
+        r.b = nullptr;   // the stages keep only the torque
         r.bd = e.d_bd;
         for (int q = 0; q < 8; ++q) r.k[q] = e.d_k[q];
     }
