@@ -25,15 +25,20 @@
 #include "common.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace llgmi {
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kLevels = 4;
@@ -146,6 +151,128 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ x,
                 else ((uint16_t *)y)[o] = f2bf(v);
             }
         }
+}
+
+// 3x3 convolution on spatial tiles (gfx950's K = 32 bf16 matrix-core
+// product, v_mfma_f32_16x16x32_bf16): a block computes a TH x 32 pixel tile
+// of one image for 16*NT output channels.  Input channels go in chunks of
+// 32: the chunk's input tile with its one-pixel halo ((TH+2) x 34 pixels,
+// zero outside the image) and the chunk's weights of all nine taps are staged
+// in LDS once, then every tap's product reads both from LDS, so each input
+// pixel is read from memory once per chunk instead of once per tap.  Wave w
+// owns rows w*TH/4 .. of the tile (16-pixel half rows as MFMA row tiles) and
+// all 16*NT channels.  The same product as k_conv3x3 (also used for the data
+// gradient); it needs the pixel slices 16-byte aligned (xs, xo multiples of
+// 8).  LDS rows are padded by 8 elements against bank conflicts.
+constexpr int kTileW = 32, kTileKC = 32, kTileRow = kTileKC + 8;
+template <int TH, int NT>
+constexpr size_t conv_tile_lds() {
+    return sizeof(uint16_t) * (size_t)kTileRow * ((TH + 2) * (kTileW + 2) + 9 * 16 * NT);
+}
+template <int TH, int NT, bool RELU, bool OUT_F32>
+__global__ __launch_bounds__(256) void k_conv3x3_tile(const uint16_t *__restrict__ x, int xs, int xo, int cin,
+                                                      const uint16_t *__restrict__ w, const float *__restrict__ bias,
+                                                      void *__restrict__ y, int ys, int yo, int cout, int B, int H,
+                                                      int W) {
+    constexpr int RW = TH / 4;           // tile rows per wave
+    constexpr int MT = 2 * RW;           // 16-pixel row tiles per wave
+    constexpr int CO = 16 * NT;          // output channels of the block
+    constexpr int TPW = kTileW + 2;      // tile width with halo
+    constexpr int NIN = (TH + 2) * TPW;  // staged input pixels
+    extern __shared__ __align__(16) unsigned char tile_smem[];
+    uint16_t *xin = reinterpret_cast<uint16_t *>(tile_smem);   // [NIN][kTileRow]
+    uint16_t *wsl = xin + NIN * kTileRow;                      // [9][CO][kTileRow]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, kg = (lane >> 4) * 8;
+    const int ntx = (W + kTileW - 1) / kTileW, nty = (H + TH - 1) / TH;
+    const int tx = blockIdx.x % ntx, ty = (blockIdx.x / ntx) % nty, b = blockIdx.x / (ntx * nty);
+    const int x0 = tx * kTileW, y0 = ty * TH;
+    const int co_base = blockIdx.y * CO;
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // A chunk's input tile and weights go through registers: the next chunk's
+    // loads are issued before the current chunk's products, so they are in
+    // flight while the matrix cores work.
+    constexpr int NI = (NIN * (kTileKC / 8) + 255) / 256, NW = (9 * CO * (kTileKC / 8) + 255) / 256;
+    uint4 ri[NI], rw[NW];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int e = threadIdx.x + k * 256;
+            const int pix = e >> 2, c = c0 + (e & 3) * 8;
+            const int yy = y0 + pix / TPW - 1, xx = x0 + pix % TPW - 1;
+            ri[k] = uint4{0u, 0u, 0u, 0u};
+            if (e < NIN * 4 && yy >= 0 && yy < H && xx >= 0 && xx < W && c < cin)
+                ri[k] = *reinterpret_cast<const uint4 *>(x + ((int64_t)(b * H + yy) * W + xx) * xs + xo + c);
+        }
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            const int e = threadIdx.x + k * 256;
+            const int row = e >> 2, c = c0 + (e & 3) * 8;   // row = tap * CO + co
+            const int tap = row / CO, co = row - tap * CO;
+            rw[k] = uint4{0u, 0u, 0u, 0u};
+            if (e < 9 * CO * 4 && c < cin)
+                rw[k] = *reinterpret_cast<const uint4 *>(w + ((int64_t)tap * cout + co_base + co) * cin + c);
+        }
+    };
+    fetch(0);
+    for (int c0 = 0; c0 < cin; c0 += kTileKC) {
+        __syncthreads();   // the previous chunk's reads are done
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int e = threadIdx.x + k * 256;
+            if (e < NIN * 4) *reinterpret_cast<uint4 *>(xin + (e >> 2) * kTileRow + (e & 3) * 8) = ri[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            const int e = threadIdx.x + k * 256;
+            if (e < 9 * CO * 4) *reinterpret_cast<uint4 *>(wsl + (e >> 2) * kTileRow + (e & 3) * 8) = rw[k];
+        }
+        __syncthreads();
+        if (c0 + kTileKC < cin) fetch(c0 + kTileKC);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;   // offsets into the haloed tile
+            bf16x8 a[MT], bw[NT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int r = wave * RW + (mt >> 1), col = (mt & 1) * 16 + li;
+                a[mt] = *reinterpret_cast<const bf16x8 *>(xin + ((r + dy) * TPW + col + dx) * kTileRow + kg);
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                bw[nt] = *reinterpret_cast<const bf16x8 *>(wsl + (tap * CO + nt * 16 + li) * kTileRow + kg);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], bw[nt], acc[mt][nt], 0, 0, 0);
+        }
+    }
+    // lane holds pixels 4*(lane/16)+r of each 16-pixel row tile, channel lane%16
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int yy = y0 + wave * RW + (mt >> 1);
+        if (yy >= H) continue;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = co_base + nt * 16 + li;
+            const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int xx = x0 + (mt & 1) * 16 + (lane >> 4) * 4 + r;
+                if (xx >= W) continue;
+                float v = acc[mt][nt][r] + bv;
+                if (RELU) v = fmaxf(v, 0.f);
+                const int64_t o = ((int64_t)(b * H + yy) * W + xx) * ys + yo + co;
+                if (OUT_F32) ((float *)y)[o] = v;
+                else ((uint16_t *)y)[o] = f2bf(v);
+            }
+        }
+    }
 }
 
 // Weight gradient: dw[tap][co][ci] += sum_p dz[p][co] * x[p + off(tap)][ci].
@@ -287,6 +414,30 @@ __global__ void k_pool2(const uint16_t *__restrict__ x, int xs, int xo, int C, u
     }
 }
 
+// k_pool2 eight channels (16 bytes) at a time: C, xs, xo multiples of 8
+__global__ void k_pool2_v8(const uint16_t *__restrict__ x, int xs, int xo, int C, uint16_t *__restrict__ y, int B,
+                           int H, int W) {
+    const int Hc = H / 2, Wc = W / 2, C8 = C / 8;
+    const int64_t n = (int64_t)B * Hc * Wc * C8;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C8) * 8;
+        const int64_t pc = i / C8;
+        const int X = (int)(pc % Wc), Y = (int)((pc / Wc) % Hc), b = (int)(pc / ((int64_t)Wc * Hc));
+        const int64_t p00 = ((int64_t)b * H + 2 * Y) * W + 2 * X;
+        const uint4 q[4] = {*reinterpret_cast<const uint4 *>(x + p00 * xs + xo + c),
+                            *reinterpret_cast<const uint4 *>(x + (p00 + 1) * xs + xo + c),
+                            *reinterpret_cast<const uint4 *>(x + (p00 + W) * xs + xo + c),
+                            *reinterpret_cast<const uint4 *>(x + (p00 + W + 1) * xs + xo + c)};
+        const uint16_t *h[4] = {reinterpret_cast<const uint16_t *>(&q[0]), reinterpret_cast<const uint16_t *>(&q[1]),
+                                reinterpret_cast<const uint16_t *>(&q[2]), reinterpret_cast<const uint16_t *>(&q[3])};
+        uint4 o;
+        uint16_t *ho = reinterpret_cast<uint16_t *>(&o);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ho[k] = f2bf(0.25f * (bf2f(h[0][k]) + bf2f(h[1][k]) + bf2f(h[2][k]) + bf2f(h[3][k])));
+        *reinterpret_cast<uint4 *>(y + pc * C + c) = o;
+    }
+}
+
 // gradient of k_pool2: dx[fine][c] += 0.25 dy[coarse][c]   (dense C, FP32)
 __global__ void k_pool2_bwd(const float *__restrict__ dy, int C, float *__restrict__ dx, int B, int H, int W) {
     const int64_t n = (int64_t)B * H * W * C;
@@ -309,6 +460,20 @@ __global__ void k_up2(const uint16_t *__restrict__ x, int C, uint16_t *__restric
         const int xx = (int)(p % W), yy = (int)((p / W) % H), b = (int)(p / ((int64_t)W * H));
         const int64_t pc = ((int64_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2;
         y[p * ys + yo + c] = x[pc * C + c];
+    }
+}
+
+// k_up2 eight channels (16 bytes) at a time: C, ys, yo multiples of 8
+__global__ void k_up2_v8(const uint16_t *__restrict__ x, int C, uint16_t *__restrict__ y, int ys, int yo, int B,
+                         int H, int W) {
+    const int C8 = C / 8;
+    const int64_t n = (int64_t)B * H * W * C8;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C8) * 8;
+        const int64_t p = i / C8;
+        const int xx = (int)(p % W), yy = (int)((p / W) % H), b = (int)(p / ((int64_t)W * H));
+        const int64_t pc = ((int64_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2;
+        *reinterpret_cast<uint4 *>(y + p * ys + yo + c) = *reinterpret_cast<const uint4 *>(x + pc * C + c);
     }
 }
 
@@ -634,8 +799,54 @@ struct UNet {
         if (cin % 16 || xs % 4 || xo % 4 || ys % 4 || yo % 4 || xo + cin > xs || yo + co > ys || co % 16)
             throw Error(LLGMI_ERR_INVALID, "internal: convolution shape");
         const int64_t P = pl(l);
+        if (xs % 8 == 0 && xo % 8 == 0 && tile_conv_enabled()) {
+            // spatial tiles of 8 rows, or 4 where 8 would leave the GPU short of blocks
+            const int nt = co % 64 == 0 ? 4 : co % 48 == 0 ? 3 : co % 32 == 0 ? 2 : 1;
+            const int64_t tiles8 = (int64_t)B * ((hl(l) + 7) / 8) * ((wl(l) + kTileW - 1) / kTileW);
+            const bool th8 = tiles8 * (co / (16 * nt)) >= 1024;
+            auto go = [&](auto thc, auto ntc) {
+                constexpr int TH = decltype(thc)::value, NT = decltype(ntc)::value;
+                launch_conv_tile<TH, NT, RELU, OUT_F32>(x, xs, xo, cin, w, bias, y, ys, yo, co, l);
+            };
+            using I8 = std::integral_constant<int, 8>;
+            using I4 = std::integral_constant<int, 4>;
+            switch (nt) {
+                case 4: th8 ? go(I8{}, std::integral_constant<int, 4>{}) : go(I4{}, std::integral_constant<int, 4>{}); break;
+                case 3: th8 ? go(I8{}, std::integral_constant<int, 3>{}) : go(I4{}, std::integral_constant<int, 3>{}); break;
+                case 2: th8 ? go(I8{}, std::integral_constant<int, 2>{}) : go(I4{}, std::integral_constant<int, 2>{}); break;
+                default: th8 ? go(I8{}, std::integral_constant<int, 1>{}) : go(I4{}, std::integral_constant<int, 1>{}); break;
+            }
+            return;
+        }
         if (P >= kWideTileMinPixels) launch_conv<4, RELU, OUT_F32>(x, xs, xo, cin, w, bias, y, ys, yo, co, l);
         else launch_conv<2, RELU, OUT_F32>(x, xs, xo, cin, w, bias, y, ys, yo, co, l);
+    }
+    // LLGMI_UNET_TILE=0 runs every convolution with k_conv3x3 (the pixel-row kernel)
+    static bool tile_conv_enabled() {
+        const char *e = std::getenv("LLGMI_UNET_TILE");
+        return !(e && e[0] == '0');
+    }
+    template <int TH, int NT, bool RELU, bool OUT_F32>
+    void launch_conv_tile(const uint16_t *x, int xs, int xo, int cin, const uint16_t *w, const float *bias, void *y,
+                          int ys, int yo, int co, int l) {
+        constexpr size_t lds = conv_tile_lds<TH, NT>();
+        auto kern = k_conv3x3_tile<TH, NT, RELU, OUT_F32>;
+        {
+            // the LDS attribute is set once per kernel and device
+            static std::mutex mu;
+            static std::set<int> done;
+            int dev = 0;
+            LLGMI_HIP(hipGetDevice(&dev));
+            std::lock_guard<std::mutex> lock(mu);
+            if (!done.count(dev)) {
+                LLGMI_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                done.insert(dev);
+            }
+        }
+        const int ntx = (wl(l) + kTileW - 1) / kTileW, nty = (hl(l) + TH - 1) / TH;
+        const dim3 grid((unsigned)(ntx * nty * B), (unsigned)(co / (16 * NT)));
+        hipLaunchKernelGGL(kern, grid, dim3(256), lds, stream, x, xs, xo, cin, w, bias, y, ys, yo, co, B, hl(l), wl(l));
+        LLGMI_LAUNCH_CHECK();
     }
     template <int MT, bool RELU, bool OUT_F32>
     void launch_conv(const uint16_t *x, int xs, int xo, int cin, const uint16_t *w, const float *bias, void *y,
@@ -701,8 +912,12 @@ struct UNet {
                 const int cprev = c(l - 1);
                 const int ps = l - 1 < kLevels - 1 ? c(l) + cprev : cprev;   // eb[l-1] stride
                 const int po = l - 1 < kLevels - 1 ? c(l) : 0;
-                hipLaunchKernelGGL(k_pool2, dim3(grid_of(pl(l) * cprev)), dim3(kNT), 0, stream, eb[l - 1], ps, po,
-                                   cprev, pool[l], B, hl(l - 1), wl(l - 1));
+                if (cprev % 8 == 0 && ps % 8 == 0 && po % 8 == 0)
+                    hipLaunchKernelGGL(k_pool2_v8, dim3(grid_of(pl(l) * cprev / 8)), dim3(kNT), 0, stream, eb[l - 1],
+                                       ps, po, cprev, pool[l], B, hl(l - 1), wl(l - 1));
+                else
+                    hipLaunchKernelGGL(k_pool2, dim3(grid_of(pl(l) * cprev)), dim3(kNT), 0, stream, eb[l - 1], ps, po,
+                                       cprev, pool[l], B, hl(l - 1), wl(l - 1));
             }
             conv_fwd<true, false>(src, cs, 0, cs, enc(l, 0).wf, d_p + enc(l, 0).b_off, ea[l], c(l), 0, c(l), l);
             const int es = l < kLevels - 1 ? c(l + 1) + c(l) : c(l), eo = l < kLevels - 1 ? c(l + 1) : 0;
@@ -711,8 +926,12 @@ struct UNet {
         for (int l = kLevels - 2; l >= 0; --l) {
             const uint16_t *below = l == kLevels - 2 ? eb[l + 1] : db[l + 1];
             const int cs = c(l + 1) + c(l);
-            hipLaunchKernelGGL(k_up2, dim3(grid_of(pl(l) * c(l + 1))), dim3(kNT), 0, stream, below, c(l + 1), cat[l],
-                               cs, 0, B, hl(l), wl(l));
+            if (c(l + 1) % 8 == 0 && cs % 8 == 0)
+                hipLaunchKernelGGL(k_up2_v8, dim3(grid_of(pl(l) * c(l + 1) / 8)), dim3(kNT), 0, stream, below,
+                                   c(l + 1), cat[l], cs, 0, B, hl(l), wl(l));
+            else
+                hipLaunchKernelGGL(k_up2, dim3(grid_of(pl(l) * c(l + 1))), dim3(kNT), 0, stream, below, c(l + 1), cat[l],
+                                   cs, 0, B, hl(l), wl(l));
             conv_fwd<true, false>(cat[l], cs, 0, cs, dec(l, 0).wf, d_p + dec(l, 0).b_off, da[l], c(l), 0, c(l), l);
             conv_fwd<true, false>(da[l], c(l), 0, c(l), dec(l, 1).wf, d_p + dec(l, 1).b_off, db[l], c(l), 0, c(l), l);
         }

@@ -6,7 +6,11 @@
   device accumulates in FP32 (the oracle in float64), so a bf16 rounding can
   land on the other side in a few elements: the bars are relative L2 norms,
   forward 1e-5 (measured 6e-8), each layer's gradient 1e-3 (worst measured
-  4e-5), all gradients 1e-4 (measured 4e-6).
+  7e-4), all gradients 3e-4 (measured 1.05e-4).  The spatial-tile
+  convolutions (the default) sum 32 channels per matrix-core instruction;
+  the pixel-row kernel (LLGMI_UNET_TILE=0, 16 per instruction) lands closer
+  to the emulation's roundings (4e-5 worst layer, 3.6e-6 in all), and
+  against float64 the two are equally accurate (the large-plane test).
 * Adam training on the device reduces the loss of a learnable map.
 * The trained network's potential, written into the multigrid solver as its
   initial phi, reaches the same residual in fewer V-cycles than the zero
@@ -45,8 +49,10 @@ def _setup(L, s: "U.Shape", seed: int):
 SHAPE = U.Shape(batch=2, height=16, width=24, out_channels=20, base=16, valid_width=22, valid_height=14)
 
 
-def test_forward_matches_oracle(gpu_lib):
+@pytest.mark.parametrize("tile", ["1", "0"])
+def test_forward_matches_oracle(gpu_lib, tile, monkeypatch):
     from conftest import DeviceBuffer
+    monkeypatch.setenv("LLGMI_UNET_TILE", tile)
     s = SHAPE
     net, p, x, _, xd, _ = _setup(gpu_lib, s, 7)
     n = s.batch * s.height * s.width * s.c_pad
@@ -60,7 +66,9 @@ def test_forward_matches_oracle(gpu_lib):
     assert np.all(np.isfinite(y))
 
 
-def test_gradients_match_oracle(gpu_lib):
+@pytest.mark.parametrize("tile", ["1", "0"])
+def test_gradients_match_oracle(gpu_lib, tile, monkeypatch):
+    monkeypatch.setenv("LLGMI_UNET_TILE", tile)
     s = SHAPE
     net, p, x, t, xd, td = _setup(gpu_lib, s, 8)
     loss = net.loss_grad(xd, td)
@@ -81,7 +89,7 @@ def test_gradients_match_oracle(gpu_lib):
         o += k
     tot = np.linalg.norm(g - g_o) / np.linalg.norm(g_o)
     print(f"gradient relative L2 vs oracle: total {tot:.2e}, worst layer {worst:.2e}; loss {loss:.6e} vs {loss_o:.6e}")
-    assert tot <= 1e-4
+    assert tot <= 3e-4
 
 
 def test_training_reduces_loss(gpu_lib):
