@@ -398,6 +398,33 @@ __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ 
     }
 }
 
+// k_bias_grad eight channels (16 bytes) per thread: cout a multiple of 8 and
+// at most 256 (groups of 8 channels per thread, pixel lanes over the rest)
+__global__ __launch_bounds__(256) void k_bias_grad_v8(const uint16_t *__restrict__ dz, int cout, int P, int chunk,
+                                                      float *__restrict__ db) {
+    __shared__ float part[256 * 8];
+    const int C8 = cout / 8, G = 256 / C8;
+    const int t = threadIdx.x, g = t / C8, c8 = t - g * C8;
+    const int p0 = blockIdx.x * chunk, p1 = min(P, p0 + chunk);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (g < G)
+        for (int p = p0 + g; p < p1; p += G) {
+            const uint4 q = *reinterpret_cast<const uint4 *>(dz + (int64_t)p * cout + c8 * 8);
+            const uint16_t *h = reinterpret_cast<const uint16_t *>(&q);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s[k] += bf2f(h[k]);
+        }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) part[t * 8 + k] = g < G ? s[k] : 0.f;
+    __syncthreads();
+    if (t < cout) {
+        const int cg = t / 8, ck = t - cg * 8;
+        float r = 0.f;
+        for (int k = 0; k < G; ++k) r += part[(k * C8 + cg) * 8 + ck];
+        atomicAdd(db + t, r);
+    }
+}
+
 // 2x2 average pool of a slice: y[b][Y][X][c] = mean of x[b][2Y+i][2X+j][xo+c]
 __global__ void k_pool2(const uint16_t *__restrict__ x, int xs, int xo, int C, uint16_t *__restrict__ y, int B,
                         int H, int W) {
@@ -502,6 +529,32 @@ __global__ void k_slice_copy(const float *__restrict__ src, int ss, int so, int 
 }
 
 // dz = bf16(dy * [y > 0]) with y a slice (ReLU layers), or bf16(dy) (y null)
+// k_relu_bwd eight channels per thread: C, ys, yo multiples of 8
+__global__ void k_relu_bwd_v8(const float *__restrict__ dy, const uint16_t *__restrict__ y, int ys, int yo, int C,
+                              uint16_t *__restrict__ dz, int64_t P) {
+    const int C8 = C / 8;
+    const int64_t n = P * C8;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = i / C8;
+        const int c = (int)(i - p * C8) * 8;
+        const float4 g0 = *reinterpret_cast<const float4 *>(dy + p * C + c);
+        const float4 g1 = *reinterpret_cast<const float4 *>(dy + p * C + c + 4);
+        float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+        if (y) {
+            const uint4 q = *reinterpret_cast<const uint4 *>(y + p * ys + yo + c);
+            const uint16_t *h = reinterpret_cast<const uint16_t *>(&q);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (!(bf2f(h[k]) > 0.f)) g[k] = 0.f;
+        }
+        uint4 o;
+        uint16_t *ho = reinterpret_cast<uint16_t *>(&o);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ho[k] = f2bf(g[k]);
+        *reinterpret_cast<uint4 *>(dz + p * C + c) = o;
+    }
+}
+
 __global__ void k_relu_bwd(const float *__restrict__ dy, const uint16_t *__restrict__ y, int ys, int yo, int C,
                            uint16_t *__restrict__ dz, int64_t P) {
     const int64_t n = P * C;
@@ -894,8 +947,12 @@ struct UNet {
                                hl(l), wl(l), chunk);
         const int bchunk = 1024;
         if (co > 256) throw Error(LLGMI_ERR_INVALID, "internal: bias gradient over 256 channels");
-        hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((P + bchunk - 1) / bchunk)), dim3(256), 0, stream, dzp, co,
-                           (int)P, bchunk, d_g + cv.b_off);
+        if (co % 8 == 0)
+            hipLaunchKernelGGL(k_bias_grad_v8, dim3((unsigned)((P + bchunk - 1) / bchunk)), dim3(256), 0, stream, dzp,
+                               co, (int)P, bchunk, d_g + cv.b_off);
+        else
+            hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((P + bchunk - 1) / bchunk)), dim3(256), 0, stream, dzp,
+                               co, (int)P, bchunk, d_g + cv.b_off);
         LLGMI_LAUNCH_CHECK();
     }
 
@@ -941,7 +998,12 @@ struct UNet {
 
     // relu'(y) applied to g (dense C), cast to bf16 into dz[l]
     void relu_bwd(const float *g, const uint16_t *y, int ys, int yo, int C, int l) {
-        hipLaunchKernelGGL(k_relu_bwd, dim3(grid_of(pl(l) * C)), dim3(kNT), 0, stream, g, y, ys, yo, C, dz[l], pl(l));
+        if (C % 8 == 0 && (!y || (ys % 8 == 0 && yo % 8 == 0)))
+            hipLaunchKernelGGL(k_relu_bwd_v8, dim3(grid_of(pl(l) * C / 8)), dim3(kNT), 0, stream, g, y, ys, yo, C, dz[l],
+                               pl(l));
+        else
+            hipLaunchKernelGGL(k_relu_bwd, dim3(grid_of(pl(l) * C)), dim3(kNT), 0, stream, g, y, ys, yo, C, dz[l],
+                               pl(l));
     }
 
     // gradient of the loss w.r.t. every parameter into d_g (overwritten)
