@@ -297,7 +297,7 @@ def test_rk45_graph_replay_equals_eager(gpu_lib, prec, monkeypatch):
             assert not rg[1]
 
 
-@pytest.mark.parametrize("path", ["fft_unfused", "fft_fused", "no_demag", "mg"])
+@pytest.mark.parametrize("path", ["fft_unfused", "fft_fused", "fft_radix3", "no_demag", "mg"])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_rk45_stage_epilogue_equals_stage_kernels(gpu_lib, prec, path, monkeypatch):
     """The next stage state formed in the epilogue of the field evaluation
@@ -305,11 +305,14 @@ def test_rk45_stage_epilogue_equals_stage_kernels(gpu_lib, prec, path, monkeypat
     (LLGMI_RK45_FUSE=0): state, eps, accept and dt bitwise equal over 10
     attempts (one rejected, one B_ext change) and over a device-controlled run
     of 6 attempts; with the FFT demag (inverse row pass fused with the local
-    field or not), without demag, and with the multigrid demag."""
+    field or not; a 384 x 8 film whose rows pad to 768 = 3 * 256, the radix-3
+    transforms), without demag, and with the multigrid demag."""
     from conftest import DeviceBuffer
     L = gpu_lib
     P, es = (L.F32, 4) if prec == "f32" else (L.F64, 8)
-    nx, ny, cell = 40, 24, (3e-9, 3e-9, 2e-9)
+    nx, ny, cell = (384, 8, (3e-9, 3e-9, 2e-9)) if path == "fft_radix3" else (40, 24, (3e-9, 3e-9, 2e-9))
+    if path == "fft_radix3":
+        monkeypatch.setenv("LLGMI_FFT_PAD", "3")
     m0, _, mat = _setup(nx, ny, 23, dmi=2e-3, k_u=3e4, demag=path != "no_demag")
     if path == "mg":
         mat.demag_method = L.DemagMethod.PoissonMG
