@@ -425,4 +425,75 @@ bool read_kernel_cache(const std::string &path, const llgmi_grid &g, int64_t pbc
     return true;
 }
 
+namespace {
+constexpr char kDkMagic[8] = {'L', 'L', 'G', 'D', 'K', 'H', '8', '\0'};
+constexpr size_t kDkFields = sizeof(DkCacheKey) / sizeof(uint64_t);
+static_assert(kDkFields == 23, "DeltaKernelKey has 23 fields");
+std::string hex16(uint64_t v) {
+    char b[24];
+    std::snprintf(b, sizeof b, "%016llx", (unsigned long long)v);
+    return b;
+}
+double from_bits(uint64_t u) {
+    double d;
+    std::memcpy(&d, &u, 8);
+    return d;
+}
+}  // namespace
+
+std::string dk_cache_path(const DkCacheKey &k) {
+    // MGOperatorSettings::tag (:806-827)
+    std::string st = k.stencil == 0 ? "7" : k.stencil == 1 ? "iso9"
+                   : (k.iso27_alpha_bits != 0 ? "iso27a" + hex16(k.iso27_alpha_bits) : "iso27");
+    const std::string op = st + "_" + (k.prolong == 0 ? "inj" : "tri") + "_" + (k.coarse_op == 0 ? "rd" : "gal");
+    const char *bc = k.bc == 0 ? "dir0" : k.bc == 1 ? "dip" : "bh";
+    const char *sm = k.smoother == 0 ? "wj" : "rb";
+    return "out/demag_cache/demag_mg_hybrid_dk_nx" + std::to_string(k.nx) + "_ny" + std::to_string(k.ny) + "_dx" +
+           rust_exp(from_bits(k.dx_bits), 3) + "_dy" + rust_exp(from_bits(k.dy_bits), 3) + "_dz" +
+           rust_exp(from_bits(k.dz_bits), 3) + "_pad" + std::to_string(k.pad_xy) + "_nvacz" +
+           std::to_string(k.n_vac_z) + "_op" + op + "_bc" + bc + "_th" + rust_exp(from_bits(k.tree_theta_bits), 3) +
+           "_leaf" + std::to_string(k.tree_leaf) + "_dep" + std::to_string(k.tree_max_depth) + "_sm" + sm + "_pre" +
+           std::to_string(k.pre_smooth) + "_post" + std::to_string(k.post_smooth) + "_om" + hex16(k.omega_bits) +
+           "_som" + hex16(k.sor_omega_bits) + "_sig" + hex16(k.sigma_bits) + "_r" + std::to_string(k.radius_xy) +
+           "_dv" + std::to_string(k.delta_v_cycles) + ".bin";
+}
+
+// save_delta_kernel_to_disk (:2966-3052)
+void write_dk_cache(const std::string &path, const DkCacheKey &k, const std::vector<double> &dk4) {
+    const size_t n = (size_t)((2 * k.radius_xy + 1) * (2 * k.radius_xy + 1));
+    if (dk4.size() != 4 * n) throw Error(LLGMI_ERR_INVALID, "correction stencil size does not match its radius");
+    ensure_parent_dir(path);
+    const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+    File f(tmp, "wb");
+    f.bytes(kDkMagic, 8);
+    f.bytes(&k, sizeof k);   // 23 little-endian u64 in the reference's order
+    f.bytes(dk4.data(), sizeof(double) * dk4.size());
+    const std::vector<double> zeros(3 * n, 0.0);
+    f.bytes(zeros.data(), sizeof(double) * zeros.size());
+    f.close();
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) {
+        std::remove(tmp.c_str());
+        throw Error(LLGMI_ERR_INVALID, "cannot rename " + tmp + " to " + path);
+    }
+}
+
+// load_delta_kernel_from_disk (:2821-2964); the PPPM-phi stencils that follow
+// are not read
+bool read_dk_cache(const std::string &path, const DkCacheKey &k, std::vector<double> &dk4) {
+    std::FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    char magic[8];
+    uint64_t u[kDkFields];
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kDkMagic, 8) == 0 &&
+              std::fread(u, sizeof(uint64_t), kDkFields, f) == kDkFields && std::memcmp(u, &k, sizeof k) == 0;
+    if (ok) {
+        const size_t n = (size_t)((2 * k.radius_xy + 1) * (2 * k.radius_xy + 1));
+        std::vector<double> v(4 * n);
+        ok = std::fread(v.data(), sizeof(double), v.size(), f) == v.size();
+        if (ok) dk4.swap(v);
+    }
+    std::fclose(f);
+    return ok;
+}
+
 }  // namespace llgmi

@@ -284,7 +284,11 @@ struct Engine : EngineBase {
     }
 
     // ensure_delta_kernel (demag_poisson_mg.rs:2412-2466): rebuilt when the MG
-    // settings, the hybrid settings or the effective radius change.
+    // settings, the hybrid settings or the effective radius change.  With
+    // LLG_DEMAG_MG_HYBRID_CACHE=1 the stencil is read from, or after a build
+    // written to, the reference's LLGDKH8 file; a missing, short or foreign
+    // file is a miss, and a failed write only costs the next process a
+    // rebuild (as in the reference).
     void ensure_delta_kernel(const llgmi_mg_hybrid_config &h, int64_t r, const llgmi_mg_config &c,
                              const llgmi_material &mat) {
         llgmi_mg_hybrid_config hk = h;
@@ -294,8 +298,65 @@ struct Engine : EngineBase {
         key.append((const char *)&hk, sizeof hk);
         key.append((const char *)&r, sizeof r);
         if (d_dk && key == dk_key) return;
+        const char *ce = std::getenv("LLG_DEMAG_MG_HYBRID_CACHE");
+        const bool disk = ce && std::string(ce) == "1";
+        const DkCacheKey dkey = dk_cache_key(h, r, c);
+        if (disk) {
+            std::vector<double> v;
+            if (read_dk_cache(dk_cache_path(dkey), dkey, v)) {
+                dk_host.swap(v);
+                upload_dk();
+                dk_key = key;
+                return;
+            }
+        }
         build_delta_kernel(h, r, c, mat);
         dk_key = key;
+        if (disk) {
+            try {
+                write_dk_cache(dk_cache_path(dkey), dkey, dk_host);
+            } catch (const Error &) {
+            }
+        }
+    }
+    // DeltaKernelKey::new (demag_poisson_mg.rs:2305-2333) for radius r
+    DkCacheKey dk_cache_key(const llgmi_mg_hybrid_config &h, int64_t r, const llgmi_mg_config &c) const {
+        auto bits = [](double d) {
+            uint64_t u;
+            std::memcpy(&u, &d, 8);
+            return u;
+        };
+        DkCacheKey k{};
+        k.nx = (uint64_t)g.nx;
+        k.ny = (uint64_t)g.ny;
+        k.dx_bits = bits(g.dx);
+        k.dy_bits = bits(g.dy);
+        k.dz_bits = bits(g.dz);
+        k.pad_xy = (uint64_t)c.pad_xy;
+        k.n_vac_z = (uint64_t)c.n_vac_z;
+        k.stencil = (uint64_t)c.stencil;
+        k.prolong = (uint64_t)c.prolong;
+        k.coarse_op = (uint64_t)c.coarse_op;
+        k.iso27_alpha_bits = c.iso27_alpha > 0.0 ? bits(c.iso27_alpha) : 0;
+        k.bc = (uint64_t)c.bc;
+        k.tree_theta_bits = bits(c.tree_theta);
+        k.tree_leaf = (uint64_t)c.tree_leaf;
+        k.tree_max_depth = (uint64_t)c.tree_max_depth;
+        k.pre_smooth = (uint64_t)c.pre_smooth;
+        k.post_smooth = (uint64_t)c.post_smooth;
+        k.smoother = (uint64_t)c.smoother;
+        k.omega_bits = bits(c.omega);
+        k.sor_omega_bits = bits(c.sor_omega);
+        k.sigma_bits = bits(h.sigma_cells);
+        k.radius_xy = (uint64_t)r;
+        k.delta_v_cycles = (uint64_t)h.delta_v_cycles;
+        return k;
+    }
+    void upload_dk() {
+        if (d_dk) (void)hipFree(d_dk);
+        d_dk = nullptr;
+        LLGMI_HIP(hipMalloc(&d_dk, sizeof(double) * dk_host.size()));
+        LLGMI_HIP(hipMemcpy(d_dk, dk_host.data(), sizeof(double) * dk_host.size(), hipMemcpyHostToDevice));
     }
 
     // build_delta_kernel_impulse (demag_poisson_mg.rs:3159-3597), without its
@@ -397,10 +458,7 @@ struct Engine : EngineBase {
         std::copy(xy.begin(), xy.end(), dk_host.begin() + nst);
         std::copy(yy.begin(), yy.end(), dk_host.begin() + 2 * nst);
         std::copy(zz.begin(), zz.end(), dk_host.begin() + 3 * nst);
-        if (d_dk) (void)hipFree(d_dk);
-        d_dk = nullptr;
-        LLGMI_HIP(hipMalloc(&d_dk, sizeof(double) * dk_host.size()));
-        LLGMI_HIP(hipMemcpy(d_dk, dk_host.data(), sizeof(double) * dk_host.size(), hipMemcpyHostToDevice));
+        upload_dk();
     }
 
     LocalArgs<T> local_args(const llgmi_params &p, const llgmi_material &mat, int mask, bool torque,

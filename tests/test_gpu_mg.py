@@ -307,3 +307,60 @@ def test_mg_hybrid_env_and_fp32(gpu_lib, monkeypatch):
     c32 = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F32)
     c32.set_mg_config(L.mg_config(**kw))
     assert rel_err(c32.compute_demag_field_poisson_mg(m, L.Material(ms=8e5, demag=True)), want) <= 1e-5
+
+
+def test_mg_hybrid_stencil_cache_file(gpu_lib, monkeypatch, tmp_path):
+    """LLG_DEMAG_MG_HYBRID_CACHE=1 writes the correction stencil to the
+    reference's LLGDKH8 file (demag_poisson_mg.rs:2811-3052) under
+    out/demag_cache, named by DeltaKernelKey::cache_path (:2335-2374) from the
+    key the file holds; a new context reads it back (bitwise the same field),
+    uses what the file holds (a doubled stencil changes the field), and treats
+    a truncated file as a miss (rebuilt and rewritten)."""
+    import struct
+    L = gpu_lib
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("LLG_DEMAG_MG_HYBRID_CACHE", "1")
+    nx, ny, cell = 24, 20, (4e-9, 4e-9, 2e-9)
+    m = _rand_m(nx * ny, 31)
+    mat = L.Material(ms=8e5, demag=True)
+
+    def field():
+        c = L.Context(L.Grid2D(nx, ny, *cell), precision=L.F64)
+        c.set_mg_config(L.mg_config(warm_start=0, pad_xy=3, n_vac_z=4, v_cycles=3))
+        c.set_mg_hybrid_config(L.mg_hybrid_config(enabled=1, radius_xy=3, delta_v_cycles=10, sigma_cells=0.8))
+        return c.compute_demag_field_poisson_mg(m, mat)
+
+    b0 = field()
+    files = sorted((tmp_path / "out" / "demag_cache").glob("*"))
+    assert len(files) == 1, files
+    path = files[0]
+    raw = path.read_bytes()
+    n = 7 * 7
+    assert raw[:8] == b"LLGDKH8\x00" and len(raw) == 8 + 23 * 8 + 7 * n * 8
+    u = struct.unpack("<23Q", raw[8:8 + 23 * 8])
+    f64 = lambda b: struct.unpack("<d", struct.pack("<Q", b))[0]
+    assert u[0:2] == (nx, ny) and [f64(x) for x in u[2:5]] == list(cell)
+    assert u[5:7] == (3, 4) and u[21:23] == (3, 10) and f64(u[20]) == 0.8
+
+    def rexp(v):
+        mant, e = f"{v:.3e}".split("e")
+        return f"{mant}e{int(e)}"
+    st = {0: "7", 1: "iso9"}.get(u[7], "iso27" if u[10] == 0 else f"iso27a{u[10]:016x}")
+    op = f"{st}_{'inj' if u[8] == 0 else 'tri'}_{'rd' if u[9] == 0 else 'gal'}"
+    name = (f"demag_mg_hybrid_dk_nx{u[0]}_ny{u[1]}_dx{rexp(f64(u[2]))}_dy{rexp(f64(u[3]))}_dz{rexp(f64(u[4]))}"
+            f"_pad{u[5]}_nvacz{u[6]}_op{op}_bc{['dir0', 'dip', 'bh'][u[11]]}_th{rexp(f64(u[12]))}_leaf{u[13]}"
+            f"_dep{u[14]}_sm{['wj', 'rb'][u[17]]}_pre{u[15]}_post{u[16]}_om{u[18]:016x}_som{u[19]:016x}"
+            f"_sig{u[20]:016x}_r{u[21]}_dv{u[22]}.bin")
+    assert path.name == name
+    dk = np.frombuffer(raw[192:192 + 4 * n * 8], dtype="<f8").reshape(4, n)
+    assert np.all(np.frombuffer(raw[192 + 4 * n * 8:], dtype="<f8") == 0.0)   # PPPM-phi stencils: none
+    for c in (0, 2, 3):   # the uniform-bias fix leaves the diagonal stencils summing to zero
+        assert abs(dk[c].sum()) <= 1e-12 * np.abs(dk[c]).max()
+
+    np.testing.assert_array_equal(field(), b0)          # read back
+    path.write_bytes(raw[:192] + (2.0 * dk).astype("<f8").tobytes() + raw[192 + 4 * n * 8:])
+    assert np.abs(field() - b0).max() > 1e-6 * np.abs(b0).max()   # the file's stencil is the one used
+    path.write_bytes(raw[:200])                          # truncated: a miss, rebuilt and rewritten
+    np.testing.assert_array_equal(field(), b0)
+    assert path.read_bytes() == raw
+
